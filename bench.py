@@ -1,0 +1,175 @@
+"""HMVM benchmark: ms per H-matrix-vector multiply at N=101,250 on one MI355X.
+
+Workload (one "step" = one HMVM y = A x of one right-hand side): the H-matrix of
+1/(4 pi r) on a jittered grid of the unit-cube surface, N = 101,250
+(6 faces x 135 x 125), leaves of <= 32 points, eta = 2, ACA eps = 1e-4; x ~ U[-1,1].
+The matrix is generated, written as hmatrix_1.bin, read back through
+loadHmatrix (the drop-in path) and uploaded once; the timed region is the GPU
+launches of K back-to-back HMVMs with x and y resident in HBM.
+
+Reported: FP64 storage (the headline `value`, ms/HMVM), FP32 storage (ms and
+relative l2 error vs the FP64 oracle), roofline of the dominant kernel from
+HIP events, and the CPU baseline: the oracle's OpenMP HMVM (restating
+src_c/hmvm_omp.c) on the host cores, plus the sequential oracle.
+"""
+import argparse
+import json
+import os
+import platform
+import subprocess
+import sys
+import tempfile
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_model():
+    try:
+        out = subprocess.run(["lscpu"], stdout=subprocess.PIPE, text=True).stdout
+        for line in out.splitlines():
+            if line.startswith("Model name"):
+                return line.split(":", 1)[1].strip()
+    except Exception:
+        pass
+    return platform.processor()
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--n", type=int, default=101250)
+    ap.add_argument("--geometry", default="cube")
+    ap.add_argument("--eps", type=float, default=1e-4)
+    ap.add_argument("--leaf", type=int, default=32)
+    ap.add_argument("--in-memory", action="store_true", help="skip the hmatrix_1.bin round trip")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--w1", type=int, default=0)
+    ap.add_argument("--w2", type=int, default=0)
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py: this build measures one GPU only (the HMVM path is single-device; "
+              "row-band sharding over several GPUs is a design note, not implemented). Use --gpus 1.",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+
+    import hacapk_hmvm_amd as H
+
+    if H.device_count() < 1:
+        raise SystemExit("bench.py: no HIP device visible")
+
+    t0 = time.time()
+    g = H.generate(args.n, geometry=args.geometry, eps=args.eps, leaf=args.leaf)
+    log("generated N=%d nlf=%d len=%d ktmax=%d in %.1fs" % (g.nd, g.mat2.nlf, g.mat2.len, g.mat2.ktmax,
+                                                              time.time() - t0))
+    if args.in_memory:
+        h = g
+    else:
+        tmpd = tempfile.mkdtemp(prefix="hmvm_bench_")
+        fname = os.path.join(tmpd, "hmatrix_1.bin")
+        t0 = time.time()
+        H.writeHmatrix1(fname, g)
+        g.free()
+        h = H.loadHmatrix(fname)
+        os.remove(fname)
+        log("hmatrix_1.bin written and loaded in %.1fs" % (time.time() - t0))
+    nd, length = h.nd, h.mat2.len
+    x = np.random.default_rng(20261202).uniform(-1.0, 1.0, nd)
+
+    results = {}
+    ys = {}
+    for storage in ("f64", "f32"):
+        t0 = time.time()
+        plan = H.Plan(h, storage=storage, w1=args.w1, w2=args.w2)
+        info = plan.info()
+        ys[storage] = plan.apply_host(x)
+        tm = plan.time(warmup=args.warmup, iters=args.steps)
+        log("%s: plan %.1fs tiles=%d+%d panels=%d ms/HMVM stream=%.4f evented=%.4f (p1 %.4f tsum %.4f p2 %.4f "
+            "ysum %.4f)" % (storage, time.time() - t0, info["ntiles1"], info["ntiles2"], info["npanels"],
+                            tm["ms_stream"], tm["ms_avg"], tm["ms_phase1"], tm["ms_tsum"], tm["ms_phase2"],
+                            tm["ms_ysum"]))
+        results[storage] = (info, tm)
+        plan.close()
+
+    info64, tm64 = results["f64"]
+    info32, tm32 = results["f32"]
+    eb = 8
+    ms = tm64["ms_stream"]
+    b_alg = info64["algorithmic_bytes"]
+    whole_gbs = b_alg / (ms * 1e-3) / 1e9
+    # dominant kernel: the larger of the two tile launches
+    if tm64["ms_phase2"] >= tm64["ms_phase1"]:
+        kname, kms, kbytes = "tile_kernel(phase2)", tm64["ms_phase2"], eb * info64["elems_phase2"]
+    else:
+        kname, kms, kbytes = "tile_kernel(phase1)", tm64["ms_phase1"], eb * info64["elems_phase1"]
+    k_gbs = kbytes / (kms * 1e-3) / 1e9
+    f32_ms = tm32["ms_stream"]
+    f32_gbs = info32["algorithmic_bytes"] / (f32_ms * 1e-3) / 1e9
+    rel_f32_vs_f64hip = float(np.linalg.norm(ys["f32"] - ys["f64"]) / np.linalg.norm(ys["f64"]))
+
+    cpu = None
+    rel64 = rel32 = None
+    if not args.no_cpu_baseline:
+        import oracle as O  # CPU baseline leg: the oracle, timed on the host cores
+        t0 = time.time()
+        y_or = O.seq_2(h, x)
+        seq_s = time.time() - t0
+        rel64 = float(np.linalg.norm(ys["f64"] - y_or) / np.linalg.norm(y_or))
+        rel32 = float(np.linalg.norm(ys["f32"] - y_or) / np.linalg.norm(y_or))
+        times = []
+        for _ in range(10):  # hmvm_omp_bench protocol: 10 runs, the last 5 averaged
+            t0 = time.perf_counter()
+            O.omp_2(h, x)
+            times.append(time.perf_counter() - t0)
+        omp_ms = 1e3 * sum(times[5:]) / 5
+        cpu = {"value": round(omp_ms, 3), "unit": "ms/HMVM", "cores": O.omp_threads(), "kind": "port",
+               "sample": "full workload (N=%d, len=%d), oracle hmvm_omp_2 restatement, 10 runs, mean of last 5; "
+                         "sequential oracle hmvm_seq_2 %.1f ms; host %s" % (nd, length, 1e3 * seq_s, cpu_model())}
+        log("cpu: omp %.2f ms (%d threads), seq %.1f ms; rel err f64 %.2e f32 %.2e" %
+            (omp_ms, O.omp_threads(), 1e3 * seq_s, rel64, rel32))
+
+    line = {
+        "metric": "ms/HMVM & % HBM peak at N=101,250: FP64 plain vs FP32-stored, rel err vs FP64",
+        "value": round(ms, 5),
+        "unit": "ms/HMVM",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 5),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic: generated H-matrix (1/(4 pi r), cube surface), x ~ U[-1,1] seed 20261202",
+        "config": {"workload": "HMVM y=A*x, N=%d %s surface, leaf<=%d, eta=2, ACA eps=%g, 1 RHS" %
+                               (nd, args.geometry, args.leaf, args.eps),
+                   "N": nd, "nlf": info64["nlf"], "len": length, "ktmax": h.mat2.ktmax,
+                   "bytes_fp64": info64["algorithmic_bytes"], "parallelism": "single GPU"},
+        "hbm_frac_whole_hmvm": round(whole_gbs / HBM_PEAK_GBS, 4),
+        "fp64": {"ms": round(ms, 5), "ms_evented": round(tm64["ms_avg"], 5), "GBps": round(whole_gbs, 1),
+                 "rel_err_vs_oracle": rel64,
+                 "phase_ms": {"tile1": tm64["ms_phase1"], "tsum": tm64["ms_tsum"], "tile2": tm64["ms_phase2"],
+                              "ysum": tm64["ms_ysum"]}},
+        "fp32": {"ms": round(f32_ms, 5), "GBps": round(f32_gbs, 1), "speedup_vs_fp64": round(ms / f32_ms, 3),
+                 "rel_err_vs_oracle": rel32, "rel_err_vs_fp64_hip": rel_f32_vs_f64hip},
+        "roofline": {"bound": "hbm", "kernel": kname, "achieved": round(k_gbs, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(k_gbs / HBM_PEAK_GBS, 4), "traffic": None},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,423 @@
+"""hacapk_hmvm_amd -- MI355X (gfx950) H-matrix-vector multiply.
+
+Python mirror of the C ABI in include/hacapk.h, include/hgen.h and
+include/hmvm_hip.h.  All compute runs in the native library
+``hacapk_hmvm_amd/libhmvm_hip.so`` (HIP kernels); this module only marshals
+arguments.  Importing it fails loudly when the library has not been built:
+there is no Python or CPU fallback for the HMVM.
+
+Names follow the reference benchmark (src_hmvm/src_c/hacapk.h): ``matrix`` /
+``matrix2`` structs, ``loadHmatrix`` / ``loadHmatrix2`` loaders and the
+``hmvm_hip_1/1t/2/2t`` kernels with the argument order of ``hmvm_seq_*``
+(v, mat, b) and the same accumulate semantics (v += A b).
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libhmvm_hip.so")
+
+c_int_p = ctypes.POINTER(ctypes.c_int)
+c_double_p = ctypes.POINTER(ctypes.c_double)
+
+
+class submatrix(ctypes.Structure):
+    """src_c/hacapk.h:5-8"""
+    _fields_ = [("ltmtx", ctypes.c_int), ("kt", ctypes.c_int), ("ndl", ctypes.c_int), ("ndt", ctypes.c_int),
+                ("nstrtl", ctypes.c_int), ("nstrtt", ctypes.c_int),
+                ("a1", c_double_p), ("a2", c_double_p), ("a2t", c_double_p)]
+
+
+class matrix(ctypes.Structure):
+    """src_c/hacapk.h:9-17"""
+    _fields_ = [("nd", ctypes.c_int), ("nlf", ctypes.c_int), ("ktmax", ctypes.c_int), ("nleavs", c_int_p),
+                ("submat", ctypes.POINTER(submatrix)),
+                ("b1", ctypes.c_int), ("b2", ctypes.c_int), ("b3", ctypes.c_int),
+                ("e1", ctypes.c_int), ("e2", ctypes.c_int), ("e3", ctypes.c_int)]
+
+
+class matrix2(ctypes.Structure):
+    """src_c/hacapk.h:19-28"""
+    _fields_ = [("nd", ctypes.c_int), ("nlf", ctypes.c_int), ("ktmax", ctypes.c_int), ("len", ctypes.c_int),
+                ("nleavs", c_int_p),
+                ("ltmtx", c_int_p), ("kt", c_int_p), ("ndl", c_int_p), ("ndt", c_int_p),
+                ("nstrtl", c_int_p), ("nstrtt", c_int_p), ("a1", c_int_p), ("a2", c_int_p),
+                ("rowmat", c_double_p), ("rowmat_t", c_double_p)]
+
+
+class hgen_params(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_int), ("geometry", ctypes.c_int), ("leaf", ctypes.c_int), ("kmax", ctypes.c_int),
+                ("eta", ctypes.c_double), ("eps", ctypes.c_double), ("aca_eps_factor", ctypes.c_double),
+                ("jitter", ctypes.c_double), ("seed", ctypes.c_ulonglong)]
+
+
+class hmvm_options(ctypes.Structure):
+    _fields_ = [("storage", ctypes.c_int), ("w1", ctypes.c_int), ("w2", ctypes.c_int),
+                ("block_threads", ctypes.c_int), ("reserved", ctypes.c_int * 4)]
+
+
+class hmvm_plan_info(ctypes.Structure):
+    _fields_ = [("nd", ctypes.c_int), ("nlf", ctypes.c_int), ("nlowrank", ctypes.c_int), ("ndense", ctypes.c_int),
+                ("len", ctypes.c_longlong), ("device_elems", ctypes.c_longlong),
+                ("storage", ctypes.c_int), ("elem_bytes", ctypes.c_int),
+                ("ntiles1", ctypes.c_int), ("ntiles2", ctypes.c_int), ("npanels", ctypes.c_int),
+                ("ncol_groups", ctypes.c_int), ("nrow_groups", ctypes.c_int),
+                ("n_t", ctypes.c_longlong), ("n_tslots", ctypes.c_longlong), ("n_zslots", ctypes.c_longlong),
+                ("elems_phase1", ctypes.c_longlong), ("elems_phase2", ctypes.c_longlong),
+                ("vec_reads_phase1", ctypes.c_longlong), ("vec_reads_phase2", ctypes.c_longlong),
+                ("algorithmic_bytes", ctypes.c_longlong), ("device_bytes", ctypes.c_longlong)]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+class hmvm_timing(ctypes.Structure):
+    _fields_ = [("iters", ctypes.c_int), ("ms_avg", ctypes.c_double), ("ms_min", ctypes.c_double),
+                ("ms_median", ctypes.c_double), ("ms_phase1", ctypes.c_double), ("ms_tsum", ctypes.c_double),
+                ("ms_phase2", ctypes.c_double), ("ms_ysum", ctypes.c_double), ("ms_stream", ctypes.c_double)]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+HGEN_SPHERE, HGEN_CUBE = 0, 1
+HMVM_STORAGE_F64, HMVM_STORAGE_F32, HMVM_STORAGE_BF16 = 0, 1, 2
+HMVM_A2_ROWMAJOR, HMVM_A2_TRANSPOSED = 0, 1
+STORAGE = {"f64": 0, "f32": 1, "bf16": 2}
+
+
+class HMVMError(RuntimeError):
+    pass
+
+
+def _load():
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(
+            "hacapk_hmvm_amd: native library %s is missing; build it with "
+            "`python hacapk_hmvm_amd/build.py` (there is no CPU fallback)" % LIB_PATH)
+    lib = ctypes.CDLL(LIB_PATH)
+    P = ctypes.POINTER
+    sig = {
+        "loadHmatrix": (ctypes.c_int, [ctypes.c_char_p, P(matrix), P(matrix2)]),
+        "loadHmatrix2": (ctypes.c_int, [ctypes.c_char_p, P(matrix2)]),
+        "dummymatrix": (ctypes.c_int, [P(matrix), P(matrix2), ctypes.c_int, ctypes.c_int, ctypes.c_int]),
+        "freeHmatrix": (None, [P(matrix), P(matrix2)]),
+        "writeHmatrix1": (ctypes.c_int, [ctypes.c_char_p, P(matrix2)]),
+        "writeHmatrix2": (ctypes.c_int, [ctypes.c_char_p, P(matrix2)]),
+        "hacapk_strerror": (ctypes.c_char_p, [ctypes.c_int]),
+        "hgen_default_params": (None, [P(hgen_params), ctypes.c_int]),
+        "hgen_generate": (ctypes.c_int, [P(hgen_params), P(matrix2)]),
+        "hgen_matrix_from_matrix2": (ctypes.c_int, [P(matrix2), P(matrix)]),
+        "hgen_points": (ctypes.c_int, [P(hgen_params), c_double_p, c_double_p]),
+        "hgen_kernel_block": (ctypes.c_int, [P(hgen_params), ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                             ctypes.c_int, c_double_p]),
+        "hgen_accuracy": (ctypes.c_double, [P(hgen_params), P(matrix2), ctypes.c_int]),
+        "hmvm_hip": (None, [P(matrix), P(matrix2), c_double_p, ctypes.c_int]),
+        "hmvm_hip_bench": (None, [P(matrix), P(matrix2), c_double_p]),
+        "hmvm_hip_1": (None, [c_double_p, P(matrix), c_double_p]),
+        "hmvm_hip_1t": (None, [c_double_p, P(matrix), c_double_p]),
+        "hmvm_hip_2": (None, [c_double_p, P(matrix2), c_double_p]),
+        "hmvm_hip_2t": (None, [c_double_p, P(matrix2), c_double_p]),
+        "hmvm_hip_last_error": (ctypes.c_int, []),
+        "hmvm_hip_last_error_message": (ctypes.c_char_p, []),
+        "hmvm_hip_release": (None, [ctypes.c_void_p]),
+        "hmvm_options_default": (None, [P(hmvm_options)]),
+        "hmvm_plan_create": (ctypes.c_int, [P(matrix2), ctypes.c_int, P(hmvm_options), P(ctypes.c_void_p)]),
+        "hmvm_plan_create_aos": (ctypes.c_int, [P(matrix), ctypes.c_int, P(hmvm_options), P(ctypes.c_void_p)]),
+        "hmvm_plan_destroy": (None, [ctypes.c_void_p]),
+        "hmvm_plan_get_info": (ctypes.c_int, [ctypes.c_void_p, P(hmvm_plan_info)]),
+        "hmvm_plan_apply": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
+                                           ctypes.c_void_p]),
+        "hmvm_plan_apply_host": (ctypes.c_int, [ctypes.c_void_p, c_double_p, c_double_p, ctypes.c_int]),
+        "hmvm_plan_time": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, P(hmvm_timing)]),
+        "hmvm_plan_check": (ctypes.c_int, [P(matrix2), P(hmvm_options), P(hmvm_plan_info)]),
+        "hmvm_device_count": (ctypes.c_int, []),
+        "hmvm_device_alloc": (ctypes.c_int, [P(ctypes.c_void_p), ctypes.c_size_t]),
+        "hmvm_device_free": (ctypes.c_int, [ctypes.c_void_p]),
+        "hmvm_memcpy_h2d": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t]),
+        "hmvm_memcpy_d2h": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t]),
+        "hmvm_device_synchronize": (ctypes.c_int, []),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    return lib
+
+
+lib = _load()
+
+
+def _dptr(a):
+    return a.ctypes.data_as(c_double_p)
+
+
+def _check_vec(a, n, name):
+    if not isinstance(a, np.ndarray) or a.dtype != np.float64 or a.ndim != 1 or a.shape[0] < n:
+        raise HMVMError("%s must be a float64 vector of length >= %d" % (name, n))
+    if not a.flags["C_CONTIGUOUS"]:
+        raise HMVMError("%s must be contiguous" % name)
+
+
+def _raise_hmvm(what):
+    code = lib.hmvm_hip_last_error()
+    if code != 0:
+        raise HMVMError("%s: %s (code %d)" % (what, lib.hmvm_hip_last_error_message().decode(), code))
+
+
+class HMatrix:
+    """Owns a ``matrix`` and/or ``matrix2`` filled by the native loaders or generator."""
+
+    def __init__(self):
+        self.mat = matrix()
+        self.mat2 = matrix2()
+        self.has_mat = False
+        self.has_mat2 = False
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+    def free(self):
+        if self.has_mat or self.has_mat2:
+            if self.has_mat:
+                lib.hmvm_hip_release(ctypes.cast(ctypes.pointer(self.mat), ctypes.c_void_p))
+            if self.has_mat2:
+                lib.hmvm_hip_release(ctypes.cast(ctypes.pointer(self.mat2), ctypes.c_void_p))
+            lib.freeHmatrix(ctypes.byref(self.mat) if self.has_mat else None,
+                            ctypes.byref(self.mat2) if self.has_mat2 else None)
+            self.has_mat = self.has_mat2 = False
+
+    @property
+    def nd(self):
+        return self.mat2.nd if self.has_mat2 else self.mat.nd
+
+    @property
+    def mat_ptr(self):
+        return ctypes.pointer(self.mat) if self.has_mat else None
+
+    @property
+    def mat2_ptr(self):
+        return ctypes.pointer(self.mat2) if self.has_mat2 else None
+
+    def arrays(self):
+        """numpy views of the matrix2 fields (no copies)."""
+        m = self.mat2
+        n = m.nlf
+        out = {k: np.ctypeslib.as_array(getattr(m, k), (n,))
+               for k in ("ltmtx", "kt", "ndl", "ndt", "nstrtl", "nstrtt", "a1", "a2")}
+        out["rowmat"] = np.ctypeslib.as_array(m.rowmat, (m.len,)) if m.len > 0 else np.zeros(0)
+        out["rowmat_t"] = np.ctypeslib.as_array(m.rowmat_t, (m.len,)) if m.len > 0 else np.zeros(0)
+        return out
+
+    def build_aos(self):
+        """Adds the array-of-structs form (with a2t) built from matrix2."""
+        if not self.has_mat2:
+            raise HMVMError("no matrix2")
+        if not self.has_mat:
+            rc = lib.hgen_matrix_from_matrix2(ctypes.byref(self.mat2), ctypes.byref(self.mat))
+            if rc:
+                raise HMVMError("hgen_matrix_from_matrix2: %s" % lib.hacapk_strerror(rc).decode())
+            self.has_mat = True
+        return self
+
+
+def _hacapk_err(rc, what):
+    if rc:
+        raise HMVMError("%s: %s (code %d)" % (what, lib.hacapk_strerror(rc).decode(), rc))
+
+
+def loadHmatrix(fname):
+    """Format 1 -> HMatrix with both matrix and matrix2 (src_c/loadmatrix.c:8)."""
+    h = HMatrix()
+    rc = lib.loadHmatrix(os.fsencode(fname), ctypes.byref(h.mat), ctypes.byref(h.mat2))
+    _hacapk_err(rc, "loadHmatrix(%s)" % fname)
+    h.has_mat = h.has_mat2 = True
+    return h
+
+
+def loadHmatrix2(fname):
+    """Format 2 -> HMatrix with matrix2 only (src_c/loadmatrix.c:134)."""
+    h = HMatrix()
+    rc = lib.loadHmatrix2(os.fsencode(fname), ctypes.byref(h.mat2))
+    _hacapk_err(rc, "loadHmatrix2(%s)" % fname)
+    h.has_mat2 = True
+    return h
+
+
+def dummymatrix(ndl, ndt, n):
+    """All-dense synthetic matrix (src_c/loadmatrix.c:240)."""
+    h = HMatrix()
+    _hacapk_err(lib.dummymatrix(ctypes.byref(h.mat), ctypes.byref(h.mat2), ndl, ndt, n), "dummymatrix")
+    h.has_mat = h.has_mat2 = True
+    return h
+
+
+def writeHmatrix1(fname, h):
+    _hacapk_err(lib.writeHmatrix1(os.fsencode(fname), ctypes.byref(h.mat2)), "writeHmatrix1")
+
+
+def writeHmatrix2(fname, h):
+    _hacapk_err(lib.writeHmatrix2(os.fsencode(fname), ctypes.byref(h.mat2)), "writeHmatrix2")
+
+
+def gen_params(n, geometry="sphere", leaf=32, eta=2.0, eps=1e-4, seed=20261202, jitter=0.25, kmax=1000):
+    p = hgen_params()
+    lib.hgen_default_params(ctypes.byref(p), int(n))
+    p.geometry = {"sphere": HGEN_SPHERE, "cube": HGEN_CUBE}[geometry] if isinstance(geometry, str) else geometry
+    p.leaf, p.eta, p.eps, p.seed, p.jitter, p.kmax = leaf, eta, eps, seed, jitter, kmax
+    return p
+
+
+def generate(n, **kw):
+    """Generates an H-matrix (matrix2 only; call .build_aos() for matrix)."""
+    p = gen_params(n, **kw)
+    h = HMatrix()
+    _hacapk_err(lib.hgen_generate(ctypes.byref(p), ctypes.byref(h.mat2)), "hgen_generate")
+    h.has_mat2 = True
+    h.params = p
+    return h
+
+
+def kernel_block(params, i0, nr, j0, nc):
+    out = np.empty((nr, nc))
+    _hacapk_err(lib.hgen_kernel_block(ctypes.byref(params), i0, nr, j0, nc, _dptr(out)), "hgen_kernel_block")
+    return out
+
+
+def accuracy(h, stride=1):
+    return lib.hgen_accuracy(ctypes.byref(h.params), ctypes.byref(h.mat2), stride)
+
+
+# ---- drop-in HMVM (v += A b) -------------------------------------------------------
+def _variant(fn, v, h, b, use_mat2):
+    _check_vec(v, h.nd, "v")
+    _check_vec(b, h.nd, "b")
+    if use_mat2 and not h.has_mat2:
+        raise HMVMError("this variant needs matrix2")
+    if not use_mat2 and not h.has_mat:
+        raise HMVMError("this variant needs matrix")
+    fn(_dptr(v), ctypes.byref(h.mat2) if use_mat2 else ctypes.byref(h.mat), _dptr(b))
+    _raise_hmvm(fn.__name__)
+    return v
+
+
+def hmvm_hip_1(v, h, b):
+    return _variant(lib.hmvm_hip_1, v, h, b, False)
+
+
+def hmvm_hip_1t(v, h, b):
+    return _variant(lib.hmvm_hip_1t, v, h, b, False)
+
+
+def hmvm_hip_2(v, h, b):
+    return _variant(lib.hmvm_hip_2, v, h, b, True)
+
+
+def hmvm_hip_2t(v, h, b):
+    return _variant(lib.hmvm_hip_2t, v, h, b, True)
+
+
+def hmvm_hip(h, b, dump_result=0):
+    _check_vec(b, h.nd, "b")
+    lib.hmvm_hip(h.mat_ptr, h.mat2_ptr, _dptr(b), int(dump_result))
+    _raise_hmvm("hmvm_hip")
+
+
+def hmvm_hip_bench(h, b):
+    _check_vec(b, h.nd, "b")
+    lib.hmvm_hip_bench(h.mat_ptr, h.mat2_ptr, _dptr(b))
+    _raise_hmvm("hmvm_hip_bench")
+
+
+def release(h):
+    for m in (h.mat_ptr, h.mat2_ptr):
+        if m is not None:
+            lib.hmvm_hip_release(ctypes.cast(m, ctypes.c_void_p))
+
+
+def device_count():
+    return lib.hmvm_device_count()
+
+
+# ---- device-resident plan ---------------------------------------------------------------
+def options(storage="f64", w1=0, w2=0, block_threads=0):
+    o = hmvm_options()
+    lib.hmvm_options_default(ctypes.byref(o))
+    o.storage = STORAGE[storage] if isinstance(storage, str) else int(storage)
+    if w1:
+        o.w1 = w1
+    if w2:
+        o.w2 = w2
+    if block_threads:
+        o.block_threads = block_threads
+    return o
+
+
+def plan_check(h, **kw):
+    """Host-only schedule check (no GPU); returns the plan info dict."""
+    info = hmvm_plan_info()
+    o = options(**kw)
+    rc = lib.hmvm_plan_check(ctypes.byref(h.mat2), ctypes.byref(o), ctypes.byref(info))
+    if rc:
+        raise HMVMError("hmvm_plan_check: %s" % lib.hmvm_hip_last_error_message().decode())
+    return info.as_dict()
+
+
+class Plan:
+    """Repacked device copy of an H-matrix; y (+)= A x on device or host vectors."""
+
+    def __init__(self, h, source="rowmat", **kw):
+        self._p = ctypes.c_void_p()
+        o = options(**kw)
+        if source in ("rowmat", "rowmat_t"):
+            rc = lib.hmvm_plan_create(ctypes.byref(h.mat2), 1 if source == "rowmat_t" else 0, ctypes.byref(o),
+                                      ctypes.byref(self._p))
+        elif source in ("a2", "a2t"):
+            rc = lib.hmvm_plan_create_aos(ctypes.byref(h.mat), 1 if source == "a2t" else 0, ctypes.byref(o),
+                                          ctypes.byref(self._p))
+        else:
+            raise HMVMError("unknown source %r" % source)
+        if rc:
+            raise HMVMError("hmvm_plan_create: %s (code %d)" % (lib.hmvm_hip_last_error_message().decode(), rc))
+        self.nd = h.nd
+
+    def __del__(self):
+        self.close()
+
+    def close(self):
+        if getattr(self, "_p", None) and self._p.value:
+            lib.hmvm_plan_destroy(self._p)
+            self._p = ctypes.c_void_p()
+
+    def info(self):
+        i = hmvm_plan_info()
+        lib.hmvm_plan_get_info(self._p, ctypes.byref(i))
+        return i.as_dict()
+
+    def apply_host(self, x, y=None, accumulate=False):
+        _check_vec(x, self.nd, "x")
+        if y is None:
+            y = np.zeros(self.nd)
+        _check_vec(y, self.nd, "y")
+        rc = lib.hmvm_plan_apply_host(self._p, _dptr(x), _dptr(y), int(bool(accumulate)))
+        if rc:
+            raise HMVMError("hmvm_plan_apply_host: %s" % lib.hmvm_hip_last_error_message().decode())
+        return y
+
+    def apply(self, d_x, d_y, accumulate=False, stream=None):
+        rc = lib.hmvm_plan_apply(self._p, ctypes.c_void_p(d_x), ctypes.c_void_p(d_y), int(bool(accumulate)),
+                                 ctypes.c_void_p(stream) if stream else None)
+        if rc:
+            raise HMVMError("hmvm_plan_apply: %s" % lib.hmvm_hip_last_error_message().decode())
+
+    def time(self, warmup=5, iters=20):
+        t = hmvm_timing()
+        rc = lib.hmvm_plan_time(self._p, warmup, iters, ctypes.byref(t))
+        if rc:
+            raise HMVMError("hmvm_plan_time: %s" % lib.hmvm_hip_last_error_message().decode())
+        return t.as_dict()

@@ -1,0 +1,748 @@
+// hmvm_runtime.hip -- plan upload, launch sequence and the C ABI of the HIP HMVM.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "hacapk.h"
+#include "hmvm_hip.h"
+#include "hmvm_kernels.hpp"
+#include "hmvm_plan.hpp"
+
+using namespace hmvm;
+
+// ---------------------------------------------------------------------------
+// error state
+namespace {
+thread_local int g_err = HMVM_OK;
+thread_local char g_msg[1024] = "ok";
+
+int set_error(int code, const char *fmt, ...) {
+  g_err = code;
+  va_list ap;
+  va_start(ap, fmt);
+  std::vsnprintf(g_msg, sizeof g_msg, fmt, ap);
+  va_end(ap);
+  if (code != HMVM_OK) std::fprintf(stderr, "hmvm_hip: %s\n", g_msg);
+  return code;
+}
+void clear_error() {
+  g_err = HMVM_OK;
+  std::snprintf(g_msg, sizeof g_msg, "ok");
+}
+
+#define HIP_TRY(expr)                                                                      \
+  do {                                                                                     \
+    hipError_t e_ = (expr);                                                                \
+    if (e_ != hipSuccess)                                                                  \
+      return set_error(HMVM_ERR_HIP, "%s failed: %s", #expr, hipGetErrorString(e_));       \
+  } while (0)
+
+bool verbose() {
+  static int v = -1;
+  if (v < 0) {
+    const char *e = std::getenv("HMVM_VERBOSE");
+    v = (e && *e && *e != '0') ? 1 : 0;
+  }
+  return v == 1;
+}
+
+template <class T>
+int dalloc(T **p, size_t n) {
+  *p = nullptr;
+  if (n == 0) return HMVM_OK;
+  HIP_TRY(hipMalloc((void **)p, sizeof(T) * n));
+  return HMVM_OK;
+}
+template <class T>
+int h2d(T *d, const T *h, size_t n) {
+  if (n == 0) return HMVM_OK;
+  HIP_TRY(hipMemcpy(d, h, sizeof(T) * n, hipMemcpyHostToDevice));
+  return HMVM_OK;
+}
+template <class T>
+int upload(T **d, const std::vector<T> &h) {
+  int rc = dalloc(d, h.size());
+  if (rc) return rc;
+  return h2d(*d, h.data(), h.size());
+}
+
+int elem_bytes(int storage) { return storage == kF64 ? 8 : storage == kF32 ? 4 : 2; }
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// the plan object
+struct hmvm_plan {
+  Plan plan;
+  int storage = kF64;
+  int block_threads = 256;
+  int device = 0;
+  hipStream_t stream = nullptr;
+  void *d_vals = nullptr;
+  TileDesc *d_tiles1 = nullptr, *d_tiles2 = nullptr;
+  PanelDesc *d_panels = nullptr;
+  double *d_t = nullptr, *d_tp = nullptr, *d_z = nullptr;
+  int32_t *d_tseg = nullptr, *d_tptr = nullptr, *d_tfac = nullptr;
+  int64_t *d_tlist = nullptr;
+  int32_t *d_yseg = nullptr, *d_yptr = nullptr;
+  int64_t *d_ylist = nullptr;
+  float *d_scales = nullptr;
+  double *d_x = nullptr, *d_y = nullptr;  // staging for host-pointer calls
+  size_t device_bytes = 0;
+
+  ~hmvm_plan() {
+    void *ptrs[] = {d_vals, d_tiles1, d_tiles2, d_panels, d_t,    d_tp,     d_z,   d_tseg, d_tptr,
+                    d_tfac, d_tlist,  d_yseg,   d_yptr,   d_ylist, d_scales, d_x, d_y};
+    for (void *p : ptrs)
+      if (p) (void)hipFree(p);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+};
+
+namespace {
+
+// Source view of the caller's matrix: descriptor arrays + one contiguous FP64
+// value buffer in host memory (rowmat, rowmat_t or a gathered copy of the
+// per-leaf arrays).
+struct Source {
+  LeafArrays la;
+  std::vector<int64_t> a1, a2;
+  const double *vals = nullptr;
+  std::vector<double> gathered;
+  std::vector<int> ltmtx, kt, ndl, ndt, nstrtl, nstrtt;
+};
+
+int source_from_matrix2(const matrix2 *m, int a2_source, Source *s) {
+  if (!m || m->nd < 1 || m->nlf < 1 || !m->ltmtx || !m->kt || !m->ndl || !m->ndt || !m->nstrtl ||
+      !m->nstrtt || !m->a1 || !m->a2)
+    return set_error(HMVM_ERR_ARG, "matrix2 is NULL or incomplete");
+  const double *vals = a2_source == kA2Transposed ? m->rowmat_t : m->rowmat;
+  if (!vals) return set_error(HMVM_ERR_ARG, "matrix2 has no %s", a2_source ? "rowmat_t" : "rowmat");
+  s->a1.resize(m->nlf);
+  s->a2.resize(m->nlf);
+  for (int i = 0; i < m->nlf; i++) {
+    s->a1[i] = m->a1[i];
+    s->a2[i] = m->a2[i];
+  }
+  LeafArrays &la = s->la;
+  la.nd = m->nd;
+  la.nlf = m->nlf;
+  la.ltmtx = m->ltmtx;
+  la.kt = m->kt;
+  la.ndl = m->ndl;
+  la.ndt = m->ndt;
+  la.nstrtl = m->nstrtl;
+  la.nstrtt = m->nstrtt;
+  la.src_a1 = s->a1.data();
+  la.src_a2 = s->a2.data();
+  la.a2_source = a2_source;
+  la.src_len = m->len;
+  s->vals = vals;
+  return HMVM_OK;
+}
+
+// matrix (array of structs): gather every leaf into one buffer in the packing
+// order of the reference matrix2 (src_c/loadmatrix.c:98-128).
+int source_from_matrix(const matrix *m, int a2_source, Source *s) {
+  if (!m || m->nd < 1 || m->nlf < 1 || !m->submat) return set_error(HMVM_ERR_ARG, "matrix is NULL or incomplete");
+  const int nlf = m->nlf;
+  s->ltmtx.resize(nlf); s->kt.resize(nlf); s->ndl.resize(nlf); s->ndt.resize(nlf);
+  s->nstrtl.resize(nlf); s->nstrtt.resize(nlf); s->a1.resize(nlf); s->a2.resize(nlf);
+  int64_t off = 0;
+  for (int i = 0; i < nlf; i++) {
+    const submatrix &q = m->submat[i];
+    s->ltmtx[i] = q.ltmtx; s->kt[i] = q.kt; s->ndl[i] = q.ndl; s->ndt[i] = q.ndt;
+    s->nstrtl[i] = q.nstrtl; s->nstrtt[i] = q.nstrtt;
+    s->a1[i] = off;
+    if (q.ltmtx == 1) {
+      if (q.kt < 0 || q.ndl < 0 || q.ndt < 0) return set_error(HMVM_ERR_MATRIX, "leaf %d: negative size", i);
+      off += (int64_t)q.kt * q.ndt;
+      s->a2[i] = off;
+      off += (int64_t)q.kt * q.ndl;
+    } else {
+      if (q.ndl < 0 || q.ndt < 0) return set_error(HMVM_ERR_MATRIX, "leaf %d: negative size", i);
+      s->a2[i] = 0;
+      off += (int64_t)q.ndl * q.ndt;
+    }
+  }
+  try {
+    s->gathered.resize((size_t)std::max<int64_t>(off, 1));
+  } catch (...) {
+    return set_error(HMVM_ERR_NOMEM, "cannot gather %lld elements", (long long)off);
+  }
+  for (int i = 0; i < nlf; i++) {
+    const submatrix &q = m->submat[i];
+    if (q.ltmtx == 1) {
+      const double *a2 = a2_source == kA2Transposed ? q.a2t : q.a2;
+      if ((q.kt > 0) && (!q.a1 || !a2)) return set_error(HMVM_ERR_ARG, "leaf %d: missing factor", i);
+      if (q.kt > 0) {
+        std::memcpy(&s->gathered[s->a1[i]], q.a1, sizeof(double) * (size_t)q.kt * q.ndt);
+        std::memcpy(&s->gathered[s->a2[i]], a2, sizeof(double) * (size_t)q.kt * q.ndl);
+      }
+    } else {
+      if (!q.a1) return set_error(HMVM_ERR_ARG, "leaf %d: missing a1", i);
+      std::memcpy(&s->gathered[s->a1[i]], q.a1, sizeof(double) * (size_t)q.ndl * q.ndt);
+    }
+  }
+  LeafArrays &la = s->la;
+  la.nd = m->nd;
+  la.nlf = nlf;
+  la.ltmtx = s->ltmtx.data(); la.kt = s->kt.data(); la.ndl = s->ndl.data(); la.ndt = s->ndt.data();
+  la.nstrtl = s->nstrtl.data(); la.nstrtt = s->nstrtt.data();
+  la.src_a1 = s->a1.data();
+  la.src_a2 = s->a2.data();
+  la.a2_source = a2_source;
+  la.src_len = off;
+  s->vals = s->gathered.data();
+  return HMVM_OK;
+}
+
+PlanOptions plan_options(const hmvm_options *o) {
+  PlanOptions po;
+  if (o) {
+    po.storage = o->storage;
+    if (o->w1 > 0) po.w1 = o->w1;
+    if (o->w2 > 0) po.w2 = o->w2;
+  }
+  return po;
+}
+
+template <typename S>
+int launch_repack(hmvm_plan *p, const double *d_src, const RepackDesc *d_desc, const RepackChunk *d_chunks,
+                  int nchunks, const int32_t *d_chunk_begin, int ndesc, float *d_chunk_max) {
+  const bool scaled = p->storage == kBF16;
+  if (scaled) {
+    hipLaunchKernelGGL(absmax_kernel, dim3(nchunks), dim3(256), 0, p->stream, d_desc, d_chunks, d_src, d_chunk_max);
+    HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL(scale_kernel, dim3((ndesc + 255) / 256), dim3(256), 0, p->stream, ndesc, d_desc,
+                       d_chunk_begin, d_chunk_max, p->d_scales);
+    HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL((repack_kernel<S, true>), dim3(nchunks), dim3(256), 0, p->stream, d_desc, d_chunks, d_src,
+                       p->d_scales, (S *)p->d_vals);
+  } else {
+    hipLaunchKernelGGL((repack_kernel<S, false>), dim3(nchunks), dim3(256), 0, p->stream, d_desc, d_chunks, d_src,
+                       (const float *)nullptr, (S *)p->d_vals);
+  }
+  HIP_TRY(hipGetLastError());
+  return HMVM_OK;
+}
+
+int create_plan(Source &src, const hmvm_options *opt, hmvm_plan **out) {
+  *out = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return set_error(HMVM_ERR_NODEVICE, "no HIP device visible");
+  const PlanOptions po = plan_options(opt);
+  if (po.storage < kF64 || po.storage > kBF16) return set_error(HMVM_ERR_ARG, "unknown storage %d", po.storage);
+  std::unique_ptr<hmvm_plan> p(new hmvm_plan());
+  p->storage = po.storage;
+  if (opt && opt->block_threads > 0) {
+    if (opt->block_threads % 64 || opt->block_threads > 1024 || opt->block_threads < 64)
+      return set_error(HMVM_ERR_ARG, "block_threads must be a multiple of 64 in [64,1024]");
+    p->block_threads = opt->block_threads;
+  }
+  std::string err = build_plan(src.la, po, &p->plan);
+  if (!err.empty()) return set_error(HMVM_ERR_MATRIX, "%s", err.c_str());
+  err = verify_plan(p->plan, src.la);
+  if (!err.empty()) return set_error(HMVM_ERR_MATRIX, "schedule check failed: %s", err.c_str());
+  Plan &P = p->plan;
+  HIP_TRY(hipGetDevice(&p->device));
+  HIP_TRY(hipStreamCreate(&p->stream));
+
+  int rc;
+  const size_t eb = (size_t)elem_bytes(p->storage);
+  // schedule
+  if ((rc = upload(&p->d_tiles1, P.tiles1)) || (rc = upload(&p->d_tiles2, P.tiles2)) ||
+      (rc = upload(&p->d_panels, P.panels)) || (rc = upload(&p->d_tseg, P.tsum.seg)) ||
+      (rc = upload(&p->d_tptr, P.tsum.ptr)) || (rc = upload(&p->d_tlist, P.tsum.list)) ||
+      (rc = upload(&p->d_yseg, P.ysum.seg)) || (rc = upload(&p->d_yptr, P.ysum.ptr)) ||
+      (rc = upload(&p->d_ylist, P.ysum.list)))
+    return rc;
+  if (p->storage == kBF16) {
+    if ((rc = upload(&p->d_tfac, P.factor_of_trow))) return rc;
+    if ((rc = dalloc(&p->d_scales, (size_t)P.nfactors))) return rc;
+    std::vector<float> ones((size_t)P.nfactors, 1.0f);
+    if ((rc = h2d(p->d_scales, ones.data(), ones.size()))) return rc;
+  }
+  if ((rc = dalloc(&p->d_t, (size_t)P.n_t)) || (rc = dalloc(&p->d_tp, (size_t)P.n_tp)) ||
+      (rc = dalloc(&p->d_z, (size_t)P.n_z)) || (rc = dalloc(&p->d_x, (size_t)P.nd)) ||
+      (rc = dalloc(&p->d_y, (size_t)P.nd)))
+    return rc;
+  HIP_TRY(hipMalloc(&p->d_vals, eb * (size_t)std::max<int64_t>(P.nvals, 1)));
+
+  // values: upload the caller's buffer once, repack on the GPU
+  std::vector<RepackChunk> chunks;
+  std::vector<int32_t> chunk_begin;
+  chunk_begin.reserve(P.repack.size() + 1);
+  for (size_t d = 0; d < P.repack.size(); d++) {
+    chunk_begin.push_back((int32_t)chunks.size());
+    const int64_t ne = (int64_t)P.repack[d].rows * P.repack[d].cols;
+    for (int64_t e0 = 0; e0 < ne; e0 += kRepackChunk) chunks.push_back({(int32_t)d, (int32_t)e0});
+  }
+  chunk_begin.push_back((int32_t)chunks.size());
+  double *d_src = nullptr;
+  RepackDesc *d_desc = nullptr;
+  RepackChunk *d_chunks = nullptr;
+  int32_t *d_chunk_begin = nullptr;
+  float *d_chunk_max = nullptr;
+  auto cleanup = [&]() {
+    if (d_src) (void)hipFree(d_src);
+    if (d_desc) (void)hipFree(d_desc);
+    if (d_chunks) (void)hipFree(d_chunks);
+    if (d_chunk_begin) (void)hipFree(d_chunk_begin);
+    if (d_chunk_max) (void)hipFree(d_chunk_max);
+  };
+  rc = dalloc(&d_src, (size_t)std::max<int64_t>(src.la.src_len, 1));
+  if (!rc) rc = h2d(d_src, src.vals, (size_t)src.la.src_len);
+  if (!rc) rc = upload(&d_desc, P.repack);
+  if (!rc) rc = upload(&d_chunks, chunks);
+  if (!rc) rc = upload(&d_chunk_begin, chunk_begin);
+  if (!rc && p->storage == kBF16) rc = dalloc(&d_chunk_max, chunks.size());
+  if (!rc && !chunks.empty()) {
+    const int nch = (int)chunks.size(), nd = (int)P.repack.size();
+    if (p->storage == kF64)
+      rc = launch_repack<double>(p.get(), d_src, d_desc, d_chunks, nch, d_chunk_begin, nd, d_chunk_max);
+    else if (p->storage == kF32)
+      rc = launch_repack<float>(p.get(), d_src, d_desc, d_chunks, nch, d_chunk_begin, nd, d_chunk_max);
+    else
+      rc = launch_repack<uint16_t>(p.get(), d_src, d_desc, d_chunks, nch, d_chunk_begin, nd, d_chunk_max);
+  }
+  if (!rc) {
+    hipError_t e = hipStreamSynchronize(p->stream);
+    if (e != hipSuccess) rc = set_error(HMVM_ERR_HIP, "repack failed: %s", hipGetErrorString(e));
+  }
+  cleanup();
+  if (rc) return rc;
+
+  p->device_bytes = eb * (size_t)P.nvals + sizeof(TileDesc) * (P.tiles1.size() + P.tiles2.size()) +
+                    sizeof(PanelDesc) * P.panels.size() + 8 * (size_t)(P.n_t + P.n_tp + P.n_z + 2 * P.nd) +
+                    4 * (P.tsum.seg.size() + P.tsum.ptr.size() + P.ysum.seg.size() + P.ysum.ptr.size()) +
+                    8 * (P.tsum.list.size() + P.ysum.list.size());
+  if (verbose())
+    std::fprintf(stderr,
+                 "hmvm_hip: plan nd=%d nlf=%d len=%lld storage=%d tiles=%zu+%zu panels=%zu t=%lld tslots=%lld "
+                 "zslots=%lld\n",
+                 P.nd, P.nlf, (long long)P.len, p->storage, P.tiles1.size(), P.tiles2.size(), P.panels.size(),
+                 (long long)P.n_t, (long long)P.n_tp, (long long)P.n_z);
+  *out = p.release();
+  return HMVM_OK;
+}
+
+template <typename S>
+int launch_tiles(hmvm_plan *p, const TileDesc *d_tiles, int ntiles, const double *x, const double *t,
+                 double *out, bool scaled, hipStream_t st) {
+  if (ntiles == 0) return HMVM_OK;
+  const int wpb = p->block_threads / 64;
+  const dim3 grid((ntiles + wpb - 1) / wpb), block(p->block_threads);
+  if (scaled)
+    hipLaunchKernelGGL((tile_kernel<S, true, 8>), grid, block, 0, st, d_tiles, ntiles, p->d_panels,
+                       (const S *)p->d_vals, x, t, p->d_scales, out);
+  else
+    hipLaunchKernelGGL((tile_kernel<S, false, 8>), grid, block, 0, st, d_tiles, ntiles, p->d_panels,
+                       (const S *)p->d_vals, x, t, (const float *)nullptr, out);
+  return HMVM_OK;
+}
+
+int launch_phase(hmvm_plan *p, int phase, const double *x, double *y, int accumulate, hipStream_t st) {
+  const Plan &P = p->plan;
+  const bool bf = p->storage == kBF16;
+  switch (phase) {
+    case 0:  // phase-1 tiles -> t slots (scales applied per t row in the t sum)
+      if (p->storage == kF64) return launch_tiles<double>(p, p->d_tiles1, (int)P.tiles1.size(), x, nullptr, p->d_tp, false, st);
+      if (p->storage == kF32) return launch_tiles<float>(p, p->d_tiles1, (int)P.tiles1.size(), x, nullptr, p->d_tp, false, st);
+      return launch_tiles<uint16_t>(p, p->d_tiles1, (int)P.tiles1.size(), x, nullptr, p->d_tp, false, st);
+    case 1:
+      if (P.tsum.n > 0) {
+        if (bf)
+          hipLaunchKernelGGL(segsum_kernel<true>, dim3((P.tsum.n + 255) / 256), dim3(256), 0, st, P.tsum.n, p->d_tseg,
+                             p->d_tptr, p->d_tlist, p->d_tp, p->d_tfac, p->d_scales, p->d_t, 0);
+        else
+          hipLaunchKernelGGL(segsum_kernel<false>, dim3((P.tsum.n + 255) / 256), dim3(256), 0, st, P.tsum.n, p->d_tseg,
+                             p->d_tptr, p->d_tlist, p->d_tp, (const int32_t *)nullptr, (const float *)nullptr, p->d_t, 0);
+      }
+      return HMVM_OK;
+    case 2:
+      if (p->storage == kF64) return launch_tiles<double>(p, p->d_tiles2, (int)P.tiles2.size(), x, p->d_t, p->d_z, false, st);
+      if (p->storage == kF32) return launch_tiles<float>(p, p->d_tiles2, (int)P.tiles2.size(), x, p->d_t, p->d_z, false, st);
+      return launch_tiles<uint16_t>(p, p->d_tiles2, (int)P.tiles2.size(), x, p->d_t, p->d_z, true, st);
+    default:
+      hipLaunchKernelGGL(segsum_kernel<false>, dim3((P.ysum.n + 255) / 256), dim3(256), 0, st, P.ysum.n, p->d_yseg,
+                         p->d_yptr, p->d_ylist, p->d_z, (const int32_t *)nullptr, (const float *)nullptr, y,
+                         accumulate);
+      return HMVM_OK;
+  }
+}
+
+int apply(hmvm_plan *p, const double *d_x, double *d_y, int accumulate, hipStream_t st) {
+  for (int ph = 0; ph < 4; ph++) {
+    int rc = launch_phase(p, ph, d_x, d_y, accumulate, st);
+    if (rc) return rc;
+  }
+  HIP_TRY(hipGetLastError());
+  return HMVM_OK;
+}
+
+void fill_info(const hmvm_plan *p, const Plan &P, int storage, hmvm_plan_info *info) {
+  std::memset(info, 0, sizeof(*info));
+  info->nd = P.nd;
+  info->nlf = P.nlf;
+  info->nlowrank = P.nlr;
+  info->ndense = P.ndense;
+  info->len = P.len;
+  info->device_elems = P.nvals;
+  info->storage = storage;
+  info->elem_bytes = elem_bytes(storage);
+  info->ntiles1 = (int)P.tiles1.size();
+  info->ntiles2 = (int)P.tiles2.size();
+  info->npanels = (int)P.panels.size();
+  info->ncol_groups = P.ngroups_col;
+  info->nrow_groups = P.ngroups_row;
+  info->n_t = P.n_t;
+  info->n_tslots = P.n_tp;
+  info->n_zslots = P.n_z;
+  info->elems_phase1 = P.elems1;
+  info->elems_phase2 = P.elems2;
+  info->vec_reads_phase1 = P.xreads1;
+  info->vec_reads_phase2 = P.xreads2;
+  info->algorithmic_bytes = (long long)info->elem_bytes * P.len + 16LL * P.nd;
+  info->device_bytes = p ? (long long)p->device_bytes : 0;
+}
+
+// ---------------------------------------------------------------------------
+// cache of device copies for the drop-in entry points
+struct CacheKey {
+  const void *mat;
+  int variant;  // 0:1 1:1t 2:2 3:2t
+  bool operator<(const CacheKey &o) const { return mat != o.mat ? mat < o.mat : variant < o.variant; }
+};
+struct CacheEntry {
+  hmvm_plan *plan = nullptr;
+  // identity of the contents the plan was built from
+  int nd = 0, nlf = 0, len = 0;
+  const void *data = nullptr;
+  const void *first_leaf_a1 = nullptr;
+};
+std::mutex g_cache_mu;
+std::map<CacheKey, CacheEntry> g_cache;
+
+hmvm_plan *cached_plan(const void *mat, int variant, int nd, int nlf, int len, const void *data,
+                       const void *first_a1) {
+  std::lock_guard<std::mutex> lk(g_cache_mu);
+  auto it = g_cache.find({mat, variant});
+  if (it == g_cache.end()) return nullptr;
+  const CacheEntry &e = it->second;
+  if (e.nd == nd && e.nlf == nlf && e.len == len && e.data == data && e.first_leaf_a1 == first_a1) return e.plan;
+  hmvm_plan_destroy(e.plan);
+  g_cache.erase(it);
+  return nullptr;
+}
+
+void cache_put(const void *mat, int variant, hmvm_plan *p, int nd, int nlf, int len, const void *data,
+               const void *first_a1) {
+  std::lock_guard<std::mutex> lk(g_cache_mu);
+  CacheEntry e;
+  e.plan = p;
+  e.nd = nd;
+  e.nlf = nlf;
+  e.len = len;
+  e.data = data;
+  e.first_leaf_a1 = first_a1;
+  g_cache[{mat, variant}] = e;
+}
+
+const char *kVariantName[4] = {"1", "1t", "2", "2t"};
+
+// Resolves (and on first use builds) the device copy for one variant.
+hmvm_plan *plan_for(const matrix *mat, const matrix2 *mat2, int variant) {
+  if (variant < 2) {
+    if (!mat || !mat->submat) { set_error(HMVM_ERR_ARG, "matrix is NULL"); return nullptr; }
+    const void *a1 = mat->nlf > 0 ? (const void *)mat->submat[0].a1 : nullptr;
+    hmvm_plan *p = cached_plan(mat, variant, mat->nd, mat->nlf, 0, mat->submat, a1);
+    if (p) return p;
+    Source s;
+    if (source_from_matrix(mat, variant == 1 ? kA2Transposed : kA2RowMajor, &s)) return nullptr;
+    if (create_plan(s, nullptr, &p)) return nullptr;
+    cache_put(mat, variant, p, mat->nd, mat->nlf, 0, mat->submat, a1);
+    return p;
+  }
+  if (!mat2) { set_error(HMVM_ERR_ARG, "matrix2 is NULL"); return nullptr; }
+  const void *data = variant == 3 ? (const void *)mat2->rowmat_t : (const void *)mat2->rowmat;
+  hmvm_plan *p = cached_plan(mat2, variant, mat2->nd, mat2->nlf, mat2->len, data, nullptr);
+  if (p) return p;
+  Source s;
+  if (source_from_matrix2(mat2, variant == 3 ? kA2Transposed : kA2RowMajor, &s)) return nullptr;
+  if (create_plan(s, nullptr, &p)) return nullptr;
+  cache_put(mat2, variant, p, mat2->nd, mat2->nlf, mat2->len, data, nullptr);
+  return p;
+}
+
+void run_variant(double *v, const matrix *mat, const matrix2 *mat2, const double *b, int variant) {
+  clear_error();
+  if (!v || !b) { set_error(HMVM_ERR_ARG, "v or b is NULL"); return; }
+  hmvm_plan *p = plan_for(mat, mat2, variant);
+  if (!p) return;
+  hmvm_plan_apply_host(p, b, v, 1);
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// C ABI
+extern "C" {
+
+int hmvm_hip_last_error(void) { return g_err; }
+const char *hmvm_hip_last_error_message(void) { return g_msg; }
+
+void hmvm_options_default(hmvm_options *opt) {
+  if (!opt) return;
+  std::memset(opt, 0, sizeof(*opt));
+  PlanOptions po;
+  opt->storage = po.storage;
+  opt->w1 = po.w1;
+  opt->w2 = po.w2;
+  opt->block_threads = 256;
+}
+
+int hmvm_plan_create(const matrix2 *mat2, int a2_source, const hmvm_options *opt, hmvm_plan **out) {
+  clear_error();
+  if (!out) return set_error(HMVM_ERR_ARG, "out is NULL");
+  *out = nullptr;
+  if (a2_source != kA2RowMajor && a2_source != kA2Transposed) return set_error(HMVM_ERR_ARG, "bad a2_source");
+  Source s;
+  int rc = source_from_matrix2(mat2, a2_source, &s);
+  if (rc) return rc;
+  return create_plan(s, opt, out);
+}
+
+int hmvm_plan_create_aos(const matrix *mat, int a2_source, const hmvm_options *opt, hmvm_plan **out) {
+  clear_error();
+  if (!out) return set_error(HMVM_ERR_ARG, "out is NULL");
+  *out = nullptr;
+  if (a2_source != kA2RowMajor && a2_source != kA2Transposed) return set_error(HMVM_ERR_ARG, "bad a2_source");
+  Source s;
+  int rc = source_from_matrix(mat, a2_source, &s);
+  if (rc) return rc;
+  return create_plan(s, opt, out);
+}
+
+void hmvm_plan_destroy(hmvm_plan *plan) { delete plan; }
+
+int hmvm_plan_get_info(const hmvm_plan *plan, hmvm_plan_info *info) {
+  if (!plan || !info) return set_error(HMVM_ERR_ARG, "NULL argument");
+  fill_info(plan, plan->plan, plan->storage, info);
+  return HMVM_OK;
+}
+
+int hmvm_plan_check(const matrix2 *mat2, const hmvm_options *opt, hmvm_plan_info *info) {
+  clear_error();
+  Source s;
+  int rc = source_from_matrix2(mat2, kA2RowMajor, &s);
+  if (rc) return rc;
+  const PlanOptions po = plan_options(opt);
+  Plan P;
+  std::string err = build_plan(s.la, po, &P);
+  if (!err.empty()) return set_error(HMVM_ERR_MATRIX, "%s", err.c_str());
+  err = verify_plan(P, s.la);
+  if (!err.empty()) return set_error(HMVM_ERR_MATRIX, "schedule check failed: %s", err.c_str());
+  if (info) fill_info(nullptr, P, po.storage, info);
+  return HMVM_OK;
+}
+
+int hmvm_plan_apply(hmvm_plan *plan, const double *d_x, double *d_y, int accumulate, void *stream) {
+  if (!plan || !d_x || !d_y) return set_error(HMVM_ERR_ARG, "NULL argument");
+  hipStream_t st = stream ? (hipStream_t)stream : plan->stream;
+  return apply(plan, d_x, d_y, accumulate, st);
+}
+
+int hmvm_plan_apply_host(hmvm_plan *plan, const double *x, double *y, int accumulate) {
+  if (!plan || !x || !y) return set_error(HMVM_ERR_ARG, "NULL argument");
+  const size_t nb = sizeof(double) * (size_t)plan->plan.nd;
+  HIP_TRY(hipMemcpyAsync(plan->d_x, x, nb, hipMemcpyHostToDevice, plan->stream));
+  if (accumulate) HIP_TRY(hipMemcpyAsync(plan->d_y, y, nb, hipMemcpyHostToDevice, plan->stream));
+  int rc = apply(plan, plan->d_x, plan->d_y, accumulate, plan->stream);
+  if (rc) return rc;
+  HIP_TRY(hipMemcpyAsync(y, plan->d_y, nb, hipMemcpyDeviceToHost, plan->stream));
+  HIP_TRY(hipStreamSynchronize(plan->stream));
+  return HMVM_OK;
+}
+
+int hmvm_plan_time(hmvm_plan *plan, int warmup, int iters, hmvm_timing *tm) {
+  if (!plan || !tm || iters < 1 || warmup < 0) return set_error(HMVM_ERR_ARG, "bad argument");
+  std::memset(tm, 0, sizeof(*tm));
+  hipStream_t st = plan->stream;
+  const int nd = plan->plan.nd;
+  {
+    std::vector<double> h((size_t)nd);
+    for (int i = 0; i < nd; i++) h[i] = std::sin((double)(i + 1));
+    HIP_TRY(hipMemcpy(plan->d_x, h.data(), sizeof(double) * nd, hipMemcpyHostToDevice));
+  }
+  for (int w = 0; w < warmup; w++) {
+    int rc = apply(plan, plan->d_x, plan->d_y, 0, st);
+    if (rc) return rc;
+  }
+  hipEvent_t ev[5];
+  for (int e = 0; e < 5; e++) HIP_TRY(hipEventCreate(&ev[e]));
+  std::vector<double> total;
+  double ph[4] = {0, 0, 0, 0};
+  int rc = HMVM_OK;
+  for (int it = 0; it < iters && !rc; it++) {
+    hipEventRecord(ev[0], st);
+    for (int phs = 0; phs < 4 && !rc; phs++) {
+      rc = launch_phase(plan, phs, plan->d_x, plan->d_y, 0, st);
+      hipEventRecord(ev[phs + 1], st);
+    }
+    if (hipEventSynchronize(ev[4]) != hipSuccess) rc = set_error(HMVM_ERR_HIP, "event sync failed");
+    float ms;
+    for (int phs = 0; phs < 4 && !rc; phs++) {
+      hipEventElapsedTime(&ms, ev[phs], ev[phs + 1]);
+      ph[phs] += ms;
+    }
+    hipEventElapsedTime(&ms, ev[0], ev[4]);
+    total.push_back(ms);
+  }
+  // back-to-back throughput: all applies enqueued, one event pair
+  if (!rc) {
+    hipEventRecord(ev[0], st);
+    for (int it = 0; it < iters && !rc; it++) rc = apply(plan, plan->d_x, plan->d_y, 0, st);
+    hipEventRecord(ev[1], st);
+    if (!rc && hipEventSynchronize(ev[1]) != hipSuccess) rc = set_error(HMVM_ERR_HIP, "event sync failed");
+    float ms = 0;
+    if (!rc) hipEventElapsedTime(&ms, ev[0], ev[1]);
+    tm->ms_stream = ms / iters;
+  }
+  for (int e = 0; e < 5; e++) (void)hipEventDestroy(ev[e]);
+  if (rc) return rc;
+  HIP_TRY(hipGetLastError());
+  std::vector<double> s = total;
+  std::sort(s.begin(), s.end());
+  double sum = 0;
+  for (double v : total) sum += v;
+  tm->iters = iters;
+  tm->ms_avg = sum / iters;
+  tm->ms_min = s.front();
+  tm->ms_median = s[s.size() / 2];
+  tm->ms_phase1 = ph[0] / iters;
+  tm->ms_tsum = ph[1] / iters;
+  tm->ms_phase2 = ph[2] / iters;
+  tm->ms_ysum = ph[3] / iters;
+  return HMVM_OK;
+}
+
+void hmvm_hip_1(double *v, const matrix *mat, const double *b) { run_variant(v, mat, nullptr, b, 0); }
+void hmvm_hip_1t(double *v, const matrix *mat, const double *b) { run_variant(v, mat, nullptr, b, 1); }
+void hmvm_hip_2(double *v, const matrix2 *mat2, const double *b) { run_variant(v, nullptr, mat2, b, 2); }
+void hmvm_hip_2t(double *v, const matrix2 *mat2, const double *b) { run_variant(v, nullptr, mat2, b, 3); }
+
+void hmvm_hip_release(const void *mat) {
+  std::lock_guard<std::mutex> lk(g_cache_mu);
+  for (auto it = g_cache.begin(); it != g_cache.end();) {
+    if (it->first.mat == mat) {
+      hmvm_plan_destroy(it->second.plan);
+      it = g_cache.erase(it);
+    } else {
+      ++it;
+    }
+  }
+}
+
+void hmvm_hip(const matrix *mat, const matrix2 *mat2, const double *b, int dump_result) {
+  clear_error();
+  const int nd = mat ? mat->nd : (mat2 ? mat2->nd : 0);
+  if (nd < 1 || !b) { set_error(HMVM_ERR_ARG, "no matrix or b"); return; }
+  std::vector<double> v((size_t)nd);
+  for (int var = 0; var < 4; var++) {
+    if ((var < 2 && !mat) || (var >= 2 && !mat2)) continue;
+    std::fill(v.begin(), v.end(), 0.0);
+    run_variant(v.data(), mat, mat2, b, var);
+    if (g_err) return;
+    if (dump_result) {
+      char fname[64];
+      std::snprintf(fname, sizeof fname, "result_hip_%s_d.txt", kVariantName[var]);
+      FILE *F = std::fopen(fname, "w");
+      if (!F) { set_error(HMVM_ERR_ARG, "cannot write %s", fname); return; }
+      for (int i = 0; i < nd; i++) std::fprintf(F, "%.3E\n", v[i]);
+      std::fclose(F);
+    }
+  }
+}
+
+void hmvm_hip_bench(const matrix *mat, const matrix2 *mat2, const double *b) {
+  clear_error();
+  const int L = 10, M = 5;
+  const int nd = mat ? mat->nd : (mat2 ? mat2->nd : 0);
+  if (nd < 1 || !b) { set_error(HMVM_ERR_ARG, "no matrix or b"); return; }
+  for (int var = 0; var < 4; var++) {
+    if ((var < 2 && !mat) || (var >= 2 && !mat2)) continue;
+    hmvm_plan *p = plan_for(mat, mat2, var);
+    if (!p) return;
+    if (hipMemcpy(p->d_x, b, sizeof(double) * nd, hipMemcpyHostToDevice) != hipSuccess) {
+      set_error(HMVM_ERR_HIP, "upload of b failed");
+      return;
+    }
+    hipEvent_t e0, e1;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+      set_error(HMVM_ERR_HIP, "hipEventCreate failed");
+      return;
+    }
+    double dt[L];
+    for (int l = 0; l < L; l++) {
+      hipEventRecord(e0, p->stream);
+      if (apply(p, p->d_x, p->d_y, 0, p->stream)) break;
+      hipEventRecord(e1, p->stream);
+      hipEventSynchronize(e1);
+      float ms = 0;
+      hipEventElapsedTime(&ms, e0, e1);
+      dt[l] = ms * 1e-3;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (g_err) return;
+    double mn = 9999.99, mx = 0.0, avg = 0.0;
+    for (int l = M; l < L; l++) {
+      avg += dt[l];
+      mn = std::min(mn, dt[l]);
+      mx = std::max(mx, dt[l]);
+    }
+    avg /= (L - M);
+    std::printf("TIME %d hmvm_hip_%s min %e max %e avg %e\n", L, kVariantName[var], mn, mx, avg);
+    std::fflush(stdout);
+  }
+}
+
+int hmvm_device_count(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+  return n;
+}
+int hmvm_device_alloc(void **ptr, size_t bytes) {
+  if (!ptr) return set_error(HMVM_ERR_ARG, "NULL argument");
+  HIP_TRY(hipMalloc(ptr, bytes));
+  return HMVM_OK;
+}
+int hmvm_device_free(void *ptr) {
+  HIP_TRY(hipFree(ptr));
+  return HMVM_OK;
+}
+int hmvm_memcpy_h2d(void *dst, const void *src, size_t bytes) {
+  HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+  return HMVM_OK;
+}
+int hmvm_memcpy_d2h(void *dst, const void *src, size_t bytes) {
+  HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+  return HMVM_OK;
+}
+int hmvm_device_synchronize(void) {
+  HIP_TRY(hipDeviceSynchronize());
+  return HMVM_OK;
+}
+
+}  // extern "C"

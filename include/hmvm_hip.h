@@ -1,0 +1,158 @@
+/*
+ * hmvm_hip.h -- C ABI of the MI355X (gfx950) H-matrix-vector multiply.
+ *
+ * Drop-in part: the functions mirror the reference HMVM entry points one for
+ * one (same argument lists and meaning), so a caller of hmvm_seq / hmvm_omp /
+ * hmvm_cuda can switch by renaming the call:
+ *
+ *   reference                                   here
+ *   hmvm_seq(mat, mat2, b, dump)  hacapk.h:36   hmvm_hip(mat, mat2, b, dump)
+ *   hmvm_seq_bench(mat, mat2, b)  hacapk.h:37   hmvm_hip_bench(mat, mat2, b)
+ *   hmvm_seq_1 (v, mat,  b)       hmvm_seq.c:9    hmvm_hip_1 (v, mat,  b)
+ *   hmvm_seq_1t(v, mat,  b)       hmvm_seq.c:61   hmvm_hip_1t(v, mat,  b)
+ *   hmvm_seq_2 (v, mat2, b)       hmvm_seq.c:113  hmvm_hip_2 (v, mat2, b)
+ *   hmvm_seq_2t(v, mat2, b)       hmvm_seq.c:169  hmvm_hip_2t(v, mat2, b)
+ *   hmvm_cuda1<T>(mat2, b, kernel, dump, nbench)  hmvm_cuda.h:5
+ *                                               hmvm_hip<T>(...) in hmvm_hip.hpp
+ *
+ * Semantics (as the sequential reference): hmvm_hip_X ACCUMULATES, v += A*b;
+ * the caller zeroes v for a plain product.  All pointers are host pointers
+ * owned by the caller.  The first call on a matrix uploads it to the GPU and
+ * keeps the device copy, keyed by the struct address; call hmvm_hip_release()
+ * after changing or freeing the matrix.  Calls block until the result is back
+ * in host memory.  Errors never exit the process: the function returns, the
+ * reason is printed once to stderr and hmvm_hip_last_error() reports it.
+ * Not re-entrant on one matrix; one host thread per device.
+ *
+ * Device-resident part (no reference counterpart; used by Krylov loops,
+ * benchmarks and batched callers): an explicit plan object that holds the
+ * repacked matrix in HBM and applies y (+)= A x to device vectors on a stream.
+ */
+#ifndef HACAPK_AMD_HMVM_HIP_H
+#define HACAPK_AMD_HMVM_HIP_H
+
+#include <stddef.h>
+
+#include "hacapk.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- error codes ---------------------------------------------------------- */
+enum {
+  HMVM_OK = 0,
+  HMVM_ERR_ARG = -101,      /* NULL pointer or invalid option               */
+  HMVM_ERR_MATRIX = -102,   /* leaf descriptors inconsistent with nd / len  */
+  HMVM_ERR_HIP = -103,      /* HIP runtime error (message has the details)  */
+  HMVM_ERR_NOMEM = -104,    /* host allocation failed                       */
+  HMVM_ERR_NODEVICE = -105, /* no HIP device is visible                     */
+  HMVM_ERR_UNSUPPORTED = -106
+};
+
+/* ---- drop-in entry points ------------------------------------------------ */
+
+/* Runs variants 1, 1t (when mat != NULL) and 2, 2t (when mat2 != NULL) on a
+ * zeroed v and, when dump_result != 0, writes result_hip_<variant>_d.txt with
+ * one "%.3E" value per line (reference src_c/hmvm_seq.c:226-286). */
+void hmvm_hip(const matrix *mat, const matrix2 *mat2, const double *b, int dump_result);
+
+/* Times each available variant: 10 runs, statistics over the last 5, printed
+ * as "TIME 10 hmvm_hip_<variant> min <s> max <s> avg <s>" (reference
+ * src_c/hmvm_seq.c:289-390).  The timed region is the GPU kernels only, with
+ * b and v already in device memory, as the reference GPU proxies time it
+ * (src_cpp/hmvm_cuda1.cu:236-302). */
+void hmvm_hip_bench(const matrix *mat, const matrix2 *mat2, const double *b);
+
+/* v += A*b from the per-leaf arrays; a2 (1) or a2t (1t) for the low-rank
+ * second stage (reference src_c/hmvm_seq.c:9, :61). */
+void hmvm_hip_1(double *v, const matrix *mat, const double *b);
+void hmvm_hip_1t(double *v, const matrix *mat, const double *b);
+/* v += A*b from matrix2.rowmat (2) or matrix2.rowmat_t (2t)
+ * (reference src_c/hmvm_seq.c:113, :169). */
+void hmvm_hip_2(double *v, const matrix2 *mat2, const double *b);
+void hmvm_hip_2t(double *v, const matrix2 *mat2, const double *b);
+
+/* Status of the last call on this thread (HMVM_OK or an error code), and its
+ * message.  No reference counterpart: the reference calls exit(-1) instead
+ * (src_cpp/hmvm_cuda0.cu:11). */
+int hmvm_hip_last_error(void);
+const char *hmvm_hip_last_error_message(void);
+
+/* Drops the cached device copy of `mat` (a matrix* or matrix2*).  No
+ * reference counterpart (the reference re-uploads and leaks per call). */
+void hmvm_hip_release(const void *mat);
+
+/* ---- device-resident plan API --------------------------------------------- */
+enum { HMVM_STORAGE_F64 = 0, HMVM_STORAGE_F32 = 1, HMVM_STORAGE_BF16 = 2 };
+enum { HMVM_A2_ROWMAJOR = 0, HMVM_A2_TRANSPOSED = 1 };
+
+typedef struct hmvm_options {
+  int storage;           /* HMVM_STORAGE_*: element type kept in HBM            */
+  int w1;                /* target elements per phase-1 tile (0: default)       */
+  int w2;                /* target elements per phase-2 tile (0: default)       */
+  int block_threads;     /* threads per workgroup, multiple of 64 (0: default)  */
+  int reserved[4];
+} hmvm_options;
+
+typedef struct hmvm_plan_info {
+  int nd, nlf, nlowrank, ndense;
+  long long len;             /* reference element count                         */
+  long long device_elems;    /* elements in HBM incl. alignment padding         */
+  int storage, elem_bytes;
+  int ntiles1, ntiles2, npanels;
+  int ncol_groups, nrow_groups;
+  long long n_t, n_tslots, n_zslots;
+  long long elems_phase1, elems_phase2;
+  long long vec_reads_phase1, vec_reads_phase2;
+  long long algorithmic_bytes; /* elem_bytes*len + 16*nd (x read, y written)    */
+  long long device_bytes;      /* HBM held by the plan                          */
+} hmvm_plan_info;
+
+typedef struct hmvm_timing {
+  int iters;
+  double ms_avg, ms_min, ms_median; /* whole HMVM (all launches)                 */
+  double ms_phase1, ms_tsum, ms_phase2, ms_ysum; /* per-launch averages          */
+  double ms_stream; /* iters applies enqueued back to back, total / iters       */
+} hmvm_timing;
+
+typedef struct hmvm_plan hmvm_plan;
+
+void hmvm_options_default(hmvm_options *opt);
+
+/* Builds the schedule from the descriptors, uploads the values and repacks
+ * them on the GPU into the plan's layout.  a2_source selects rowmat
+ * (HMVM_A2_ROWMAJOR) or rowmat_t (HMVM_A2_TRANSPOSED) of mat2. */
+int hmvm_plan_create(const matrix2 *mat2, int a2_source, const hmvm_options *opt, hmvm_plan **out);
+/* Same from the array-of-structs form: a2 (ROWMAJOR) or a2t (TRANSPOSED). */
+int hmvm_plan_create_aos(const matrix *mat, int a2_source, const hmvm_options *opt, hmvm_plan **out);
+void hmvm_plan_destroy(hmvm_plan *plan);
+int hmvm_plan_get_info(const hmvm_plan *plan, hmvm_plan_info *info);
+
+/* d_y (+)= A d_x on `stream` (a hipStream_t; NULL = the plan's own stream).
+ * d_x and d_y are device pointers of nd doubles and must not overlap.
+ * Asynchronous; no host synchronisation and no allocation. */
+int hmvm_plan_apply(hmvm_plan *plan, const double *d_x, double *d_y, int accumulate, void *stream);
+/* Host-pointer form: copies x in, applies, copies y back (blocking). */
+int hmvm_plan_apply_host(hmvm_plan *plan, const double *x, double *y, int accumulate);
+/* Times `iters` applies on the plan's internal device vectors after `warmup`
+ * untimed ones, with HIP events around every launch. */
+int hmvm_plan_time(hmvm_plan *plan, int warmup, int iters, hmvm_timing *timing);
+
+/* Host-only check of the schedule a plan would use for this matrix (no GPU
+ * access): every element streamed once, every access in bounds. */
+int hmvm_plan_check(const matrix2 *mat2, const hmvm_options *opt, hmvm_plan_info *info);
+
+/* ---- small device helpers for callers without a HIP runtime of their own -- */
+int hmvm_device_count(void);
+int hmvm_device_alloc(void **ptr, size_t bytes);
+int hmvm_device_free(void *ptr);
+int hmvm_memcpy_h2d(void *dst, const void *src, size_t bytes);
+int hmvm_memcpy_d2h(void *dst, const void *src, size_t bytes);
+int hmvm_device_synchronize(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* HACAPK_AMD_HMVM_HIP_H */

@@ -1,0 +1,197 @@
+"""GPU parity of the HIP HMVM against the CPU oracle (hmvm_seq restated).
+
+Bar: FP64 y within 1e-13 relative l2 of the sequential
+FP64 oracle; the FP32-stored form within 1e-6 of it.  Every test calls the
+library through its C ABI (ctypes)."""
+import os
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def rel(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+@pytest.fixture(scope="module")
+def sphere4096(H):
+    h = H.generate(4096, geometry="sphere", eps=1e-4)
+    h.build_aos()
+    return h
+
+
+@pytest.fixture(scope="module")
+def cube3000(H):
+    h = H.generate(3000, geometry="cube", eps=1e-6)
+    h.build_aos()
+    return h
+
+
+@pytest.mark.parametrize("variant", ["1", "1t", "2", "2t"])
+@pytest.mark.parametrize("mname", ["sphere4096", "cube3000"])
+def test_variants_match_oracle(gpu, H, O, request, variant, mname):
+    h = request.getfixturevalue(mname)
+    rng = np.random.default_rng(11)
+    b = rng.uniform(-1, 1, h.nd)
+    ref = O.seq_2(h, b)
+    v = np.zeros(h.nd)
+    getattr(H, "hmvm_hip_" + variant)(v, h, b)
+    assert rel(v, ref) <= 1e-13
+
+
+def test_reference_input_vector(gpu, H, O, sphere4096):
+    h = sphere4096
+    b = np.sin(np.arange(1, h.nd + 1, dtype=np.float64))  # hmvm1.c:32-35
+    ref = O.seq_2(h, b)
+    v = np.zeros(h.nd)
+    H.hmvm_hip_2(v, h, b)
+    assert rel(v, ref) <= 1e-13
+
+
+def test_accumulates(gpu, H, O, sphere4096):
+    h = sphere4096
+    rng = np.random.default_rng(3)
+    b = rng.uniform(-1, 1, h.nd)
+    y0 = rng.uniform(-1, 1, h.nd)
+    v = y0.copy()
+    H.hmvm_hip_2(v, h, b)
+    ref = O.seq_2(h, b, y0.copy())
+    assert rel(v, ref) <= 1e-13
+
+
+def test_deterministic(gpu, H, sphere4096):
+    h = sphere4096
+    b = np.random.default_rng(5).uniform(-1, 1, h.nd)
+    runs = []
+    for _ in range(3):
+        v = np.zeros(h.nd)
+        H.hmvm_hip_2(v, h, b)
+        runs.append(v)
+    assert np.array_equal(runs[0], runs[1]) and np.array_equal(runs[0], runs[2])
+
+
+def test_linearity(gpu, H, cube3000):
+    h = cube3000
+    rng = np.random.default_rng(9)
+    b1, b2 = rng.uniform(-1, 1, h.nd), rng.uniform(-1, 1, h.nd)
+    y = [np.zeros(h.nd) for _ in range(3)]
+    H.hmvm_hip_2(y[0], h, b1)
+    H.hmvm_hip_2(y[1], h, b2)
+    H.hmvm_hip_2(y[2], h, 2.0 * b1 - 3.0 * b2)
+    assert rel(y[2], 2.0 * y[0] - 3.0 * y[1]) <= 1e-13
+
+
+def test_all_dense_closed_form(gpu, H):
+    """dummymatrix (loadmatrix.c:240): an all-dense grid equals a plain GEMV."""
+    h = H.dummymatrix(24, 24, 9)
+    b = np.random.default_rng(1).uniform(-1, 1, h.nd)
+    A = np.zeros((h.nd, h.nd))
+    m = h.mat
+    for i in range(m.nlf):
+        s = m.submat[i]
+        blk = np.ctypeslib.as_array(s.a1, (s.ndl * s.ndt,)).reshape(s.ndl, s.ndt)
+        A[s.nstrtl - 1:s.nstrtl - 1 + s.ndl, s.nstrtt - 1:s.nstrtt - 1 + s.ndt] = blk
+    for fn in (H.hmvm_hip_1, H.hmvm_hip_2):
+        v = np.zeros(h.nd)
+        fn(v, h, b)
+        assert rel(v, A @ b) <= 1e-13
+
+
+@pytest.mark.parametrize("storage,tol", [("f32", 1e-6), ("bf16", 2e-2)])
+def test_reduced_storage(gpu, H, O, sphere4096, storage, tol):
+    h = sphere4096
+    b = np.random.default_rng(2).uniform(-1, 1, h.nd)
+    ref = O.seq_2(h, b)
+    p = H.Plan(h, storage=storage)
+    y = p.apply_host(b)
+    err = rel(y, ref)
+    print("storage", storage, "rel err vs FP64 oracle", err)
+    assert err <= tol
+    p.close()
+
+
+@pytest.mark.parametrize("w", [64, 1000, 100000])
+def test_tile_sizes(gpu, H, O, cube3000, w):
+    h = cube3000
+    b = np.random.default_rng(4).uniform(-1, 1, h.nd)
+    p = H.Plan(h, w1=w, w2=w)
+    assert rel(p.apply_host(b), O.seq_2(h, b)) <= 1e-13
+    p.close()
+
+
+def test_plan_sources_agree(gpu, H, sphere4096):
+    h = sphere4096
+    b = np.random.default_rng(6).uniform(-1, 1, h.nd)
+    ys = [H.Plan(h, source=s).apply_host(b) for s in ("rowmat", "rowmat_t", "a2", "a2t")]
+    for y in ys[1:]:
+        assert np.array_equal(y, ys[0])  # same device layout -> same bits
+
+
+def test_golden_fixtures(gpu, H):
+    """Committed matrices + y produced by the compiled reference hmvm_seq_2."""
+    for name in sorted(os.listdir(GOLDEN)):
+        if not name.endswith("_1.bin"):
+            continue
+        stem = name[:-6]
+        h = H.loadHmatrix(os.path.join(GOLDEN, name))
+        h.build_aos()
+        g = np.load(os.path.join(GOLDEN, stem + "_y.npz"))
+        for key in ("sin", "rand"):
+            b, y = g["x_" + key], g["y_" + key]
+            v = np.zeros(h.nd)
+            H.hmvm_hip_2(v, h, b)
+            assert rel(v, y) <= 1e-13, (stem, key)
+
+
+def test_release_and_reupload(gpu, H, O):
+    h = H.generate(1500, geometry="cube", eps=1e-4)
+    b = np.random.default_rng(8).uniform(-1, 1, h.nd)
+    v = np.zeros(h.nd)
+    H.hmvm_hip_2(v, h, b)
+    a = h.arrays()
+    a["rowmat"][:] *= 2.0
+    a["rowmat_t"][:] *= 2.0
+    H.release(h)  # contents changed: drop the cached device copy
+    w = np.zeros(h.nd)
+    H.hmvm_hip_2(w, h, b)
+    assert rel(w, 2.0 * v) <= 1e-13
+
+
+def test_dump_and_bench(gpu, H, O, tmp_path, capfd, sphere4096):
+    h = sphere4096
+    b = np.sin(np.arange(1, h.nd + 1, dtype=np.float64))
+    cwd = os.getcwd()
+    os.chdir(tmp_path)
+    try:
+        H.hmvm_hip(h, b, 1)
+        H.hmvm_hip_bench(h, b)
+    finally:
+        os.chdir(cwd)
+    ref = O.seq_2(h, b)
+    for var in ("1", "1t", "2", "2t"):
+        lines = open(tmp_path / ("result_hip_%s_d.txt" % var)).read().split()
+        assert len(lines) == h.nd
+        assert lines[0] == "%.3E" % ref[0]
+    out = capfd.readouterr().out
+    assert out.count("TIME 10 hmvm_hip_") == 4
+
+
+def test_plan_time(gpu, H, sphere4096):
+    p = H.Plan(sphere4096)
+    t = p.time(warmup=2, iters=5)
+    assert t["ms_avg"] > 0 and t["ms_phase1"] > 0 and t["ms_phase2"] > 0
+    info = p.info()
+    assert info["len"] == sphere4096.mat2.len
+    p.close()
+
+
+def test_bad_arguments_report_errors(gpu, H):
+    h = H.generate(600, geometry="sphere")
+    with pytest.raises(H.HMVMError):
+        H.hmvm_hip_2(np.zeros(10), h, np.zeros(h.nd))
+    with pytest.raises(H.HMVMError):
+        H.Plan(h, storage=7)
