@@ -1,0 +1,71 @@
+"""Host logic of the HIP path (CPU only, no GPU calls): the C ABI exports, and
+the schedule checker on generated and synthetic matrices."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+from conftest import ROOT
+
+
+def declared_functions():
+    names = set()
+    for h in ("hacapk.h", "hgen.h", "hmvm_hip.h"):
+        text = open(os.path.join(ROOT, "include", h)).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        for m in re.finditer(r"^[A-Za-z_][\w \*]*?\b([a-zA-Z_]\w*)\s*\([^;{]*\)\s*;", text, flags=re.M):
+            if not m.group(0).lstrip().startswith(("typedef", "#")):
+                names.add(m.group(1))
+    return sorted(names)
+
+
+def test_c_abi_exports_every_declared_symbol(H):
+    names = declared_functions()
+    assert "hmvm_hip_2" in names and "loadHmatrix" in names and "hgen_generate" in names
+    lib = ctypes.CDLL(H.LIB_PATH)
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+@pytest.mark.parametrize("w", [64, 512, 16384, 1 << 20])
+def test_schedule_check_generated(H, w):
+    h = H.generate(3000, geometry="cube", eps=1e-4)
+    info = H.plan_check(h, w1=w, w2=w)
+    assert info["len"] == h.mat2.len
+    assert info["elems_phase1"] + info["elems_phase2"] == h.mat2.len
+    assert info["device_elems"] >= h.mat2.len
+    assert info["algorithmic_bytes"] == 8 * h.mat2.len + 16 * h.nd
+    a = h.arrays()
+    lr = a["ltmtx"] == 1
+    assert info["nlowrank"] == lr.sum() and info["ndense"] == (~lr).sum()
+    assert info["n_t"] == a["kt"][lr].sum()
+
+
+def test_schedule_check_dense_grid(H):
+    h = H.dummymatrix(40, 40, 5)  # rows of 40 -> one 40-row tile per row cluster
+    info = H.plan_check(h)
+    assert info["ntiles1"] == 0 and info["elems_phase2"] == h.mat2.len
+
+
+def test_schedule_rejects_bad_descriptors(H):
+    h = H.generate(500, geometry="sphere", leaf=16)
+    a = h.arrays()
+    saved = a["nstrtl"][3]
+    a["nstrtl"][3] = h.nd  # leaf now runs past the last row
+    with pytest.raises(H.HMVMError):
+        H.plan_check(h)
+    a["nstrtl"][3] = saved
+    saved = a["a1"][5]
+    a["a1"][5] = h.mat2.len  # factor outside the value buffer
+    with pytest.raises(H.HMVMError):
+        H.plan_check(h)
+    a["a1"][5] = saved
+    H.plan_check(h)
+
+
+def test_storage_option_validated(H):
+    h = H.generate(400, geometry="sphere", leaf=16)
+    assert H.plan_check(h, storage="bf16")["elem_bytes"] == 2
+    assert H.plan_check(h, storage="f32")["algorithmic_bytes"] == 4 * h.mat2.len + 16 * h.nd
