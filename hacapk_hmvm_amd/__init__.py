@@ -55,18 +55,52 @@ class hgen_params(ctypes.Structure):
 
 class hmvm_options(ctypes.Structure):
     _fields_ = [("storage", ctypes.c_int), ("w1", ctypes.c_int), ("w2", ctypes.c_int),
-                ("block_threads", ctypes.c_int), ("reserved", ctypes.c_int * 4)]
+                ("block_threads", ctypes.c_int), ("kernel", ctypes.c_int), ("order", ctypes.c_int),
+                ("reserved", ctypes.c_int * 2)]
+
+
+class hmvm_tile(ctypes.Structure):
+    _fields_ = [("nrows", ctypes.c_int), ("panel0", ctypes.c_int), ("npanel", ctypes.c_int), ("out_off", ctypes.c_int)]
+
+
+class hmvm_panel(ctypes.Structure):
+    _fields_ = [("val_off", ctypes.c_longlong), ("ld", ctypes.c_int), ("ncols", ctypes.c_int),
+                ("vec_off", ctypes.c_int), ("vec_sel", ctypes.c_int), ("factor", ctypes.c_int),
+                ("vec_chunks", ctypes.c_int)]
+
+
+class hmvm_repack(ctypes.Structure):
+    _fields_ = [("src_off", ctypes.c_longlong), ("dst_off", ctypes.c_longlong), ("rows", ctypes.c_int),
+                ("cols", ctypes.c_int), ("src_rs", ctypes.c_int), ("src_cs", ctypes.c_int), ("dst_cs", ctypes.c_int),
+                ("factor", ctypes.c_int)]
+
+
+class hmvm_segsum(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_int), ("nseg", ctypes.c_int), ("seg", ctypes.POINTER(ctypes.c_int)),
+                ("ptr", ctypes.POINTER(ctypes.c_int)), ("list", ctypes.POINTER(ctypes.c_longlong))]
+
+
+class hmvm_schedule(ctypes.Structure):
+    _fields_ = [("ntiles_a", ctypes.c_int), ("ntiles_b", ctypes.c_int), ("npanels", ctypes.c_int),
+                ("nrepack", ctypes.c_int), ("t_fused", ctypes.c_int),
+                ("nvals", ctypes.c_longlong), ("n_t", ctypes.c_longlong), ("n_tslots", ctypes.c_longlong),
+                ("n_zslots", ctypes.c_longlong),
+                ("tiles_a", ctypes.POINTER(hmvm_tile)), ("tiles_b", ctypes.POINTER(hmvm_tile)),
+                ("panels", ctypes.POINTER(hmvm_panel)), ("repack", ctypes.POINTER(hmvm_repack)),
+                ("tsum", hmvm_segsum), ("ysum", hmvm_segsum)]
 
 
 class hmvm_plan_info(ctypes.Structure):
     _fields_ = [("nd", ctypes.c_int), ("nlf", ctypes.c_int), ("nlowrank", ctypes.c_int), ("ndense", ctypes.c_int),
                 ("len", ctypes.c_longlong), ("device_elems", ctypes.c_longlong),
                 ("storage", ctypes.c_int), ("elem_bytes", ctypes.c_int),
-                ("ntiles1", ctypes.c_int), ("ntiles2", ctypes.c_int), ("npanels", ctypes.c_int),
-                ("ncol_groups", ctypes.c_int), ("nrow_groups", ctypes.c_int),
+                ("ntiles_a", ctypes.c_int), ("ntiles_b", ctypes.c_int), ("npanels", ctypes.c_int),
+                ("ncol_groups", ctypes.c_int), ("nrow_groups", ctypes.c_int), ("t_fused", ctypes.c_int),
                 ("n_t", ctypes.c_longlong), ("n_tslots", ctypes.c_longlong), ("n_zslots", ctypes.c_longlong),
-                ("elems_phase1", ctypes.c_longlong), ("elems_phase2", ctypes.c_longlong),
-                ("vec_reads_phase1", ctypes.c_longlong), ("vec_reads_phase2", ctypes.c_longlong),
+                ("elems_a", ctypes.c_longlong), ("elems_b", ctypes.c_longlong),
+                ("elems_stage1", ctypes.c_longlong), ("elems_dense", ctypes.c_longlong),
+                ("elems_stage2", ctypes.c_longlong),
+                ("vec_reads_a", ctypes.c_longlong), ("vec_reads_b", ctypes.c_longlong),
                 ("algorithmic_bytes", ctypes.c_longlong), ("device_bytes", ctypes.c_longlong)]
 
     def as_dict(self):
@@ -75,8 +109,8 @@ class hmvm_plan_info(ctypes.Structure):
 
 class hmvm_timing(ctypes.Structure):
     _fields_ = [("iters", ctypes.c_int), ("ms_avg", ctypes.c_double), ("ms_min", ctypes.c_double),
-                ("ms_median", ctypes.c_double), ("ms_phase1", ctypes.c_double), ("ms_tsum", ctypes.c_double),
-                ("ms_phase2", ctypes.c_double), ("ms_ysum", ctypes.c_double), ("ms_stream", ctypes.c_double)]
+                ("ms_median", ctypes.c_double), ("ms_a", ctypes.c_double), ("ms_tsum", ctypes.c_double),
+                ("ms_b", ctypes.c_double), ("ms_ysum", ctypes.c_double), ("ms_stream", ctypes.c_double)]
 
     def as_dict(self):
         return {name: getattr(self, name) for name, _ in self._fields_}
@@ -133,6 +167,10 @@ def _load():
         "hmvm_plan_apply_host": (ctypes.c_int, [ctypes.c_void_p, c_double_p, c_double_p, ctypes.c_int]),
         "hmvm_plan_time": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, P(hmvm_timing)]),
         "hmvm_plan_check": (ctypes.c_int, [P(matrix2), P(hmvm_options), P(hmvm_plan_info)]),
+        "hmvm_plan_set_launch": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]),
+        "hmvm_schedule_export": (ctypes.c_int, [P(matrix2), ctypes.c_int, P(hmvm_options), P(hmvm_schedule)]),
+        "hmvm_schedule_release": (None, [P(hmvm_schedule)]),
+        "hmvm_bandwidth_probe": (ctypes.c_int, [ctypes.c_size_t, ctypes.c_int, c_double_p, c_double_p]),
         "hmvm_device_count": (ctypes.c_int, []),
         "hmvm_device_alloc": (ctypes.c_int, [P(ctypes.c_void_p), ctypes.c_size_t]),
         "hmvm_device_free": (ctypes.c_int, [ctypes.c_void_p]),
@@ -345,9 +383,20 @@ def device_count():
 
 
 # ---- device-resident plan ---------------------------------------------------------------
-def options(storage="f64", w1=0, w2=0, block_threads=0):
+def bandwidth_probe(nbytes=2 << 30, iters=10):
+    """Achievable (read, copy) GB/s of this device with plain streaming kernels."""
+    r, c = ctypes.c_double(), ctypes.c_double()
+    rc = lib.hmvm_bandwidth_probe(nbytes, iters, ctypes.byref(r), ctypes.byref(c))
+    if rc:
+        raise HMVMError("hmvm_bandwidth_probe: %s" % lib.hmvm_hip_last_error_message().decode())
+    return r.value, c.value
+
+
+def options(storage="f64", w1=0, w2=0, block_threads=0, kernel=0, order=0):
     o = hmvm_options()
     lib.hmvm_options_default(ctypes.byref(o))
+    o.kernel = kernel
+    o.order = order
     o.storage = STORAGE[storage] if isinstance(storage, str) else int(storage)
     if w1:
         o.w1 = w1
@@ -366,6 +415,44 @@ def plan_check(h, **kw):
     if rc:
         raise HMVMError("hmvm_plan_check: %s" % lib.hmvm_hip_last_error_message().decode())
     return info.as_dict()
+
+
+def _struct_array(ptr, n, dtype):
+    if n == 0:
+        return np.zeros(0, dtype=dtype)
+    raw = np.ctypeslib.as_array(ctypes.cast(ptr, ctypes.POINTER(ctypes.c_uint8)), (n * dtype.itemsize,))
+    return raw.view(dtype).copy()
+
+
+TILE_DT = np.dtype([("nrows", "<i4"), ("panel0", "<i4"), ("npanel", "<i4"), ("out_off", "<i4")])
+PANEL_DT = np.dtype([("val_off", "<i8"), ("ld", "<i4"), ("ncols", "<i4"), ("vec_off", "<i4"), ("vec_sel", "<i4"),
+                     ("factor", "<i4"), ("vec_chunks", "<i4")])
+REPACK_DT = np.dtype([("src_off", "<i8"), ("dst_off", "<i8"), ("rows", "<i4"), ("cols", "<i4"), ("src_rs", "<i4"),
+                      ("src_cs", "<i4"), ("dst_cs", "<i4"), ("factor", "<i4")])
+
+
+def schedule(h, source="rowmat", **kw):
+    """The tile schedule a plan would run for this matrix, as numpy arrays (host only)."""
+    s = hmvm_schedule()
+    o = options(**kw)
+    rc = lib.hmvm_schedule_export(ctypes.byref(h.mat2), 1 if source == "rowmat_t" else 0, ctypes.byref(o),
+                                  ctypes.byref(s))
+    if rc:
+        raise HMVMError("hmvm_schedule_export: %s" % lib.hmvm_hip_last_error_message().decode())
+    try:
+        def seg(ss):
+            return {"n": ss.n, "seg": np.ctypeslib.as_array(ss.seg, (max(ss.n, 1),))[:ss.n].copy(),
+                    "ptr": np.ctypeslib.as_array(ss.ptr, (ss.nseg + 1,)).copy(),
+                    "list": (np.ctypeslib.as_array(ss.list, (max(ss.ptr[ss.nseg], 1),))[:ss.ptr[ss.nseg]].copy())}
+        return {"t_fused": bool(s.t_fused), "nvals": s.nvals, "n_t": s.n_t, "n_tslots": s.n_tslots,
+                "n_zslots": s.n_zslots,
+                "tiles_a": _struct_array(s.tiles_a, s.ntiles_a, TILE_DT),
+                "tiles_b": _struct_array(s.tiles_b, s.ntiles_b, TILE_DT),
+                "panels": _struct_array(s.panels, s.npanels, PANEL_DT),
+                "repack": _struct_array(s.repack, s.nrepack, REPACK_DT),
+                "tsum": seg(s.tsum), "ysum": seg(s.ysum)}
+    finally:
+        lib.hmvm_schedule_release(ctypes.byref(s))
 
 
 class Plan:
@@ -414,6 +501,11 @@ class Plan:
                                  ctypes.c_void_p(stream) if stream else None)
         if rc:
             raise HMVMError("hmvm_plan_apply: %s" % lib.hmvm_hip_last_error_message().decode())
+
+    def set_launch(self, kernel, block_threads=256):
+        rc = lib.hmvm_plan_set_launch(self._p, kernel, block_threads)
+        if rc:
+            raise HMVMError("hmvm_plan_set_launch: %s" % lib.hmvm_hip_last_error_message().decode())
 
     def time(self, warmup=5, iters=20):
         t = hmvm_timing()

@@ -1,10 +1,13 @@
 // hmvm_kernels.hpp -- gfx950 device code of the HMVM (included by the runtime).
 //
-// Kernel roles (one HMVM = four launches on one stream):
-//   tile_kernel (phase 1)  t-slots  = stacked a1 panels x x-slices
-//   segsum_kernel (t)      t        = sum of t-slots            (fixed order)
-//   tile_kernel (phase 2)  z-slots  = dense / a2 panels x x- or t-slices
-//   segsum_kernel (y)      y        = y + sum of z-slots         (fixed order)
+// Kernel roles (one HMVM = three launches on one stream, four when a leaf's
+// rank rows straddle two stage-1 row groups):
+//   tile_kernel (launch A)  stage-1 slots = stacked a1 panels x x-slices,
+//                           dense slots   = transposed dense leaves x x-slices
+//   [segsum_kernel (t)      t = sum of stage-1 slots            (fixed order)]
+//   tile_kernel (launch B)  stage-2 slots = a2 panels x t (summed from the
+//                           stage-1 slots on the fly, or the materialised t)
+//   segsum_kernel (y)       y = y + sum of dense and stage-2 slots (fixed order)
 // plus the one-time repack kernels that move the caller's rowmat layout into
 // the plan's layout (and, for BF16 storage, compute per-factor scales).
 #pragma once
@@ -48,64 +51,227 @@ __device__ __forceinline__ uint16_t store_val<uint16_t>(double v, float inv_scal
   return (uint16_t)((b + 0x7FFFu + ((b >> 16) & 1u)) >> 16);
 }
 
-// One wavefront per tile; lane = (row r, column group jg).  With nrows rows
-// the wave holds g = 64/nrows column groups; lane (r, jg) sums columns
-// jg, jg+g, jg+2g, ... of every panel of the tile, so for a contiguous panel
-// (ld == nrows) each load instruction reads g*nrows consecutive elements.
-// The g partial sums of a row are combined by shuffles at the end, in a fixed
-// order, and written to the tile's private output slot.
-template <typename S, bool kScaled, int kUnroll>
-__global__ __launch_bounds__(256) void tile_kernel(const TileDesc *__restrict__ tiles, int ntiles,
-                                                   const PanelDesc *__restrict__ panels,
-                                                   const S *__restrict__ vals,
-                                                   const double *__restrict__ x,
-                                                   const double *__restrict__ t,
-                                                   const float *__restrict__ scales,
-                                                   double *__restrict__ out) {
+template <bool kNT, typename S>
+__device__ __forceinline__ auto load_stream(const S *p) {
+  if constexpr (kNT) {
+    if constexpr (sizeof(S) == 2) {
+      return __uint_as_float(((uint32_t)__builtin_nontemporal_load(p)) << 16);
+    } else {
+      return __builtin_nontemporal_load(p);
+    }
+  } else {
+    return load_val(p);
+  }
+}
+
+// Kernel variants (bit field): bit0 16 loads in flight per batch instead of 8,
+// bit1 software pipelining (next batch issued before the current one is
+// consumed), bit2 non-temporal matrix loads.
+constexpr int kNumKernelVariants = 8;
+
+// Vector entry j of a panel, for a lane-varying j.
+__device__ __forceinline__ double vec_entry(const PanelDesc &pd, const double *__restrict__ x,
+                                            const double *__restrict__ t, const double *__restrict__ tp, int j) {
+  if (pd.vec_sel == kVecX) return x[pd.vec_off + j];
+  if (pd.vec_sel == kVecT) return t[pd.vec_off + j];
+  const int nch = pd.vec_chunks >> 16, stride = pd.vec_chunks & 0xFFFF;
+  double s = 0;
+  for (int c = 0; c < nch; ++c) s += tp[pd.vec_off + c * stride + j];
+  return s;
+}
+
+// Scale of a panel's product under BF16 storage: its own factor, times the
+// a1 factor of the same leaf when the panel sums unscaled stage-1 partials.
+__device__ __forceinline__ float panel_scale(const PanelDesc &pd, const float *__restrict__ scales) {
+  const float s = scales[pd.factor];
+  return pd.vec_sel == kVecTP ? s * scales[pd.factor - 1] : s;
+}
+
+// nrows in (32, 256]: lane owns rows lane + 64q, q < R; every column is read
+// whole (R coalesced loads), the vector entry is wave-uniform.
+template <typename S, int R, int kLoads, bool kPipe, bool kNT>
+__device__ __forceinline__ void rows_panel(const S *__restrict__ vals, const PanelDesc &pd, int lane, int nrows,
+                                           const double *__restrict__ x, const double *__restrict__ t,
+                                           const double *__restrict__ tp,
+                                           typename AccOf<S>::T (&accp)[R]) {
   using A = typename AccOf<S>::T;
-  const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
-  const int lane = (int)(threadIdx.x & 63);
-  const int tid = (int)blockIdx.x * (int)(blockDim.x >> 6) + wave;
-  if (tid >= ntiles) return;
-  const TileDesc td = tiles[tid];
+  constexpr int U = kLoads / R > 0 ? kLoads / R : 1;
+  const int ld = pd.ld, nc = pd.ncols;
+  const S *__restrict__ base[R];
+#pragma unroll
+  for (int q = 0; q < R; ++q) {
+    const int row = lane + 64 * q;
+    base[q] = vals + pd.val_off + (row < nrows ? row : 0);
+  }
+  int j = 0;
+  if (kPipe && nc >= 2 * U) {
+    A a0[U][R], b0[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      b0[u] = (A)vec_entry(pd, x, t, tp, u);
+#pragma unroll
+      for (int q = 0; q < R; ++q) a0[u][q] = (A)load_stream<kNT>(base[q] + (int64_t)u * ld);
+    }
+    for (j = U; j + U <= nc; j += U) {
+      A a1[U][R], b1[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        b1[u] = (A)vec_entry(pd, x, t, tp, j + u);
+#pragma unroll
+        for (int q = 0; q < R; ++q) a1[u][q] = (A)load_stream<kNT>(base[q] + (int64_t)(j + u) * ld);
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+#pragma unroll
+        for (int q = 0; q < R; ++q) accp[q] += a0[u][q] * b0[u];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        b0[u] = b1[u];
+#pragma unroll
+        for (int q = 0; q < R; ++q) a0[u][q] = a1[u][q];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+#pragma unroll
+      for (int q = 0; q < R; ++q) accp[q] += a0[u][q] * b0[u];
+  } else {
+    for (; j + U <= nc; j += U) {
+      A a[U][R], b[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        b[u] = (A)vec_entry(pd, x, t, tp, j + u);
+#pragma unroll
+        for (int q = 0; q < R; ++q) a[u][q] = (A)load_stream<kNT>(base[q] + (int64_t)(j + u) * ld);
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+#pragma unroll
+        for (int q = 0; q < R; ++q) accp[q] += a[u][q] * b[u];
+    }
+  }
+  for (; j < nc; ++j) {
+    const A b = (A)vec_entry(pd, x, t, tp, j);
+#pragma unroll
+    for (int q = 0; q < R; ++q) accp[q] += (A)load_stream<kNT>(base[q] + (int64_t)j * ld) * b;
+  }
+}
+
+template <typename S, int R, int kLoads, bool kPipe, bool kNT, bool kScaled>
+__device__ __forceinline__ void rows_tile(const TileDesc &td, const PanelDesc *__restrict__ panels,
+                                          const S *__restrict__ vals, const double *__restrict__ x,
+                                          const double *__restrict__ t, const double *__restrict__ tp,
+                                          const float *__restrict__ scales, double *__restrict__ out, int lane) {
+  using A = typename AccOf<S>::T;
+  A acc[R];
+#pragma unroll
+  for (int q = 0; q < R; ++q) acc[q] = 0;
+  for (int p = 0; p < td.npanel; ++p) {
+    const PanelDesc pd = panels[td.panel0 + p];
+    A accp[R];
+#pragma unroll
+    for (int q = 0; q < R; ++q) accp[q] = 0;
+    rows_panel<S, R, kLoads, kPipe, kNT>(vals, pd, lane, td.nrows, x, t, tp, accp);
+    const A sc = kScaled ? (A)panel_scale(pd, scales) : (A)1;
+#pragma unroll
+    for (int q = 0; q < R; ++q) acc[q] += kScaled ? accp[q] * sc : accp[q];
+  }
+#pragma unroll
+  for (int q = 0; q < R; ++q) {
+    const int row = lane + 64 * q;
+    if (row < td.nrows) out[td.out_off + row] = (double)acc[q];
+  }
+}
+
+// nrows <= 32: the wave holds g = 64/nrows column groups; lane (r, jg) sums
+// columns jg, jg+g, ... so for a contiguous panel (ld == nrows) each load
+// instruction reads g*nrows consecutive elements.  The g partial sums of a row
+// are combined by shuffles at the end, in a fixed order.
+template <typename S, int kLoads, bool kNT, bool kScaled>
+__device__ __forceinline__ void small_tile(const TileDesc &td, const PanelDesc *__restrict__ panels,
+                                           const S *__restrict__ vals, const double *__restrict__ x,
+                                           const double *__restrict__ t, const double *__restrict__ tp,
+                                           const float *__restrict__ scales, double *__restrict__ out, int lane) {
+  using A = typename AccOf<S>::T;
   const int nrows = td.nrows;
   const int g = 64 / nrows;
   const int jg = lane / nrows;
   const int r = lane - jg * nrows;
-  const bool active = jg < g;
   A acc = 0;
   for (int p = 0; p < td.npanel; ++p) {
     const PanelDesc pd = panels[td.panel0 + p];
     const int ld = pd.ld, nc = pd.ncols;
-    const double *__restrict__ v = (pd.vec_sel ? t : x) + pd.vec_off;
     const int64_t gs = (int64_t)g * ld;
-    int j = active ? jg : nc;
+    int j = jg < g ? jg : nc;
     const S *__restrict__ pj = vals + pd.val_off + r + (int64_t)j * ld;
     A accp = 0;
-    for (; j + (kUnroll - 1) * g < nc; j += kUnroll * g) {
-      A a[kUnroll], b[kUnroll];
+    for (; j + (kLoads - 1) * g < nc; j += kLoads * g) {
+      A a[kLoads], b[kLoads];
 #pragma unroll
-      for (int u = 0; u < kUnroll; ++u) {
-        a[u] = (A)load_val(pj + u * gs);
-        b[u] = (A)v[j + u * g];
+      for (int u = 0; u < kLoads; ++u) {
+        a[u] = (A)load_stream<kNT>(pj + u * gs);
+        b[u] = (A)vec_entry(pd, x, t, tp, j + u * g);
       }
 #pragma unroll
-      for (int u = 0; u < kUnroll; ++u) accp += a[u] * b[u];
-      pj += kUnroll * gs;
+      for (int u = 0; u < kLoads; ++u) accp += a[u] * b[u];
+      pj += kLoads * gs;
     }
     for (; j < nc; j += g) {
-      accp += (A)load_val(pj) * (A)v[j];
+      accp += (A)load_stream<kNT>(pj) * (A)vec_entry(pd, x, t, tp, j);
       pj += gs;
     }
-    if (kScaled)
-      acc += accp * (A)scales[pd.factor];
-    else
-      acc += accp;
+    acc += kScaled ? accp * (A)panel_scale(pd, scales) : accp;
   }
   const double mine = (double)acc;
   double s = 0;
   for (int q = 0; q < g; ++q) s += __shfl(mine, r + q * nrows, 64);
   if (lane < nrows) out[td.out_off + lane] = s;
+}
+
+// One wavefront per tile (see hmvm_plan.hpp for the tile model).
+template <typename S, bool kScaled, int kLoads, bool kPipe, bool kNT>
+__global__ __launch_bounds__(256) void tile_kernel(const TileDesc *__restrict__ tiles, int ntiles,
+                                                   const PanelDesc *__restrict__ panels,
+                                                   const S *__restrict__ vals,
+                                                   const double *__restrict__ x,
+                                                   const double *__restrict__ t,
+                                                   const double *__restrict__ tp,
+                                                   const float *__restrict__ scales,
+                                                   double *__restrict__ out) {
+  const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  const int lane = (int)(threadIdx.x & 63);
+  const int tid = (int)blockIdx.x * (int)(blockDim.x >> 6) + wave;
+  if (tid >= ntiles) return;
+  const TileDesc td = tiles[tid];
+  if (td.nrows <= 32) {
+    small_tile<S, kLoads, kNT, kScaled>(td, panels, vals, x, t, tp, scales, out, lane);
+  } else if (td.nrows <= 64) {
+    rows_tile<S, 1, kLoads, kPipe, kNT, kScaled>(td, panels, vals, x, t, tp, scales, out, lane);
+  } else if (td.nrows <= 128) {
+    rows_tile<S, 2, kLoads, kPipe, kNT, kScaled>(td, panels, vals, x, t, tp, scales, out, lane);
+  } else if (td.nrows <= 192) {
+    rows_tile<S, 3, kLoads, kPipe, kNT, kScaled>(td, panels, vals, x, t, tp, scales, out, lane);
+  } else {
+    rows_tile<S, 4, kLoads, kPipe, kNT, kScaled>(td, panels, vals, x, t, tp, scales, out, lane);
+  }
+}
+
+// Achievable-bandwidth probes: a read-only stream (sum) and a copy, each
+// element touched once, grid-stride.
+__global__ __launch_bounds__(256) void probe_read_kernel(const double4 *__restrict__ src, size_t n,
+                                                         double *__restrict__ sink) {
+  double s = 0;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+    const double4 v = src[i];
+    s += v.x + v.y + v.z + v.w;
+  }
+  if (s == 1.2345e-300) sink[0] = s;  // keeps the loads alive
+}
+
+__global__ __launch_bounds__(256) void probe_copy_kernel(const double4 *__restrict__ src, double4 *__restrict__ dst,
+                                                         size_t n) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    dst[i] = src[i];
 }
 
 // out[i] = (accumulate ? out[i] : 0) + scale(i) * sum_e buf[list[e] + i]

@@ -3,9 +3,10 @@
 #include "hmvm_plan.hpp"
 
 #include <algorithm>
+#include <climits>
 #include <cstdio>
 #include <map>
-#include <tuple>
+#include <numeric>
 #include <utility>
 
 namespace hmvm {
@@ -14,10 +15,10 @@ namespace {
 
 inline int64_t align_up(int64_t v, int64_t a) { return (v + a - 1) / a * a; }
 
-// Splits n rows into ceil(n/64) near-equal chunks (sizes differ by <= 1).
+// Splits n rows into ceil(n/kMaxTileRows) near-equal chunks.
 std::vector<std::pair<int, int>> row_chunks(int n) {
   std::vector<std::pair<int, int>> out;
-  const int nc = (n + 63) / 64;
+  const int nc = (n + kMaxTileRows - 1) / kMaxTileRows;
   int r0 = 0;
   for (int c = 0; c < nc; c++) {
     const int sz = n / nc + (c < n % nc ? 1 : 0);
@@ -27,10 +28,27 @@ std::vector<std::pair<int, int>> row_chunks(int n) {
   return out;
 }
 
+// Columns per tile for a tile of nr rows and a target of w elements; for
+// nr <= 32 a multiple of the 64/nr lane groups.
+int chunk_cols(int nr, int w) {
+  int c = std::max(1, w / nr);
+  if (nr <= 32) {
+    const int g = 64 / nr;
+    c = (c + g - 1) / g * g;
+  }
+  return c;
+}
+
 std::string fmt(const char *f, long long a, long long b = 0) {
   char buf[256];
   std::snprintf(buf, sizeof buf, f, a, b);
   return buf;
+}
+
+int64_t tile_work(const Plan &P, const TileDesc &t) {
+  int64_t w = 0;
+  for (int p = t.panel0; p < t.panel0 + t.npanel; p++) w += (int64_t)t.nrows * P.panels[p].ncols;
+  return w;
 }
 
 }  // namespace
@@ -42,6 +60,7 @@ std::string build_plan(const LeafArrays &la, const PlanOptions &opt, Plan *plan)
   P.nlf = la.nlf;
   if (la.nd < 1 || la.nlf < 1) return "empty matrix";
   if (opt.w1 < 64 || opt.w2 < 64 || opt.align < 1) return "bad plan options";
+  if (la.nd > (1 << 30)) return "matrix order too large";
   const int nlf = la.nlf;
 
   // ---- validation and reference element count --------------------------------
@@ -54,16 +73,14 @@ std::string build_plan(const LeafArrays &la, const PlanOptions &opt, Plan *plan)
     if (lt == 1 && kt < 0) return fmt("leaf %lld: negative kt %lld", i, kt);
     const int64_t n1 = lt == 1 ? (int64_t)kt * ndt : (int64_t)ndl * ndt;
     const int64_t n2 = lt == 1 ? (int64_t)kt * ndl : 0;
-    if (la.src_a1[i] < 0 || la.src_a1[i] + n1 > la.src_len || (lt == 1 && (la.src_a2[i] < 0 || la.src_a2[i] + n2 > la.src_len)))
+    if (la.src_a1[i] < 0 || la.src_a1[i] + n1 > la.src_len ||
+        (lt == 1 && (la.src_a2[i] < 0 || la.src_a2[i] + n2 > la.src_len)))
       return fmt("leaf %lld: factor outside the value buffer (len %lld)", i, la.src_len);
     P.len += n1 + n2;
     if (lt == 1) P.nlr++; else P.ndense++;
   }
-  if (la.nd > (1 << 30)) return "matrix order too large";
 
   // ---- groups -------------------------------------------------------------------
-  // column clusters of the low-rank leaves (phase 1) and row clusters of all
-  // leaves (phase 2), both in ascending (start, size) order.
   std::map<std::pair<int, int>, std::vector<int>> colg, rowg;
   for (int i = 0; i < nlf; i++) {
     const bool lr = la.ltmtx[i] == 1;
@@ -76,80 +93,120 @@ std::string build_plan(const LeafArrays &la, const PlanOptions &opt, Plan *plan)
 
   // Factor ids: a1 of leaf i -> 2i, a2 / dense -> 2i+1.
   P.nfactors = 2 * nlf;
-  std::vector<int64_t> t_off(nlf, -1);
-  std::vector<int64_t> dbase(nlf, -1);  // device base of a2 / dense factor
+  std::vector<int64_t> t_idx(nlf, -1);     // index of t_b[0] in the materialised t
+  std::vector<int64_t> tp_off(nlf, -1);    // partial slot of t_b[0], chunk 0
+  std::vector<int32_t> tp_chunks(nlf, 0);  // (nch << 16) | stride
+  std::vector<int64_t> dbase(nlf, -1);     // device base of a2 / dense factor
   int64_t off = 0;
+  std::vector<TileDesc> t1, td, t2;        // stage-1, dense, stage-2 tiles
 
-  // ---- phase 1 layout + tiles -----------------------------------------------------
+  // ---- stage 1 layout + tiles -----------------------------------------------------
   int64_t tpos = 0;
   for (auto &g : colg) {
     const int nstrtt = g.first.first, ndt = g.first.second;
     const std::vector<int> &ls = g.second;
-    int K = 0;
-    for (int i : ls) K += la.kt[i];
+    int64_t K64 = 0;
+    for (int i : ls) K64 += la.kt[i];
+    if (K64 > INT_MAX / 2) return "rank stack too large";
+    const int K = (int)K64;
     const int64_t base = align_up(off, opt.align);
     off = base + (int64_t)ndt * K;
+    // row groups of <= kMaxTileRows rank rows, cut at leaf boundaries when a
+    // leaf fits; a leaf of more than kMaxTileRows ranks is cut inside
+    struct Piece { int leaf, k0, n; };  // leaf rows [k0, k0+n) at stack row kofs
+    std::vector<std::vector<Piece>> groups(1);
+    std::vector<int> gsize(1, 0);
+    std::vector<int> kofs_of(ls.size());
     int kofs = 0;
-    for (int i : ls) {
+    for (size_t q = 0; q < ls.size(); q++) {
+      const int i = ls[q], kt = la.kt[i];
+      kofs_of[q] = kofs;
       RepackDesc r;
       r.src_off = la.src_a1[i];
       r.dst_off = base + kofs;
-      r.rows = la.kt[i];  // k
-      r.cols = ndt;       // it
+      r.rows = kt;   // k
+      r.cols = ndt;  // it
       r.src_rs = ndt;
       r.src_cs = 1;
       r.dst_cs = K;
       r.factor = 2 * i;
       P.repack.push_back(r);
-      t_off[i] = tpos + kofs;
-      for (int k = 0; k < la.kt[i]; k++) P.factor_of_trow.push_back(2 * i);
-      kofs += la.kt[i];
+      t_idx[i] = tpos + kofs;
+      for (int k = 0; k < kt; k++) P.factor_of_trow.push_back(2 * i);
+      int done = 0;
+      while (done < kt) {
+        if (gsize.back() == kMaxTileRows || (done == 0 && gsize.back() + kt > kMaxTileRows && gsize.back() > 0 &&
+                                              kt <= kMaxTileRows)) {
+          groups.emplace_back();
+          gsize.push_back(0);
+        }
+        const int take = std::min(kt - done, kMaxTileRows - gsize.back());
+        groups.back().push_back({i, done, take});
+        gsize.back() += take;
+        done += take;
+      }
+      kofs += kt;
     }
-    // tiles: balanced <=64-row groups x column chunks of ~w1/nrows columns
-    for (auto rc : row_chunks(K)) {
-      const int k0 = rc.first, nr = rc.second;
-      const int g_lanes = 64 / nr;
-      int C = std::max(1, opt.w1 / nr);
-      C = (C + g_lanes - 1) / g_lanes * g_lanes;
+    int k0 = 0;
+    for (size_t gi = 0; gi < groups.size(); gi++) {
+      const int nr = gsize[gi];
+      if (nr == 0) continue;
+      const int C = chunk_cols(nr, opt.w1);
+      const int nch = (ndt + C - 1) / C;
+      const int64_t slot0 = P.n_tp;
       const int seg = (int)P.tsum.ptr.size();
       P.tsum.ptr.push_back((int)P.tsum.list.size());
-      for (int c0 = 0; c0 < ndt; c0 += C) {
-        const int nc = std::min(C, ndt - c0);
+      for (int c = 0; c < nch; c++) {
+        const int c0 = c * C, nc = std::min(C, ndt - c0);
         PanelDesc pd;
         pd.val_off = base + (int64_t)c0 * K + k0;
         pd.ld = K;
         pd.ncols = nc;
         pd.vec_off = nstrtt - 1 + c0;
-        pd.vec_sel = 0;
-        pd.factor = -1;  // phase-1 scales are applied per t row
-        pd.pad = 0;
-        TileDesc td;
-        td.nrows = nr;
-        td.panel0 = (int)P.panels.size();
-        td.npanel = 1;
-        td.out_off = (int)P.n_tp;
+        pd.vec_sel = kVecX;
+        pd.factor = -1;  // stage-1 scales are applied with t
+        pd.vec_chunks = 0;
+        TileDesc tdsc;
+        tdsc.nrows = nr;
+        tdsc.panel0 = (int)P.panels.size();
+        tdsc.npanel = 1;
+        tdsc.out_off = (int)P.n_tp;
         P.panels.push_back(pd);
-        P.tiles1.push_back(td);
+        t1.push_back(tdsc);
         P.tsum.list.push_back(P.n_tp - (tpos + k0));
         P.n_tp += nr;
-        P.elems1 += (int64_t)nr * nc;
-        P.xreads1 += nc;
+        P.elems_stage1 += (int64_t)nr * nc;
+        P.vec_a += nc;
       }
       for (int k = 0; k < nr; k++) P.tsum.seg.push_back(seg);
+      for (const Piece &pc : groups[gi]) {
+        if (pc.k0 != 0 || pc.n != la.kt[pc.leaf]) {
+          P.t_fused = false;  // rank rows of this leaf straddle two row groups
+        } else {
+          int kk0 = -1;
+          for (size_t q = 0; q < ls.size(); q++)
+            if (ls[q] == pc.leaf) kk0 = kofs_of[q] - k0;
+          tp_off[pc.leaf] = slot0 + kk0;
+          tp_chunks[pc.leaf] = (nch << 16) | nr;
+          if (nch >= (1 << 15) || nr >= (1 << 16)) P.t_fused = false;
+        }
+      }
+      k0 += nr;
     }
     tpos += K;
   }
   P.tsum.ptr.push_back((int)P.tsum.list.size());
   P.n_t = tpos;
   P.tsum.n = (int)tpos;
-  if (P.n_tp > INT32_MAX || P.n_t > INT32_MAX) return "phase-1 slots exceed int32";
+  if (P.n_tp > INT32_MAX || P.n_t > INT32_MAX) return "stage-1 slots exceed int32";
 
-  // ---- phase 2 layout + tiles -----------------------------------------------------
+  // ---- dense / stage 2 layout + tiles ------------------------------------------------
   struct Slot { int64_t row0; int nr; int64_t z; };
   std::vector<Slot> slots;
   for (auto &g : rowg) {
     const int nstrtl = g.first.first, ndl = g.first.second;
     const std::vector<int> &ls = g.second;
+    std::vector<int> dense, lowrank;
     for (int i : ls) {
       const int64_t base = align_up(off, opt.align);
       dbase[i] = base;
@@ -164,6 +221,7 @@ std::string build_plan(const LeafArrays &la, const PlanOptions &opt, Plan *plan)
         r.src_rs = la.ndt[i];
         r.src_cs = 1;
         off = base + (int64_t)ndl * la.ndt[i];
+        dense.push_back(i);
       } else {
         r.src_off = la.src_a2[i];
         r.cols = la.kt[i];  // k
@@ -175,48 +233,87 @@ std::string build_plan(const LeafArrays &la, const PlanOptions &opt, Plan *plan)
           r.src_cs = 1;
         }
         off = base + (int64_t)ndl * la.kt[i];
+        lowrank.push_back(i);
       }
       P.repack.push_back(r);
     }
     for (auto rc : row_chunks(ndl)) {
       const int r0 = rc.first, nr = rc.second;
-      size_t q = 0;
-      while (q < ls.size()) {
-        TileDesc td;
-        td.nrows = nr;
-        td.panel0 = (int)P.panels.size();
-        td.npanel = 0;
-        td.out_off = (int)P.n_z;
-        int64_t work = 0;
-        while (q < ls.size()) {
-          const int i = ls[q];
-          const bool dense = la.ltmtx[i] == 2;
-          const int nc = dense ? la.ndt[i] : la.kt[i];
-          if (td.npanel > 0 && work + (int64_t)nr * nc > opt.w2) break;
-          PanelDesc pd;
-          pd.val_off = dbase[i] + r0;
-          pd.ld = ndl;
-          pd.ncols = nc;
-          pd.vec_off = dense ? la.nstrtt[i] - 1 : (int32_t)t_off[i];
-          pd.vec_sel = dense ? 0 : 1;
-          pd.factor = 2 * i + 1;
-          pd.pad = 0;
-          P.panels.push_back(pd);
-          td.npanel++;
-          work += (int64_t)nr * nc;
-          P.elems2 += (int64_t)nr * nc;
-          P.xreads2 += nc;
-          q++;
+      for (int kind = 0; kind < 2; kind++) {
+        const std::vector<int> &lst = kind == 0 ? dense : lowrank;
+        size_t q = 0;
+        while (q < lst.size()) {
+          TileDesc tdsc;
+          tdsc.nrows = nr;
+          tdsc.panel0 = (int)P.panels.size();
+          tdsc.npanel = 0;
+          tdsc.out_off = (int)(P.n_tp + P.n_z);  // output slots follow the stage-1 slots
+          int64_t work = 0;
+          while (q < lst.size()) {
+            const int i = lst[q];
+            const int nc = kind == 0 ? la.ndt[i] : la.kt[i];
+            if (tdsc.npanel > 0 && work + (int64_t)nr * nc > opt.w2) break;
+            PanelDesc pd;
+            pd.val_off = dbase[i] + r0;
+            pd.ld = ndl;
+            pd.ncols = nc;
+            pd.factor = 2 * i + 1;
+            if (kind == 0) {
+              pd.vec_off = la.nstrtt[i] - 1;
+              pd.vec_sel = kVecX;
+              pd.vec_chunks = 0;
+            } else {
+              pd.vec_off = (int32_t)tp_off[i];
+              pd.vec_sel = kVecTP;
+              pd.vec_chunks = tp_chunks[i];
+            }
+            P.panels.push_back(pd);
+            tdsc.npanel++;
+            work += (int64_t)nr * nc;
+            if (kind == 0) {
+              P.elems_dense += (int64_t)nr * nc;
+              P.vec_a += nc;
+            } else {
+              P.elems_stage2 += (int64_t)nr * nc;
+              P.vec_b += nc;
+            }
+            q++;
+          }
+          (kind == 0 ? td : t2).push_back(tdsc);
+          slots.push_back({(int64_t)nstrtl - 1 + r0, nr, P.n_tp + P.n_z});
+          P.n_z += nr;
         }
-        P.tiles2.push_back(td);
-        slots.push_back({(int64_t)nstrtl - 1 + r0, nr, P.n_z});
-        P.n_z += nr;
       }
     }
   }
-  if (P.n_z > INT32_MAX) return "phase-2 slots exceed int32";
+  if (P.n_tp + P.n_z > INT32_MAX) return "output slots exceed int32";
   if (P.panels.size() > (size_t)INT32_MAX) return "too many panels";
   P.nvals = align_up(off, opt.align);
+  // a leaf whose t could not be summed on the fly reads the materialised t
+  if (!P.t_fused) {
+    for (PanelDesc &pd : P.panels)
+      if (pd.vec_sel == kVecTP) {
+        const int i = pd.factor / 2;
+        pd.vec_sel = kVecT;
+        pd.vec_off = (int32_t)t_idx[i];
+        pd.vec_chunks = 0;
+      }
+  }
+  P.elems_a = P.elems_stage1 + P.elems_dense;
+  P.elems_b = P.elems_stage2;
+
+  // ---- launch lists -----------------------------------------------------------------
+  P.tiles_a = t1;
+  P.tiles_a.insert(P.tiles_a.end(), td.begin(), td.end());
+  P.tiles_b = t2;
+  if (opt.order == 1) {
+    auto by_work = [&](const TileDesc &a, const TileDesc &b) {
+      const int64_t wa = tile_work(P, a), wb = tile_work(P, b);
+      return wa != wb ? wa > wb : a.panel0 < b.panel0;
+    };
+    std::stable_sort(P.tiles_a.begin(), P.tiles_a.end(), by_work);
+    std::stable_sort(P.tiles_b.begin(), P.tiles_b.end(), by_work);
+  }
 
   // ---- y segment sum: strips of rows covered by the same set of slots ----------
   std::vector<int64_t> cuts;
@@ -248,59 +345,75 @@ std::string build_plan(const LeafArrays &la, const PlanOptions &opt, Plan *plan)
 }
 
 std::string verify_plan(const Plan &P, const LeafArrays &la) {
-  // 1. every source element copied once, destinations disjoint and in range
+  // 1. every source element copied once, destinations and sources in range
   int64_t copied = 0;
-  std::vector<std::pair<int64_t, int64_t>> dst;
   for (const RepackDesc &r : P.repack) {
     copied += (int64_t)r.rows * r.cols;
     if (r.rows == 0 || r.cols == 0) continue;
+    if (r.rows < 0 || r.cols < 0) return "negative repack shape";
     const int64_t last = r.dst_off + (int64_t)(r.rows - 1) + (int64_t)(r.cols - 1) * r.dst_cs;
     if (r.dst_off < 0 || last >= P.nvals) return "repack destination out of range";
     const int64_t slast = r.src_off + (int64_t)(r.rows - 1) * r.src_rs + (int64_t)(r.cols - 1) * r.src_cs;
     if (r.src_off < 0 || slast >= la.src_len) return "repack source out of range";
+    if (r.factor < 0 || r.factor >= P.nfactors) return "repack factor id out of range";
   }
   if (copied != P.len) return "repack does not copy len elements";
-  // 2. phase-2 coverage: count, per leaf, rows x columns streamed
-  std::vector<int64_t> seen(P.nlf, 0);
-  std::map<int64_t, int> factor_leaf;
-  for (int i = 0; i < P.nlf; i++) factor_leaf[2 * i + 1] = i;
+
+  // 2. tiles: shapes, panels, value and vector ranges, output slots
   const int64_t npan = (int64_t)P.panels.size();
-  for (const TileDesc &t : P.tiles2) {
-    if (t.nrows < 1 || t.nrows > 64) return "phase-2 tile rows out of range";
-    if (t.panel0 < 0 || t.npanel < 1 || (int64_t)t.panel0 + t.npanel > npan) return "phase-2 panel range";
-    if (t.out_off < 0 || t.out_off + t.nrows > P.n_z) return "phase-2 slot out of range";
-    for (int p = t.panel0; p < t.panel0 + t.npanel; p++) {
-      const PanelDesc &d = P.panels[p];
-      if (d.factor < 0 || d.factor >= P.nfactors || d.factor % 2 != 1) return "phase-2 factor id out of range";
-      const int i = d.factor / 2;
-      seen[i] += (int64_t)t.nrows * d.ncols;
-      const int64_t last = d.val_off + (int64_t)(t.nrows - 1) + (int64_t)(d.ncols - 1) * d.ld;
-      if (d.val_off < 0 || last >= P.nvals) return "phase-2 panel outside values";
-      const int64_t vlen = d.vec_sel ? P.n_t : P.nd;
-      if (d.vec_off < 0 || d.vec_off + d.ncols > vlen) return "phase-2 vector range out of bounds";
+  std::vector<int64_t> seen2(P.nlf, 0);
+  int64_t seen1 = 0;
+  for (int list = 0; list < 2; list++) {
+    const std::vector<TileDesc> &tl = list == 0 ? P.tiles_a : P.tiles_b;
+    for (const TileDesc &t : tl) {
+      if (t.nrows < 1 || t.nrows > kMaxTileRows) return "tile rows out of range";
+      if (t.panel0 < 0 || t.npanel < 1 || (int64_t)t.panel0 + t.npanel > npan) return "tile panel range";
+      if (t.out_off < 0) return "negative output slot";
+      for (int p = t.panel0; p < t.panel0 + t.npanel; p++) {
+        const PanelDesc &d = P.panels[p];
+        if (d.ncols < 1 || d.ld < t.nrows) return "panel shape";
+        const int64_t last = d.val_off + (int64_t)(t.nrows - 1) + (int64_t)(d.ncols - 1) * d.ld;
+        if (d.val_off < 0 || last >= P.nvals) return "panel outside values";
+        switch (d.vec_sel) {
+          case kVecX:
+            if (d.vec_off < 0 || (int64_t)d.vec_off + d.ncols > P.nd) return "x range out of bounds";
+            break;
+          case kVecT:
+            if (d.vec_off < 0 || (int64_t)d.vec_off + d.ncols > P.n_t) return "t range out of bounds";
+            break;
+          case kVecTP: {
+            const int nch = d.vec_chunks >> 16, stride = d.vec_chunks & 0xFFFF;
+            if (nch < 1 || stride < d.ncols) return "partial-slot chunking";
+            if (d.vec_off < 0 || (int64_t)d.vec_off + (int64_t)(nch - 1) * stride + d.ncols > P.n_tp)
+              return "partial slots out of bounds";
+            break;
+          }
+          default:
+            return "unknown vector selector";
+        }
+        if (d.vec_sel == kVecX && d.factor < 0) {  // stage-1 panel
+          if (list != 0 || t.npanel != 1) return "stage-1 tile malformed";
+          if ((int64_t)t.out_off + t.nrows > P.n_tp) return "stage-1 slot out of range";
+          seen1 += (int64_t)t.nrows * d.ncols;
+        } else {
+          if (d.factor < 0 || d.factor >= P.nfactors || d.factor % 2 != 1) return "factor id out of range";
+          if (t.out_off < P.n_tp || (int64_t)t.out_off + t.nrows > P.n_tp + P.n_z) return "output slot out of range";
+          if ((d.vec_sel == kVecX) != (la.ltmtx[d.factor / 2] == 2)) return "panel kind disagrees with leaf";
+          if ((list == 1) != (d.vec_sel != kVecX)) return "panel in the wrong launch";
+          seen2[d.factor / 2] += (int64_t)t.nrows * d.ncols;
+        }
+      }
     }
   }
+  int64_t want1 = 0;
   for (int i = 0; i < P.nlf; i++) {
     const int64_t want = la.ltmtx[i] == 2 ? (int64_t)la.ndl[i] * la.ndt[i] : (int64_t)la.ndl[i] * la.kt[i];
-    if (seen[i] != want) return fmt("leaf %lld: phase-2 coverage %lld wrong", i, seen[i]);
-  }
-  // 3. phase-1 coverage
-  int64_t e1 = 0;
-  for (const TileDesc &t : P.tiles1) {
-    if (t.nrows < 1 || t.nrows > 64 || t.npanel != 1) return "phase-1 tile malformed";
-    if (t.panel0 < 0 || t.panel0 >= npan) return "phase-1 panel range";
-    const PanelDesc &d = P.panels[t.panel0];
-    const int64_t last = d.val_off + (int64_t)(t.nrows - 1) + (int64_t)(d.ncols - 1) * d.ld;
-    if (d.val_off < 0 || last >= P.nvals) return "phase-1 panel outside values";
-    if (d.vec_off < 0 || d.vec_off + d.ncols > P.nd) return "phase-1 x range out of bounds";
-    if (t.out_off < 0 || t.out_off + t.nrows > P.n_tp) return "phase-1 slot out of range";
-    e1 += (int64_t)t.nrows * d.ncols;
-  }
-  int64_t want1 = 0;
-  for (int i = 0; i < P.nlf; i++)
+    if (seen2[i] != want) return fmt("leaf %lld: coverage %lld wrong", i, seen2[i]);
     if (la.ltmtx[i] == 1) want1 += (int64_t)la.kt[i] * la.ndt[i];
-  if (e1 != want1) return "phase-1 coverage wrong";
-  // 4. segment sums index inside their buffers
+  }
+  if (seen1 != want1) return "stage-1 coverage wrong";
+
+  // 3. segment sums index inside their buffers
   if ((int64_t)P.tsum.seg.size() != P.tsum.n || (int64_t)P.ysum.seg.size() != P.ysum.n || P.ysum.n != P.nd ||
       P.tsum.n != P.n_t)
     return "segment-sum sizes";
@@ -321,7 +434,7 @@ std::string verify_plan(const Plan &P, const LeafArrays &la) {
     if (s < 0 || s >= nyseg) return "y segment id out of range";
     for (int e = P.ysum.ptr[s]; e < P.ysum.ptr[s + 1]; e++) {
       const int64_t a = P.ysum.list[e] + i;
-      if (a < 0 || a >= P.n_z) return "y segment sum reads outside the slots";
+      if (a < P.n_tp || a >= P.n_tp + P.n_z) return "y segment sum reads outside the slots";
     }
   }
   return "";

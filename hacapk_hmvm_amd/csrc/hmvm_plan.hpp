@@ -1,25 +1,31 @@
 // hmvm_plan.hpp -- host-side schedule of the HIP HMVM (no HIP dependency).
 //
-// The HMVM y += A x over the leaves of an H-matrix is executed in two phases
-// of one primitive, "column-major panel times vector":
+// The HMVM y += A x over the leaves of an H-matrix is executed with one
+// primitive, "column-major panel times vector":
 //
-//     out[r] = sum_j P[j*ld + r] * v[j]          r < nrows <= 64
+//     out[r] = sum_j P[j*ld + r] * v[j]          r < nrows <= 256
 //
-// phase 1: t_b = A1_b x_sigma for every low-rank leaf b.  The a1 factors of all
-//          low-rank leaves that share a column cluster sigma (same nstrtt, ndt)
-//          are stacked into one ndt x K_sigma panel (element (it, k) at
-//          it*K_sigma + k), so one tile reads the x slice once for up to 64
-//          rank rows of several leaves.
-// phase 2: for every row cluster tau (same nstrtl, ndl) the dense leaves
-//          (stored transposed, element (it, il) at it*ndl + il) and the a2
-//          factors (element (k, il) at k*ndl + il, the reference order) of
-//          all leaves in tau, restricted to a <=64-row chunk of tau, are
-//          summed by one tile into its own slot.
-// Each tile is one wavefront; lanes own rows, so no lane-crossing reduction
-// happens inside the streaming loop.  Partial results go to private slots and
-// two "segment sums" (t = sum of phase-1 slots, y += sum of phase-2 slots)
-// combine them in a fixed order: y is accumulated without atomics and the
-// result is bit-reproducible run to run.
+// low-rank stage 1: t_b = A1_b x_sigma for every low-rank leaf b.  The a1
+//          factors of all low-rank leaves that share a column cluster sigma
+//          (same nstrtt, ndt) are stacked into one ndt x K_sigma panel (element
+//          (it, k) at it*K_sigma + k), so a tile reads whole columns of the
+//          stack and the x slice once for up to 256 rank rows of several leaves.
+// dense:   the dense leaves of a row cluster tau (same nstrtl, ndl), stored
+//          transposed (element (it, il) at it*ndl + il), times x.
+// low-rank stage 2: the a2 factors (element (k, il) at k*ndl + il, the
+//          reference order) of all low-rank leaves of tau times their t_b.
+// A tile is one wavefront: with nrows <= 32 the wave splits the columns over
+// 64/nrows lane groups; with nrows > 32 each lane owns up to four rows
+// (lane, lane+64, ...) and reads every column whole, with the vector entry a
+// wave-uniform scalar load.  No lane-crossing reduction happens inside the
+// streaming loop.
+//
+// Launch A runs the stage-1 tiles and the dense tiles (independent of each
+// other); launch B runs the stage-2 tiles, which sum the stage-1 partial slots
+// of their leaf on the fly (or read a materialised t when a leaf's rank rows
+// straddle two stage-1 row groups); a final segment sum adds the private
+// output slots of all tiles into y in a fixed order: y is accumulated without
+// atomics and the result is bit-reproducible run to run.
 #pragma once
 
 #include <cstdint>
@@ -33,18 +39,27 @@ enum Storage : int { kF64 = 0, kF32 = 1, kBF16 = 2 };
 // Layout of the a2 factor in the caller's buffer.
 enum A2Source : int { kA2RowMajor = 0 /* a2[k*ndl+il] */, kA2Transposed = 1 /* a2t[il*kt+k] */ };
 
+// Input vector of a panel.
+enum VecSel : int {
+  kVecX = 0,   // x[vec_off + j]
+  kVecT = 1,   // t[vec_off + j]  (materialised by the t segment sum)
+  kVecTP = 2   // sum_{c < nch} tp[vec_off + c*stride + j]  (stage-1 partial slots)
+};
+
+constexpr int kMaxTileRows = 256;
+
 struct PanelDesc {        // 32 bytes, read with scalar loads
   int64_t val_off;        // element offset of (row 0 of the tile, column 0)
   int32_t ld;             // element stride between columns
   int32_t ncols;          // summation length
-  int32_t vec_off;        // index of column 0 in the input vector
-  int32_t vec_sel;        // 0: x, 1: t
-  int32_t factor;         // factor id: index of the BF16 scale of this factor
-  int32_t pad;
+  int32_t vec_off;        // see VecSel
+  int32_t vec_sel;        // VecSel
+  int32_t factor;         // BF16 scale slot of this factor (-1: none)
+  int32_t vec_chunks;     // kVecTP: (nch << 16) | stride
 };
 
 struct TileDesc {         // 16 bytes
-  int32_t nrows;          // 1..64
+  int32_t nrows;          // 1..kMaxTileRows
   int32_t panel0;
   int32_t npanel;
   int32_t out_off;        // first element of the tile's output slot
@@ -69,9 +84,10 @@ struct SegSum {
 
 struct PlanOptions {
   int storage = kF64;
-  int w1 = 16384;         // target elements per phase-1 tile
-  int w2 = 16384;         // target elements per phase-2 tile
+  int w1 = 32768;         // target elements per stage-1 tile
+  int w2 = 16384;         // target elements per dense / stage-2 tile
   int align = 16;         // factor start alignment in elements
+  int order = 0;          // 0: memory order, 1: largest tiles first
 };
 
 struct LeafArrays {       // the caller's descriptors (matrix2 or matrix)
@@ -89,15 +105,19 @@ struct Plan {
   int64_t nvals = 0;                 // device value buffer (elements)
   int64_t len = 0;                   // reference element count
   int nfactors = 0;
+  bool t_fused = true;               // stage 2 sums the partial slots itself
   std::vector<RepackDesc> repack;
   std::vector<int32_t> factor_of_trow;  // t index -> a1 factor id (BF16 t scaling)
-  std::vector<TileDesc> tiles1, tiles2;
+  std::vector<TileDesc> tiles_a, tiles_b;
   std::vector<PanelDesc> panels;
+  // slot buffer: stage-1 partial slots [0, n_tp), then the dense and stage-2
+  // output slots [n_tp, n_tp + n_z)
   int64_t n_t = 0, n_tp = 0, n_z = 0;
   SegSum tsum, ysum;
   // accounting
-  int64_t elems1 = 0, elems2 = 0;    // value elements streamed per phase
-  int64_t xreads1 = 0, xreads2 = 0;  // vector elements read per phase
+  int64_t elems_a = 0, elems_b = 0;     // value elements streamed per launch
+  int64_t elems_stage1 = 0, elems_dense = 0, elems_stage2 = 0;
+  int64_t vec_a = 0, vec_b = 0;         // vector elements read per launch
   int nlr = 0, ndense = 0, ngroups_col = 0, ngroups_row = 0;
 };
 
@@ -105,8 +125,9 @@ struct Plan {
 std::string build_plan(const LeafArrays &la, const PlanOptions &opt, Plan *plan);
 
 // CPU check of the plan's bookkeeping: every source element is copied exactly
-// once, every row of every leaf is covered by exactly one phase-2 panel row,
-// and every t row by the phase-1 panels.  Returns "" when consistent.
+// once, every element of every leaf is streamed by exactly one tile, and every
+// index a kernel derives from the descriptors stays inside its buffer.
+// Returns "" when consistent.
 std::string verify_plan(const Plan &plan, const LeafArrays &la);
 
 }  // namespace hmvm

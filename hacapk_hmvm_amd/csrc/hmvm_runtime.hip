@@ -86,12 +86,13 @@ struct hmvm_plan {
   Plan plan;
   int storage = kF64;
   int block_threads = 256;
+  int kernel = 0;  // tile-kernel variant, 0 .. kNumKernelVariants-1
   int device = 0;
   hipStream_t stream = nullptr;
   void *d_vals = nullptr;
-  TileDesc *d_tiles1 = nullptr, *d_tiles2 = nullptr;
+  TileDesc *d_tiles_a = nullptr, *d_tiles_b = nullptr;
   PanelDesc *d_panels = nullptr;
-  double *d_t = nullptr, *d_tp = nullptr, *d_z = nullptr;
+  double *d_t = nullptr, *d_slots = nullptr;  // stage-1 partial slots, then output slots
   int32_t *d_tseg = nullptr, *d_tptr = nullptr, *d_tfac = nullptr;
   int64_t *d_tlist = nullptr;
   int32_t *d_yseg = nullptr, *d_yptr = nullptr;
@@ -101,8 +102,8 @@ struct hmvm_plan {
   size_t device_bytes = 0;
 
   ~hmvm_plan() {
-    void *ptrs[] = {d_vals, d_tiles1, d_tiles2, d_panels, d_t,    d_tp,     d_z,   d_tseg, d_tptr,
-                    d_tfac, d_tlist,  d_yseg,   d_yptr,   d_ylist, d_scales, d_x, d_y};
+    void *ptrs[] = {d_vals, d_tiles_a, d_tiles_b, d_panels, d_t,      d_slots, d_tseg, d_tptr,
+                    d_tfac, d_tlist,   d_yseg,    d_yptr,   d_ylist, d_scales, d_x,     d_y};
     for (void *p : ptrs)
       if (p) (void)hipFree(p);
     if (stream) (void)hipStreamDestroy(stream);
@@ -213,8 +214,28 @@ PlanOptions plan_options(const hmvm_options *o) {
     po.storage = o->storage;
     if (o->w1 > 0) po.w1 = o->w1;
     if (o->w2 > 0) po.w2 = o->w2;
+    po.order = o->order;
   }
   return po;
+}
+
+static_assert(sizeof(hmvm_tile) == sizeof(TileDesc), "tile record layout");
+static_assert(sizeof(hmvm_panel) == sizeof(PanelDesc), "panel record layout");
+static_assert(sizeof(hmvm_repack) == sizeof(RepackDesc), "repack record layout");
+
+template <class T>
+T *copy_out(const std::vector<T> &v) {
+  T *p = (T *)std::malloc(sizeof(T) * std::max<size_t>(v.size(), 1));
+  if (p && !v.empty()) std::memcpy(p, v.data(), sizeof(T) * v.size());
+  return p;
+}
+
+void export_segsum(const SegSum &s, hmvm_segsum *o) {
+  o->n = s.n;
+  o->nseg = (int)s.ptr.size() - 1;
+  o->seg = copy_out(s.seg);
+  o->ptr = copy_out(s.ptr);
+  o->list = (long long *)copy_out(s.list);
 }
 
 template <typename S>
@@ -246,9 +267,14 @@ int create_plan(Source &src, const hmvm_options *opt, hmvm_plan **out) {
   std::unique_ptr<hmvm_plan> p(new hmvm_plan());
   p->storage = po.storage;
   if (opt && opt->block_threads > 0) {
-    if (opt->block_threads % 64 || opt->block_threads > 1024 || opt->block_threads < 64)
-      return set_error(HMVM_ERR_ARG, "block_threads must be a multiple of 64 in [64,1024]");
+    if (opt->block_threads % 64 || opt->block_threads > 256 || opt->block_threads < 64)
+      return set_error(HMVM_ERR_ARG, "block_threads must be 64, 128, 192 or 256");
     p->block_threads = opt->block_threads;
+  }
+  if (opt) {
+    if (opt->kernel < 0 || opt->kernel >= kNumKernelVariants)
+      return set_error(HMVM_ERR_ARG, "kernel variant must be in [0,%d)", kNumKernelVariants);
+    p->kernel = opt->kernel;
   }
   std::string err = build_plan(src.la, po, &p->plan);
   if (!err.empty()) return set_error(HMVM_ERR_MATRIX, "%s", err.c_str());
@@ -261,7 +287,7 @@ int create_plan(Source &src, const hmvm_options *opt, hmvm_plan **out) {
   int rc;
   const size_t eb = (size_t)elem_bytes(p->storage);
   // schedule
-  if ((rc = upload(&p->d_tiles1, P.tiles1)) || (rc = upload(&p->d_tiles2, P.tiles2)) ||
+  if ((rc = upload(&p->d_tiles_a, P.tiles_a)) || (rc = upload(&p->d_tiles_b, P.tiles_b)) ||
       (rc = upload(&p->d_panels, P.panels)) || (rc = upload(&p->d_tseg, P.tsum.seg)) ||
       (rc = upload(&p->d_tptr, P.tsum.ptr)) || (rc = upload(&p->d_tlist, P.tsum.list)) ||
       (rc = upload(&p->d_yseg, P.ysum.seg)) || (rc = upload(&p->d_yptr, P.ysum.ptr)) ||
@@ -273,8 +299,8 @@ int create_plan(Source &src, const hmvm_options *opt, hmvm_plan **out) {
     std::vector<float> ones((size_t)P.nfactors, 1.0f);
     if ((rc = h2d(p->d_scales, ones.data(), ones.size()))) return rc;
   }
-  if ((rc = dalloc(&p->d_t, (size_t)P.n_t)) || (rc = dalloc(&p->d_tp, (size_t)P.n_tp)) ||
-      (rc = dalloc(&p->d_z, (size_t)P.n_z)) || (rc = dalloc(&p->d_x, (size_t)P.nd)) ||
+  if ((rc = dalloc(&p->d_t, (size_t)P.n_t)) || (rc = dalloc(&p->d_slots, (size_t)(P.n_tp + P.n_z))) ||
+      (rc = dalloc(&p->d_x, (size_t)P.nd)) ||
       (rc = dalloc(&p->d_y, (size_t)P.nd)))
     return rc;
   HIP_TRY(hipMalloc(&p->d_vals, eb * (size_t)std::max<int64_t>(P.nvals, 1)));
@@ -323,7 +349,7 @@ int create_plan(Source &src, const hmvm_options *opt, hmvm_plan **out) {
   cleanup();
   if (rc) return rc;
 
-  p->device_bytes = eb * (size_t)P.nvals + sizeof(TileDesc) * (P.tiles1.size() + P.tiles2.size()) +
+  p->device_bytes = eb * (size_t)P.nvals + sizeof(TileDesc) * (P.tiles_a.size() + P.tiles_b.size()) +
                     sizeof(PanelDesc) * P.panels.size() + 8 * (size_t)(P.n_t + P.n_tp + P.n_z + 2 * P.nd) +
                     4 * (P.tsum.seg.size() + P.tsum.ptr.size() + P.ysum.seg.size() + P.ysum.ptr.size()) +
                     8 * (P.tsum.list.size() + P.ysum.list.size());
@@ -331,62 +357,90 @@ int create_plan(Source &src, const hmvm_options *opt, hmvm_plan **out) {
     std::fprintf(stderr,
                  "hmvm_hip: plan nd=%d nlf=%d len=%lld storage=%d tiles=%zu+%zu panels=%zu t=%lld tslots=%lld "
                  "zslots=%lld\n",
-                 P.nd, P.nlf, (long long)P.len, p->storage, P.tiles1.size(), P.tiles2.size(), P.panels.size(),
+                 P.nd, P.nlf, (long long)P.len, p->storage, P.tiles_a.size(), P.tiles_b.size(), P.panels.size(),
                  (long long)P.n_t, (long long)P.n_tp, (long long)P.n_z);
   *out = p.release();
   return HMVM_OK;
 }
 
+template <typename S, bool kScaled, int kVar>
+void launch_tiles_v(hmvm_plan *p, dim3 grid, dim3 block, hipStream_t st, const TileDesc *d_tiles, int ntiles,
+                    const double *x, const double *t, const double *tp, double *out) {
+  constexpr int L = (kVar & 1) ? 16 : 8;
+  constexpr bool kPipe = (kVar & 2) != 0, kNT = (kVar & 4) != 0;
+  hipLaunchKernelGGL((tile_kernel<S, kScaled, L, kPipe, kNT>), grid, block, 0, st, d_tiles, ntiles, p->d_panels,
+                     (const S *)p->d_vals, x, t, tp, (const float *)p->d_scales, out);
+}
+
+template <typename S, bool kScaled>
+void launch_tiles_s(hmvm_plan *p, dim3 grid, dim3 block, hipStream_t st, const TileDesc *d_tiles, int ntiles,
+                    const double *x, const double *t, const double *tp, double *out) {
+  switch (p->kernel) {
+    case 0: launch_tiles_v<S, kScaled, 0>(p, grid, block, st, d_tiles, ntiles, x, t, tp, out); break;
+    case 1: launch_tiles_v<S, kScaled, 1>(p, grid, block, st, d_tiles, ntiles, x, t, tp, out); break;
+    case 2: launch_tiles_v<S, kScaled, 2>(p, grid, block, st, d_tiles, ntiles, x, t, tp, out); break;
+    case 3: launch_tiles_v<S, kScaled, 3>(p, grid, block, st, d_tiles, ntiles, x, t, tp, out); break;
+    case 4: launch_tiles_v<S, kScaled, 4>(p, grid, block, st, d_tiles, ntiles, x, t, tp, out); break;
+    case 5: launch_tiles_v<S, kScaled, 5>(p, grid, block, st, d_tiles, ntiles, x, t, tp, out); break;
+    case 6: launch_tiles_v<S, kScaled, 6>(p, grid, block, st, d_tiles, ntiles, x, t, tp, out); break;
+    default: launch_tiles_v<S, kScaled, 7>(p, grid, block, st, d_tiles, ntiles, x, t, tp, out); break;
+  }
+}
+
 template <typename S>
-int launch_tiles(hmvm_plan *p, const TileDesc *d_tiles, int ntiles, const double *x, const double *t,
-                 double *out, bool scaled, hipStream_t st) {
-  if (ntiles == 0) return HMVM_OK;
+void launch_tiles(hmvm_plan *p, const TileDesc *d_tiles, int ntiles, const double *x, const double *t,
+                  const double *tp, double *out, bool scaled, hipStream_t st) {
+  if (ntiles == 0) return;
   const int wpb = p->block_threads / 64;
   const dim3 grid((ntiles + wpb - 1) / wpb), block(p->block_threads);
   if (scaled)
-    hipLaunchKernelGGL((tile_kernel<S, true, 8>), grid, block, 0, st, d_tiles, ntiles, p->d_panels,
-                       (const S *)p->d_vals, x, t, p->d_scales, out);
+    launch_tiles_s<S, true>(p, grid, block, st, d_tiles, ntiles, x, t, tp, out);
   else
-    hipLaunchKernelGGL((tile_kernel<S, false, 8>), grid, block, 0, st, d_tiles, ntiles, p->d_panels,
-                       (const S *)p->d_vals, x, t, (const float *)nullptr, out);
-  return HMVM_OK;
+    launch_tiles_s<S, false>(p, grid, block, st, d_tiles, ntiles, x, t, tp, out);
 }
 
-int launch_phase(hmvm_plan *p, int phase, const double *x, double *y, int accumulate, hipStream_t st) {
+template <typename S>
+void launch_tiles_both(hmvm_plan *p, int which, const double *x, hipStream_t st) {
   const Plan &P = p->plan;
   const bool bf = p->storage == kBF16;
-  switch (phase) {
-    case 0:  // phase-1 tiles -> t slots (scales applied per t row in the t sum)
-      if (p->storage == kF64) return launch_tiles<double>(p, p->d_tiles1, (int)P.tiles1.size(), x, nullptr, p->d_tp, false, st);
-      if (p->storage == kF32) return launch_tiles<float>(p, p->d_tiles1, (int)P.tiles1.size(), x, nullptr, p->d_tp, false, st);
-      return launch_tiles<uint16_t>(p, p->d_tiles1, (int)P.tiles1.size(), x, nullptr, p->d_tp, false, st);
-    case 1:
-      if (P.tsum.n > 0) {
-        if (bf)
-          hipLaunchKernelGGL(segsum_kernel<true>, dim3((P.tsum.n + 255) / 256), dim3(256), 0, st, P.tsum.n, p->d_tseg,
-                             p->d_tptr, p->d_tlist, p->d_tp, p->d_tfac, p->d_scales, p->d_t, 0);
-        else
-          hipLaunchKernelGGL(segsum_kernel<false>, dim3((P.tsum.n + 255) / 256), dim3(256), 0, st, P.tsum.n, p->d_tseg,
-                             p->d_tptr, p->d_tlist, p->d_tp, (const int32_t *)nullptr, (const float *)nullptr, p->d_t, 0);
-      }
-      return HMVM_OK;
-    case 2:
-      if (p->storage == kF64) return launch_tiles<double>(p, p->d_tiles2, (int)P.tiles2.size(), x, p->d_t, p->d_z, false, st);
-      if (p->storage == kF32) return launch_tiles<float>(p, p->d_tiles2, (int)P.tiles2.size(), x, p->d_t, p->d_z, false, st);
-      return launch_tiles<uint16_t>(p, p->d_tiles2, (int)P.tiles2.size(), x, p->d_t, p->d_z, true, st);
-    default:
-      hipLaunchKernelGGL(segsum_kernel<false>, dim3((P.ysum.n + 255) / 256), dim3(256), 0, st, P.ysum.n, p->d_yseg,
-                         p->d_yptr, p->d_ylist, p->d_z, (const int32_t *)nullptr, (const float *)nullptr, y,
-                         accumulate);
-      return HMVM_OK;
+  if (which == 0)
+    launch_tiles<S>(p, p->d_tiles_a, (int)P.tiles_a.size(), x, nullptr, nullptr, p->d_slots, bf, st);
+  else
+    launch_tiles<S>(p, p->d_tiles_b, (int)P.tiles_b.size(), x, p->d_t, p->d_slots, p->d_slots, bf, st);
+}
+
+// Launch `phase` of one HMVM: 0 = tiles A (stage 1 + dense), 1 = t segment sum
+// (only when some leaf's rank rows straddle two row groups), 2 = tiles B
+// (stage 2), 3 = y segment sum.
+void launch_phase(hmvm_plan *p, int phase, const double *x, double *y, int accumulate, hipStream_t st) {
+  const Plan &P = p->plan;
+  const bool bf = p->storage == kBF16;
+  if (phase == 0 || phase == 2) {
+    const int which = phase == 0 ? 0 : 1;
+    if (p->storage == kF64)
+      launch_tiles_both<double>(p, which, x, st);
+    else if (p->storage == kF32)
+      launch_tiles_both<float>(p, which, x, st);
+    else
+      launch_tiles_both<uint16_t>(p, which, x, st);
+  } else if (phase == 1) {
+    if (P.t_fused || P.tsum.n == 0) return;
+    if (bf)
+      hipLaunchKernelGGL(segsum_kernel<true>, dim3((P.tsum.n + 255) / 256), dim3(256), 0, st, P.tsum.n, p->d_tseg,
+                         p->d_tptr, p->d_tlist, p->d_slots, p->d_tfac, p->d_scales, p->d_t, 0);
+    else
+      hipLaunchKernelGGL(segsum_kernel<false>, dim3((P.tsum.n + 255) / 256), dim3(256), 0, st, P.tsum.n, p->d_tseg,
+                         p->d_tptr, p->d_tlist, p->d_slots, (const int32_t *)nullptr, (const float *)nullptr,
+                         p->d_t, 0);
+  } else {
+    hipLaunchKernelGGL(segsum_kernel<false>, dim3((P.ysum.n + 255) / 256), dim3(256), 0, st, P.ysum.n, p->d_yseg,
+                       p->d_yptr, p->d_ylist, p->d_slots, (const int32_t *)nullptr, (const float *)nullptr, y,
+                       accumulate);
   }
 }
 
 int apply(hmvm_plan *p, const double *d_x, double *d_y, int accumulate, hipStream_t st) {
-  for (int ph = 0; ph < 4; ph++) {
-    int rc = launch_phase(p, ph, d_x, d_y, accumulate, st);
-    if (rc) return rc;
-  }
+  for (int ph = 0; ph < 4; ph++) launch_phase(p, ph, d_x, d_y, accumulate, st);
   HIP_TRY(hipGetLastError());
   return HMVM_OK;
 }
@@ -401,18 +455,22 @@ void fill_info(const hmvm_plan *p, const Plan &P, int storage, hmvm_plan_info *i
   info->device_elems = P.nvals;
   info->storage = storage;
   info->elem_bytes = elem_bytes(storage);
-  info->ntiles1 = (int)P.tiles1.size();
-  info->ntiles2 = (int)P.tiles2.size();
+  info->ntiles_a = (int)P.tiles_a.size();
+  info->ntiles_b = (int)P.tiles_b.size();
   info->npanels = (int)P.panels.size();
   info->ncol_groups = P.ngroups_col;
   info->nrow_groups = P.ngroups_row;
+  info->t_fused = P.t_fused ? 1 : 0;
   info->n_t = P.n_t;
   info->n_tslots = P.n_tp;
   info->n_zslots = P.n_z;
-  info->elems_phase1 = P.elems1;
-  info->elems_phase2 = P.elems2;
-  info->vec_reads_phase1 = P.xreads1;
-  info->vec_reads_phase2 = P.xreads2;
+  info->elems_a = P.elems_a;
+  info->elems_b = P.elems_b;
+  info->elems_stage1 = P.elems_stage1;
+  info->elems_dense = P.elems_dense;
+  info->elems_stage2 = P.elems_stage2;
+  info->vec_reads_a = P.vec_a;
+  info->vec_reads_b = P.vec_b;
   info->algorithmic_bytes = (long long)info->elem_bytes * P.len + 16LL * P.nd;
   info->device_bytes = p ? (long long)p->device_bytes : 0;
 }
@@ -510,6 +568,7 @@ void hmvm_options_default(hmvm_options *opt) {
   opt->w1 = po.w1;
   opt->w2 = po.w2;
   opt->block_threads = 256;
+  opt->kernel = 0;
 }
 
 int hmvm_plan_create(const matrix2 *mat2, int a2_source, const hmvm_options *opt, hmvm_plan **out) {
@@ -536,6 +595,63 @@ int hmvm_plan_create_aos(const matrix *mat, int a2_source, const hmvm_options *o
 
 void hmvm_plan_destroy(hmvm_plan *plan) { delete plan; }
 
+int hmvm_plan_set_launch(hmvm_plan *plan, int kernel, int block_threads) {
+  if (!plan) return set_error(HMVM_ERR_ARG, "NULL plan");
+  if (kernel < 0 || kernel >= kNumKernelVariants)
+    return set_error(HMVM_ERR_ARG, "kernel variant must be in [0,%d)", kNumKernelVariants);
+  if (block_threads % 64 || block_threads < 64 || block_threads > 256)
+    return set_error(HMVM_ERR_ARG, "block_threads must be 64, 128, 192 or 256");
+  plan->kernel = kernel;
+  plan->block_threads = block_threads;
+  return HMVM_OK;
+}
+
+int hmvm_bandwidth_probe(size_t bytes, int iters, double *read_gbs, double *copy_gbs) {
+  clear_error();
+  if (bytes < (1u << 20) || iters < 1) return set_error(HMVM_ERR_ARG, "bytes >= 1 MiB and iters >= 1");
+  const size_t n4 = bytes / sizeof(double4);
+  double4 *a = nullptr, *b = nullptr;
+  double *sink = nullptr;
+  HIP_TRY(hipMalloc(&a, n4 * sizeof(double4)));
+  hipError_t e = hipMalloc(&b, n4 * sizeof(double4));
+  if (e == hipSuccess) e = hipMalloc(&sink, sizeof(double));
+  if (e == hipSuccess) e = hipMemset(a, 0, n4 * sizeof(double4));
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (e == hipSuccess) e = hipEventCreate(&e0);
+  if (e == hipSuccess) e = hipEventCreate(&e1);
+  int ncu = 256;
+  hipDeviceProp_t prop;
+  if (e == hipSuccess && hipGetDeviceProperties(&prop, 0) == hipSuccess) ncu = prop.multiProcessorCount;
+  const dim3 grid(ncu * 8), block(256);
+  float ms_r = 0, ms_c = 0;
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(probe_read_kernel, grid, block, 0, 0, a, n4, sink);  // warm-up
+    hipEventRecord(e0, 0);
+    for (int i = 0; i < iters; i++) hipLaunchKernelGGL(probe_read_kernel, grid, block, 0, 0, a, n4, sink);
+    hipEventRecord(e1, 0);
+    e = hipEventSynchronize(e1);
+    if (e == hipSuccess) hipEventElapsedTime(&ms_r, e0, e1);
+  }
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(probe_copy_kernel, grid, block, 0, 0, a, b, n4);
+    hipEventRecord(e0, 0);
+    for (int i = 0; i < iters; i++) hipLaunchKernelGGL(probe_copy_kernel, grid, block, 0, 0, a, b, n4);
+    hipEventRecord(e1, 0);
+    e = hipEventSynchronize(e1);
+    if (e == hipSuccess) hipEventElapsedTime(&ms_c, e0, e1);
+  }
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  (void)hipFree(a);
+  if (b) (void)hipFree(b);
+  if (sink) (void)hipFree(sink);
+  if (e != hipSuccess) return set_error(HMVM_ERR_HIP, "bandwidth probe: %s", hipGetErrorString(e));
+  const double nbytes = (double)n4 * sizeof(double4);
+  if (read_gbs) *read_gbs = nbytes * iters / (ms_r * 1e-3) / 1e9;
+  if (copy_gbs) *copy_gbs = 2.0 * nbytes * iters / (ms_c * 1e-3) / 1e9;
+  return HMVM_OK;
+}
+
 int hmvm_plan_get_info(const hmvm_plan *plan, hmvm_plan_info *info) {
   if (!plan || !info) return set_error(HMVM_ERR_ARG, "NULL argument");
   fill_info(plan, plan->plan, plan->storage, info);
@@ -555,6 +671,49 @@ int hmvm_plan_check(const matrix2 *mat2, const hmvm_options *opt, hmvm_plan_info
   if (!err.empty()) return set_error(HMVM_ERR_MATRIX, "schedule check failed: %s", err.c_str());
   if (info) fill_info(nullptr, P, po.storage, info);
   return HMVM_OK;
+}
+
+int hmvm_schedule_export(const matrix2 *mat2, int a2_source, const hmvm_options *opt, hmvm_schedule *out) {
+  clear_error();
+  if (!out) return set_error(HMVM_ERR_ARG, "out is NULL");
+  std::memset(out, 0, sizeof(*out));
+  if (a2_source != kA2RowMajor && a2_source != kA2Transposed) return set_error(HMVM_ERR_ARG, "bad a2_source");
+  Source s;
+  int rc = source_from_matrix2(mat2, a2_source, &s);
+  if (rc) return rc;
+  Plan P;
+  std::string err = build_plan(s.la, plan_options(opt), &P);
+  if (!err.empty()) return set_error(HMVM_ERR_MATRIX, "%s", err.c_str());
+  err = verify_plan(P, s.la);
+  if (!err.empty()) return set_error(HMVM_ERR_MATRIX, "schedule check failed: %s", err.c_str());
+  out->ntiles_a = (int)P.tiles_a.size();
+  out->ntiles_b = (int)P.tiles_b.size();
+  out->npanels = (int)P.panels.size();
+  out->nrepack = (int)P.repack.size();
+  out->t_fused = P.t_fused ? 1 : 0;
+  out->nvals = P.nvals;
+  out->n_t = P.n_t;
+  out->n_tslots = P.n_tp;
+  out->n_zslots = P.n_z;
+  out->tiles_a = (hmvm_tile *)copy_out(P.tiles_a);
+  out->tiles_b = (hmvm_tile *)copy_out(P.tiles_b);
+  out->panels = (hmvm_panel *)copy_out(P.panels);
+  out->repack = (hmvm_repack *)copy_out(P.repack);
+  export_segsum(P.tsum, &out->tsum);
+  export_segsum(P.ysum, &out->ysum);
+  if (!out->tiles_a || !out->tiles_b || !out->panels || !out->repack || !out->tsum.seg || !out->ysum.seg) {
+    hmvm_schedule_release(out);
+    return set_error(HMVM_ERR_NOMEM, "schedule export");
+  }
+  return HMVM_OK;
+}
+
+void hmvm_schedule_release(hmvm_schedule *s) {
+  if (!s) return;
+  void *ptrs[] = {s->tiles_a, s->tiles_b, s->panels, s->repack, s->tsum.seg, s->tsum.ptr, s->tsum.list,
+                  s->ysum.seg, s->ysum.ptr, s->ysum.list};
+  for (void *p : ptrs) std::free(p);
+  std::memset(s, 0, sizeof(*s));
 }
 
 int hmvm_plan_apply(hmvm_plan *plan, const double *d_x, double *d_y, int accumulate, void *stream) {
@@ -596,8 +755,8 @@ int hmvm_plan_time(hmvm_plan *plan, int warmup, int iters, hmvm_timing *tm) {
   int rc = HMVM_OK;
   for (int it = 0; it < iters && !rc; it++) {
     hipEventRecord(ev[0], st);
-    for (int phs = 0; phs < 4 && !rc; phs++) {
-      rc = launch_phase(plan, phs, plan->d_x, plan->d_y, 0, st);
+    for (int phs = 0; phs < 4; phs++) {
+      launch_phase(plan, phs, plan->d_x, plan->d_y, 0, st);
       hipEventRecord(ev[phs + 1], st);
     }
     if (hipEventSynchronize(ev[4]) != hipSuccess) rc = set_error(HMVM_ERR_HIP, "event sync failed");
@@ -630,9 +789,9 @@ int hmvm_plan_time(hmvm_plan *plan, int warmup, int iters, hmvm_timing *tm) {
   tm->ms_avg = sum / iters;
   tm->ms_min = s.front();
   tm->ms_median = s[s.size() / 2];
-  tm->ms_phase1 = ph[0] / iters;
+  tm->ms_a = ph[0] / iters;
   tm->ms_tsum = ph[1] / iters;
-  tm->ms_phase2 = ph[2] / iters;
+  tm->ms_b = ph[2] / iters;
   tm->ms_ysum = ph[3] / iters;
   return HMVM_OK;
 }

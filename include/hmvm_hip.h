@@ -91,8 +91,11 @@ typedef struct hmvm_options {
   int storage;           /* HMVM_STORAGE_*: element type kept in HBM            */
   int w1;                /* target elements per phase-1 tile (0: default)       */
   int w2;                /* target elements per phase-2 tile (0: default)       */
-  int block_threads;     /* threads per workgroup, multiple of 64 (0: default)  */
-  int reserved[4];
+  int block_threads;     /* threads per workgroup, 64..256 step 64 (0: default) */
+  int kernel;            /* tile-kernel variant 0..7: bit0 16 loads per batch,  */
+                         /* bit1 software pipelining, bit2 non-temporal loads   */
+  int order;             /* tile order in a launch: 0 memory, 1 largest first   */
+  int reserved[2];
 } hmvm_options;
 
 typedef struct hmvm_plan_info {
@@ -100,11 +103,14 @@ typedef struct hmvm_plan_info {
   long long len;             /* reference element count                         */
   long long device_elems;    /* elements in HBM incl. alignment padding         */
   int storage, elem_bytes;
-  int ntiles1, ntiles2, npanels;
+  int ntiles_a, ntiles_b;    /* tiles of launch A (stage 1 + dense), B (stage 2) */
+  int npanels;
   int ncol_groups, nrow_groups;
+  int t_fused;               /* 1: stage 2 sums stage-1 partials itself         */
   long long n_t, n_tslots, n_zslots;
-  long long elems_phase1, elems_phase2;
-  long long vec_reads_phase1, vec_reads_phase2;
+  long long elems_a, elems_b;                     /* elements per launch        */
+  long long elems_stage1, elems_dense, elems_stage2;
+  long long vec_reads_a, vec_reads_b;
   long long algorithmic_bytes; /* elem_bytes*len + 16*nd (x read, y written)    */
   long long device_bytes;      /* HBM held by the plan                          */
 } hmvm_plan_info;
@@ -112,7 +118,7 @@ typedef struct hmvm_plan_info {
 typedef struct hmvm_timing {
   int iters;
   double ms_avg, ms_min, ms_median; /* whole HMVM (all launches)                 */
-  double ms_phase1, ms_tsum, ms_phase2, ms_ysum; /* per-launch averages          */
+  double ms_a, ms_tsum, ms_b, ms_ysum; /* per-launch averages (A, t sum, B, y sum)   */
   double ms_stream; /* iters applies enqueued back to back, total / iters       */
 } hmvm_timing;
 
@@ -139,9 +145,52 @@ int hmvm_plan_apply_host(hmvm_plan *plan, const double *x, double *y, int accumu
  * untimed ones, with HIP events around every launch. */
 int hmvm_plan_time(hmvm_plan *plan, int warmup, int iters, hmvm_timing *timing);
 
+/* Switches the tile-kernel variant and workgroup size of an existing plan
+ * (used by the autotuner; results are identical across variants up to the
+ * summation order inside a tile, which every variant keeps). */
+int hmvm_plan_set_launch(hmvm_plan *plan, int kernel, int block_threads);
+
+/* Achievable HBM bandwidth on this device: a read-only stream and a copy of
+ * `bytes` (each repeated `iters` times); GB/s counts bytes read (+ written). */
+int hmvm_bandwidth_probe(size_t bytes, int iters, double *read_gbs, double *copy_gbs);
+
 /* Host-only check of the schedule a plan would use for this matrix (no GPU
  * access): every element streamed once, every access in bounds. */
 int hmvm_plan_check(const matrix2 *mat2, const hmvm_options *opt, hmvm_plan_info *info);
+
+/* ---- schedule introspection (host only) -------------------------------------
+ * The schedule a plan runs, as plain arrays: the autotuner's features, and
+ * the CPU tests that replay it.  Element (r, j) of a tile's panel p is
+ * values[val_off + r + j*ld]; the panel's vector entry j is x[vec_off + j]
+ * (vec_sel 0), t[vec_off + j] (1) or sum_{c<nch} slots[vec_off + c*stride + j]
+ * (2, vec_chunks = nch<<16 | stride).  Repack record: values[dst_off + r +
+ * c*dst_cs] = source[src_off + r*src_rs + c*src_cs].  Segment sums:
+ * out[i] (+)= sum_{e in ptr[seg[i]]..ptr[seg[i]+1]} slots[list[e] + i]. */
+typedef struct hmvm_tile { int nrows, panel0, npanel, out_off; } hmvm_tile;
+typedef struct hmvm_panel {
+  long long val_off;
+  int ld, ncols, vec_off, vec_sel, factor, vec_chunks;
+} hmvm_panel;
+typedef struct hmvm_repack {
+  long long src_off, dst_off;
+  int rows, cols, src_rs, src_cs, dst_cs, factor;
+} hmvm_repack;
+typedef struct hmvm_segsum {
+  int n, nseg;
+  int *seg, *ptr;
+  long long *list;
+} hmvm_segsum;
+typedef struct hmvm_schedule {
+  int ntiles_a, ntiles_b, npanels, nrepack, t_fused;
+  long long nvals, n_t, n_tslots, n_zslots;
+  hmvm_tile *tiles_a, *tiles_b;
+  hmvm_panel *panels;
+  hmvm_repack *repack;
+  hmvm_segsum tsum, ysum;
+} hmvm_schedule;
+
+int hmvm_schedule_export(const matrix2 *mat2, int a2_source, const hmvm_options *opt, hmvm_schedule *out);
+void hmvm_schedule_release(hmvm_schedule *s);
 
 /* ---- small device helpers for callers without a HIP runtime of their own -- */
 int hmvm_device_count(void);

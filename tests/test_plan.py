@@ -34,7 +34,8 @@ def test_schedule_check_generated(H, w):
     h = H.generate(3000, geometry="cube", eps=1e-4)
     info = H.plan_check(h, w1=w, w2=w)
     assert info["len"] == h.mat2.len
-    assert info["elems_phase1"] + info["elems_phase2"] == h.mat2.len
+    assert info["elems_a"] + info["elems_b"] == h.mat2.len
+    assert info["elems_stage1"] + info["elems_dense"] + info["elems_stage2"] == h.mat2.len
     assert info["device_elems"] >= h.mat2.len
     assert info["algorithmic_bytes"] == 8 * h.mat2.len + 16 * h.nd
     a = h.arrays()
@@ -46,7 +47,7 @@ def test_schedule_check_generated(H, w):
 def test_schedule_check_dense_grid(H):
     h = H.dummymatrix(40, 40, 5)  # rows of 40 -> one 40-row tile per row cluster
     info = H.plan_check(h)
-    assert info["ntiles1"] == 0 and info["elems_phase2"] == h.mat2.len
+    assert info["ntiles_b"] == 0 and info["elems_dense"] == h.mat2.len
 
 
 def test_schedule_rejects_bad_descriptors(H):
