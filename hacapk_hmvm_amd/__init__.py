@@ -56,7 +56,7 @@ class hgen_params(ctypes.Structure):
 class hmvm_options(ctypes.Structure):
     _fields_ = [("storage", ctypes.c_int), ("w1", ctypes.c_int), ("w2", ctypes.c_int),
                 ("block_threads", ctypes.c_int), ("kernel", ctypes.c_int), ("order", ctypes.c_int),
-                ("reserved", ctypes.c_int * 2)]
+                ("mode", ctypes.c_int), ("reserved", ctypes.c_int * 1)]
 
 
 class hmvm_tile(ctypes.Structure):
@@ -166,8 +166,11 @@ def _load():
                                            ctypes.c_void_p]),
         "hmvm_plan_apply_host": (ctypes.c_int, [ctypes.c_void_p, c_double_p, c_double_p, ctypes.c_int]),
         "hmvm_plan_time": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, P(hmvm_timing)]),
+        "hmvm_plan_bench_apply": (ctypes.c_int, [ctypes.c_void_p, c_double_p, ctypes.c_int, c_double_p]),
         "hmvm_plan_check": (ctypes.c_int, [P(matrix2), P(hmvm_options), P(hmvm_plan_info)]),
         "hmvm_plan_set_launch": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]),
+        "hmvm_plan_set_mode": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+        "hmvm_plan_fault": (ctypes.c_int, [ctypes.c_void_p]),
         "hmvm_schedule_export": (ctypes.c_int, [P(matrix2), ctypes.c_int, P(hmvm_options), P(hmvm_schedule)]),
         "hmvm_schedule_release": (None, [P(hmvm_schedule)]),
         "hmvm_bandwidth_probe": (ctypes.c_int, [ctypes.c_size_t, ctypes.c_int, c_double_p, c_double_p]),
@@ -392,11 +395,12 @@ def bandwidth_probe(nbytes=2 << 30, iters=10):
     return r.value, c.value
 
 
-def options(storage="f64", w1=0, w2=0, block_threads=0, kernel=0, order=0):
+def options(storage="f64", w1=0, w2=0, block_threads=0, kernel=0, order=0, mode=0):
     o = hmvm_options()
     lib.hmvm_options_default(ctypes.byref(o))
     o.kernel = kernel
     o.order = order
+    o.mode = mode
     o.storage = STORAGE[storage] if isinstance(storage, str) else int(storage)
     if w1:
         o.w1 = w1
@@ -506,6 +510,15 @@ class Plan:
         rc = lib.hmvm_plan_set_launch(self._p, kernel, block_threads)
         if rc:
             raise HMVMError("hmvm_plan_set_launch: %s" % lib.hmvm_hip_last_error_message().decode())
+
+    def set_mode(self, mode):
+        rc = lib.hmvm_plan_set_mode(self._p, mode)
+        if rc:
+            raise HMVMError("hmvm_plan_set_mode: %s" % lib.hmvm_hip_last_error_message().decode())
+
+    def fault(self):
+        """1 when a bounded wait of the persistent launch gave up."""
+        return lib.hmvm_plan_fault(self._p)
 
     def time(self, warmup=5, iters=20):
         t = hmvm_timing()

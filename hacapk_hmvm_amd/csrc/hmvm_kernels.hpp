@@ -69,30 +69,41 @@ __device__ __forceinline__ auto load_stream(const S *p) {
 // consumed), bit2 non-temporal matrix loads.
 constexpr int kNumKernelVariants = 8;
 
-// Vector entry j of a panel, for a lane-varying j.
+// Vector entry j of a panel.  kCoherent: the stage-1 partial slots were
+// written earlier in the same launch (persistent mode), so they are read with
+// device-coherent vector loads instead of possibly-scalar cached ones.
+template <bool kCoherent = false>
 __device__ __forceinline__ double vec_entry(const PanelDesc &pd, const double *__restrict__ x,
-                                            const double *__restrict__ t, const double *__restrict__ tp, int j) {
+                                            const double *__restrict__ t, const double *tp, int j) {
   if (pd.vec_sel == kVecX) return x[pd.vec_off + j];
   if (pd.vec_sel == kVecT) return t[pd.vec_off + j];
   const int nch = pd.vec_chunks >> 16, stride = pd.vec_chunks & 0xFFFF;
   double s = 0;
-  for (int c = 0; c < nch; ++c) s += tp[pd.vec_off + c * stride + j];
+  for (int c = 0; c < nch; ++c) {
+    const double *q = tp + pd.vec_off + c * stride + j;
+    if constexpr (kCoherent)
+      s += __hip_atomic_load(q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    else
+      s += *q;
+  }
   return s;
 }
 
 // Scale of a panel's product under BF16 storage: its own factor, times the
-// a1 factor of the same leaf when the panel sums unscaled stage-1 partials.
+// a1 factor of the same leaf when the panel sums unscaled stage-1 partials;
+// stage-1 panels (no factor) stay unscaled.
 __device__ __forceinline__ float panel_scale(const PanelDesc &pd, const float *__restrict__ scales) {
+  if (pd.factor < 0) return 1.0f;  // stage-1 partials: scaled later, with t
   const float s = scales[pd.factor];
   return pd.vec_sel == kVecTP ? s * scales[pd.factor - 1] : s;
 }
 
 // nrows in (32, 256]: lane owns rows lane + 64q, q < R; every column is read
 // whole (R coalesced loads), the vector entry is wave-uniform.
-template <typename S, int R, int kLoads, bool kPipe, bool kNT>
+template <typename S, int R, int kLoads, bool kPipe, bool kNT, bool kCoh>
 __device__ __forceinline__ void rows_panel(const S *__restrict__ vals, const PanelDesc &pd, int lane, int nrows,
                                            const double *__restrict__ x, const double *__restrict__ t,
-                                           const double *__restrict__ tp,
+                                           const double *tp,
                                            typename AccOf<S>::T (&accp)[R]) {
   using A = typename AccOf<S>::T;
   constexpr int U = kLoads / R > 0 ? kLoads / R : 1;
@@ -108,7 +119,7 @@ __device__ __forceinline__ void rows_panel(const S *__restrict__ vals, const Pan
     A a0[U][R], b0[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-      b0[u] = (A)vec_entry(pd, x, t, tp, u);
+      b0[u] = (A)vec_entry<kCoh>(pd, x, t, tp, u);
 #pragma unroll
       for (int q = 0; q < R; ++q) a0[u][q] = (A)load_stream<kNT>(base[q] + (int64_t)u * ld);
     }
@@ -116,7 +127,7 @@ __device__ __forceinline__ void rows_panel(const S *__restrict__ vals, const Pan
       A a1[U][R], b1[U];
 #pragma unroll
       for (int u = 0; u < U; ++u) {
-        b1[u] = (A)vec_entry(pd, x, t, tp, j + u);
+        b1[u] = (A)vec_entry<kCoh>(pd, x, t, tp, j + u);
 #pragma unroll
         for (int q = 0; q < R; ++q) a1[u][q] = (A)load_stream<kNT>(base[q] + (int64_t)(j + u) * ld);
       }
@@ -140,7 +151,7 @@ __device__ __forceinline__ void rows_panel(const S *__restrict__ vals, const Pan
       A a[U][R], b[U];
 #pragma unroll
       for (int u = 0; u < U; ++u) {
-        b[u] = (A)vec_entry(pd, x, t, tp, j + u);
+        b[u] = (A)vec_entry<kCoh>(pd, x, t, tp, j + u);
 #pragma unroll
         for (int q = 0; q < R; ++q) a[u][q] = (A)load_stream<kNT>(base[q] + (int64_t)(j + u) * ld);
       }
@@ -151,16 +162,16 @@ __device__ __forceinline__ void rows_panel(const S *__restrict__ vals, const Pan
     }
   }
   for (; j < nc; ++j) {
-    const A b = (A)vec_entry(pd, x, t, tp, j);
+    const A b = (A)vec_entry<kCoh>(pd, x, t, tp, j);
 #pragma unroll
     for (int q = 0; q < R; ++q) accp[q] += (A)load_stream<kNT>(base[q] + (int64_t)j * ld) * b;
   }
 }
 
-template <typename S, int R, int kLoads, bool kPipe, bool kNT, bool kScaled>
+template <typename S, int R, int kLoads, bool kPipe, bool kNT, bool kScaled, bool kCoh>
 __device__ __forceinline__ void rows_tile(const TileDesc &td, const PanelDesc *__restrict__ panels,
                                           const S *__restrict__ vals, const double *__restrict__ x,
-                                          const double *__restrict__ t, const double *__restrict__ tp,
+                                          const double *__restrict__ t, const double *tp,
                                           const float *__restrict__ scales, double *__restrict__ out, int lane) {
   using A = typename AccOf<S>::T;
   A acc[R];
@@ -171,7 +182,7 @@ __device__ __forceinline__ void rows_tile(const TileDesc &td, const PanelDesc *_
     A accp[R];
 #pragma unroll
     for (int q = 0; q < R; ++q) accp[q] = 0;
-    rows_panel<S, R, kLoads, kPipe, kNT>(vals, pd, lane, td.nrows, x, t, tp, accp);
+    rows_panel<S, R, kLoads, kPipe, kNT, kCoh>(vals, pd, lane, td.nrows, x, t, tp, accp);
     const A sc = kScaled ? (A)panel_scale(pd, scales) : (A)1;
 #pragma unroll
     for (int q = 0; q < R; ++q) acc[q] += kScaled ? accp[q] * sc : accp[q];
@@ -187,10 +198,10 @@ __device__ __forceinline__ void rows_tile(const TileDesc &td, const PanelDesc *_
 // columns jg, jg+g, ... so for a contiguous panel (ld == nrows) each load
 // instruction reads g*nrows consecutive elements.  The g partial sums of a row
 // are combined by shuffles at the end, in a fixed order.
-template <typename S, int kLoads, bool kNT, bool kScaled>
+template <typename S, int kLoads, bool kNT, bool kScaled, bool kCoh>
 __device__ __forceinline__ void small_tile(const TileDesc &td, const PanelDesc *__restrict__ panels,
                                            const S *__restrict__ vals, const double *__restrict__ x,
-                                           const double *__restrict__ t, const double *__restrict__ tp,
+                                           const double *__restrict__ t, const double *tp,
                                            const float *__restrict__ scales, double *__restrict__ out, int lane) {
   using A = typename AccOf<S>::T;
   const int nrows = td.nrows;
@@ -210,14 +221,14 @@ __device__ __forceinline__ void small_tile(const TileDesc &td, const PanelDesc *
 #pragma unroll
       for (int u = 0; u < kLoads; ++u) {
         a[u] = (A)load_stream<kNT>(pj + u * gs);
-        b[u] = (A)vec_entry(pd, x, t, tp, j + u * g);
+        b[u] = (A)vec_entry<kCoh>(pd, x, t, tp, j + u * g);
       }
 #pragma unroll
       for (int u = 0; u < kLoads; ++u) accp += a[u] * b[u];
       pj += kLoads * gs;
     }
     for (; j < nc; j += g) {
-      accp += (A)load_stream<kNT>(pj) * (A)vec_entry(pd, x, t, tp, j);
+      accp += (A)load_stream<kNT>(pj) * (A)vec_entry<kCoh>(pd, x, t, tp, j);
       pj += gs;
     }
     acc += kScaled ? accp * (A)panel_scale(pd, scales) : accp;
@@ -226,6 +237,24 @@ __device__ __forceinline__ void small_tile(const TileDesc &td, const PanelDesc *
   double s = 0;
   for (int q = 0; q < g; ++q) s += __shfl(mine, r + q * nrows, 64);
   if (lane < nrows) out[td.out_off + lane] = s;
+}
+
+template <typename S, bool kScaled, int kLoads, bool kPipe, bool kNT, bool kCoh>
+__device__ __forceinline__ void run_tile(const TileDesc &td, const PanelDesc *__restrict__ panels,
+                                         const S *__restrict__ vals, const double *__restrict__ x,
+                                         const double *__restrict__ t, const double *tp,
+                                         const float *__restrict__ scales, double *out, int lane) {
+  if (td.nrows <= 32) {
+    small_tile<S, kLoads, kNT, kScaled, kCoh>(td, panels, vals, x, t, tp, scales, out, lane);
+  } else if (td.nrows <= 64) {
+    rows_tile<S, 1, kLoads, kPipe, kNT, kScaled, kCoh>(td, panels, vals, x, t, tp, scales, out, lane);
+  } else if (td.nrows <= 128) {
+    rows_tile<S, 2, kLoads, kPipe, kNT, kScaled, kCoh>(td, panels, vals, x, t, tp, scales, out, lane);
+  } else if (td.nrows <= 192) {
+    rows_tile<S, 3, kLoads, kPipe, kNT, kScaled, kCoh>(td, panels, vals, x, t, tp, scales, out, lane);
+  } else {
+    rows_tile<S, 4, kLoads, kPipe, kNT, kScaled, kCoh>(td, panels, vals, x, t, tp, scales, out, lane);
+  }
 }
 
 // One wavefront per tile (see hmvm_plan.hpp for the tile model).
@@ -243,16 +272,107 @@ __global__ __launch_bounds__(256) void tile_kernel(const TileDesc *__restrict__ 
   const int tid = (int)blockIdx.x * (int)(blockDim.x >> 6) + wave;
   if (tid >= ntiles) return;
   const TileDesc td = tiles[tid];
-  if (td.nrows <= 32) {
-    small_tile<S, kLoads, kNT, kScaled>(td, panels, vals, x, t, tp, scales, out, lane);
-  } else if (td.nrows <= 64) {
-    rows_tile<S, 1, kLoads, kPipe, kNT, kScaled>(td, panels, vals, x, t, tp, scales, out, lane);
-  } else if (td.nrows <= 128) {
-    rows_tile<S, 2, kLoads, kPipe, kNT, kScaled>(td, panels, vals, x, t, tp, scales, out, lane);
-  } else if (td.nrows <= 192) {
-    rows_tile<S, 3, kLoads, kPipe, kNT, kScaled>(td, panels, vals, x, t, tp, scales, out, lane);
-  } else {
-    rows_tile<S, 4, kLoads, kPipe, kNT, kScaled>(td, panels, vals, x, t, tp, scales, out, lane);
+  run_tile<S, kScaled, kLoads, kPipe, kNT, false>(td, panels, vals, x, t, tp, scales, out, lane);
+}
+
+// ---- persistent single-launch mode ------------------------------------------
+// Work items, in queue order: stage-1 tiles [0, n1), dense tiles [n1, na),
+// stage-2 tiles [na, nt), y-sum items of 64 rows [nt, nitems).  A wave takes
+// the next item from a device-wide counter; a stage-2 tile first waits until
+// all stage-1 tiles have completed, a y-sum item until all tiles have.  Every
+// item a wave waits for was taken earlier by a running wave that waits for
+// nothing later in the queue, so the waits end; each wait is additionally
+// bounded (the kernel sets the fault word and carries on rather than hang).
+// The last wave to leave resets the counters for the next launch.
+struct PersistCounters {
+  int head;        // next work item
+  int s1_done;     // completed stage-1 tiles
+  int tiles_done;  // completed tiles
+  int exited;      // waves that left the loop
+  int fault;       // set when a bounded wait gave up
+  int pad[3];
+};
+
+constexpr int kMaxSpins = 1 << 22;
+
+__device__ __forceinline__ int wave_take(int *head, int lane) {
+  int v = 0;
+  if (lane == 0) v = __hip_atomic_fetch_add(head, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  return __builtin_amdgcn_readfirstlane(v);
+}
+
+// Publishes the wave's earlier stores, then counts one completion.
+__device__ __forceinline__ void wave_signal(int *ctr, int lane) {
+  if (lane == 0) __hip_atomic_fetch_add(ctr, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Waits until *ctr >= target, bounded; relaxed polls, one acquire fence at
+// the end (so the caches are invalidated once, not on every poll).
+__device__ __forceinline__ void wave_wait(int *ctr, int target, int *fault, int lane) {
+  if (lane == 0) {
+    int ok = 0;
+    for (int s = 0; s < kMaxSpins; ++s) {
+      if (__hip_atomic_load(ctr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= target) {
+        ok = 1;
+        break;
+      }
+      __builtin_amdgcn_s_sleep(2);
+    }
+    if (!ok) __hip_atomic_store(fault, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+}
+
+template <typename S, bool kScaled, int kLoads, bool kPipe, bool kNT>
+__global__ __launch_bounds__(256) void persistent_kernel(
+    const TileDesc *__restrict__ tiles_a, int n1, int na, const TileDesc *__restrict__ tiles_b, int nt,
+    int nitems, int nwaves, const PanelDesc *__restrict__ panels, const S *__restrict__ vals,
+    const double *__restrict__ x, const double *__restrict__ t, double *slots, const float *__restrict__ scales,
+    const int32_t *__restrict__ yseg, const int32_t *__restrict__ yptr, const int64_t *__restrict__ ylist,
+    int nd, double *__restrict__ y, int accumulate, PersistCounters *ctr) {
+  const int lane = (int)(threadIdx.x & 63);
+  bool s1_seen = false, all_seen = false;
+  for (;;) {
+    const int item = wave_take(&ctr->head, lane);
+    if (item >= nitems) break;
+    if (item < na) {
+      const TileDesc td = tiles_a[item];
+      run_tile<S, kScaled, kLoads, kPipe, kNT, true>(td, panels, vals, x, t, slots, scales, slots, lane);
+      if (item < n1) wave_signal(&ctr->s1_done, lane);
+      wave_signal(&ctr->tiles_done, lane);
+    } else if (item < nt) {
+      if (!s1_seen) {
+        wave_wait(&ctr->s1_done, n1, &ctr->fault, lane);
+        s1_seen = true;
+      }
+      const TileDesc td = tiles_b[item - na];
+      run_tile<S, kScaled, kLoads, kPipe, kNT, true>(td, panels, vals, x, t, slots, scales, slots, lane);
+      wave_signal(&ctr->tiles_done, lane);
+    } else {
+      if (!all_seen) {
+        wave_wait(&ctr->tiles_done, nt, &ctr->fault, lane);
+        all_seen = true;
+      }
+      const int i = (item - nt) * 64 + lane;
+      if (i < nd) {
+        const int sg = yseg[i];
+        const int e1 = yptr[sg + 1];
+        double a = 0;
+        for (int e = yptr[sg]; e < e1; ++e) a += __hip_atomic_load(slots + ylist[e] + i, __ATOMIC_RELAXED,
+                                                                   __HIP_MEMORY_SCOPE_AGENT);
+        y[i] = accumulate ? y[i] + a : a;
+      }
+    }
+  }
+  // leave; the last wave out resets the counters for the next launch
+  if (lane == 0) {
+    const int e = __hip_atomic_fetch_add(&ctr->exited, 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    if (e == nwaves - 1) {
+      __hip_atomic_store(&ctr->head, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_store(&ctr->s1_done, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_store(&ctr->tiles_done, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_store(&ctr->exited, 0, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    }
   }
 }
 

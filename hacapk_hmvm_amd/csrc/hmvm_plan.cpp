@@ -303,17 +303,19 @@ std::string build_plan(const LeafArrays &la, const PlanOptions &opt, Plan *plan)
   P.elems_b = P.elems_stage2;
 
   // ---- launch lists -----------------------------------------------------------------
-  P.tiles_a = t1;
-  P.tiles_a.insert(P.tiles_a.end(), td.begin(), td.end());
-  P.tiles_b = t2;
   if (opt.order == 1) {
     auto by_work = [&](const TileDesc &a, const TileDesc &b) {
       const int64_t wa = tile_work(P, a), wb = tile_work(P, b);
       return wa != wb ? wa > wb : a.panel0 < b.panel0;
     };
-    std::stable_sort(P.tiles_a.begin(), P.tiles_a.end(), by_work);
-    std::stable_sort(P.tiles_b.begin(), P.tiles_b.end(), by_work);
+    std::stable_sort(t1.begin(), t1.end(), by_work);
+    std::stable_sort(td.begin(), td.end(), by_work);
+    std::stable_sort(t2.begin(), t2.end(), by_work);
   }
+  P.n_stage1_tiles = (int)t1.size();
+  P.tiles_a = t1;
+  P.tiles_a.insert(P.tiles_a.end(), td.begin(), td.end());
+  P.tiles_b = t2;
 
   // ---- y segment sum: strips of rows covered by the same set of slots ----------
   std::vector<int64_t> cuts;
@@ -391,6 +393,8 @@ std::string verify_plan(const Plan &P, const LeafArrays &la) {
           default:
             return "unknown vector selector";
         }
+        if (d.factor < -1 || d.factor >= P.nfactors) return "panel factor id out of range";
+        if (d.vec_sel == kVecTP && d.factor < 1) return "stage-2 panel without an a1 factor";
         if (d.vec_sel == kVecX && d.factor < 0) {  // stage-1 panel
           if (list != 0 || t.npanel != 1) return "stage-1 tile malformed";
           if ((int64_t)t.out_off + t.nrows > P.n_tp) return "stage-1 slot out of range";

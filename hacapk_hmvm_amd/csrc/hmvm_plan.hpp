@@ -108,7 +108,8 @@ struct Plan {
   bool t_fused = true;               // stage 2 sums the partial slots itself
   std::vector<RepackDesc> repack;
   std::vector<int32_t> factor_of_trow;  // t index -> a1 factor id (BF16 t scaling)
-  std::vector<TileDesc> tiles_a, tiles_b;
+  std::vector<TileDesc> tiles_a, tiles_b;  // tiles_a: stage-1 tiles first, then dense
+  int n_stage1_tiles = 0;
   std::vector<PanelDesc> panels;
   // slot buffer: stage-1 partial slots [0, n_tp), then the dense and stage-2
   // output slots [n_tp, n_tp + n_z)

@@ -87,6 +87,9 @@ struct hmvm_plan {
   int storage = kF64;
   int block_threads = 256;
   int kernel = 0;  // tile-kernel variant, 0 .. kNumKernelVariants-1
+  int mode = 0;    // 0: launches A, [t sum], B, y sum; 1: one persistent launch
+  PersistCounters *d_ctr = nullptr;
+  int ncu = 0;     // compute units of the device
   int device = 0;
   hipStream_t stream = nullptr;
   void *d_vals = nullptr;
@@ -102,8 +105,8 @@ struct hmvm_plan {
   size_t device_bytes = 0;
 
   ~hmvm_plan() {
-    void *ptrs[] = {d_vals, d_tiles_a, d_tiles_b, d_panels, d_t,      d_slots, d_tseg, d_tptr,
-                    d_tfac, d_tlist,   d_yseg,    d_yptr,   d_ylist, d_scales, d_x,     d_y};
+    void *ptrs[] = {d_vals, d_tiles_a, d_tiles_b, d_panels, d_t,      d_slots, d_tseg, d_tptr,  d_tfac,
+                    d_tlist, d_yseg,   d_yptr,    d_ylist,  d_scales, d_x,     d_y,    d_ctr};
     for (void *p : ptrs)
       if (p) (void)hipFree(p);
     if (stream) (void)hipStreamDestroy(stream);
@@ -275,6 +278,8 @@ int create_plan(Source &src, const hmvm_options *opt, hmvm_plan **out) {
     if (opt->kernel < 0 || opt->kernel >= kNumKernelVariants)
       return set_error(HMVM_ERR_ARG, "kernel variant must be in [0,%d)", kNumKernelVariants);
     p->kernel = opt->kernel;
+    if (opt->mode < 0 || opt->mode > 1) return set_error(HMVM_ERR_ARG, "mode must be 0 or 1");
+    p->mode = opt->mode;
   }
   std::string err = build_plan(src.la, po, &p->plan);
   if (!err.empty()) return set_error(HMVM_ERR_MATRIX, "%s", err.c_str());
@@ -283,6 +288,13 @@ int create_plan(Source &src, const hmvm_options *opt, hmvm_plan **out) {
   Plan &P = p->plan;
   HIP_TRY(hipGetDevice(&p->device));
   HIP_TRY(hipStreamCreate(&p->stream));
+  {
+    hipDeviceProp_t prop;
+    HIP_TRY(hipGetDeviceProperties(&prop, p->device));
+    p->ncu = prop.multiProcessorCount;
+  }
+  HIP_TRY(hipMalloc(&p->d_ctr, sizeof(PersistCounters)));
+  HIP_TRY(hipMemset(p->d_ctr, 0, sizeof(PersistCounters)));
 
   int rc;
   const size_t eb = (size_t)elem_bytes(p->storage);
@@ -439,8 +451,57 @@ void launch_phase(hmvm_plan *p, int phase, const double *x, double *y, int accum
   }
 }
 
+template <typename S, bool kScaled, int kVar>
+int launch_persistent_v(hmvm_plan *p, const double *x, double *y, int accumulate, hipStream_t st) {
+  constexpr int L = (kVar & 1) ? 16 : 8;
+  constexpr bool kPipe = (kVar & 2) != 0, kNT = (kVar & 4) != 0;
+  const Plan &P = p->plan;
+  auto kfn = persistent_kernel<S, kScaled, L, kPipe, kNT>;
+  int per_cu = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kfn, p->block_threads, 0) != hipSuccess || per_cu < 1)
+    per_cu = 1;
+  const int blocks = per_cu * std::max(p->ncu, 1);
+  const int nwaves = blocks * (p->block_threads / 64);
+  const int ns1 = P.n_stage1_tiles;
+  const int na = (int)P.tiles_a.size();
+  const int nt = na + (int)P.tiles_b.size();
+  const int nitems = nt + (P.nd + 63) / 64;
+  hipLaunchKernelGGL(kfn, dim3(blocks), dim3(p->block_threads), 0, st, p->d_tiles_a, ns1, na, p->d_tiles_b, nt,
+                     nitems, nwaves, p->d_panels, (const S *)p->d_vals, x, p->d_t, p->d_slots,
+                     (const float *)p->d_scales, p->d_yseg, p->d_yptr, p->d_ylist, P.nd, y, accumulate, p->d_ctr);
+  return HMVM_OK;
+}
+
+template <typename S, bool kScaled>
+int launch_persistent_s(hmvm_plan *p, const double *x, double *y, int accumulate, hipStream_t st) {
+  switch (p->kernel) {
+    case 0: return launch_persistent_v<S, kScaled, 0>(p, x, y, accumulate, st);
+    case 1: return launch_persistent_v<S, kScaled, 1>(p, x, y, accumulate, st);
+    case 2: return launch_persistent_v<S, kScaled, 2>(p, x, y, accumulate, st);
+    case 3: return launch_persistent_v<S, kScaled, 3>(p, x, y, accumulate, st);
+    case 4: return launch_persistent_v<S, kScaled, 4>(p, x, y, accumulate, st);
+    case 5: return launch_persistent_v<S, kScaled, 5>(p, x, y, accumulate, st);
+    case 6: return launch_persistent_v<S, kScaled, 6>(p, x, y, accumulate, st);
+    default: return launch_persistent_v<S, kScaled, 7>(p, x, y, accumulate, st);
+  }
+}
+
+int launch_persistent(hmvm_plan *p, const double *x, double *y, int accumulate, hipStream_t st) {
+  if (p->storage == kF64) return launch_persistent_s<double, false>(p, x, y, accumulate, st);
+  if (p->storage == kF32) return launch_persistent_s<float, false>(p, x, y, accumulate, st);
+  return launch_persistent_s<uint16_t, true>(p, x, y, accumulate, st);
+}
+
+// The persistent mode needs every stage-2 tile to read its leaf's stage-1
+// partial slots directly (no materialised t).
+bool persistent_ok(const hmvm_plan *p) { return p->mode == 1 && p->plan.t_fused; }
+
 int apply(hmvm_plan *p, const double *d_x, double *d_y, int accumulate, hipStream_t st) {
-  for (int ph = 0; ph < 4; ph++) launch_phase(p, ph, d_x, d_y, accumulate, st);
+  if (persistent_ok(p)) {
+    launch_persistent(p, d_x, d_y, accumulate, st);
+  } else {
+    for (int ph = 0; ph < 4; ph++) launch_phase(p, ph, d_x, d_y, accumulate, st);
+  }
   HIP_TRY(hipGetLastError());
   return HMVM_OK;
 }
@@ -569,6 +630,8 @@ void hmvm_options_default(hmvm_options *opt) {
   opt->w2 = po.w2;
   opt->block_threads = 256;
   opt->kernel = 0;
+  opt->order = 0;
+  opt->mode = 0;
 }
 
 int hmvm_plan_create(const matrix2 *mat2, int a2_source, const hmvm_options *opt, hmvm_plan **out) {
@@ -604,6 +667,21 @@ int hmvm_plan_set_launch(hmvm_plan *plan, int kernel, int block_threads) {
   plan->kernel = kernel;
   plan->block_threads = block_threads;
   return HMVM_OK;
+}
+
+int hmvm_plan_set_mode(hmvm_plan *plan, int mode) {
+  if (!plan) return set_error(HMVM_ERR_ARG, "NULL plan");
+  if (mode < 0 || mode > 1) return set_error(HMVM_ERR_ARG, "mode must be 0 or 1");
+  plan->mode = mode;
+  return HMVM_OK;
+}
+
+int hmvm_plan_fault(hmvm_plan *plan) {
+  if (!plan) return set_error(HMVM_ERR_ARG, "NULL plan");
+  PersistCounters c;
+  HIP_TRY(hipStreamSynchronize(plan->stream));
+  HIP_TRY(hipMemcpy(&c, plan->d_ctr, sizeof(c), hipMemcpyDeviceToHost));
+  return c.fault ? 1 : 0;
 }
 
 int hmvm_bandwidth_probe(size_t bytes, int iters, double *read_gbs, double *copy_gbs) {
@@ -734,6 +812,29 @@ int hmvm_plan_apply_host(hmvm_plan *plan, const double *x, double *y, int accumu
   return HMVM_OK;
 }
 
+int hmvm_plan_bench_apply(hmvm_plan *plan, const double *x, int iters, double *seconds) {
+  if (!plan || !x || !seconds || iters < 1) return set_error(HMVM_ERR_ARG, "bad argument");
+  hipStream_t st = plan->stream;
+  HIP_TRY(hipMemcpyAsync(plan->d_x, x, sizeof(double) * (size_t)plan->plan.nd, hipMemcpyHostToDevice, st));
+  hipEvent_t e0, e1;
+  HIP_TRY(hipEventCreate(&e0));
+  if (hipEventCreate(&e1) != hipSuccess) {
+    (void)hipEventDestroy(e0);
+    return set_error(HMVM_ERR_HIP, "hipEventCreate failed");
+  }
+  hipEventRecord(e0, st);
+  int rc = HMVM_OK;
+  for (int i = 0; i < iters && !rc; i++) rc = apply(plan, plan->d_x, plan->d_y, 0, st);
+  hipEventRecord(e1, st);
+  float ms = 0;
+  if (!rc && hipEventSynchronize(e1) == hipSuccess) hipEventElapsedTime(&ms, e0, e1);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (rc) return rc;
+  *seconds = ms * 1e-3 / iters;
+  return HMVM_OK;
+}
+
 int hmvm_plan_time(hmvm_plan *plan, int warmup, int iters, hmvm_timing *tm) {
   if (!plan || !tm || iters < 1 || warmup < 0) return set_error(HMVM_ERR_ARG, "bad argument");
   std::memset(tm, 0, sizeof(*tm));
@@ -755,9 +856,14 @@ int hmvm_plan_time(hmvm_plan *plan, int warmup, int iters, hmvm_timing *tm) {
   int rc = HMVM_OK;
   for (int it = 0; it < iters && !rc; it++) {
     hipEventRecord(ev[0], st);
-    for (int phs = 0; phs < 4; phs++) {
-      launch_phase(plan, phs, plan->d_x, plan->d_y, 0, st);
-      hipEventRecord(ev[phs + 1], st);
+    if (persistent_ok(plan)) {
+      launch_persistent(plan, plan->d_x, plan->d_y, 0, st);
+      for (int phs = 0; phs < 4; phs++) hipEventRecord(ev[phs + 1], st);
+    } else {
+      for (int phs = 0; phs < 4; phs++) {
+        launch_phase(plan, phs, plan->d_x, plan->d_y, 0, st);
+        hipEventRecord(ev[phs + 1], st);
+      }
     }
     if (hipEventSynchronize(ev[4]) != hipSuccess) rc = set_error(HMVM_ERR_HIP, "event sync failed");
     float ms;

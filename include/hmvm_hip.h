@@ -95,7 +95,9 @@ typedef struct hmvm_options {
   int kernel;            /* tile-kernel variant 0..7: bit0 16 loads per batch,  */
                          /* bit1 software pipelining, bit2 non-temporal loads   */
   int order;             /* tile order in a launch: 0 memory, 1 largest first   */
-  int reserved[2];
+  int mode;              /* 0: three launches (A, B, y sum); 1: one persistent  */
+                         /* launch with a device-wide work queue                */
+  int reserved[1];
 } hmvm_options;
 
 typedef struct hmvm_plan_info {
@@ -141,6 +143,10 @@ int hmvm_plan_get_info(const hmvm_plan *plan, hmvm_plan_info *info);
 int hmvm_plan_apply(hmvm_plan *plan, const double *d_x, double *d_y, int accumulate, void *stream);
 /* Host-pointer form: copies x in, applies, copies y back (blocking). */
 int hmvm_plan_apply_host(hmvm_plan *plan, const double *x, double *y, int accumulate);
+/* Copies x in once, then times `iters` back-to-back applies on the device
+ * (kernels only) and returns the seconds per apply (the C++ hmvm_hip<T>
+ * benchmark loop). */
+int hmvm_plan_bench_apply(hmvm_plan *plan, const double *x, int iters, double *seconds);
 /* Times `iters` applies on the plan's internal device vectors after `warmup`
  * untimed ones, with HIP events around every launch. */
 int hmvm_plan_time(hmvm_plan *plan, int warmup, int iters, hmvm_timing *timing);
@@ -149,6 +155,12 @@ int hmvm_plan_time(hmvm_plan *plan, int warmup, int iters, hmvm_timing *timing);
  * (used by the autotuner; results are identical across variants up to the
  * summation order inside a tile, which every variant keeps). */
 int hmvm_plan_set_launch(hmvm_plan *plan, int kernel, int block_threads);
+
+/* Selects the execution mode of an existing plan (see hmvm_options.mode). */
+int hmvm_plan_set_mode(hmvm_plan *plan, int mode);
+/* 1 when a bounded wait inside the persistent launch gave up (the results of
+ * that launch are then invalid), 0 otherwise; synchronises the plan's stream. */
+int hmvm_plan_fault(hmvm_plan *plan);
 
 /* Achievable HBM bandwidth on this device: a read-only stream and a copy of
  * `bytes` (each repeated `iters` times); GB/s counts bytes read (+ written). */

@@ -234,3 +234,25 @@ def test_rank_rows_straddle_row_groups(gpu, H, O, tmp_path):
         p = H.Plan(h, storage=storage)
         assert rel(p.apply_host(b), ref) <= tol
         p.close()
+
+
+@pytest.mark.parametrize("storage", ["f64", "f32", "bf16"])
+def test_persistent_mode_matches_launches(gpu, H, sphere4096, storage):
+    """The single persistent launch (device-wide work queue) computes the same
+    bits as the three-launch mode, repeatedly (its counters reset themselves)."""
+    h = sphere4096
+    b = np.random.default_rng(14).uniform(-1, 1, h.nd)
+    p = H.Plan(h, storage=storage)
+    y0 = p.apply_host(b)
+    p.set_mode(1)
+    for k in (0, 6):
+        p.set_launch(k, 256)
+        for _ in range(3):
+            assert np.array_equal(p.apply_host(b), y0)
+    y1 = y0.copy()
+    p.apply_host(b, y1, accumulate=True)
+    assert np.allclose(y1, 2 * y0, rtol=1e-14, atol=0)
+    assert p.fault() == 0
+    t = p.time(2, 5)
+    assert t["ms_stream"] > 0 and p.fault() == 0
+    p.close()
