@@ -116,6 +116,14 @@ class hmvm_timing(ctypes.Structure):
         return {name: getattr(self, name) for name, _ in self._fields_}
 
 
+class hmvm_krylov_result(ctypes.Structure):
+    _fields_ = [("iters", ctypes.c_int), ("hmvms", ctypes.c_int), ("relres", ctypes.c_double),
+                ("ms_loop", ctypes.c_double), ("ms_per_iter", ctypes.c_double), ("graph_used", ctypes.c_int)]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
 HGEN_SPHERE, HGEN_CUBE = 0, 1
 HMVM_STORAGE_F64, HMVM_STORAGE_F32, HMVM_STORAGE_BF16 = 0, 1, 2
 HMVM_A2_ROWMAJOR, HMVM_A2_TRANSPOSED = 0, 1
@@ -167,6 +175,8 @@ def _load():
         "hmvm_plan_apply_host": (ctypes.c_int, [ctypes.c_void_p, c_double_p, c_double_p, ctypes.c_int]),
         "hmvm_plan_time": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, P(hmvm_timing)]),
         "hmvm_plan_bench_apply": (ctypes.c_int, [ctypes.c_void_p, c_double_p, ctypes.c_int, c_double_p]),
+        "hmvm_bicgstab": (ctypes.c_int, [ctypes.c_void_p, c_double_p, c_double_p, ctypes.c_int, ctypes.c_double,
+                                         ctypes.c_int, ctypes.c_int, P(hmvm_krylov_result)]),
         "hmvm_plan_check": (ctypes.c_int, [P(matrix2), P(hmvm_options), P(hmvm_plan_info)]),
         "hmvm_plan_set_launch": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]),
         "hmvm_plan_set_mode": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
@@ -519,6 +529,17 @@ class Plan:
     def fault(self):
         """1 when a bounded wait of the persistent launch gave up."""
         return lib.hmvm_plan_fault(self._p)
+
+    def bicgstab(self, b, u0=None, max_iter=100, tol=1e-8, use_graph=True, check_every=10):
+        """BiCGSTAB for A u = b on the device (m_HACApK_solve.f90:306-380); returns (u, result dict)."""
+        _check_vec(b, self.nd, "b")
+        u = np.zeros(self.nd) if u0 is None else np.array(u0, dtype=np.float64)
+        res = hmvm_krylov_result()
+        rc = lib.hmvm_bicgstab(self._p, _dptr(b), _dptr(u), int(max_iter), float(tol), int(bool(use_graph)),
+                               int(check_every), ctypes.byref(res))
+        if rc:
+            raise HMVMError("hmvm_bicgstab failed (code %d): %s" % (rc, lib.hmvm_hip_last_error_message().decode()))
+        return u, res.as_dict()
 
     def time(self, warmup=5, iters=20):
         t = hmvm_timing()

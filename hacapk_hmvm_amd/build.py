@@ -26,7 +26,7 @@ HIPFLAGS = ["-O3", "-fPIC", "-std=c++17", "--offload-arch=gfx950", "-I" + INC, "
             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-Wall"]
 
 CPP_SOURCES = ["hacapk_io.cpp", "hgen.cpp", "hmvm_plan.cpp", "hmvm_cpp_api.cpp"]
-HIP_SOURCES = ["hmvm_runtime.hip"]
+HIP_SOURCES = ["hmvm_runtime.hip", "hmvm_krylov.hip"]
 
 
 def _run(cmd):

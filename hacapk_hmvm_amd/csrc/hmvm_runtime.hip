@@ -88,6 +88,7 @@ struct hmvm_plan {
   int block_threads = 256;
   int kernel = 0;  // tile-kernel variant, 0 .. kNumKernelVariants-1
   int mode = 0;    // 0: launches A, [t sum], B, y sum; 1: one persistent launch
+  int persist_key = -1, persist_blocks = 0;  // occupancy cache (kernel variant, block size)
   PersistCounters *d_ctr = nullptr;
   int ncu = 0;     // compute units of the device
   int device = 0;
@@ -457,10 +458,15 @@ int launch_persistent_v(hmvm_plan *p, const double *x, double *y, int accumulate
   constexpr bool kPipe = (kVar & 2) != 0, kNT = (kVar & 4) != 0;
   const Plan &P = p->plan;
   auto kfn = persistent_kernel<S, kScaled, L, kPipe, kNT>;
-  int per_cu = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kfn, p->block_threads, 0) != hipSuccess || per_cu < 1)
-    per_cu = 1;
-  const int blocks = per_cu * std::max(p->ncu, 1);
+  const int key = kVar * 1024 + p->block_threads;
+  if (p->persist_key != key) {  // resident workgroups per CU, queried once per launch shape
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kfn, p->block_threads, 0) != hipSuccess || per_cu < 1)
+      per_cu = 1;
+    p->persist_blocks = per_cu * std::max(p->ncu, 1);
+    p->persist_key = key;
+  }
+  const int blocks = p->persist_blocks;
   const int nwaves = blocks * (p->block_threads / 64);
   const int ns1 = P.n_stage1_tiles;
   const int na = (int)P.tiles_a.size();

@@ -170,6 +170,26 @@ int hmvm_bandwidth_probe(size_t bytes, int iters, double *read_gbs, double *copy
  * access): every element streamed once, every access in bounds. */
 int hmvm_plan_check(const matrix2 *mat2, const hmvm_options *opt, hmvm_plan_info *info);
 
+/* ---- Krylov driver -----------------------------------------------------------
+ * BiCGSTAB for A u = b with this plan as A, iterating as the reference's
+ * HACApK_bicgstab_lfmtx_hyp (src_hdump/m_HACApK_solve.f90:306-380): two HMVMs
+ * per iteration, all scalars on the device.  use_graph != 0 captures one
+ * iteration in a hipGraph and replays it.  The residual is read back every
+ * check_every iterations (0: only at the end) and the loop stops when
+ * ||r||/||b|| < tol (tol <= 0: run max_iter iterations).  u is the initial
+ * guess on entry and the iterate on return (host pointers). */
+typedef struct hmvm_krylov_result {
+  int iters;          /* iterations run                                        */
+  int hmvms;          /* HMVMs applied, 2 per iteration + 1 for the residual   */
+  double relres;      /* ||r||/||b|| of the recurrence at the end              */
+  double ms_loop;     /* device time of the iteration loop                     */
+  double ms_per_iter;
+  int graph_used;
+} hmvm_krylov_result;
+
+int hmvm_bicgstab(hmvm_plan *plan, const double *b, double *u, int max_iter, double tol, int use_graph,
+                  int check_every, hmvm_krylov_result *result);
+
 /* ---- schedule introspection (host only) -------------------------------------
  * The schedule a plan runs, as plain arrays: the autotuner's features, and
  * the CPU tests that replay it.  Element (r, j) of a tile's panel p is
