@@ -175,6 +175,13 @@ def _load():
         "hmvm_plan_apply_host": (ctypes.c_int, [ctypes.c_void_p, c_double_p, c_double_p, ctypes.c_int]),
         "hmvm_plan_time": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, P(hmvm_timing)]),
         "hmvm_plan_bench_apply": (ctypes.c_int, [ctypes.c_void_p, c_double_p, ctypes.c_int, c_double_p]),
+        "hmvm_plan_reserve_multi": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+        "hmvm_plan_apply_multi": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
+                                                 ctypes.c_int, ctypes.c_void_p]),
+        "hmvm_plan_apply_multi_host": (ctypes.c_int, [ctypes.c_void_p, c_double_p, c_double_p, ctypes.c_int,
+                                                      ctypes.c_int]),
+        "hmvm_plan_time_multi": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                                P(hmvm_timing)]),
         "hmvm_bicgstab": (ctypes.c_int, [ctypes.c_void_p, c_double_p, c_double_p, ctypes.c_int, ctypes.c_double,
                                          ctypes.c_int, ctypes.c_int, P(hmvm_krylov_result)]),
         "hmvm_plan_check": (ctypes.c_int, [P(matrix2), P(hmvm_options), P(hmvm_plan_info)]),
@@ -529,6 +536,28 @@ class Plan:
     def fault(self):
         """1 when a bounded wait of the persistent launch gave up."""
         return lib.hmvm_plan_fault(self._p)
+
+    def apply_multi_host(self, X, Y=None, accumulate=False):
+        """Y (+)= A X for X of shape (nd, nrhs), nrhs in {16, 32, 64}, on the matrix cores."""
+        X = np.ascontiguousarray(X, dtype=np.float64)
+        if X.ndim != 2 or X.shape[0] != self.nd:
+            raise HMVMError("X must be (nd, nrhs)")
+        nrhs = X.shape[1]
+        if Y is None:
+            Y = np.zeros_like(X)
+        if Y.shape != X.shape or Y.dtype != np.float64 or not Y.flags["C_CONTIGUOUS"]:
+            raise HMVMError("Y must be a contiguous float64 array shaped like X")
+        rc = lib.hmvm_plan_apply_multi_host(self._p, _dptr(X), _dptr(Y), nrhs, int(bool(accumulate)))
+        if rc:
+            raise HMVMError("hmvm_plan_apply_multi_host: %s" % lib.hmvm_hip_last_error_message().decode())
+        return Y
+
+    def time_multi(self, nrhs=32, warmup=3, iters=10):
+        t = hmvm_timing()
+        rc = lib.hmvm_plan_time_multi(self._p, nrhs, warmup, iters, ctypes.byref(t))
+        if rc:
+            raise HMVMError("hmvm_plan_time_multi: %s" % lib.hmvm_hip_last_error_message().decode())
+        return t.as_dict()
 
     def bicgstab(self, b, u0=None, max_iter=100, tol=1e-8, use_graph=True, check_every=10):
         """BiCGSTAB for A u = b on the device (m_HACApK_solve.f90:306-380); returns (u, result dict)."""

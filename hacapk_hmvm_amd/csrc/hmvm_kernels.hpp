@@ -98,43 +98,92 @@ __device__ __forceinline__ float panel_scale(const PanelDesc &pd, const float *_
   return pd.vec_sel == kVecTP ? s * scales[pd.factor - 1] : s;
 }
 
-// nrows in (32, 256]: lane owns rows lane + 64q, q < R; every column is read
-// whole (R coalesced loads), the vector entry is wave-uniform.
+// ---- wide lane loads ----------------------------------------------------------
+// Every lane loads 16 bytes = V consecutive rows of one column (V = 2 FP64,
+// 4 FP32, 8 BF16).  Column strides (ld) and factor bases are multiples of V,
+// so with shift = val_off % V (the same for every panel of a tile) the address
+// val_off - shift + V*c + j*ld is 16-byte aligned for every j.
+typedef double wl_d2 __attribute__((ext_vector_type(2)));
+typedef float wl_f4 __attribute__((ext_vector_type(4)));
+typedef unsigned int wl_u4 __attribute__((ext_vector_type(4)));
+
+template <typename S>
+struct Wide;
+template <>
+struct Wide<double> {
+  static constexpr int V = 2;
+  using T = wl_d2;
+  __device__ static __forceinline__ double get(const T &w, int v) { return w[v]; }
+};
+template <>
+struct Wide<float> {
+  static constexpr int V = 4;
+  using T = wl_f4;
+  __device__ static __forceinline__ float get(const T &w, int v) { return w[v]; }
+};
+template <>
+struct Wide<uint16_t> {
+  static constexpr int V = 8;
+  using T = wl_u4;
+  __device__ static __forceinline__ float get(const T &w, int v) {
+    const unsigned int u = w[v >> 1];
+    return __uint_as_float((v & 1) ? (u & 0xFFFF0000u) : (u << 16));
+  }
+};
+
+template <typename S, bool kNT>
+__device__ __forceinline__ typename Wide<S>::T load_wide(const S *p) {
+  using T = typename Wide<S>::T;
+  if constexpr (kNT)
+    return __builtin_nontemporal_load((const T *)p);
+  else
+    return *(const T *)p;
+}
+
+// lanes_per_col > 32: lane owns V-row groups lane + 64q (q < R); the vector
+// entry of a column is wave-uniform.
 template <typename S, int R, int kLoads, bool kPipe, bool kNT, bool kCoh>
-__device__ __forceinline__ void rows_panel(const S *__restrict__ vals, const PanelDesc &pd, int lane, int nrows,
-                                           const double *__restrict__ x, const double *__restrict__ t,
-                                           const double *tp,
-                                           typename AccOf<S>::T (&accp)[R]) {
+__device__ __forceinline__ void wide_panel(const S *__restrict__ vals, const PanelDesc &pd, int lane, int shift,
+                                           int lpc, const double *__restrict__ x, const double *__restrict__ t,
+                                           const double *tp, typename AccOf<S>::T (&accp)[R][Wide<S>::V]) {
   using A = typename AccOf<S>::T;
+  using W = Wide<S>;
+  constexpr int V = W::V;
   constexpr int U = kLoads / R > 0 ? kLoads / R : 1;
   const int ld = pd.ld, nc = pd.ncols;
   const S *__restrict__ base[R];
 #pragma unroll
   for (int q = 0; q < R; ++q) {
-    const int row = lane + 64 * q;
-    base[q] = vals + pd.val_off + (row < nrows ? row : 0);
+    const int c = lane + 64 * q;
+    base[q] = vals + (pd.val_off - shift) + (int64_t)V * (c < lpc ? c : 0);
   }
+  auto consume = [&](const typename W::T (&a)[R], A b) {
+#pragma unroll
+    for (int q = 0; q < R; ++q)
+#pragma unroll
+      for (int v = 0; v < V; ++v) accp[q][v] += (A)W::get(a[q], v) * b;
+  };
   int j = 0;
   if (kPipe && nc >= 2 * U) {
-    A a0[U][R], b0[U];
+    typename W::T a0[U][R];
+    A b0[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       b0[u] = (A)vec_entry<kCoh>(pd, x, t, tp, u);
 #pragma unroll
-      for (int q = 0; q < R; ++q) a0[u][q] = (A)load_stream<kNT>(base[q] + (int64_t)u * ld);
+      for (int q = 0; q < R; ++q) a0[u][q] = load_wide<S, kNT>(base[q] + (int64_t)u * ld);
     }
     for (j = U; j + U <= nc; j += U) {
-      A a1[U][R], b1[U];
+      typename W::T a1[U][R];
+      A b1[U];
 #pragma unroll
       for (int u = 0; u < U; ++u) {
         b1[u] = (A)vec_entry<kCoh>(pd, x, t, tp, j + u);
 #pragma unroll
-        for (int q = 0; q < R; ++q) a1[u][q] = (A)load_stream<kNT>(base[q] + (int64_t)(j + u) * ld);
+        for (int q = 0; q < R; ++q) a1[u][q] = load_wide<S, kNT>(base[q] + (int64_t)(j + u) * ld);
       }
 #pragma unroll
-      for (int u = 0; u < U; ++u)
-#pragma unroll
-        for (int q = 0; q < R; ++q) accp[q] += a0[u][q] * b0[u];
+      for (int u = 0; u < U; ++u) consume(a0[u], b0[u]);
 #pragma unroll
       for (int u = 0; u < U; ++u) {
         b0[u] = b1[u];
@@ -143,100 +192,125 @@ __device__ __forceinline__ void rows_panel(const S *__restrict__ vals, const Pan
       }
     }
 #pragma unroll
-    for (int u = 0; u < U; ++u)
-#pragma unroll
-      for (int q = 0; q < R; ++q) accp[q] += a0[u][q] * b0[u];
+    for (int u = 0; u < U; ++u) consume(a0[u], b0[u]);
   } else {
     for (; j + U <= nc; j += U) {
-      A a[U][R], b[U];
+      typename W::T a[U][R];
+      A b[U];
 #pragma unroll
       for (int u = 0; u < U; ++u) {
         b[u] = (A)vec_entry<kCoh>(pd, x, t, tp, j + u);
 #pragma unroll
-        for (int q = 0; q < R; ++q) a[u][q] = (A)load_stream<kNT>(base[q] + (int64_t)(j + u) * ld);
+        for (int q = 0; q < R; ++q) a[u][q] = load_wide<S, kNT>(base[q] + (int64_t)(j + u) * ld);
       }
 #pragma unroll
-      for (int u = 0; u < U; ++u)
-#pragma unroll
-        for (int q = 0; q < R; ++q) accp[q] += a[u][q] * b[u];
+      for (int u = 0; u < U; ++u) consume(a[u], b[u]);
     }
   }
   for (; j < nc; ++j) {
-    const A b = (A)vec_entry<kCoh>(pd, x, t, tp, j);
+    typename W::T a[R];
 #pragma unroll
-    for (int q = 0; q < R; ++q) accp[q] += (A)load_stream<kNT>(base[q] + (int64_t)j * ld) * b;
+    for (int q = 0; q < R; ++q) a[q] = load_wide<S, kNT>(base[q] + (int64_t)j * ld);
+    consume(a, (A)vec_entry<kCoh>(pd, x, t, tp, j));
   }
 }
 
 template <typename S, int R, int kLoads, bool kPipe, bool kNT, bool kScaled, bool kCoh>
-__device__ __forceinline__ void rows_tile(const TileDesc &td, const PanelDesc *__restrict__ panels,
+__device__ __forceinline__ void wide_tile(const TileDesc &td, int shift, int lpc, const PanelDesc *__restrict__ panels,
                                           const S *__restrict__ vals, const double *__restrict__ x,
                                           const double *__restrict__ t, const double *tp,
                                           const float *__restrict__ scales, double *__restrict__ out, int lane) {
   using A = typename AccOf<S>::T;
-  A acc[R];
+  constexpr int V = Wide<S>::V;
+  A acc[R][V];
 #pragma unroll
-  for (int q = 0; q < R; ++q) acc[q] = 0;
+  for (int q = 0; q < R; ++q)
+#pragma unroll
+    for (int v = 0; v < V; ++v) acc[q][v] = 0;
   for (int p = 0; p < td.npanel; ++p) {
     const PanelDesc pd = panels[td.panel0 + p];
-    A accp[R];
+    A accp[R][V];
 #pragma unroll
-    for (int q = 0; q < R; ++q) accp[q] = 0;
-    rows_panel<S, R, kLoads, kPipe, kNT, kCoh>(vals, pd, lane, td.nrows, x, t, tp, accp);
+    for (int q = 0; q < R; ++q)
+#pragma unroll
+      for (int v = 0; v < V; ++v) accp[q][v] = 0;
+    wide_panel<S, R, kLoads, kPipe, kNT, kCoh>(vals, pd, lane, shift, lpc, x, t, tp, accp);
     const A sc = kScaled ? (A)panel_scale(pd, scales) : (A)1;
 #pragma unroll
-    for (int q = 0; q < R; ++q) acc[q] += kScaled ? accp[q] * sc : accp[q];
+    for (int q = 0; q < R; ++q)
+#pragma unroll
+      for (int v = 0; v < V; ++v) acc[q][v] += kScaled ? accp[q][v] * sc : accp[q][v];
   }
 #pragma unroll
-  for (int q = 0; q < R; ++q) {
-    const int row = lane + 64 * q;
-    if (row < td.nrows) out[td.out_off + row] = (double)acc[q];
-  }
+  for (int q = 0; q < R; ++q)
+#pragma unroll
+    for (int v = 0; v < V; ++v) {
+      const int row = V * (lane + 64 * q) + v - shift;
+      if (row >= 0 && row < td.nrows) out[td.out_off + row] = (double)acc[q][v];
+    }
 }
 
-// nrows <= 32: the wave holds g = 64/nrows column groups; lane (r, jg) sums
-// columns jg, jg+g, ... so for a contiguous panel (ld == nrows) each load
-// instruction reads g*nrows consecutive elements.  The g partial sums of a row
-// are combined by shuffles at the end, in a fixed order.
+// lanes_per_col <= 32: the wave holds g = 64/lpc column groups; lane (c, jg)
+// sums columns jg, jg+g, ... so for a contiguous panel each load instruction
+// reads g*lpc*V consecutive elements.  The g partial sums of each row meet in
+// a fixed-order shuffle reduction at the end.
 template <typename S, int kLoads, bool kNT, bool kScaled, bool kCoh>
-__device__ __forceinline__ void small_tile(const TileDesc &td, const PanelDesc *__restrict__ panels,
-                                           const S *__restrict__ vals, const double *__restrict__ x,
-                                           const double *__restrict__ t, const double *tp,
-                                           const float *__restrict__ scales, double *__restrict__ out, int lane) {
+__device__ __forceinline__ void grouped_tile(const TileDesc &td, int shift, int lpc,
+                                             const PanelDesc *__restrict__ panels, const S *__restrict__ vals,
+                                             const double *__restrict__ x, const double *__restrict__ t,
+                                             const double *tp, const float *__restrict__ scales,
+                                             double *__restrict__ out, int lane) {
   using A = typename AccOf<S>::T;
-  const int nrows = td.nrows;
-  const int g = 64 / nrows;
-  const int jg = lane / nrows;
-  const int r = lane - jg * nrows;
-  A acc = 0;
+  using W = Wide<S>;
+  constexpr int V = W::V;
+  const int g = 64 / lpc;
+  const int jg = lane / lpc;
+  const int c = lane - jg * lpc;
+  A acc[V];
+#pragma unroll
+  for (int v = 0; v < V; ++v) acc[v] = 0;
   for (int p = 0; p < td.npanel; ++p) {
     const PanelDesc pd = panels[td.panel0 + p];
     const int ld = pd.ld, nc = pd.ncols;
     const int64_t gs = (int64_t)g * ld;
     int j = jg < g ? jg : nc;
-    const S *__restrict__ pj = vals + pd.val_off + r + (int64_t)j * ld;
-    A accp = 0;
+    const S *__restrict__ pj = vals + (pd.val_off - shift) + (int64_t)V * c + (int64_t)j * ld;
+    A accp[V];
+#pragma unroll
+    for (int v = 0; v < V; ++v) accp[v] = 0;
     for (; j + (kLoads - 1) * g < nc; j += kLoads * g) {
-      A a[kLoads], b[kLoads];
+      typename W::T a[kLoads];
+      A b[kLoads];
 #pragma unroll
       for (int u = 0; u < kLoads; ++u) {
-        a[u] = (A)load_stream<kNT>(pj + u * gs);
+        a[u] = load_wide<S, kNT>(pj + u * gs);
         b[u] = (A)vec_entry<kCoh>(pd, x, t, tp, j + u * g);
       }
 #pragma unroll
-      for (int u = 0; u < kLoads; ++u) accp += a[u] * b[u];
+      for (int u = 0; u < kLoads; ++u)
+#pragma unroll
+        for (int v = 0; v < V; ++v) accp[v] += (A)W::get(a[u], v) * b[u];
       pj += kLoads * gs;
     }
     for (; j < nc; j += g) {
-      accp += (A)load_stream<kNT>(pj) * (A)vec_entry<kCoh>(pd, x, t, tp, j);
+      const typename W::T a = load_wide<S, kNT>(pj);
+      const A b = (A)vec_entry<kCoh>(pd, x, t, tp, j);
+#pragma unroll
+      for (int v = 0; v < V; ++v) accp[v] += (A)W::get(a, v) * b;
       pj += gs;
     }
-    acc += kScaled ? accp * (A)panel_scale(pd, scales) : accp;
+    const A sc = kScaled ? (A)panel_scale(pd, scales) : (A)1;
+#pragma unroll
+    for (int v = 0; v < V; ++v) acc[v] += kScaled ? accp[v] * sc : accp[v];
   }
-  const double mine = (double)acc;
-  double s = 0;
-  for (int q = 0; q < g; ++q) s += __shfl(mine, r + q * nrows, 64);
-  if (lane < nrows) out[td.out_off + lane] = s;
+#pragma unroll
+  for (int v = 0; v < V; ++v) {
+    const double mine = (double)acc[v];
+    double s = 0;
+    for (int q = 0; q < g; ++q) s += __shfl(mine, c + q * lpc, 64);
+    const int row = V * c + v - shift;
+    if (lane < lpc && row >= 0 && row < td.nrows) out[td.out_off + row] = s;
+  }
 }
 
 template <typename S, bool kScaled, int kLoads, bool kPipe, bool kNT, bool kCoh>
@@ -244,16 +318,19 @@ __device__ __forceinline__ void run_tile(const TileDesc &td, const PanelDesc *__
                                          const S *__restrict__ vals, const double *__restrict__ x,
                                          const double *__restrict__ t, const double *tp,
                                          const float *__restrict__ scales, double *out, int lane) {
-  if (td.nrows <= 32) {
-    small_tile<S, kLoads, kNT, kScaled, kCoh>(td, panels, vals, x, t, tp, scales, out, lane);
-  } else if (td.nrows <= 64) {
-    rows_tile<S, 1, kLoads, kPipe, kNT, kScaled, kCoh>(td, panels, vals, x, t, tp, scales, out, lane);
-  } else if (td.nrows <= 128) {
-    rows_tile<S, 2, kLoads, kPipe, kNT, kScaled, kCoh>(td, panels, vals, x, t, tp, scales, out, lane);
-  } else if (td.nrows <= 192) {
-    rows_tile<S, 3, kLoads, kPipe, kNT, kScaled, kCoh>(td, panels, vals, x, t, tp, scales, out, lane);
+  constexpr int V = Wide<S>::V;
+  const int shift = (int)(panels[td.panel0].val_off % V);
+  const int lpc = (td.nrows + shift + V - 1) / V;  // lanes per column
+  if (lpc <= 32) {
+    grouped_tile<S, kLoads, kNT, kScaled, kCoh>(td, shift, lpc, panels, vals, x, t, tp, scales, out, lane);
+  } else if (lpc <= 64) {
+    wide_tile<S, 1, kLoads, kPipe, kNT, kScaled, kCoh>(td, shift, lpc, panels, vals, x, t, tp, scales, out, lane);
+  } else if (lpc <= 128) {
+    if constexpr (V <= 4)
+      wide_tile<S, 2, kLoads, kPipe, kNT, kScaled, kCoh>(td, shift, lpc, panels, vals, x, t, tp, scales, out, lane);
   } else {
-    rows_tile<S, 4, kLoads, kPipe, kNT, kScaled, kCoh>(td, panels, vals, x, t, tp, scales, out, lane);
+    if constexpr (V <= 2)
+      wide_tile<S, 3, kLoads, kPipe, kNT, kScaled, kCoh>(td, shift, lpc, panels, vals, x, t, tp, scales, out, lane);
   }
 }
 
@@ -374,6 +451,194 @@ __global__ __launch_bounds__(256) void persistent_kernel(
       __hip_atomic_store(&ctr->exited, 0, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
     }
   }
+}
+
+// ---- several right-hand sides: MFMA tiles --------------------------------
+// Y (nd x nrhs, row-major) += A X.  A tile's output is nrows x nrhs; the wave
+// walks the tile in chunks of 64 rows (four 16-row MFMA blocks) and the
+// right-hand sides in 16-wide blocks.  Per panel and per K step the wave
+// loads one A operand per row block (16 rows x K columns of the column-major
+// panel) and one B operand per rhs block (K entries of the panel's vector for
+// 16 right-hand sides) and issues one MFMA per (row block, rhs block):
+//   FP64 storage: v_mfma_f64_16x16x4_f64   (K = 4, FP64 accumulate)
+//   FP32 storage: v_mfma_f32_16x16x4_f32   (K = 4, FP32 accumulate)
+//   BF16 storage: v_mfma_f32_16x16x16_bf16 (K = 16, vector entries rounded to
+//                 BF16 after the factor scale is applied, FP32 accumulate)
+// Operand layouts (wave64): A lane l holds rows l%16 of K-slice l/16, B lane l
+// holds rhs l%16 of K-slice l/16, the 16x16 result gives lane l the four rows
+// 4*(l/16)+v of rhs l%16.  Every address is formed from (lane, v) with the
+// row and column bounds applied, so a layout mismatch could only produce wrong
+// values, never an access outside the panel.
+typedef double mf_d4 __attribute__((ext_vector_type(4)));
+typedef float mf_f4 __attribute__((ext_vector_type(4)));
+typedef short mf_s4 __attribute__((ext_vector_type(4)));
+
+template <typename S>
+struct Mfma;
+template <>
+struct Mfma<double> {
+  static constexpr int K = 4;
+  using Acc = mf_d4;
+  using Op = double;
+};
+template <>
+struct Mfma<float> {
+  static constexpr int K = 4;
+  using Acc = mf_f4;
+  using Op = float;
+};
+template <>
+struct Mfma<uint16_t> {
+  static constexpr int K = 16;
+  using Acc = mf_f4;
+  using Op = mf_s4;
+};
+
+__device__ __forceinline__ mf_d4 mfma_step(double a, double b, mf_d4 c) {
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+__device__ __forceinline__ mf_f4 mfma_step(float a, float b, mf_f4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+__device__ __forceinline__ mf_f4 mfma_step(mf_s4 a, mf_s4 b, mf_f4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
+}
+
+__device__ __forceinline__ short bf16_bits(float f) {
+  const uint32_t b = __float_as_uint(f);
+  return (short)(uint16_t)((b + 0x7FFFu + ((b >> 16) & 1u)) >> 16);
+}
+
+// Vector entry (col, rhs) of a panel in the multi-RHS layout.
+__device__ __forceinline__ double vec_entry_multi(const PanelDesc &pd, const double *__restrict__ X,
+                                                  const double *__restrict__ T, const double *__restrict__ TP,
+                                                  int col, int rhs, int nrhs) {
+  if (pd.vec_sel == kVecX) return X[(int64_t)(pd.vec_off + col) * nrhs + rhs];
+  if (pd.vec_sel == kVecT) return T[(int64_t)(pd.vec_off + col) * nrhs + rhs];
+  const int nch = pd.vec_chunks >> 16, stride = pd.vec_chunks & 0xFFFF;
+  double s = 0;
+  for (int c = 0; c < nch; ++c) s += TP[(int64_t)(pd.vec_off + c * stride + col) * nrhs + rhs];
+  return s;
+}
+
+// A operand of row block rb at K step k0 (zero outside the panel).
+template <typename S>
+__device__ __forceinline__ typename Mfma<S>::Op a_operand(const S *__restrict__ vals, const PanelDesc &pd, int row,
+                                                          int nrows, int k0, int lane) {
+  if constexpr (sizeof(S) == 2) {
+    mf_s4 a;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int col = k0 + 4 * (lane >> 4) + q;
+      const bool ok = row < nrows && col < pd.ncols;
+      const int64_t off = pd.val_off + (ok ? row + (int64_t)col * pd.ld : 0);
+      a[q] = ok ? (short)vals[off] : (short)0;
+    }
+    return a;
+  } else {
+    const int col = k0 + (lane >> 4);
+    const bool ok = row < nrows && col < pd.ncols;
+    const int64_t off = pd.val_off + (ok ? row + (int64_t)col * pd.ld : 0);
+    const S v = vals[off];
+    return ok ? v : (S)0;
+  }
+}
+
+// B operand of rhs block cb at K step k0, times the panel scale.
+template <typename S>
+__device__ __forceinline__ typename Mfma<S>::Op b_operand(const PanelDesc &pd, const double *__restrict__ X,
+                                                          const double *__restrict__ T, const double *__restrict__ TP,
+                                                          int rhs, int nrhs, int k0, int lane, float scale) {
+  if constexpr (sizeof(S) == 2) {
+    mf_s4 b;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int col = k0 + 4 * (lane >> 4) + q;
+      const double v = col < pd.ncols ? vec_entry_multi(pd, X, T, TP, col, rhs, nrhs) : 0.0;
+      b[q] = bf16_bits((float)v * scale);
+    }
+    return b;
+  } else {
+    const int col = k0 + (lane >> 4);
+    const double v = col < pd.ncols ? vec_entry_multi(pd, X, T, TP, col, rhs, nrhs) : 0.0;
+    return (S)v;
+  }
+}
+
+// nrhs = 16*NB; one wave per tile.
+template <typename S, bool kScaled, int NB>
+__global__ __launch_bounds__(256) void tile_multi_kernel(const TileDesc *__restrict__ tiles, int ntiles,
+                                                         const PanelDesc *__restrict__ panels,
+                                                         const S *__restrict__ vals, const double *__restrict__ X,
+                                                         const double *__restrict__ T,
+                                                         const double *__restrict__ TP,
+                                                         const float *__restrict__ scales,
+                                                         double *__restrict__ out) {
+  using M = Mfma<S>;
+  using Acc = typename M::Acc;
+  constexpr int nrhs = 16 * NB;
+  const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  const int lane = (int)(threadIdx.x & 63);
+  const int tid = (int)blockIdx.x * (int)(blockDim.x >> 6) + wave;
+  if (tid >= ntiles) return;
+  const TileDesc td = tiles[tid];
+  for (int rc = 0; rc < td.nrows; rc += 64) {
+    const int nrb = (td.nrows - rc + 15) >> 4 < 4 ? (td.nrows - rc + 15) >> 4 : 4;  // live row blocks
+    Acc acc[4][NB];
+#pragma unroll
+    for (int rb = 0; rb < 4; ++rb)
+#pragma unroll
+      for (int cb = 0; cb < NB; ++cb) acc[rb][cb] = Acc{0, 0, 0, 0};
+    for (int p = 0; p < td.npanel; ++p) {
+      const PanelDesc pd = panels[td.panel0 + p];
+      float scale = 1.0f;
+      if constexpr (kScaled) scale = panel_scale(pd, scales);
+      for (int k0 = 0; k0 < pd.ncols; k0 += M::K) {
+        typename M::Op a[4], b[NB];
+#pragma unroll
+        for (int rb = 0; rb < 4; ++rb)
+          if (rb < nrb) a[rb] = a_operand<S>(vals, pd, rc + 16 * rb + (lane & 15), td.nrows, k0, lane);
+#pragma unroll
+        for (int cb = 0; cb < NB; ++cb) b[cb] = b_operand<S>(pd, X, T, TP, 16 * cb + (lane & 15), nrhs, k0, lane, scale);
+#pragma unroll
+        for (int rb = 0; rb < 4; ++rb)
+          if (rb < nrb)
+#pragma unroll
+            for (int cb = 0; cb < NB; ++cb) acc[rb][cb] = mfma_step(a[rb], b[cb], acc[rb][cb]);
+      }
+    }
+#pragma unroll
+    for (int rb = 0; rb < 4; ++rb) {
+      if (rb >= nrb) continue;
+#pragma unroll
+      for (int cb = 0; cb < NB; ++cb)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int row = rc + 16 * rb + 4 * (lane >> 4) + v;
+          if (row < td.nrows) out[(int64_t)(td.out_off + row) * nrhs + 16 * cb + (lane & 15)] = (double)acc[rb][cb][v];
+        }
+    }
+  }
+}
+
+// out[i*nrhs+c] = (accumulate ? out : 0) + scale(i) * sum_e buf[(list[e]+i)*nrhs + c]
+template <bool kScaled>
+__global__ __launch_bounds__(256) void segsum_multi_kernel(int n, int nrhs, const int32_t *__restrict__ seg,
+                                                           const int32_t *__restrict__ ptr,
+                                                           const int64_t *__restrict__ list,
+                                                           const double *__restrict__ buf,
+                                                           const int32_t *__restrict__ factor_of,
+                                                           const float *__restrict__ scales,
+                                                           double *__restrict__ out, int accumulate) {
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= (int64_t)n * nrhs) return;
+  const int i = (int)(g / nrhs), c = (int)(g - (int64_t)i * nrhs);
+  const int s = seg[i];
+  const int e1 = ptr[s + 1];
+  double a = 0;
+  for (int e = ptr[s]; e < e1; ++e) a += buf[(list[e] + i) * nrhs + c];
+  if (kScaled) a *= (double)scales[factor_of[i]];
+  out[g] = accumulate ? out[g] + a : a;
 }
 
 // Achievable-bandwidth probes: a read-only stream (sum) and a copy, each

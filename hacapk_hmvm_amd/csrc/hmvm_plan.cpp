@@ -39,6 +39,12 @@ int chunk_cols(int nr, int w) {
   return c;
 }
 
+}  // namespace
+
+int lane_elems(int storage) { return storage == kF64 ? 2 : storage == kF32 ? 4 : 8; }
+
+namespace {
+
 std::string fmt(const char *f, long long a, long long b = 0) {
   char buf[256];
   std::snprintf(buf, sizeof buf, f, a, b);
@@ -93,6 +99,11 @@ std::string build_plan(const LeafArrays &la, const PlanOptions &opt, Plan *plan)
 
   // Factor ids: a1 of leaf i -> 2i, a2 / dense -> 2i+1.
   P.nfactors = 2 * nlf;
+  // A lane loads 16 bytes (V elements) of a column: column strides and factor
+  // bases are multiples of V.
+  const int V = lane_elems(opt.storage);
+  P.lane_elems = V;
+  if (opt.align % V) return "alignment must be a multiple of the lane width";
   std::vector<int64_t> t_idx(nlf, -1);     // index of t_b[0] in the materialised t
   std::vector<int64_t> tp_off(nlf, -1);    // partial slot of t_b[0], chunk 0
   std::vector<int32_t> tp_chunks(nlf, 0);  // (nch << 16) | stride
@@ -109,8 +120,9 @@ std::string build_plan(const LeafArrays &la, const PlanOptions &opt, Plan *plan)
     for (int i : ls) K64 += la.kt[i];
     if (K64 > INT_MAX / 2) return "rank stack too large";
     const int K = (int)K64;
+    const int Kp = (int)align_up(K, V);  // padded stack height = column stride
     const int64_t base = align_up(off, opt.align);
-    off = base + (int64_t)ndt * K;
+    off = base + (int64_t)ndt * Kp;
     // row groups of <= kMaxTileRows rank rows, cut at leaf boundaries when a
     // leaf fits; a leaf of more than kMaxTileRows ranks is cut inside
     struct Piece { int leaf, k0, n; };  // leaf rows [k0, k0+n) at stack row kofs
@@ -128,7 +140,7 @@ std::string build_plan(const LeafArrays &la, const PlanOptions &opt, Plan *plan)
       r.cols = ndt;  // it
       r.src_rs = ndt;
       r.src_cs = 1;
-      r.dst_cs = K;
+      r.dst_cs = Kp;
       r.factor = 2 * i;
       P.repack.push_back(r);
       t_idx[i] = tpos + kofs;
@@ -159,8 +171,8 @@ std::string build_plan(const LeafArrays &la, const PlanOptions &opt, Plan *plan)
       for (int c = 0; c < nch; c++) {
         const int c0 = c * C, nc = std::min(C, ndt - c0);
         PanelDesc pd;
-        pd.val_off = base + (int64_t)c0 * K + k0;
-        pd.ld = K;
+        pd.val_off = base + (int64_t)c0 * Kp + k0;
+        pd.ld = Kp;
         pd.ncols = nc;
         pd.vec_off = nstrtt - 1 + c0;
         pd.vec_sel = kVecX;
@@ -207,12 +219,13 @@ std::string build_plan(const LeafArrays &la, const PlanOptions &opt, Plan *plan)
     const int nstrtl = g.first.first, ndl = g.first.second;
     const std::vector<int> &ls = g.second;
     std::vector<int> dense, lowrank;
+    const int ldp = (int)align_up(ndl, V);  // padded column stride of the cluster's factors
     for (int i : ls) {
       const int64_t base = align_up(off, opt.align);
       dbase[i] = base;
       RepackDesc r;
       r.dst_off = base;
-      r.dst_cs = ndl;
+      r.dst_cs = ldp;
       r.factor = 2 * i + 1;
       r.rows = ndl;  // il
       if (la.ltmtx[i] == 2) {
@@ -220,7 +233,7 @@ std::string build_plan(const LeafArrays &la, const PlanOptions &opt, Plan *plan)
         r.cols = la.ndt[i];  // it
         r.src_rs = la.ndt[i];
         r.src_cs = 1;
-        off = base + (int64_t)ndl * la.ndt[i];
+        off = base + (int64_t)ldp * la.ndt[i];
         dense.push_back(i);
       } else {
         r.src_off = la.src_a2[i];
@@ -232,7 +245,7 @@ std::string build_plan(const LeafArrays &la, const PlanOptions &opt, Plan *plan)
           r.src_rs = la.kt[i];
           r.src_cs = 1;
         }
-        off = base + (int64_t)ndl * la.kt[i];
+        off = base + (int64_t)ldp * la.kt[i];
         lowrank.push_back(i);
       }
       P.repack.push_back(r);
@@ -255,7 +268,7 @@ std::string build_plan(const LeafArrays &la, const PlanOptions &opt, Plan *plan)
             if (tdsc.npanel > 0 && work + (int64_t)nr * nc > opt.w2) break;
             PanelDesc pd;
             pd.val_off = dbase[i] + r0;
-            pd.ld = ndl;
+            pd.ld = ldp;
             pd.ncols = nc;
             pd.factor = 2 * i + 1;
             if (kind == 0) {
@@ -371,11 +384,19 @@ std::string verify_plan(const Plan &P, const LeafArrays &la) {
       if (t.nrows < 1 || t.nrows > kMaxTileRows) return "tile rows out of range";
       if (t.panel0 < 0 || t.npanel < 1 || (int64_t)t.panel0 + t.npanel > npan) return "tile panel range";
       if (t.out_off < 0) return "negative output slot";
+      const int64_t shift0 = P.panels[t.panel0].val_off % P.lane_elems;
       for (int p = t.panel0; p < t.panel0 + t.npanel; p++) {
         const PanelDesc &d = P.panels[p];
         if (d.ncols < 1 || d.ld < t.nrows) return "panel shape";
-        const int64_t last = d.val_off + (int64_t)(t.nrows - 1) + (int64_t)(d.ncols - 1) * d.ld;
-        if (d.val_off < 0 || last >= P.nvals) return "panel outside values";
+        if (d.ld % P.lane_elems) return "column stride not a multiple of the lane width";
+        // the kernel reads whole lane groups: [val_off - shift, val_off - shift + V*lpc) per column
+        const int64_t shift = d.val_off % P.lane_elems;
+        if (shift != shift0) return "panels of a tile disagree on the row alignment";
+        const int64_t lpc = (t.nrows + shift + P.lane_elems - 1) / P.lane_elems;
+        const int64_t first = d.val_off - shift;
+        const int64_t last = first + P.lane_elems * lpc - 1 + (int64_t)(d.ncols - 1) * d.ld;
+        if (first < 0 || last >= P.nvals) return "panel outside values";
+
         switch (d.vec_sel) {
           case kVecX:
             if (d.vec_off < 0 || (int64_t)d.vec_off + d.ncols > P.nd) return "x range out of bounds";

@@ -14,11 +14,12 @@
 //          transposed (element (it, il) at it*ndl + il), times x.
 // low-rank stage 2: the a2 factors (element (k, il) at k*ndl + il, the
 //          reference order) of all low-rank leaves of tau times their t_b.
-// A tile is one wavefront: with nrows <= 32 the wave splits the columns over
-// 64/nrows lane groups; with nrows > 32 each lane owns up to four rows
-// (lane, lane+64, ...) and reads every column whole, with the vector entry a
-// wave-uniform scalar load.  No lane-crossing reduction happens inside the
-// streaming loop.
+// A tile is one wavefront and every lane loads 16 bytes (V consecutive rows
+// of one column; column strides are padded to multiples of V).  When a column
+// of the tile needs at most 32 lanes, the wave splits the columns over
+// 64/lanes-per-column groups; otherwise each lane owns up to three V-row
+// groups and reads every column whole, with the vector entry a wave-uniform
+// scalar load.  No lane-crossing reduction happens inside the streaming loop.
 //
 // Launch A runs the stage-1 tiles and the dense tiles (independent of each
 // other); launch B runs the stage-2 tiles, which sum the stage-1 partial slots
@@ -100,8 +101,12 @@ struct LeafArrays {       // the caller's descriptors (matrix2 or matrix)
   int64_t src_len = 0;              // length of the source buffer
 };
 
+// Elements a lane loads at once (16 bytes): 2 FP64, 4 FP32, 8 BF16.
+int lane_elems(int storage);
+
 struct Plan {
   int nd = 0, nlf = 0;
+  int lane_elems = 2;                // column strides and factor bases are multiples of this
   int64_t nvals = 0;                 // device value buffer (elements)
   int64_t len = 0;                   // reference element count
   int nfactors = 0;

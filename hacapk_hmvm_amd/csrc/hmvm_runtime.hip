@@ -89,6 +89,9 @@ struct hmvm_plan {
   int kernel = 0;  // tile-kernel variant, 0 .. kNumKernelVariants-1
   int mode = 0;    // 0: launches A, [t sum], B, y sum; 1: one persistent launch
   int persist_key = -1, persist_blocks = 0;  // occupancy cache (kernel variant, block size)
+  // several right-hand sides: buffers sized for multi_cap columns
+  int multi_cap = 0;
+  double *d_mslots = nullptr, *d_mt = nullptr, *d_mX = nullptr, *d_mY = nullptr;
   PersistCounters *d_ctr = nullptr;
   int ncu = 0;     // compute units of the device
   int device = 0;
@@ -106,8 +109,9 @@ struct hmvm_plan {
   size_t device_bytes = 0;
 
   ~hmvm_plan() {
-    void *ptrs[] = {d_vals, d_tiles_a, d_tiles_b, d_panels, d_t,      d_slots, d_tseg, d_tptr,  d_tfac,
-                    d_tlist, d_yseg,   d_yptr,    d_ylist,  d_scales, d_x,     d_y,    d_ctr};
+    void *ptrs[] = {d_vals,  d_tiles_a, d_tiles_b, d_panels, d_t,      d_slots, d_tseg, d_tptr,
+                    d_tfac,  d_tlist,   d_yseg,    d_yptr,   d_ylist,  d_scales, d_x,    d_y,
+                    d_ctr,   d_mslots,  d_mt,      d_mX,     d_mY};
     for (void *p : ptrs)
       if (p) (void)hipFree(p);
     if (stream) (void)hipStreamDestroy(stream);
@@ -317,6 +321,8 @@ int create_plan(Source &src, const hmvm_options *opt, hmvm_plan **out) {
       (rc = dalloc(&p->d_y, (size_t)P.nd)))
     return rc;
   HIP_TRY(hipMalloc(&p->d_vals, eb * (size_t)std::max<int64_t>(P.nvals, 1)));
+  // column padding is read (and discarded) by the 16-byte lane loads: keep it zero
+  HIP_TRY(hipMemsetAsync(p->d_vals, 0, eb * (size_t)std::max<int64_t>(P.nvals, 1), p->stream));
 
   // values: upload the caller's buffer once, repack on the GPU
   std::vector<RepackChunk> chunks;
@@ -450,6 +456,81 @@ void launch_phase(hmvm_plan *p, int phase, const double *x, double *y, int accum
                        p->d_yptr, p->d_ylist, p->d_slots, (const int32_t *)nullptr, (const float *)nullptr, y,
                        accumulate);
   }
+}
+
+// ---- several right-hand sides -------------------------------------------------
+int reserve_multi(hmvm_plan *p, int nrhs) {
+  if (nrhs != 16 && nrhs != 32 && nrhs != 64) return set_error(HMVM_ERR_ARG, "nrhs must be 16, 32 or 64");
+  if (p->multi_cap >= nrhs) return HMVM_OK;
+  const Plan &P = p->plan;
+  double *bufs[4] = {p->d_mslots, p->d_mt, p->d_mX, p->d_mY};
+  for (double *b : bufs)
+    if (b) (void)hipFree(b);
+  p->d_mslots = p->d_mt = p->d_mX = p->d_mY = nullptr;
+  p->multi_cap = 0;
+  int rc;
+  if ((rc = dalloc(&p->d_mslots, (size_t)(P.n_tp + P.n_z) * nrhs)) || (rc = dalloc(&p->d_mt, (size_t)P.n_t * nrhs)) ||
+      (rc = dalloc(&p->d_mX, (size_t)P.nd * nrhs)) || (rc = dalloc(&p->d_mY, (size_t)P.nd * nrhs)))
+    return rc;
+  p->multi_cap = nrhs;
+  return HMVM_OK;
+}
+
+template <typename S, bool kScaled, int NB>
+void launch_multi_tiles(hmvm_plan *p, const TileDesc *d_tiles, int ntiles, const double *X, hipStream_t st) {
+  if (ntiles == 0) return;
+  const int wpb = p->block_threads / 64;
+  hipLaunchKernelGGL((tile_multi_kernel<S, kScaled, NB>), dim3((ntiles + wpb - 1) / wpb), dim3(p->block_threads), 0,
+                     st, d_tiles, ntiles, p->d_panels, (const S *)p->d_vals, X, p->d_mt, p->d_mslots,
+                     (const float *)p->d_scales, p->d_mslots);
+}
+
+template <typename S, bool kScaled>
+void launch_multi_nb(hmvm_plan *p, const TileDesc *d_tiles, int ntiles, const double *X, int nrhs, hipStream_t st) {
+  if (nrhs == 16)
+    launch_multi_tiles<S, kScaled, 1>(p, d_tiles, ntiles, X, st);
+  else if (nrhs == 32)
+    launch_multi_tiles<S, kScaled, 2>(p, d_tiles, ntiles, X, st);
+  else
+    launch_multi_tiles<S, kScaled, 4>(p, d_tiles, ntiles, X, st);
+}
+
+void launch_multi_phase(hmvm_plan *p, int phase, const double *X, double *Y, int nrhs, int accumulate,
+                        hipStream_t st) {
+  const Plan &P = p->plan;
+  const bool bf = p->storage == kBF16;
+  if (phase == 0 || phase == 2) {
+    const TileDesc *tl = phase == 0 ? p->d_tiles_a : p->d_tiles_b;
+    const int nt = (int)(phase == 0 ? P.tiles_a.size() : P.tiles_b.size());
+    if (p->storage == kF64)
+      launch_multi_nb<double, false>(p, tl, nt, X, nrhs, st);
+    else if (p->storage == kF32)
+      launch_multi_nb<float, false>(p, tl, nt, X, nrhs, st);
+    else
+      launch_multi_nb<uint16_t, true>(p, tl, nt, X, nrhs, st);
+  } else if (phase == 1) {
+    if (P.t_fused || P.tsum.n == 0) return;
+    const int64_t tot = (int64_t)P.tsum.n * nrhs;
+    if (bf)
+      hipLaunchKernelGGL(segsum_multi_kernel<true>, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, P.tsum.n,
+                         nrhs, p->d_tseg, p->d_tptr, p->d_tlist, p->d_mslots, p->d_tfac, p->d_scales, p->d_mt, 0);
+    else
+      hipLaunchKernelGGL(segsum_multi_kernel<false>, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, P.tsum.n,
+                         nrhs, p->d_tseg, p->d_tptr, p->d_tlist, p->d_mslots, (const int32_t *)nullptr,
+                         (const float *)nullptr, p->d_mt, 0);
+  } else {
+    const int64_t tot = (int64_t)P.ysum.n * nrhs;
+    hipLaunchKernelGGL(segsum_multi_kernel<false>, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, P.ysum.n,
+                       nrhs, p->d_yseg, p->d_yptr, p->d_ylist, p->d_mslots, (const int32_t *)nullptr,
+                       (const float *)nullptr, Y, accumulate);
+  }
+}
+
+int apply_multi(hmvm_plan *p, const double *X, double *Y, int nrhs, int accumulate, hipStream_t st) {
+  if (p->multi_cap < nrhs) return set_error(HMVM_ERR_ARG, "call hmvm_plan_reserve_multi(plan, %d) first", nrhs);
+  for (int ph = 0; ph < 4; ph++) launch_multi_phase(p, ph, X, Y, nrhs, accumulate, st);
+  HIP_TRY(hipGetLastError());
+  return HMVM_OK;
 }
 
 template <typename S, bool kScaled, int kVar>
@@ -672,6 +753,85 @@ int hmvm_plan_set_launch(hmvm_plan *plan, int kernel, int block_threads) {
     return set_error(HMVM_ERR_ARG, "block_threads must be 64, 128, 192 or 256");
   plan->kernel = kernel;
   plan->block_threads = block_threads;
+  return HMVM_OK;
+}
+
+int hmvm_plan_reserve_multi(hmvm_plan *plan, int nrhs) {
+  if (!plan) return set_error(HMVM_ERR_ARG, "NULL plan");
+  return reserve_multi(plan, nrhs);
+}
+
+int hmvm_plan_apply_multi(hmvm_plan *plan, const double *d_X, double *d_Y, int nrhs, int accumulate, void *stream) {
+  if (!plan || !d_X || !d_Y) return set_error(HMVM_ERR_ARG, "NULL argument");
+  if (nrhs != 16 && nrhs != 32 && nrhs != 64) return set_error(HMVM_ERR_ARG, "nrhs must be 16, 32 or 64");
+  return apply_multi(plan, d_X, d_Y, nrhs, accumulate, stream ? (hipStream_t)stream : plan->stream);
+}
+
+int hmvm_plan_apply_multi_host(hmvm_plan *plan, const double *X, double *Y, int nrhs, int accumulate) {
+  if (!plan || !X || !Y) return set_error(HMVM_ERR_ARG, "NULL argument");
+  int rc = reserve_multi(plan, nrhs);
+  if (rc) return rc;
+  const size_t nb = sizeof(double) * (size_t)plan->plan.nd * nrhs;
+  HIP_TRY(hipMemcpyAsync(plan->d_mX, X, nb, hipMemcpyHostToDevice, plan->stream));
+  if (accumulate) HIP_TRY(hipMemcpyAsync(plan->d_mY, Y, nb, hipMemcpyHostToDevice, plan->stream));
+  if ((rc = apply_multi(plan, plan->d_mX, plan->d_mY, nrhs, accumulate, plan->stream))) return rc;
+  HIP_TRY(hipMemcpyAsync(Y, plan->d_mY, nb, hipMemcpyDeviceToHost, plan->stream));
+  HIP_TRY(hipStreamSynchronize(plan->stream));
+  return HMVM_OK;
+}
+
+int hmvm_plan_time_multi(hmvm_plan *plan, int nrhs, int warmup, int iters, hmvm_timing *tm) {
+  if (!plan || !tm || iters < 1 || warmup < 0) return set_error(HMVM_ERR_ARG, "bad argument");
+  int rc = reserve_multi(plan, nrhs);
+  if (rc) return rc;
+  std::memset(tm, 0, sizeof(*tm));
+  hipStream_t st = plan->stream;
+  {
+    const size_t n = (size_t)plan->plan.nd * nrhs;
+    std::vector<double> h(n);
+    for (size_t i = 0; i < n; i++) h[i] = std::sin((double)(i + 1));
+    HIP_TRY(hipMemcpy(plan->d_mX, h.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+  }
+  for (int w = 0; w < warmup; w++)
+    if ((rc = apply_multi(plan, plan->d_mX, plan->d_mY, nrhs, 0, st))) return rc;
+  hipEvent_t ev[5];
+  for (int e = 0; e < 5; e++) HIP_TRY(hipEventCreate(&ev[e]));
+  double ph[4] = {0, 0, 0, 0}, total = 0, mn = 1e30;
+  for (int it = 0; it < iters; it++) {
+    hipEventRecord(ev[0], st);
+    for (int phs = 0; phs < 4; phs++) {
+      launch_multi_phase(plan, phs, plan->d_mX, plan->d_mY, nrhs, 0, st);
+      hipEventRecord(ev[phs + 1], st);
+    }
+    if (hipEventSynchronize(ev[4]) != hipSuccess) { rc = set_error(HMVM_ERR_HIP, "event sync failed"); break; }
+    float ms;
+    for (int phs = 0; phs < 4; phs++) {
+      hipEventElapsedTime(&ms, ev[phs], ev[phs + 1]);
+      ph[phs] += ms;
+    }
+    hipEventElapsedTime(&ms, ev[0], ev[4]);
+    total += ms;
+    mn = std::min(mn, (double)ms);
+  }
+  if (!rc) {
+    hipEventRecord(ev[0], st);
+    for (int it = 0; it < iters && !rc; it++) rc = apply_multi(plan, plan->d_mX, plan->d_mY, nrhs, 0, st);
+    hipEventRecord(ev[1], st);
+    float ms = 0;
+    if (!rc && hipEventSynchronize(ev[1]) == hipSuccess) hipEventElapsedTime(&ms, ev[0], ev[1]);
+    tm->ms_stream = ms / iters;
+  }
+  for (int e = 0; e < 5; e++) (void)hipEventDestroy(ev[e]);
+  if (rc) return rc;
+  HIP_TRY(hipGetLastError());
+  tm->iters = iters;
+  tm->ms_avg = total / iters;
+  tm->ms_min = mn;
+  tm->ms_median = tm->ms_avg;
+  tm->ms_a = ph[0] / iters;
+  tm->ms_tsum = ph[1] / iters;
+  tm->ms_b = ph[2] / iters;
+  tm->ms_ysum = ph[3] / iters;
   return HMVM_OK;
 }
 

@@ -156,6 +156,18 @@ int hmvm_plan_time(hmvm_plan *plan, int warmup, int iters, hmvm_timing *timing);
  * summation order inside a tile, which every variant keeps). */
 int hmvm_plan_set_launch(hmvm_plan *plan, int kernel, int block_threads);
 
+/* Several right-hand sides on the matrix cores: Y (+)= A X with X, Y of
+ * nd x nrhs doubles, row-major (X[i*nrhs + c]); nrhs = 16, 32 or 64.  FP64
+ * storage runs v_mfma_f64_16x16x4, FP32 v_mfma_f32_16x16x4 (FP32 accumulate
+ * in a tile, FP64 across tiles), BF16 v_mfma_f32_16x16x16_bf16 with the
+ * vector entries rounded to BF16.  hmvm_plan_reserve_multi allocates the
+ * per-plan work buffers for up to nrhs columns (the apply itself allocates
+ * nothing); the host form reserves by itself. */
+int hmvm_plan_reserve_multi(hmvm_plan *plan, int nrhs);
+int hmvm_plan_apply_multi(hmvm_plan *plan, const double *d_X, double *d_Y, int nrhs, int accumulate, void *stream);
+int hmvm_plan_apply_multi_host(hmvm_plan *plan, const double *X, double *Y, int nrhs, int accumulate);
+int hmvm_plan_time_multi(hmvm_plan *plan, int nrhs, int warmup, int iters, hmvm_timing *timing);
+
 /* Selects the execution mode of an existing plan (see hmvm_options.mode). */
 int hmvm_plan_set_mode(hmvm_plan *plan, int mode);
 /* 1 when a bounded wait inside the persistent launch gave up (the results of
