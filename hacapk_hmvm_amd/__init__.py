@@ -47,6 +47,15 @@ class matrix2(ctypes.Structure):
                 ("rowmat", c_double_p), ("rowmat_t", c_double_p)]
 
 
+class matrix2_64(ctypes.Structure):
+    """int64 extension of matrix2 (include/hacapk.h)."""
+    _fields_ = [("nd", ctypes.c_int), ("nlf", ctypes.c_int), ("ktmax", ctypes.c_int), ("len", ctypes.c_longlong),
+                ("ltmtx", c_int_p), ("kt", c_int_p), ("ndl", c_int_p), ("ndt", c_int_p),
+                ("nstrtl", c_int_p), ("nstrtt", c_int_p),
+                ("a1", ctypes.POINTER(ctypes.c_longlong)), ("a2", ctypes.POINTER(ctypes.c_longlong)),
+                ("rowmat", c_double_p), ("rowmat_t", c_double_p)]
+
+
 class hgen_params(ctypes.Structure):
     _fields_ = [("n", ctypes.c_int), ("geometry", ctypes.c_int), ("leaf", ctypes.c_int), ("kmax", ctypes.c_int),
                 ("eta", ctypes.c_double), ("eps", ctypes.c_double), ("aca_eps_factor", ctypes.c_double),
@@ -56,7 +65,7 @@ class hgen_params(ctypes.Structure):
 class hmvm_options(ctypes.Structure):
     _fields_ = [("storage", ctypes.c_int), ("w1", ctypes.c_int), ("w2", ctypes.c_int),
                 ("block_threads", ctypes.c_int), ("kernel", ctypes.c_int), ("order", ctypes.c_int),
-                ("mode", ctypes.c_int), ("reserved", ctypes.c_int * 1)]
+                ("mode", ctypes.c_int), ("t_mode", ctypes.c_int)]
 
 
 class hmvm_tile(ctypes.Structure):
@@ -128,6 +137,8 @@ HGEN_SPHERE, HGEN_CUBE = 0, 1
 HMVM_STORAGE_F64, HMVM_STORAGE_F32, HMVM_STORAGE_BF16 = 0, 1, 2
 HMVM_A2_ROWMAJOR, HMVM_A2_TRANSPOSED = 0, 1
 STORAGE = {"f64": 0, "f32": 1, "bf16": 2}
+HACAPK_WITH_ROWMAT_T = 1
+HMVM_NFEAT = 15
 
 
 class HMVMError(RuntimeError):
@@ -156,6 +167,48 @@ def _load():
         "hgen_kernel_block": (ctypes.c_int, [P(hgen_params), ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                              ctypes.c_int, c_double_p]),
         "hgen_accuracy": (ctypes.c_double, [P(hgen_params), P(matrix2), ctypes.c_int]),
+        "hgen_generate64": (ctypes.c_int, [P(hgen_params), P(matrix2_64), ctypes.c_int]),
+        "hgen_accuracy64": (ctypes.c_double, [P(hgen_params), P(matrix2_64), ctypes.c_int]),
+        "loadHmatrix_64": (ctypes.c_int, [ctypes.c_char_p, P(matrix2_64), ctypes.c_int]),
+        "writeHmatrix2_64": (ctypes.c_int, [ctypes.c_char_p, P(matrix2_64)]),
+        "writeHmatrix1_64": (ctypes.c_int, [ctypes.c_char_p, P(matrix2_64)]),
+        "hacapk_widen_matrix2": (ctypes.c_int, [P(matrix2), P(matrix2_64)]),
+        "freeHmatrix64": (None, [P(matrix2_64)]),
+        "hacapk_scan_file": (ctypes.c_int, [ctypes.c_char_p, P(matrix2_64), P(ctypes.c_longlong), c_int_p]),
+        "hmvm_plan_create64": (ctypes.c_int, [P(matrix2_64), ctypes.c_int, P(hmvm_options), P(ctypes.c_void_p)]),
+        "hmvm_plan_create_file": (ctypes.c_int, [ctypes.c_char_p, P(hmvm_options), P(ctypes.c_void_p)]),
+        "hmvm_plan_create_batch": (ctypes.c_int, [P(P(matrix2)), ctypes.c_int, ctypes.c_int, P(hmvm_options),
+                                                  P(ctypes.c_void_p), c_int_p]),
+        "hmvm_hip_2_64": (None, [c_double_p, P(matrix2_64), c_double_p]),
+        "hmvm_hip_2t_64": (None, [c_double_p, P(matrix2_64), c_double_p]),
+        "hmvm_plan_graph_create": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
+                                                  ctypes.c_int, P(ctypes.c_void_p)]),
+        "hmvm_graph_launch": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
+        "hmvm_graph_destroy": (None, [ctypes.c_void_p]),
+        "hmvm_plan_time_graph": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                                P(hmvm_timing)]),
+        "hmvm_plan_set_phase_launch": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
+        "hmvm_plan_tile_features": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
+                                                   ctypes.POINTER(ctypes.c_float)]),
+        "hmvm_plan_profile_tiles": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, c_double_p,
+                                                   c_double_p, c_double_p]),
+        "hmvm_plan_set_tile_order": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, c_int_p]),
+        "hmvm_mlp_create": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_ulonglong, P(ctypes.c_void_p)]),
+        "hmvm_mlp_destroy": (None, [ctypes.c_void_p]),
+        "hmvm_mlp_num_params": (ctypes.c_int, [ctypes.c_void_p]),
+        "hmvm_mlp_get_params": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_float),
+                                               ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float)]),
+        "hmvm_mlp_set_params": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_float),
+                                               ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float)]),
+        "hmvm_mlp_fit_scaler": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_float), ctypes.c_int]),
+        "hmvm_mlp_predict": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_float), ctypes.c_int,
+                                            ctypes.POINTER(ctypes.c_float)]),
+        "hmvm_mlp_loss_grad": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_float),
+                                              ctypes.POINTER(ctypes.c_float), ctypes.c_int,
+                                              ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float)]),
+        "hmvm_mlp_train": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_float),
+                                          ctypes.POINTER(ctypes.c_float), ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                          ctypes.c_float, ctypes.c_ulonglong, ctypes.POINTER(ctypes.c_float)]),
         "hmvm_hip": (None, [P(matrix), P(matrix2), c_double_p, ctypes.c_int]),
         "hmvm_hip_bench": (None, [P(matrix), P(matrix2), c_double_p]),
         "hmvm_hip_1": (None, [c_double_p, P(matrix), c_double_p]),
@@ -351,6 +404,91 @@ def accuracy(h, stride=1):
     return lib.hgen_accuracy(ctypes.byref(h.params), ctypes.byref(h.mat2), stride)
 
 
+class HMatrix64:
+    """Owns a ``matrix2_64`` (int64 offsets) filled by the native loaders or generator."""
+
+    def __init__(self):
+        self.mat2 = matrix2_64()
+        self.has_mat2 = False
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+    def free(self):
+        if self.has_mat2:
+            lib.hmvm_hip_release(ctypes.cast(ctypes.pointer(self.mat2), ctypes.c_void_p))
+            lib.freeHmatrix64(ctypes.byref(self.mat2))
+            self.has_mat2 = False
+
+    @property
+    def nd(self):
+        return self.mat2.nd
+
+    def arrays(self):
+        m = self.mat2
+        n = m.nlf
+        out = {k: np.ctypeslib.as_array(getattr(m, k), (n,))
+               for k in ("ltmtx", "kt", "ndl", "ndt", "nstrtl", "nstrtt", "a1", "a2")}
+        out["rowmat"] = np.ctypeslib.as_array(m.rowmat, (m.len,)) if m.len > 0 and m.rowmat else np.zeros(0)
+        out["rowmat_t"] = (np.ctypeslib.as_array(m.rowmat_t, (m.len,)) if m.len > 0 and m.rowmat_t
+                           else None)
+        return out
+
+
+def loadHmatrix_64(fname, with_rowmat_t=False):
+    """Format 1, 2 or 2L -> HMatrix64."""
+    h = HMatrix64()
+    rc = lib.loadHmatrix_64(os.fsencode(fname), ctypes.byref(h.mat2), HACAPK_WITH_ROWMAT_T if with_rowmat_t else 0)
+    _hacapk_err(rc, "loadHmatrix_64(%s)" % fname)
+    h.has_mat2 = True
+    return h
+
+
+def writeHmatrix2_64(fname, h):
+    _hacapk_err(lib.writeHmatrix2_64(os.fsencode(fname), ctypes.byref(h.mat2)), "writeHmatrix2_64")
+
+
+def writeHmatrix1_64(fname, h):
+    _hacapk_err(lib.writeHmatrix1_64(os.fsencode(fname), ctypes.byref(h.mat2)), "writeHmatrix1_64")
+
+
+def widen(h):
+    """Deep copy of an HMatrix's matrix2 into the int64 form."""
+    w = HMatrix64()
+    _hacapk_err(lib.hacapk_widen_matrix2(ctypes.byref(h.mat2), ctypes.byref(w.mat2)), "hacapk_widen_matrix2")
+    w.has_mat2 = True
+    return w
+
+
+def scan_file(fname):
+    """Leaf descriptors of a format 1/2/2L file: (HMatrix64 without values, data_pos, format)."""
+    h = HMatrix64()
+    pos = ctypes.c_longlong()
+    fmt = ctypes.c_int()
+    _hacapk_err(lib.hacapk_scan_file(os.fsencode(fname), ctypes.byref(h.mat2), ctypes.byref(pos), ctypes.byref(fmt)),
+                "hacapk_scan_file(%s)" % fname)
+    h.has_mat2 = True
+    return h, pos.value, fmt.value
+
+
+def generate64(n, with_rowmat_t=False, **kw):
+    """Generates an H-matrix in the int64 form (for more than 2^31-1 elements)."""
+    p = gen_params(n, **kw)
+    h = HMatrix64()
+    _hacapk_err(lib.hgen_generate64(ctypes.byref(p), ctypes.byref(h.mat2),
+                                    HACAPK_WITH_ROWMAT_T if with_rowmat_t else 0), "hgen_generate64")
+    h.has_mat2 = True
+    h.params = p
+    return h
+
+
+def accuracy64(h, stride=1):
+    return lib.hgen_accuracy64(ctypes.byref(h.params), ctypes.byref(h.mat2), stride)
+
+
 # ---- drop-in HMVM (v += A b) -------------------------------------------------------
 def _variant(fn, v, h, b, use_mat2):
     _check_vec(v, h.nd, "v")
@@ -378,6 +516,22 @@ def hmvm_hip_2(v, h, b):
 
 def hmvm_hip_2t(v, h, b):
     return _variant(lib.hmvm_hip_2t, v, h, b, True)
+
+
+def hmvm_hip_2_64(v, h, b):
+    _check_vec(v, h.nd, "v")
+    _check_vec(b, h.nd, "b")
+    lib.hmvm_hip_2_64(_dptr(v), ctypes.byref(h.mat2), _dptr(b))
+    _raise_hmvm("hmvm_hip_2_64")
+    return v
+
+
+def hmvm_hip_2t_64(v, h, b):
+    _check_vec(v, h.nd, "v")
+    _check_vec(b, h.nd, "b")
+    lib.hmvm_hip_2t_64(_dptr(v), ctypes.byref(h.mat2), _dptr(b))
+    _raise_hmvm("hmvm_hip_2t_64")
+    return v
 
 
 def hmvm_hip(h, b, dump_result=0):
@@ -412,11 +566,16 @@ def bandwidth_probe(nbytes=2 << 30, iters=10):
     return r.value, c.value
 
 
-def options(storage="f64", w1=0, w2=0, block_threads=0, kernel=0, order=0, mode=0):
+def options(storage="f64", w1=0, w2=0, block_threads=0, kernel=None, order=None, mode=0, t_mode=None):
+    """hmvm_options; None / 0 keep the library defaults."""
     o = hmvm_options()
     lib.hmvm_options_default(ctypes.byref(o))
-    o.kernel = kernel
-    o.order = order
+    if kernel is not None:
+        o.kernel = kernel
+    if order is not None:
+        o.order = order
+    if t_mode is not None:
+        o.t_mode = t_mode
     o.mode = mode
     o.storage = STORAGE[storage] if isinstance(storage, str) else int(storage)
     if w1:
@@ -477,11 +636,42 @@ def schedule(h, source="rowmat", **kw):
 
 
 class Plan:
-    """Repacked device copy of an H-matrix; y (+)= A x on device or host vectors."""
+    """Repacked device copy of an H-matrix; y (+)= A x on device or host vectors.
+
+    ``h`` is an HMatrix (source rowmat / rowmat_t / a2 / a2t), an HMatrix64
+    (rowmat / rowmat_t), a file name (streamed, formats 1/2/2L) or a list of
+    HMatrix for a block-diagonal batch (``self.row_offsets`` then gives each
+    matrix's rows)."""
 
     def __init__(self, h, source="rowmat", **kw):
         self._p = ctypes.c_void_p()
+        self.row_offsets = None
         o = options(**kw)
+        if isinstance(h, (str, bytes, os.PathLike)):
+            rc = lib.hmvm_plan_create_file(os.fsencode(h), ctypes.byref(o), ctypes.byref(self._p))
+            if rc:
+                raise HMVMError("hmvm_plan_create_file: %s (code %d)" % (lib.hmvm_hip_last_error_message().decode(),
+                                                                         rc))
+            self.nd = self.info()["nd"]
+            return
+        if isinstance(h, (list, tuple)):
+            arr = (ctypes.POINTER(matrix2) * len(h))(*[ctypes.pointer(m.mat2) for m in h])
+            offs = (ctypes.c_int * (len(h) + 1))()
+            rc = lib.hmvm_plan_create_batch(arr, len(h), 1 if source == "rowmat_t" else 0, ctypes.byref(o),
+                                            ctypes.byref(self._p), offs)
+            if rc:
+                raise HMVMError("hmvm_plan_create_batch: %s (code %d)" % (lib.hmvm_hip_last_error_message().decode(),
+                                                                          rc))
+            self.row_offsets = np.array(offs[:], dtype=np.int64)
+            self.nd = int(self.row_offsets[-1])
+            return
+        if isinstance(h, HMatrix64):
+            rc = lib.hmvm_plan_create64(ctypes.byref(h.mat2), 1 if source == "rowmat_t" else 0, ctypes.byref(o),
+                                        ctypes.byref(self._p))
+            if rc:
+                raise HMVMError("hmvm_plan_create64: %s (code %d)" % (lib.hmvm_hip_last_error_message().decode(), rc))
+            self.nd = h.nd
+            return
         if source in ("rowmat", "rowmat_t"):
             rc = lib.hmvm_plan_create(ctypes.byref(h.mat2), 1 if source == "rowmat_t" else 0, ctypes.byref(o),
                                       ctypes.byref(self._p))
@@ -527,6 +717,54 @@ class Plan:
         rc = lib.hmvm_plan_set_launch(self._p, kernel, block_threads)
         if rc:
             raise HMVMError("hmvm_plan_set_launch: %s" % lib.hmvm_hip_last_error_message().decode())
+
+    def set_phase_launch(self, which, kernel, block_threads=256):
+        rc = lib.hmvm_plan_set_phase_launch(self._p, which, kernel, block_threads)
+        if rc:
+            raise HMVMError("hmvm_plan_set_phase_launch: %s" % lib.hmvm_hip_last_error_message().decode())
+
+    def tile_features(self, which, nrhs=1):
+        n = self.info()["ntiles_a" if which == 0 else "ntiles_b"]
+        f = np.zeros((n, HMVM_NFEAT), dtype=np.float32)
+        if n:
+            rc = lib.hmvm_plan_tile_features(self._p, which, nrhs, f.ctypes.data_as(ctypes.POINTER(ctypes.c_float)))
+            if rc:
+                raise HMVMError("hmvm_plan_tile_features: %s" % lib.hmvm_hip_last_error_message().decode())
+        return f
+
+    def profile_tiles(self, which, iters=3):
+        """(per-tile duration us, per-tile start offset us, mean launch ms) of launch A (0) or B (1)."""
+        n = self.info()["ntiles_a" if which == 0 else "ntiles_b"]
+        d = np.zeros(max(n, 1))
+        s0 = np.zeros(max(n, 1))
+        lm = ctypes.c_double()
+        rc = lib.hmvm_plan_profile_tiles(self._p, which, iters, _dptr(d), _dptr(s0), ctypes.byref(lm))
+        if rc:
+            raise HMVMError("hmvm_plan_profile_tiles: %s" % lib.hmvm_hip_last_error_message().decode())
+        return d[:n], s0[:n], lm.value
+
+    def set_tile_order(self, which, perm):
+        perm = np.ascontiguousarray(perm, dtype=np.int32)
+        rc = lib.hmvm_plan_set_tile_order(self._p, which, perm.ctypes.data_as(c_int_p))
+        if rc:
+            raise HMVMError("hmvm_plan_set_tile_order: %s" % lib.hmvm_hip_last_error_message().decode())
+
+    def time_graph(self, count=1, warmup=3, iters=20):
+        """Timing of hipGraph replays of ``count`` applies (per apply)."""
+        t = hmvm_timing()
+        rc = lib.hmvm_plan_time_graph(self._p, count, warmup, iters, ctypes.byref(t))
+        if rc:
+            raise HMVMError("hmvm_plan_time_graph: %s" % lib.hmvm_hip_last_error_message().decode())
+        return t.as_dict()
+
+    def graph(self, d_x, d_y, accumulate=False, count=1):
+        """hipGraph of ``count`` applies on fixed device vectors; returns a Graph."""
+        g = ctypes.c_void_p()
+        rc = lib.hmvm_plan_graph_create(self._p, ctypes.c_void_p(d_x), ctypes.c_void_p(d_y), int(bool(accumulate)),
+                                        count, ctypes.byref(g))
+        if rc:
+            raise HMVMError("hmvm_plan_graph_create: %s" % lib.hmvm_hip_last_error_message().decode())
+        return Graph(g)
 
     def set_mode(self, mode):
         rc = lib.hmvm_plan_set_mode(self._p, mode)
@@ -576,3 +814,125 @@ class Plan:
         if rc:
             raise HMVMError("hmvm_plan_time: %s" % lib.hmvm_hip_last_error_message().decode())
         return t.as_dict()
+
+
+class Graph:
+    """A captured hipGraph of plan applies (see Plan.graph)."""
+
+    def __init__(self, handle):
+        self._g = handle
+
+    def launch(self, stream=None):
+        rc = lib.hmvm_graph_launch(self._g, ctypes.c_void_p(stream) if stream else None)
+        if rc:
+            raise HMVMError("hmvm_graph_launch: %s" % lib.hmvm_hip_last_error_message().decode())
+
+    def close(self):
+        if getattr(self, "_g", None) and self._g.value:
+            lib.hmvm_graph_destroy(self._g)
+            self._g = ctypes.c_void_p()
+
+    def __del__(self):
+        self.close()
+
+
+class DeviceBuffer:
+    """A device allocation through the library's helpers (for device-resident tests and benchmarks)."""
+
+    def __init__(self, nbytes):
+        self.ptr = ctypes.c_void_p()
+        self.nbytes = int(nbytes)
+        if lib.hmvm_device_alloc(ctypes.byref(self.ptr), self.nbytes):
+            raise HMVMError("hmvm_device_alloc: %s" % lib.hmvm_hip_last_error_message().decode())
+
+    def upload(self, a):
+        a = np.ascontiguousarray(a)
+        if a.nbytes > self.nbytes or lib.hmvm_memcpy_h2d(self.ptr, a.ctypes.data_as(ctypes.c_void_p), a.nbytes):
+            raise HMVMError("upload failed")
+
+    def download(self, a):
+        if a.nbytes > self.nbytes or lib.hmvm_memcpy_d2h(a.ctypes.data_as(ctypes.c_void_p), self.ptr, a.nbytes):
+            raise HMVMError("download failed")
+        return a
+
+    @property
+    def value(self):
+        return self.ptr.value
+
+    def close(self):
+        if getattr(self, "ptr", None) and self.ptr.value:
+            lib.hmvm_device_free(self.ptr)
+            self.ptr = ctypes.c_void_p()
+
+    def __del__(self):
+        self.close()
+
+
+# ---- the learned cost model -----------------------------------------------------------
+def _fptr(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+
+
+class CostMLP:
+    """nin -> nh -> nh -> 1 ReLU MLP whose training and inference run as HIP kernels."""
+
+    def __init__(self, nin=HMVM_NFEAT, nh=64, seed=1):
+        self._m = ctypes.c_void_p()
+        rc = lib.hmvm_mlp_create(nin, nh, seed, ctypes.byref(self._m))
+        if rc:
+            raise HMVMError("hmvm_mlp_create failed (code %d)" % rc)
+        self.nin, self.nh = nin, nh
+
+    def close(self):
+        if getattr(self, "_m", None) and self._m.value:
+            lib.hmvm_mlp_destroy(self._m)
+            self._m = ctypes.c_void_p()
+
+    def __del__(self):
+        self.close()
+
+    @property
+    def nparams(self):
+        return lib.hmvm_mlp_num_params(self._m)
+
+    def get(self):
+        p = np.zeros(self.nparams, dtype=np.float32)
+        mu = np.zeros(self.nin, dtype=np.float32)
+        sd = np.zeros(self.nin, dtype=np.float32)
+        lib.hmvm_mlp_get_params(self._m, _fptr(p), _fptr(mu), _fptr(sd))
+        return p, mu, sd
+
+    def set(self, params=None, mean=None, stdev=None):
+        arrs = [None if a is None else np.ascontiguousarray(a, dtype=np.float32) for a in (params, mean, stdev)]
+        lib.hmvm_mlp_set_params(self._m, *[None if a is None else _fptr(a) for a in arrs])
+
+    def fit_scaler(self, X):
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        lib.hmvm_mlp_fit_scaler(self._m, _fptr(X), X.shape[0])
+
+    def predict(self, X):
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        y = np.zeros(X.shape[0], dtype=np.float32)
+        rc = lib.hmvm_mlp_predict(self._m, _fptr(X), X.shape[0], _fptr(y))
+        if rc:
+            raise HMVMError("hmvm_mlp_predict failed (code %d)" % rc)
+        return y
+
+    def loss_grad(self, X, y):
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        y = np.ascontiguousarray(y, dtype=np.float32)
+        loss = ctypes.c_float()
+        g = np.zeros(self.nparams, dtype=np.float32)
+        rc = lib.hmvm_mlp_loss_grad(self._m, _fptr(X), _fptr(y), X.shape[0], ctypes.byref(loss), _fptr(g))
+        if rc:
+            raise HMVMError("hmvm_mlp_loss_grad failed (code %d)" % rc)
+        return loss.value, g
+
+    def train(self, X, y, epochs=100, batch=1024, lr=1e-3, seed=7):
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        y = np.ascontiguousarray(y, dtype=np.float32)
+        loss = ctypes.c_float()
+        rc = lib.hmvm_mlp_train(self._m, _fptr(X), _fptr(y), X.shape[0], epochs, batch, lr, seed, ctypes.byref(loss))
+        if rc:
+            raise HMVMError("hmvm_mlp_train failed (code %d)" % rc)
+        return loss.value

@@ -25,8 +25,8 @@ CXXFLAGS = ["-O3", "-fPIC", "-fopenmp", "-std=c++17", "-Wall", "-I" + INC]
 HIPFLAGS = ["-O3", "-fPIC", "-std=c++17", "--offload-arch=gfx950", "-I" + INC, "-I" + CSRC,
             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-Wall"]
 
-CPP_SOURCES = ["hacapk_io.cpp", "hgen.cpp", "hmvm_plan.cpp", "hmvm_cpp_api.cpp"]
-HIP_SOURCES = ["hmvm_runtime.hip", "hmvm_krylov.hip"]
+CPP_SOURCES = ["hacapk_io.cpp", "hacapk_io64.cpp", "hgen.cpp", "hmvm_plan.cpp", "hmvm_cpp_api.cpp"]
+HIP_SOURCES = ["hmvm_runtime.hip", "hmvm_krylov.hip", "hmvm_mlp.hip"]
 
 
 def _run(cmd):

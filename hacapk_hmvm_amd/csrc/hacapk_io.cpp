@@ -20,66 +20,11 @@
 #include <vector>
 
 #include "hacapk.h"
+#include "io_records.hpp"
+
+using namespace hacapk_io;
 
 namespace {
-
-struct Reader {
-  FILE *f = nullptr;
-  explicit Reader(const char *name) { f = std::fopen(name, "rb"); }
-  ~Reader() {
-    if (f) std::fclose(f);
-  }
-  // Reads one record whose payload must be exactly `bytes` long.
-  int record(void *dst, size_t bytes) {
-    int32_t m0 = 0, m1 = 0;
-    if (std::fread(&m0, 4, 1, f) != 1) return HACAPK_ERR_TRUNCATED;
-    if (m0 < 0 || (size_t)(uint32_t)m0 != bytes) return HACAPK_ERR_MARKER;
-    if (bytes && std::fread(dst, 1, bytes, f) != bytes) return HACAPK_ERR_TRUNCATED;
-    if (std::fread(&m1, 4, 1, f) != 1) return HACAPK_ERR_TRUNCATED;
-    if (m1 != m0) return HACAPK_ERR_MARKER;
-    return HACAPK_OK;
-  }
-  // Reads a record header marker only (used for the variable header record).
-  int marker(int32_t *m) {
-    if (std::fread(m, 4, 1, f) != 1) return HACAPK_ERR_TRUNCATED;
-    return HACAPK_OK;
-  }
-};
-
-struct Writer {
-  FILE *f = nullptr;
-  bool ok = true;
-  explicit Writer(const char *name) { f = std::fopen(name, "wb"); }
-  ~Writer() {
-    if (f) std::fclose(f);
-  }
-  void record(const void *src, size_t bytes) {
-    if (bytes > (size_t)INT32_MAX) { ok = false; return; }
-    int32_t m = (int32_t)bytes;
-    ok = ok && std::fwrite(&m, 4, 1, f) == 1;
-    if (bytes) ok = ok && std::fwrite(src, 1, bytes, f) == bytes;
-    ok = ok && std::fwrite(&m, 4, 1, f) == 1;
-  }
-  int close() {
-    int rc = std::fclose(f);
-    f = nullptr;
-    return (ok && rc == 0) ? HACAPK_OK : HACAPK_ERR_OPEN;
-  }
-};
-
-// Element counts of one leaf (reference src_c/loadmatrix.c:70,84).
-inline int64_t leaf_len(int ltmtx, int kt, int ndl, int ndt) {
-  return ltmtx == 1 ? (int64_t)kt * ndt + (int64_t)kt * ndl : (int64_t)ndl * ndt;
-}
-
-int check_leaf(int nd, int ktmax, int ltmtx, int kt, int ndl, int ndt, int nstrtl, int nstrtt) {
-  if (ltmtx != 1 && ltmtx != 2) return HACAPK_ERR_LEAF;
-  if (ndl < 1 || ndt < 1) return HACAPK_ERR_LEAF;
-  if (nstrtl < 1 || nstrtt < 1) return HACAPK_ERR_LEAF;
-  if ((int64_t)nstrtl + ndl - 1 > nd || (int64_t)nstrtt + ndt - 1 > nd) return HACAPK_ERR_LEAF;
-  if (ltmtx == 1 && (kt < 0 || kt > ktmax)) return HACAPK_ERR_LEAF;
-  return HACAPK_OK;
-}
 
 template <class P>
 P *alloc_n(int64_t n) {
@@ -100,12 +45,6 @@ int alloc_index_arrays(matrix2 *m, int nlf) {
   if (!m->ltmtx || !m->kt || !m->ndl || !m->ndt || !m->nstrtl || !m->nstrtt || !m->a1 || !m->a2)
     return HACAPK_ERR_NOMEM;
   return HACAPK_OK;
-}
-
-// a2 (kt x ndl row-major) -> a2t (ndl x kt row-major).
-void transpose_a2(const double *a2, double *a2t, int kt, int ndl) {
-  for (int k = 0; k < kt; k++)
-    for (int il = 0; il < ndl; il++) a2t[k + il * kt] = a2[k * ndl + il];
 }
 
 }  // namespace

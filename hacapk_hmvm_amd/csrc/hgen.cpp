@@ -306,7 +306,11 @@ void fill_leaf(const Problem &pb, Leaf &lf) {
   }
 }
 
-int pack_matrix2(const Problem &pb, std::vector<Leaf> &leaves, matrix2 *m) {
+// Packs the filled leaves in leaf order as reference src_c/loadmatrix.c:88-128
+// does (a1 part, then a2 = a1 + kt*ndt; rowmat_t holds a2t).  M is matrix2
+// (int offsets, rowmat_t always built) or matrix2_64 (rowmat_t optional).
+template <class M, class Off>
+int pack_leaves(const Problem &pb, std::vector<Leaf> &leaves, M *m, bool with_t, int64_t max_len) {
   std::memset(m, 0, sizeof(*m));
   const int nlf = (int)leaves.size();
   int64_t len = 0;
@@ -319,21 +323,20 @@ int pack_matrix2(const Problem &pb, std::vector<Leaf> &leaves, matrix2 *m) {
       len += (int64_t)lf.ndl * lf.ndt;
     }
   }
-  if (len > INT_MAX) return HACAPK_ERR_OVERFLOW;
+  if (len > max_len) return HACAPK_ERR_OVERFLOW;
   m->nd = pb.p.n;
   m->nlf = nlf;
   m->ktmax = ktmax;
-  m->len = (int)len;
+  m->len = len;
   auto ia = [&]() { return (int *)std::malloc(sizeof(int) * (size_t)nlf); };
+  auto oa = [&]() { return (Off *)std::malloc(sizeof(Off) * (size_t)nlf); };
   m->ltmtx = ia(); m->kt = ia(); m->ndl = ia(); m->ndt = ia();
-  m->nstrtl = ia(); m->nstrtt = ia(); m->a1 = ia(); m->a2 = ia();
+  m->nstrtl = ia(); m->nstrtt = ia(); m->a1 = oa(); m->a2 = oa();
   m->rowmat = (double *)std::malloc(sizeof(double) * (size_t)std::max<int64_t>(len, 1));
-  m->rowmat_t = (double *)std::malloc(sizeof(double) * (size_t)std::max<int64_t>(len, 1));
+  if (with_t) m->rowmat_t = (double *)std::malloc(sizeof(double) * (size_t)std::max<int64_t>(len, 1));
   if (!m->ltmtx || !m->kt || !m->ndl || !m->ndt || !m->nstrtl || !m->nstrtt || !m->a1 || !m->a2 ||
-      !m->rowmat || !m->rowmat_t) {
-    freeHmatrix(nullptr, m);
+      !m->rowmat || (with_t && !m->rowmat_t))
     return HACAPK_ERR_NOMEM;
-  }
   int64_t off = 0;
   for (int i = 0; i < nlf; i++) {
     Leaf &lf = leaves[i];
@@ -343,21 +346,23 @@ int pack_matrix2(const Problem &pb, std::vector<Leaf> &leaves, matrix2 *m) {
     m->ndt[i] = lf.ndt;
     m->nstrtl[i] = lf.nstrtl;
     m->nstrtt[i] = lf.nstrtt;
-    m->a1[i] = (int)off;
+    m->a1[i] = (Off)off;
     if (lf.ltmtx == 1) {
       const size_t n1 = (size_t)lf.kt * lf.ndt, n2 = (size_t)lf.kt * lf.ndl;
       std::memcpy(m->rowmat + off, lf.a1.data(), sizeof(double) * n1);
-      std::memcpy(m->rowmat_t + off, lf.a1.data(), sizeof(double) * n1);
+      if (with_t) std::memcpy(m->rowmat_t + off, lf.a1.data(), sizeof(double) * n1);
       off += (int64_t)n1;
-      m->a2[i] = (int)off;
+      m->a2[i] = (Off)off;
       std::memcpy(m->rowmat + off, lf.a2.data(), sizeof(double) * n2);
-      for (int k = 0; k < lf.kt; k++)
-        for (int il = 0; il < lf.ndl; il++) m->rowmat_t[off + k + (int64_t)il * lf.kt] = lf.a2[(size_t)k * lf.ndl + il];
+      if (with_t)
+        for (int k = 0; k < lf.kt; k++)
+          for (int il = 0; il < lf.ndl; il++)
+            m->rowmat_t[off + k + (int64_t)il * lf.kt] = lf.a2[(size_t)k * lf.ndl + il];
       off += (int64_t)n2;
     } else {
       const size_t n1 = (size_t)lf.ndl * lf.ndt;
       std::memcpy(m->rowmat + off, lf.a1.data(), sizeof(double) * n1);
-      std::memcpy(m->rowmat_t + off, lf.a1.data(), sizeof(double) * n1);
+      if (with_t) std::memcpy(m->rowmat_t + off, lf.a1.data(), sizeof(double) * n1);
       off += (int64_t)n1;
       m->a2[i] = 0;
     }
@@ -365,6 +370,55 @@ int pack_matrix2(const Problem &pb, std::vector<Leaf> &leaves, matrix2 *m) {
     std::vector<double>().swap(lf.a2);
   }
   return HACAPK_OK;
+}
+
+// Cluster tree, block tree, leaf sort and the parallel fill
+// (largest leaves first for the dynamic schedule).
+int build_leaves(const hgen_params *p, Problem &pb, std::vector<Leaf> &leaves) {
+  int rc = make_problem(p, pb);
+  if (rc) return rc;
+  build_blocks(pb, 0, 0, leaves);
+  std::sort(leaves.begin(), leaves.end(), [](const Leaf &a, const Leaf &b) {
+    return a.nstrtl != b.nstrtl ? a.nstrtl < b.nstrtl : a.nstrtt < b.nstrtt;
+  });
+  std::vector<int> order(leaves.size());
+  for (size_t i = 0; i < order.size(); i++) order[i] = (int)i;
+  std::sort(order.begin(), order.end(), [&](int a, int b) {
+    const int64_t wa = (int64_t)leaves[a].ndl * leaves[a].ndt, wb = (int64_t)leaves[b].ndl * leaves[b].ndt;
+    return wa != wb ? wa > wb : a < b;
+  });
+  const int nl = (int)leaves.size();
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int q = 0; q < nl; q++) fill_leaf(pb, leaves[order[q]]);
+  return HACAPK_OK;
+}
+
+// ||A - H||_F / ||A||_F over every stride-th leaf (HACApK_accuracy_leafmtx,
+// m_HACApK_base.f90:407-466), entries of A recomputed.
+template <class M>
+double accuracy(const Problem &pb, const M *m, int stride) {
+  if (stride < 1) stride = 1;
+  double num = 0, den = 0;
+#pragma omp parallel for schedule(dynamic, 4) reduction(+ : num, den)
+  for (int i = 0; i < m->nlf; i += stride) {
+    const int ndl = m->ndl[i], ndt = m->ndt[i], r0 = m->nstrtl[i] - 1, c0 = m->nstrtt[i] - 1;
+    const double *a1 = m->rowmat + m->a1[i];
+    for (int il = 0; il < ndl; il++)
+      for (int it = 0; it < ndt; it++) {
+        const double a = pb.entry(r0 + il, c0 + it);
+        double h;
+        if (m->ltmtx[i] == 1) {
+          const double *a2 = m->rowmat + m->a2[i];
+          h = 0;
+          for (int k = 0; k < m->kt[i]; k++) h += a2[(size_t)k * ndl + il] * a1[(size_t)k * ndt + it];
+        } else {
+          h = a1[(size_t)il * ndt + it];
+        }
+        num += (a - h) * (a - h);
+        den += a * a;
+      }
+  }
+  return den > 0 ? std::sqrt(num / den) : 0.0;
 }
 
 }  // namespace
@@ -387,24 +441,21 @@ void hgen_default_params(hgen_params *p, int n) {
 int hgen_generate(const hgen_params *p, matrix2 *mat2) {
   if (!mat2) return HACAPK_ERR_ARG;
   Problem pb;
-  int rc = make_problem(p, pb);
-  if (rc) return rc;
   std::vector<Leaf> leaves;
-  build_blocks(pb, 0, 0, leaves);
-  std::sort(leaves.begin(), leaves.end(), [](const Leaf &a, const Leaf &b) {
-    return a.nstrtl != b.nstrtl ? a.nstrtl < b.nstrtl : a.nstrtt < b.nstrtt;
-  });
-  // Largest leaves first for the dynamic schedule.
-  std::vector<int> order(leaves.size());
-  for (size_t i = 0; i < order.size(); i++) order[i] = (int)i;
-  std::sort(order.begin(), order.end(), [&](int a, int b) {
-    const int64_t wa = (int64_t)leaves[a].ndl * leaves[a].ndt, wb = (int64_t)leaves[b].ndl * leaves[b].ndt;
-    return wa != wb ? wa > wb : a < b;
-  });
-  const int nl = (int)leaves.size();
-#pragma omp parallel for schedule(dynamic, 1)
-  for (int q = 0; q < nl; q++) fill_leaf(pb, leaves[order[q]]);
-  return pack_matrix2(pb, leaves, mat2);
+  int rc = build_leaves(p, pb, leaves);
+  if (!rc) rc = pack_leaves<matrix2, int>(pb, leaves, mat2, true, INT_MAX);
+  if (rc) freeHmatrix(nullptr, mat2);
+  return rc;
+}
+
+int hgen_generate64(const hgen_params *p, matrix2_64 *mat2, int flags) {
+  if (!mat2) return HACAPK_ERR_ARG;
+  Problem pb;
+  std::vector<Leaf> leaves;
+  int rc = build_leaves(p, pb, leaves);
+  if (!rc) rc = pack_leaves<matrix2_64, long long>(pb, leaves, mat2, (flags & HACAPK_WITH_ROWMAT_T) != 0, INT64_MAX);
+  if (rc) freeHmatrix64(mat2);
+  return rc;
 }
 
 int hgen_matrix_from_matrix2(const matrix2 *m, matrix *mat) {
@@ -463,29 +514,14 @@ int hgen_kernel_block(const hgen_params *p, int i0, int nr, int j0, int nc, doub
 
 double hgen_accuracy(const hgen_params *p, const matrix2 *m, int stride) {
   Problem pb;
-  if (make_problem(p, pb) || !m) return -1.0;
-  if (stride < 1) stride = 1;
-  double num = 0, den = 0;
-#pragma omp parallel for schedule(dynamic, 4) reduction(+ : num, den)
-  for (int i = 0; i < m->nlf; i += stride) {
-    const int ndl = m->ndl[i], ndt = m->ndt[i], r0 = m->nstrtl[i] - 1, c0 = m->nstrtt[i] - 1;
-    const double *a1 = m->rowmat + m->a1[i];
-    for (int il = 0; il < ndl; il++)
-      for (int it = 0; it < ndt; it++) {
-        const double a = pb.entry(r0 + il, c0 + it);
-        double h;
-        if (m->ltmtx[i] == 1) {
-          const double *a2 = m->rowmat + m->a2[i];
-          h = 0;
-          for (int k = 0; k < m->kt[i]; k++) h += a2[(size_t)k * ndl + il] * a1[(size_t)k * ndt + it];
-        } else {
-          h = a1[(size_t)il * ndt + it];
-        }
-        num += (a - h) * (a - h);
-        den += a * a;
-      }
-  }
-  return den > 0 ? std::sqrt(num / den) : 0.0;
+  if (!m || make_problem(p, pb)) return -1.0;
+  return accuracy(pb, m, stride);
+}
+
+double hgen_accuracy64(const hgen_params *p, const matrix2_64 *m, int stride) {
+  Problem pb;
+  if (!m || make_problem(p, pb)) return -1.0;
+  return accuracy(pb, m, stride);
 }
 
 }  // extern "C"

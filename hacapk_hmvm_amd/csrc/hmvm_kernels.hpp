@@ -352,6 +352,34 @@ __global__ __launch_bounds__(256) void tile_kernel(const TileDesc *__restrict__ 
   run_tile<S, kScaled, kLoads, kPipe, kNT, false>(td, panels, vals, x, t, tp, scales, out, lane);
 }
 
+// tile_kernel with lane 0 of every wave recording the device wall clock
+// (constant rate, read with s_memrealtime) before and after its tile: the
+// in-situ per-tile durations the cost model learns from.  Same results as
+// tile_kernel.
+template <typename S, bool kScaled, int kLoads, bool kPipe, bool kNT>
+__global__ __launch_bounds__(256) void tile_kernel_timed(const TileDesc *__restrict__ tiles, int ntiles,
+                                                         const PanelDesc *__restrict__ panels,
+                                                         const S *__restrict__ vals,
+                                                         const double *__restrict__ x,
+                                                         const double *__restrict__ t,
+                                                         const double *__restrict__ tp,
+                                                         const float *__restrict__ scales,
+                                                         double *__restrict__ out,
+                                                         long long *__restrict__ stamps) {
+  const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  const int lane = (int)(threadIdx.x & 63);
+  const int tid = (int)blockIdx.x * (int)(blockDim.x >> 6) + wave;
+  if (tid >= ntiles) return;
+  const long long t0 = (long long)wall_clock64();
+  const TileDesc td = tiles[tid];
+  run_tile<S, kScaled, kLoads, kPipe, kNT, false>(td, panels, vals, x, t, tp, scales, out, lane);
+  const long long t1 = (long long)wall_clock64();
+  if (lane == 0) {
+    stamps[2 * (int64_t)tid] = t0;
+    stamps[2 * (int64_t)tid + 1] = t1;
+  }
+}
+
 // ---- persistent single-launch mode ------------------------------------------
 // Work items, in queue order: stage-1 tiles [0, n1), dense tiles [n1, na),
 // stage-2 tiles [na, nt), y-sum items of 64 rows [nt, nitems).  A wave takes

@@ -65,7 +65,7 @@ std::string build_plan(const LeafArrays &la, const PlanOptions &opt, Plan *plan)
   P.nd = la.nd;
   P.nlf = la.nlf;
   if (la.nd < 1 || la.nlf < 1) return "empty matrix";
-  if (opt.w1 < 64 || opt.w2 < 64 || opt.align < 1) return "bad plan options";
+  if (opt.w1 < 64 || opt.w2 < 64 || opt.align < 1 || opt.t_mode < 0 || opt.t_mode > 2) return "bad plan options";
   if (la.nd > (1 << 30)) return "matrix order too large";
   const int nlf = la.nlf;
 
@@ -107,6 +107,7 @@ std::string build_plan(const LeafArrays &la, const PlanOptions &opt, Plan *plan)
   std::vector<int64_t> t_idx(nlf, -1);     // index of t_b[0] in the materialised t
   std::vector<int64_t> tp_off(nlf, -1);    // partial slot of t_b[0], chunk 0
   std::vector<int32_t> tp_chunks(nlf, 0);  // (nch << 16) | stride
+  std::vector<char> forced(nlf, 0);        // stage 2 of this leaf must read the materialised t
   std::vector<int64_t> dbase(nlf, -1);     // device base of a2 / dense factor
   int64_t off = 0;
   std::vector<TileDesc> t1, td, t2;        // stage-1, dense, stage-2 tiles
@@ -193,14 +194,14 @@ std::string build_plan(const LeafArrays &la, const PlanOptions &opt, Plan *plan)
       for (int k = 0; k < nr; k++) P.tsum.seg.push_back(seg);
       for (const Piece &pc : groups[gi]) {
         if (pc.k0 != 0 || pc.n != la.kt[pc.leaf]) {
-          P.t_fused = false;  // rank rows of this leaf straddle two row groups
+          forced[pc.leaf] = 1;  // rank rows of this leaf straddle two row groups
         } else {
           int kk0 = -1;
           for (size_t q = 0; q < ls.size(); q++)
             if (ls[q] == pc.leaf) kk0 = kofs_of[q] - k0;
           tp_off[pc.leaf] = slot0 + kk0;
           tp_chunks[pc.leaf] = (nch << 16) | nr;
-          if (nch >= (1 << 15) || nr >= (1 << 16)) P.t_fused = false;
+          if (nch >= (1 << 15) || nr >= (1 << 16)) forced[pc.leaf] = 1;
         }
       }
       k0 += nr;
@@ -302,16 +303,30 @@ std::string build_plan(const LeafArrays &la, const PlanOptions &opt, Plan *plan)
   if (P.n_tp + P.n_z > INT32_MAX) return "output slots exceed int32";
   if (P.panels.size() > (size_t)INT32_MAX) return "too many panels";
   P.nvals = align_up(off, opt.align);
-  // a leaf whose t could not be summed on the fly reads the materialised t
-  if (!P.t_fused) {
-    for (PanelDesc &pd : P.panels)
-      if (pd.vec_sel == kVecTP) {
-        const int i = pd.factor / 2;
-        pd.vec_sel = kVecT;
-        pd.vec_off = (int32_t)t_idx[i];
-        pd.vec_chunks = 0;
-      }
+  // Which stage-2 panels read the materialised t (computed by the t segment
+  // sum) instead of summing the stage-1 partial slots themselves:
+  //   t_mode 0: none, unless some leaf cannot be summed on the fly; then all
+  //   t_mode 1: all
+  //   t_mode 2: the leaves that cannot, and those whose partial sums would be
+  //             repeated by several row tiles or span more than two chunks
+  bool any_forced = false;
+  for (int i = 0; i < nlf; i++) any_forced = any_forced || forced[i];
+  std::vector<char> mat(nlf, 0);
+  for (int i = 0; i < nlf; i++) {
+    if (la.ltmtx[i] != 1) continue;
+    if (opt.t_mode == 1 || (opt.t_mode == 0 && any_forced) || forced[i])
+      mat[i] = 1;
+    else if (opt.t_mode == 2)
+      mat[i] = la.ndl[i] > kMaxTileRows || (tp_chunks[i] >> 16) > 2;
   }
+  for (PanelDesc &pd : P.panels)
+    if (pd.vec_sel == kVecTP && mat[pd.factor / 2]) {
+      const int i = pd.factor / 2;
+      pd.vec_sel = kVecT;
+      pd.vec_off = (int32_t)t_idx[i];
+      pd.vec_chunks = 0;
+      P.t_fused = false;
+    }
   P.elems_a = P.elems_stage1 + P.elems_dense;
   P.elems_b = P.elems_stage2;
 

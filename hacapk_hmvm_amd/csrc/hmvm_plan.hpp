@@ -89,6 +89,7 @@ struct PlanOptions {
   int w2 = 16384;         // target elements per dense / stage-2 tile
   int align = 16;         // factor start alignment in elements
   int order = 0;          // 0: memory order, 1: largest tiles first
+  int t_mode = 0;         // stage-2 vector: 0 partial sums on the fly, 1 materialised t, 2 per leaf
 };
 
 struct LeafArrays {       // the caller's descriptors (matrix2 or matrix)
@@ -110,7 +111,7 @@ struct Plan {
   int64_t nvals = 0;                 // device value buffer (elements)
   int64_t len = 0;                   // reference element count
   int nfactors = 0;
-  bool t_fused = true;               // stage 2 sums the partial slots itself
+  bool t_fused = true;               // no stage-2 panel reads the materialised t (no t sum launch)
   std::vector<RepackDesc> repack;
   std::vector<int32_t> factor_of_trow;  // t index -> a1 factor id (BF16 t scaling)
   std::vector<TileDesc> tiles_a, tiles_b;  // tiles_a: stage-1 tiles first, then dense

@@ -17,6 +17,7 @@
 #include "hmvm_hip.h"
 #include "hmvm_kernels.hpp"
 #include "hmvm_plan.hpp"
+#include "io_records.hpp"
 
 using namespace hmvm;
 
@@ -86,8 +87,10 @@ struct hmvm_plan {
   Plan plan;
   int storage = kF64;
   int block_threads = 256;
-  int kernel = 0;  // tile-kernel variant, 0 .. kNumKernelVariants-1
+  int kernel = 0;  // tile-kernel variant, 0 .. kNumKernelVariants-1 (launch A, persistent, multi)
+  int kernel_b = 0, block_threads_b = 256;  // launch B
   int mode = 0;    // 0: launches A, [t sum], B, y sum; 1: one persistent launch
+  bool a_mixed = false;  // launch A order mixes stage-1 and dense tiles (no persistent mode)
   int persist_key = -1, persist_blocks = 0;  // occupancy cache (kernel variant, block size)
   // several right-hand sides: buffers sized for multi_cap columns
   int multi_cap = 0;
@@ -120,13 +123,22 @@ struct hmvm_plan {
 
 namespace {
 
-// Source view of the caller's matrix: descriptor arrays + one contiguous FP64
-// value buffer in host memory (rowmat, rowmat_t or a gathered copy of the
-// per-leaf arrays).
+// Supplies elements [w0, w1) of a source value buffer that is not one
+// contiguous host array (a file being streamed, several matrices of a batch).
+// Calls come in increasing w0 order and never overlap.
+struct Feeder {
+  virtual ~Feeder() = default;
+  virtual int fill(double *dst, int64_t w0, int64_t w1) = 0;
+};
+
+// Source view of the caller's matrix: descriptor arrays + the FP64 values,
+// either one contiguous host buffer (rowmat, rowmat_t or a gathered copy of
+// the per-leaf arrays) or a feeder.
 struct Source {
   LeafArrays la;
   std::vector<int64_t> a1, a2;
   const double *vals = nullptr;
+  Feeder *feeder = nullptr;
   std::vector<double> gathered;
   std::vector<int> ltmtx, kt, ndl, ndt, nstrtl, nstrtt;
 };
@@ -216,14 +228,135 @@ int source_from_matrix(const matrix *m, int a2_source, Source *s) {
   return HMVM_OK;
 }
 
-PlanOptions plan_options(const hmvm_options *o) {
-  PlanOptions po;
-  if (o) {
-    po.storage = o->storage;
-    if (o->w1 > 0) po.w1 = o->w1;
-    if (o->w2 > 0) po.w2 = o->w2;
-    po.order = o->order;
+int source_from_matrix2_64(const matrix2_64 *m, int a2_source, Source *s) {
+  if (!m || m->nd < 1 || m->nlf < 1 || !m->ltmtx || !m->kt || !m->ndl || !m->ndt || !m->nstrtl ||
+      !m->nstrtt || !m->a1 || !m->a2)
+    return set_error(HMVM_ERR_ARG, "matrix2_64 is NULL or incomplete");
+  const double *vals = a2_source == kA2Transposed ? m->rowmat_t : m->rowmat;
+  if (!vals) return set_error(HMVM_ERR_ARG, "matrix2_64 has no %s", a2_source ? "rowmat_t" : "rowmat");
+  s->a1.assign(m->a1, m->a1 + m->nlf);
+  s->a2.assign(m->a2, m->a2 + m->nlf);
+  LeafArrays &la = s->la;
+  la.nd = m->nd;
+  la.nlf = m->nlf;
+  la.ltmtx = m->ltmtx; la.kt = m->kt; la.ndl = m->ndl; la.ndt = m->ndt;
+  la.nstrtl = m->nstrtl; la.nstrtt = m->nstrtt;
+  la.src_a1 = s->a1.data();
+  la.src_a2 = s->a2.data();
+  la.a2_source = a2_source;
+  la.src_len = m->len;
+  s->vals = vals;
+  return HMVM_OK;
+}
+
+// Streams the leaf records of a format 1/2/2L file (scanned beforehand) in
+// file order; the file order is the packing order of the scan's offsets.
+struct FileFeeder : Feeder {
+  hacapk_io::Reader rd;
+  hacapk_io::LeafStream ls;
+  const matrix2_64 *m;
+  int leaf = 0;
+  FileFeeder(const char *fname, int format, const matrix2_64 *mm) : rd(fname), ls(rd, format), m(mm) {}
+  int begin(long long data_pos) { return rd.f ? ls.begin(data_pos) : HACAPK_ERR_OPEN; }
+  int fill(double *dst, int64_t w0, int64_t w1) override {
+    while (leaf < m->nlf && m->a1[leaf] < w1) {
+      const int64_t n = hacapk_io::leaf_len(m->ltmtx[leaf], m->kt[leaf], m->ndl[leaf], m->ndt[leaf]);
+      if (m->a1[leaf] < w0 || m->a1[leaf] + n > w1)
+        return set_error(HMVM_ERR_MATRIX, "leaf %d does not fit the upload window", leaf);
+      int rc = ls.read(m, leaf, dst + (m->a1[leaf] - w0));
+      if (rc) return set_error(HMVM_ERR_MATRIX, "reading leaf %d: %s", leaf, hacapk_strerror(rc));
+      leaf++;
+    }
+    return HMVM_OK;
   }
+};
+
+// Several matrices laid end to end: segment q holds elements
+// [base[q], base[q] + len[q]) of the virtual buffer.
+struct ConcatFeeder : Feeder {
+  std::vector<const double *> ptr;
+  std::vector<int64_t> base, len;
+  int fill(double *dst, int64_t w0, int64_t w1) override {
+    for (size_t q = 0; q < ptr.size(); q++) {
+      const int64_t a = std::max(w0, base[q]), b = std::min(w1, base[q] + len[q]);
+      if (a < b) std::memcpy(dst + (a - w0), ptr[q] + (a - base[q]), sizeof(double) * (size_t)(b - a));
+    }
+    return HMVM_OK;
+  }
+};
+
+// Block-diagonal union of several matrices: matrix q occupies rows and
+// columns [row0[q], row0[q] + nd_q) and its values follow those of matrix q-1.
+int source_from_batch(const matrix2 *const *mats, int count, int a2_source, Source *s, ConcatFeeder *feed,
+                      std::vector<int> *row0) {
+  if (!mats || count < 1) return set_error(HMVM_ERR_ARG, "empty batch");
+  int64_t nd = 0, nlf = 0, len = 0;
+  for (int q = 0; q < count; q++) {
+    const matrix2 *m = mats[q];
+    if (!m || m->nd < 1 || m->nlf < 1 || !m->ltmtx || !m->kt || !m->ndl || !m->ndt || !m->nstrtl || !m->nstrtt ||
+        !m->a1 || !m->a2)
+      return set_error(HMVM_ERR_ARG, "batch matrix %d is NULL or incomplete", q);
+    if (!(a2_source == kA2Transposed ? m->rowmat_t : m->rowmat))
+      return set_error(HMVM_ERR_ARG, "batch matrix %d has no values", q);
+    nd += m->nd;
+    nlf += m->nlf;
+    len += m->len;
+  }
+  if (nd > INT32_MAX / 2 || nlf > INT32_MAX / 2) return set_error(HMVM_ERR_ARG, "batch too large");
+  const size_t n = (size_t)nlf;
+  s->ltmtx.resize(n); s->kt.resize(n); s->ndl.resize(n); s->ndt.resize(n);
+  s->nstrtl.resize(n); s->nstrtt.resize(n); s->a1.resize(n); s->a2.resize(n);
+  row0->assign((size_t)count + 1, 0);
+  int64_t r0 = 0, v0 = 0;
+  size_t i = 0;
+  for (int q = 0; q < count; q++) {
+    const matrix2 *m = mats[q];
+    (*row0)[q] = (int)r0;
+    for (int j = 0; j < m->nlf; j++, i++) {
+      s->ltmtx[i] = m->ltmtx[j]; s->kt[i] = m->kt[j]; s->ndl[i] = m->ndl[j]; s->ndt[i] = m->ndt[j];
+      s->nstrtl[i] = m->nstrtl[j] + (int)r0;
+      s->nstrtt[i] = m->nstrtt[j] + (int)r0;
+      s->a1[i] = m->a1[j] + v0;
+      s->a2[i] = m->ltmtx[j] == 1 ? m->a2[j] + v0 : 0;
+    }
+    feed->ptr.push_back(a2_source == kA2Transposed ? m->rowmat_t : m->rowmat);
+    feed->base.push_back(v0);
+    feed->len.push_back(m->len);
+    r0 += m->nd;
+    v0 += m->len;
+  }
+  (*row0)[count] = (int)r0;
+  LeafArrays &la = s->la;
+  la.nd = (int)nd;
+  la.nlf = (int)nlf;
+  la.ltmtx = s->ltmtx.data(); la.kt = s->kt.data(); la.ndl = s->ndl.data(); la.ndt = s->ndt.data();
+  la.nstrtl = s->nstrtl.data(); la.nstrtt = s->nstrtt.data();
+  la.src_a1 = s->a1.data();
+  la.src_a2 = s->a2.data();
+  la.a2_source = a2_source;
+  la.src_len = len;
+  s->feeder = feed;
+  return HMVM_OK;
+}
+
+// Tile widths default to a fixed number of bytes per tile (256 KiB for a
+// stage-1 tile, 128 KiB for a dense or stage-2 tile), i.e. twice the elements
+// for FP32 and four times for BF16.
+constexpr int kStage1TileBytes = 262144, kStage2TileBytes = 131072;
+
+PlanOptions plan_options(const hmvm_options *o) {
+  hmvm_options d;
+  if (!o) {
+    hmvm_options_default(&d);
+    o = &d;
+  }
+  PlanOptions po;
+  po.storage = o->storage;
+  const int eb = (po.storage >= kF64 && po.storage <= kBF16) ? elem_bytes(po.storage) : 8;
+  po.w1 = o->w1 > 0 ? o->w1 : kStage1TileBytes / eb;
+  po.w2 = o->w2 > 0 ? o->w2 : kStage2TileBytes / eb;
+  po.order = o->order;
+  po.t_mode = o->t_mode;
   return po;
 }
 
@@ -246,28 +379,208 @@ void export_segsum(const SegSum &s, hmvm_segsum *o) {
   o->list = (long long *)copy_out(s.list);
 }
 
+// Repack of one window: `descs` are all descriptors (chunks index them),
+// `wdescs`/`chunk_begin` the window's own; chunk maxima are indexed globally
+// (the window's chunks start at chunk0).
 template <typename S>
-int launch_repack(hmvm_plan *p, const double *d_src, const RepackDesc *d_desc, const RepackChunk *d_chunks,
-                  int nchunks, const int32_t *d_chunk_begin, int ndesc, float *d_chunk_max) {
-  const bool scaled = p->storage == kBF16;
-  if (scaled) {
-    hipLaunchKernelGGL(absmax_kernel, dim3(nchunks), dim3(256), 0, p->stream, d_desc, d_chunks, d_src, d_chunk_max);
+int launch_repack(hmvm_plan *p, const double *d_src, const RepackDesc *descs, const RepackDesc *wdescs,
+                  const RepackChunk *d_chunks, int nchunks, const int32_t *chunk_begin, int ndesc,
+                  float *d_chunk_max, int chunk0) {
+  if (p->storage == kBF16) {
+    hipLaunchKernelGGL(absmax_kernel, dim3(nchunks), dim3(256), 0, p->stream, descs, d_chunks, d_src,
+                       d_chunk_max + chunk0);
     HIP_TRY(hipGetLastError());
-    hipLaunchKernelGGL(scale_kernel, dim3((ndesc + 255) / 256), dim3(256), 0, p->stream, ndesc, d_desc,
-                       d_chunk_begin, d_chunk_max, p->d_scales);
+    hipLaunchKernelGGL(scale_kernel, dim3((ndesc + 255) / 256), dim3(256), 0, p->stream, ndesc, wdescs, chunk_begin,
+                       (const float *)d_chunk_max, p->d_scales);
     HIP_TRY(hipGetLastError());
-    hipLaunchKernelGGL((repack_kernel<S, true>), dim3(nchunks), dim3(256), 0, p->stream, d_desc, d_chunks, d_src,
+    hipLaunchKernelGGL((repack_kernel<S, true>), dim3(nchunks), dim3(256), 0, p->stream, descs, d_chunks, d_src,
                        p->d_scales, (S *)p->d_vals);
   } else {
-    hipLaunchKernelGGL((repack_kernel<S, false>), dim3(nchunks), dim3(256), 0, p->stream, d_desc, d_chunks, d_src,
+    hipLaunchKernelGGL((repack_kernel<S, false>), dim3(nchunks), dim3(256), 0, p->stream, descs, d_chunks, d_src,
                        (const float *)nullptr, (S *)p->d_vals);
   }
   HIP_TRY(hipGetLastError());
   return HMVM_OK;
 }
 
+// Elements per upload window of the source values: 256 MiB of FP64, or the
+// value of HMVM_UPLOAD_WINDOW (elements, >= 1024) when set.
+int64_t window_elems() {
+  const char *e = std::getenv("HMVM_UPLOAD_WINDOW");
+  if (e && *e) {
+    const long long v = std::atoll(e);
+    if (v >= 1024) return v;
+  }
+  return int64_t(1) << 25;
+}
+
+struct Window {
+  int64_t w0, w1;  // source elements [w0, w1)
+  int d0, d1;      // repack descriptors (window order)
+  int c0, c1;      // repack chunks
+};
+
+// Uploads the source values window by window and repacks each window on the
+// GPU.  Windows are cut between records (a factor of one leaf; a whole leaf
+// for a file), so every repack descriptor reads inside one window; with a
+// feeder the host side fills pinned buffers while the GPU repacks the
+// previous window.  Device memory beyond the plan: two windows.
+int upload_values(hmvm_plan *p, const Source &src) {
+  const Plan &P = p->plan;
+  const LeafArrays &la = src.la;
+  // source records, in source order
+  std::vector<std::pair<int64_t, int64_t>> units;
+  units.reserve((size_t)la.nlf * 2);
+  for (int i = 0; i < la.nlf; i++) {
+    if (la.ltmtx[i] == 1) {
+      const int64_t n1 = (int64_t)la.kt[i] * la.ndt[i], n2 = (int64_t)la.kt[i] * la.ndl[i];
+      if (src.feeder) {
+        if (n1 + n2 > 0) units.push_back({la.src_a1[i], la.src_a1[i] + n1 + n2});
+      } else {
+        if (n1 > 0) units.push_back({la.src_a1[i], la.src_a1[i] + n1});
+        if (n2 > 0) units.push_back({la.src_a2[i], la.src_a2[i] + n2});
+      }
+    } else {
+      units.push_back({la.src_a1[i], la.src_a1[i] + (int64_t)la.ndl[i] * la.ndt[i]});
+    }
+  }
+  std::stable_sort(units.begin(), units.end());
+  const int64_t kWindowElems = window_elems();
+  std::vector<Window> win;
+  for (const auto &u : units) {
+    if (win.empty() || u.second - win.back().w0 > kWindowElems)
+      win.push_back({u.first, u.second, 0, 0, 0, 0});
+    else
+      win.back().w1 = std::max(win.back().w1, u.second);
+  }
+  // descriptors -> windows
+  const int nrep = (int)P.repack.size();
+  std::vector<int> wof(nrep, -1);
+  bool fits = true;
+  for (int d = 0; d < nrep && fits; d++) {
+    const RepackDesc &r = P.repack[d];
+    if ((int64_t)r.rows * r.cols == 0) continue;
+    const int64_t e = r.src_off + (int64_t)(r.rows - 1) * r.src_rs + (int64_t)(r.cols - 1) * r.src_cs + 1;
+    auto it = std::upper_bound(win.begin(), win.end(), r.src_off,
+                               [](int64_t v, const Window &w) { return v < w.w0; });
+    if (it == win.begin()) { fits = false; break; }
+    --it;
+    if (e > it->w1) { fits = false; break; }
+    wof[d] = (int)(it - win.begin());
+  }
+  if (!fits) {
+    // records overlap or interleave: one window over the whole buffer
+    if (src.feeder) return set_error(HMVM_ERR_MATRIX, "source records do not fit the upload windows");
+    win.assign(1, Window{0, la.src_len, 0, 0, 0, 0});
+    for (int d = 0; d < nrep; d++) wof[d] = (int64_t)P.repack[d].rows * P.repack[d].cols ? 0 : -1;
+  }
+  // window-ordered, rebased descriptors and their chunks
+  std::vector<std::vector<int>> byw(win.size());
+  for (int d = 0; d < nrep; d++)
+    if (wof[d] >= 0) byw[wof[d]].push_back(d);
+  std::vector<RepackDesc> descs;
+  std::vector<RepackChunk> chunks;
+  std::vector<int32_t> chunk_begin;
+  descs.reserve(nrep);
+  chunk_begin.reserve(nrep + 1);
+  int64_t wmax = 0;
+  for (size_t w = 0; w < win.size(); w++) {
+    Window &W = win[w];
+    W.d0 = (int)descs.size();
+    W.c0 = (int)chunks.size();
+    for (int d : byw[w]) {
+      RepackDesc r = P.repack[d];
+      r.src_off -= W.w0;
+      chunk_begin.push_back((int32_t)chunks.size());
+      const int64_t ne = (int64_t)r.rows * r.cols;
+      for (int64_t e0 = 0; e0 < ne; e0 += kRepackChunk) chunks.push_back({(int32_t)descs.size(), (int32_t)e0});
+      descs.push_back(r);
+    }
+    W.d1 = (int)descs.size();
+    W.c1 = (int)chunks.size();
+    wmax = std::max(wmax, W.w1 - W.w0);
+  }
+  chunk_begin.push_back((int32_t)chunks.size());
+
+  RepackDesc *d_desc = nullptr;
+  RepackChunk *d_chunks = nullptr;
+  int32_t *d_chunk_begin = nullptr;
+  float *d_chunk_max = nullptr;
+  double *d_stage[2] = {nullptr, nullptr};
+  double *h_stage[2] = {nullptr, nullptr};
+  hipEvent_t copied[2] = {nullptr, nullptr};
+  const int nbuf = win.size() > 1 ? 2 : 1;
+  auto cleanup = [&]() {
+    if (d_desc) (void)hipFree(d_desc);
+    if (d_chunks) (void)hipFree(d_chunks);
+    if (d_chunk_begin) (void)hipFree(d_chunk_begin);
+    if (d_chunk_max) (void)hipFree(d_chunk_max);
+    for (int b = 0; b < 2; b++) {
+      if (d_stage[b]) (void)hipFree(d_stage[b]);
+      if (h_stage[b]) (void)hipHostFree(h_stage[b]);
+      if (copied[b]) (void)hipEventDestroy(copied[b]);
+    }
+  };
+  int rc = upload(&d_desc, descs);
+  if (!rc) rc = upload(&d_chunks, chunks);
+  if (!rc) rc = upload(&d_chunk_begin, chunk_begin);
+  if (!rc && p->storage == kBF16) rc = dalloc(&d_chunk_max, chunks.size());
+  for (int b = 0; b < nbuf && !rc; b++) {
+    rc = dalloc(&d_stage[b], (size_t)std::max<int64_t>(wmax, 1));
+    if (!rc && src.feeder) {
+      if (hipHostMalloc((void **)&h_stage[b], sizeof(double) * (size_t)std::max<int64_t>(wmax, 1)) != hipSuccess)
+        rc = set_error(HMVM_ERR_HIP, "pinned staging buffer of %lld elements", (long long)wmax);
+      else if (hipEventCreateWithFlags(&copied[b], hipEventDisableTiming) != hipSuccess)
+        rc = set_error(HMVM_ERR_HIP, "hipEventCreate failed");
+    }
+  }
+  for (size_t w = 0; w < win.size() && !rc; w++) {
+    const Window &W = win[w];
+    const int b = (int)(w % nbuf);
+    const size_t n = (size_t)(W.w1 - W.w0);
+    if (src.feeder) {
+      // the pinned buffer is free once its previous copy has landed
+      if (w >= 2 && hipEventSynchronize(copied[b]) != hipSuccess) { rc = set_error(HMVM_ERR_HIP, "staging copy"); break; }
+      if ((rc = src.feeder->fill(h_stage[b], W.w0, W.w1))) break;
+      if (hipMemcpyAsync(d_stage[b], h_stage[b], sizeof(double) * n, hipMemcpyHostToDevice, p->stream) != hipSuccess ||
+          hipEventRecord(copied[b], p->stream) != hipSuccess) {
+        rc = set_error(HMVM_ERR_HIP, "window upload failed");
+        break;
+      }
+    } else if (n > 0 && hipMemcpyAsync(d_stage[b], src.vals + W.w0, sizeof(double) * n, hipMemcpyHostToDevice,
+                                       p->stream) != hipSuccess) {
+      rc = set_error(HMVM_ERR_HIP, "window upload failed");
+      break;
+    }
+    const int nch = W.c1 - W.c0, nd = W.d1 - W.d0;
+    if (nch == 0) continue;
+    if (p->storage == kF64)
+      rc = launch_repack<double>(p, d_stage[b], d_desc, d_desc + W.d0, d_chunks + W.c0, nch, d_chunk_begin + W.d0,
+                                 nd, d_chunk_max, W.c0);
+    else if (p->storage == kF32)
+      rc = launch_repack<float>(p, d_stage[b], d_desc, d_desc + W.d0, d_chunks + W.c0, nch, d_chunk_begin + W.d0,
+                                nd, d_chunk_max, W.c0);
+    else
+      rc = launch_repack<uint16_t>(p, d_stage[b], d_desc, d_desc + W.d0, d_chunks + W.c0, nch,
+                                   d_chunk_begin + W.d0, nd, d_chunk_max, W.c0);
+  }
+  if (!rc) {
+    hipError_t e = hipStreamSynchronize(p->stream);
+    if (e != hipSuccess) rc = set_error(HMVM_ERR_HIP, "repack failed: %s", hipGetErrorString(e));
+  } else {
+    (void)hipStreamSynchronize(p->stream);
+  }
+  cleanup();
+  return rc;
+}
+
 int create_plan(Source &src, const hmvm_options *opt, hmvm_plan **out) {
   *out = nullptr;
+  hmvm_options dflt;
+  if (!opt) {
+    hmvm_options_default(&dflt);
+    opt = &dflt;
+  }
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return set_error(HMVM_ERR_NODEVICE, "no HIP device visible");
   const PlanOptions po = plan_options(opt);
@@ -278,11 +591,13 @@ int create_plan(Source &src, const hmvm_options *opt, hmvm_plan **out) {
     if (opt->block_threads % 64 || opt->block_threads > 256 || opt->block_threads < 64)
       return set_error(HMVM_ERR_ARG, "block_threads must be 64, 128, 192 or 256");
     p->block_threads = opt->block_threads;
+    p->block_threads_b = opt->block_threads;
   }
   if (opt) {
     if (opt->kernel < 0 || opt->kernel >= kNumKernelVariants)
       return set_error(HMVM_ERR_ARG, "kernel variant must be in [0,%d)", kNumKernelVariants);
     p->kernel = opt->kernel;
+    p->kernel_b = opt->kernel;
     if (opt->mode < 0 || opt->mode > 1) return set_error(HMVM_ERR_ARG, "mode must be 0 or 1");
     p->mode = opt->mode;
   }
@@ -324,49 +639,8 @@ int create_plan(Source &src, const hmvm_options *opt, hmvm_plan **out) {
   // column padding is read (and discarded) by the 16-byte lane loads: keep it zero
   HIP_TRY(hipMemsetAsync(p->d_vals, 0, eb * (size_t)std::max<int64_t>(P.nvals, 1), p->stream));
 
-  // values: upload the caller's buffer once, repack on the GPU
-  std::vector<RepackChunk> chunks;
-  std::vector<int32_t> chunk_begin;
-  chunk_begin.reserve(P.repack.size() + 1);
-  for (size_t d = 0; d < P.repack.size(); d++) {
-    chunk_begin.push_back((int32_t)chunks.size());
-    const int64_t ne = (int64_t)P.repack[d].rows * P.repack[d].cols;
-    for (int64_t e0 = 0; e0 < ne; e0 += kRepackChunk) chunks.push_back({(int32_t)d, (int32_t)e0});
-  }
-  chunk_begin.push_back((int32_t)chunks.size());
-  double *d_src = nullptr;
-  RepackDesc *d_desc = nullptr;
-  RepackChunk *d_chunks = nullptr;
-  int32_t *d_chunk_begin = nullptr;
-  float *d_chunk_max = nullptr;
-  auto cleanup = [&]() {
-    if (d_src) (void)hipFree(d_src);
-    if (d_desc) (void)hipFree(d_desc);
-    if (d_chunks) (void)hipFree(d_chunks);
-    if (d_chunk_begin) (void)hipFree(d_chunk_begin);
-    if (d_chunk_max) (void)hipFree(d_chunk_max);
-  };
-  rc = dalloc(&d_src, (size_t)std::max<int64_t>(src.la.src_len, 1));
-  if (!rc) rc = h2d(d_src, src.vals, (size_t)src.la.src_len);
-  if (!rc) rc = upload(&d_desc, P.repack);
-  if (!rc) rc = upload(&d_chunks, chunks);
-  if (!rc) rc = upload(&d_chunk_begin, chunk_begin);
-  if (!rc && p->storage == kBF16) rc = dalloc(&d_chunk_max, chunks.size());
-  if (!rc && !chunks.empty()) {
-    const int nch = (int)chunks.size(), nd = (int)P.repack.size();
-    if (p->storage == kF64)
-      rc = launch_repack<double>(p.get(), d_src, d_desc, d_chunks, nch, d_chunk_begin, nd, d_chunk_max);
-    else if (p->storage == kF32)
-      rc = launch_repack<float>(p.get(), d_src, d_desc, d_chunks, nch, d_chunk_begin, nd, d_chunk_max);
-    else
-      rc = launch_repack<uint16_t>(p.get(), d_src, d_desc, d_chunks, nch, d_chunk_begin, nd, d_chunk_max);
-  }
-  if (!rc) {
-    hipError_t e = hipStreamSynchronize(p->stream);
-    if (e != hipSuccess) rc = set_error(HMVM_ERR_HIP, "repack failed: %s", hipGetErrorString(e));
-  }
-  cleanup();
-  if (rc) return rc;
+  // values: streamed to the GPU in windows and repacked there
+  if ((rc = upload_values(p.get(), src))) return rc;
 
   p->device_bytes = eb * (size_t)P.nvals + sizeof(TileDesc) * (P.tiles_a.size() + P.tiles_b.size()) +
                     sizeof(PanelDesc) * P.panels.size() + 8 * (size_t)(P.n_t + P.n_tp + P.n_z + 2 * P.nd) +
@@ -392,9 +666,9 @@ void launch_tiles_v(hmvm_plan *p, dim3 grid, dim3 block, hipStream_t st, const T
 }
 
 template <typename S, bool kScaled>
-void launch_tiles_s(hmvm_plan *p, dim3 grid, dim3 block, hipStream_t st, const TileDesc *d_tiles, int ntiles,
-                    const double *x, const double *t, const double *tp, double *out) {
-  switch (p->kernel) {
+void launch_tiles_s(hmvm_plan *p, int kern, dim3 grid, dim3 block, hipStream_t st, const TileDesc *d_tiles,
+                    int ntiles, const double *x, const double *t, const double *tp, double *out) {
+  switch (kern) {
     case 0: launch_tiles_v<S, kScaled, 0>(p, grid, block, st, d_tiles, ntiles, x, t, tp, out); break;
     case 1: launch_tiles_v<S, kScaled, 1>(p, grid, block, st, d_tiles, ntiles, x, t, tp, out); break;
     case 2: launch_tiles_v<S, kScaled, 2>(p, grid, block, st, d_tiles, ntiles, x, t, tp, out); break;
@@ -407,15 +681,17 @@ void launch_tiles_s(hmvm_plan *p, dim3 grid, dim3 block, hipStream_t st, const T
 }
 
 template <typename S>
-void launch_tiles(hmvm_plan *p, const TileDesc *d_tiles, int ntiles, const double *x, const double *t,
+void launch_tiles(hmvm_plan *p, int which, const TileDesc *d_tiles, int ntiles, const double *x, const double *t,
                   const double *tp, double *out, bool scaled, hipStream_t st) {
   if (ntiles == 0) return;
-  const int wpb = p->block_threads / 64;
-  const dim3 grid((ntiles + wpb - 1) / wpb), block(p->block_threads);
+  const int bt = which ? p->block_threads_b : p->block_threads;
+  const int kern = which ? p->kernel_b : p->kernel;
+  const int wpb = bt / 64;
+  const dim3 grid((ntiles + wpb - 1) / wpb), block(bt);
   if (scaled)
-    launch_tiles_s<S, true>(p, grid, block, st, d_tiles, ntiles, x, t, tp, out);
+    launch_tiles_s<S, true>(p, kern, grid, block, st, d_tiles, ntiles, x, t, tp, out);
   else
-    launch_tiles_s<S, false>(p, grid, block, st, d_tiles, ntiles, x, t, tp, out);
+    launch_tiles_s<S, false>(p, kern, grid, block, st, d_tiles, ntiles, x, t, tp, out);
 }
 
 template <typename S>
@@ -423,9 +699,53 @@ void launch_tiles_both(hmvm_plan *p, int which, const double *x, hipStream_t st)
   const Plan &P = p->plan;
   const bool bf = p->storage == kBF16;
   if (which == 0)
-    launch_tiles<S>(p, p->d_tiles_a, (int)P.tiles_a.size(), x, nullptr, nullptr, p->d_slots, bf, st);
+    launch_tiles<S>(p, 0, p->d_tiles_a, (int)P.tiles_a.size(), x, nullptr, nullptr, p->d_slots, bf, st);
   else
-    launch_tiles<S>(p, p->d_tiles_b, (int)P.tiles_b.size(), x, p->d_t, p->d_slots, p->d_slots, bf, st);
+    launch_tiles<S>(p, 1, p->d_tiles_b, (int)P.tiles_b.size(), x, p->d_t, p->d_slots, p->d_slots, bf, st);
+}
+
+// ---- per-tile timing (cost-model training data) -----------------------------
+template <typename S, bool kScaled, int kVar>
+void launch_timed_v(hmvm_plan *p, int bt, const TileDesc *d_tiles, int ntiles, const double *x, const double *t,
+                    const double *tp, double *out, long long *stamps, hipStream_t st) {
+  constexpr int L = (kVar & 1) ? 16 : 8;
+  constexpr bool kPipe = (kVar & 2) != 0, kNT = (kVar & 4) != 0;
+  const int wpb = bt / 64;
+  hipLaunchKernelGGL((tile_kernel_timed<S, kScaled, L, kPipe, kNT>), dim3((ntiles + wpb - 1) / wpb),
+                     dim3(bt), 0, st, d_tiles, ntiles, p->d_panels, (const S *)p->d_vals, x, t, tp,
+                     (const float *)p->d_scales, out, stamps);
+}
+
+template <typename S, bool kScaled>
+void launch_timed_s(hmvm_plan *p, int which, const TileDesc *d_tiles, int ntiles, const double *x, const double *t,
+                    const double *tp, double *out, long long *stamps, hipStream_t st) {
+  const int bt = which ? p->block_threads_b : p->block_threads;
+  switch (which ? p->kernel_b : p->kernel) {
+    case 0: launch_timed_v<S, kScaled, 0>(p, bt, d_tiles, ntiles, x, t, tp, out, stamps, st); break;
+    case 1: launch_timed_v<S, kScaled, 1>(p, bt, d_tiles, ntiles, x, t, tp, out, stamps, st); break;
+    case 2: launch_timed_v<S, kScaled, 2>(p, bt, d_tiles, ntiles, x, t, tp, out, stamps, st); break;
+    case 3: launch_timed_v<S, kScaled, 3>(p, bt, d_tiles, ntiles, x, t, tp, out, stamps, st); break;
+    case 4: launch_timed_v<S, kScaled, 4>(p, bt, d_tiles, ntiles, x, t, tp, out, stamps, st); break;
+    case 5: launch_timed_v<S, kScaled, 5>(p, bt, d_tiles, ntiles, x, t, tp, out, stamps, st); break;
+    case 6: launch_timed_v<S, kScaled, 6>(p, bt, d_tiles, ntiles, x, t, tp, out, stamps, st); break;
+    default: launch_timed_v<S, kScaled, 7>(p, bt, d_tiles, ntiles, x, t, tp, out, stamps, st); break;
+  }
+}
+
+// Timed launch of the tiles of launch A (which = 0) or B (which = 1).
+void launch_timed(hmvm_plan *p, int which, const double *x, long long *stamps, hipStream_t st) {
+  const Plan &P = p->plan;
+  const TileDesc *tl = which == 0 ? p->d_tiles_a : p->d_tiles_b;
+  const int nt = (int)(which == 0 ? P.tiles_a.size() : P.tiles_b.size());
+  if (nt == 0) return;
+  const double *t = which == 0 ? nullptr : p->d_t;
+  const double *tp = which == 0 ? nullptr : p->d_slots;
+  if (p->storage == kF64)
+    launch_timed_s<double, false>(p, which, tl, nt, x, t, tp, p->d_slots, stamps, st);
+  else if (p->storage == kF32)
+    launch_timed_s<float, false>(p, which, tl, nt, x, t, tp, p->d_slots, stamps, st);
+  else
+    launch_timed_s<uint16_t, true>(p, which, tl, nt, x, t, tp, p->d_slots, stamps, st);
 }
 
 // Launch `phase` of one HMVM: 0 = tiles A (stage 1 + dense), 1 = t segment sum
@@ -581,7 +901,7 @@ int launch_persistent(hmvm_plan *p, const double *x, double *y, int accumulate, 
 
 // The persistent mode needs every stage-2 tile to read its leaf's stage-1
 // partial slots directly (no materialised t).
-bool persistent_ok(const hmvm_plan *p) { return p->mode == 1 && p->plan.t_fused; }
+bool persistent_ok(const hmvm_plan *p) { return p->mode == 1 && p->plan.t_fused && !p->a_mixed; }
 
 int apply(hmvm_plan *p, const double *d_x, double *d_y, int accumulate, hipStream_t st) {
   if (persistent_ok(p)) {
@@ -627,20 +947,21 @@ void fill_info(const hmvm_plan *p, const Plan &P, int storage, hmvm_plan_info *i
 // cache of device copies for the drop-in entry points
 struct CacheKey {
   const void *mat;
-  int variant;  // 0:1 1:1t 2:2 3:2t
+  int variant;  // 0:1 1:1t 2:2 3:2t 4:2 (int64 form) 5:2t (int64 form)
   bool operator<(const CacheKey &o) const { return mat != o.mat ? mat < o.mat : variant < o.variant; }
 };
 struct CacheEntry {
   hmvm_plan *plan = nullptr;
   // identity of the contents the plan was built from
-  int nd = 0, nlf = 0, len = 0;
+  int nd = 0, nlf = 0;
+  int64_t len = 0;
   const void *data = nullptr;
   const void *first_leaf_a1 = nullptr;
 };
 std::mutex g_cache_mu;
 std::map<CacheKey, CacheEntry> g_cache;
 
-hmvm_plan *cached_plan(const void *mat, int variant, int nd, int nlf, int len, const void *data,
+hmvm_plan *cached_plan(const void *mat, int variant, int nd, int nlf, int64_t len, const void *data,
                        const void *first_a1) {
   std::lock_guard<std::mutex> lk(g_cache_mu);
   auto it = g_cache.find({mat, variant});
@@ -652,7 +973,7 @@ hmvm_plan *cached_plan(const void *mat, int variant, int nd, int nlf, int len, c
   return nullptr;
 }
 
-void cache_put(const void *mat, int variant, hmvm_plan *p, int nd, int nlf, int len, const void *data,
+void cache_put(const void *mat, int variant, hmvm_plan *p, int nd, int nlf, int64_t len, const void *data,
                const void *first_a1) {
   std::lock_guard<std::mutex> lk(g_cache_mu);
   CacheEntry e;
@@ -691,6 +1012,18 @@ hmvm_plan *plan_for(const matrix *mat, const matrix2 *mat2, int variant) {
   return p;
 }
 
+hmvm_plan *plan_for64(const matrix2_64 *m, int variant) {
+  if (!m) { set_error(HMVM_ERR_ARG, "matrix2_64 is NULL"); return nullptr; }
+  const void *data = variant == 5 ? (const void *)m->rowmat_t : (const void *)m->rowmat;
+  hmvm_plan *p = cached_plan(m, variant, m->nd, m->nlf, m->len, data, nullptr);
+  if (p) return p;
+  Source s;
+  if (source_from_matrix2_64(m, variant == 5 ? kA2Transposed : kA2RowMajor, &s)) return nullptr;
+  if (create_plan(s, nullptr, &p)) return nullptr;
+  cache_put(m, variant, p, m->nd, m->nlf, m->len, data, nullptr);
+  return p;
+}
+
 void run_variant(double *v, const matrix *mat, const matrix2 *mat2, const double *b, int variant) {
   clear_error();
   if (!v || !b) { set_error(HMVM_ERR_ARG, "v or b is NULL"); return; }
@@ -711,14 +1044,16 @@ const char *hmvm_hip_last_error_message(void) { return g_msg; }
 void hmvm_options_default(hmvm_options *opt) {
   if (!opt) return;
   std::memset(opt, 0, sizeof(*opt));
-  PlanOptions po;
-  opt->storage = po.storage;
-  opt->w1 = po.w1;
-  opt->w2 = po.w2;
+  // measured best on MI355X for N = 101,250 (cube, eps 1e-4) in all three
+  // storages: pipelined non-temporal loads, largest tiles first
+  opt->storage = kF64;
+  opt->w1 = 0;  // by storage: 256 KiB per stage-1 tile
+  opt->w2 = 0;  // by storage: 128 KiB per dense / stage-2 tile
   opt->block_threads = 256;
-  opt->kernel = 0;
-  opt->order = 0;
+  opt->kernel = 6;
+  opt->order = 1;
   opt->mode = 0;
+  opt->t_mode = 0;
 }
 
 int hmvm_plan_create(const matrix2 *mat2, int a2_source, const hmvm_options *opt, hmvm_plan **out) {
@@ -743,6 +1078,68 @@ int hmvm_plan_create_aos(const matrix *mat, int a2_source, const hmvm_options *o
   return create_plan(s, opt, out);
 }
 
+int hmvm_plan_create64(const matrix2_64 *mat2, int a2_source, const hmvm_options *opt, hmvm_plan **out) {
+  clear_error();
+  if (!out) return set_error(HMVM_ERR_ARG, "out is NULL");
+  *out = nullptr;
+  if (a2_source != kA2RowMajor && a2_source != kA2Transposed) return set_error(HMVM_ERR_ARG, "bad a2_source");
+  Source s;
+  int rc = source_from_matrix2_64(mat2, a2_source, &s);
+  if (rc) return rc;
+  return create_plan(s, opt, out);
+}
+
+int hmvm_plan_create_file(const char *fname, const hmvm_options *opt, hmvm_plan **out) {
+  clear_error();
+  if (!out || !fname) return set_error(HMVM_ERR_ARG, "NULL argument");
+  *out = nullptr;
+  matrix2_64 m;
+  long long data_pos = -1;
+  int format = 0;
+  int rc = hacapk_scan_file(fname, &m, &data_pos, &format);
+  if (rc) return set_error(HMVM_ERR_MATRIX, "%s: %s", fname, hacapk_strerror(rc));
+  Source s;
+  s.a1.assign(m.a1, m.a1 + m.nlf);
+  s.a2.assign(m.a2, m.a2 + m.nlf);
+  LeafArrays &la = s.la;
+  la.nd = m.nd;
+  la.nlf = m.nlf;
+  la.ltmtx = m.ltmtx; la.kt = m.kt; la.ndl = m.ndl; la.ndt = m.ndt;
+  la.nstrtl = m.nstrtl; la.nstrtt = m.nstrtt;
+  la.src_a1 = s.a1.data();
+  la.src_a2 = s.a2.data();
+  la.a2_source = kA2RowMajor;
+  la.src_len = m.len;
+  {
+    FileFeeder feed(fname, format, &m);
+    rc = feed.begin(data_pos);
+    if (rc) {
+      rc = set_error(HMVM_ERR_MATRIX, "%s: %s", fname, hacapk_strerror(rc));
+    } else {
+      s.feeder = &feed;
+      rc = create_plan(s, opt, out);
+    }
+  }
+  freeHmatrix64(&m);
+  return rc;
+}
+
+int hmvm_plan_create_batch(const matrix2 *const *mats, int count, int a2_source, const hmvm_options *opt,
+                           hmvm_plan **out, int *row_offsets) {
+  clear_error();
+  if (!out) return set_error(HMVM_ERR_ARG, "out is NULL");
+  *out = nullptr;
+  if (a2_source != kA2RowMajor && a2_source != kA2Transposed) return set_error(HMVM_ERR_ARG, "bad a2_source");
+  Source s;
+  ConcatFeeder feed;
+  std::vector<int> row0;
+  int rc = source_from_batch(mats, count, a2_source, &s, &feed, &row0);
+  if (rc) return rc;
+  rc = create_plan(s, opt, out);
+  if (!rc && row_offsets) std::memcpy(row_offsets, row0.data(), sizeof(int) * row0.size());
+  return rc;
+}
+
 void hmvm_plan_destroy(hmvm_plan *plan) { delete plan; }
 
 int hmvm_plan_set_launch(hmvm_plan *plan, int kernel, int block_threads) {
@@ -751,8 +1148,24 @@ int hmvm_plan_set_launch(hmvm_plan *plan, int kernel, int block_threads) {
     return set_error(HMVM_ERR_ARG, "kernel variant must be in [0,%d)", kNumKernelVariants);
   if (block_threads % 64 || block_threads < 64 || block_threads > 256)
     return set_error(HMVM_ERR_ARG, "block_threads must be 64, 128, 192 or 256");
-  plan->kernel = kernel;
-  plan->block_threads = block_threads;
+  plan->kernel = plan->kernel_b = kernel;
+  plan->block_threads = plan->block_threads_b = block_threads;
+  return HMVM_OK;
+}
+
+int hmvm_plan_set_phase_launch(hmvm_plan *plan, int which, int kernel, int block_threads) {
+  if (!plan || (which != 0 && which != 1)) return set_error(HMVM_ERR_ARG, "bad argument");
+  if (kernel < 0 || kernel >= kNumKernelVariants)
+    return set_error(HMVM_ERR_ARG, "kernel variant must be in [0,%d)", kNumKernelVariants);
+  if (block_threads % 64 || block_threads < 64 || block_threads > 256)
+    return set_error(HMVM_ERR_ARG, "block_threads must be 64, 128, 192 or 256");
+  if (which == 0) {
+    plan->kernel = kernel;
+    plan->block_threads = block_threads;
+  } else {
+    plan->kernel_b = kernel;
+    plan->block_threads_b = block_threads;
+  }
   return HMVM_OK;
 }
 
@@ -1068,10 +1481,240 @@ int hmvm_plan_time(hmvm_plan *plan, int warmup, int iters, hmvm_timing *tm) {
   return HMVM_OK;
 }
 
+// ---- hipGraph form of the apply -------------------------------------------
+struct hmvm_graph {
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  ~hmvm_graph() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+  }
+};
+
+int hmvm_plan_graph_create(hmvm_plan *plan, const double *d_x, double *d_y, int accumulate, int count,
+                           hmvm_graph **out) {
+  if (!plan || !d_x || !d_y || !out || count < 1) return set_error(HMVM_ERR_ARG, "bad argument");
+  *out = nullptr;
+  std::unique_ptr<hmvm_graph> g(new hmvm_graph());
+  hipStream_t st = plan->stream;
+  HIP_TRY(hipStreamSynchronize(st));
+  HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+  int rc = HMVM_OK;
+  for (int i = 0; i < count && !rc; i++) rc = apply(plan, d_x, d_y, accumulate, st);
+  hipGraph_t graph = nullptr;
+  const hipError_t e = hipStreamEndCapture(st, &graph);
+  g->graph = graph;
+  if (rc) return rc;
+  if (e != hipSuccess || !graph) return set_error(HMVM_ERR_HIP, "graph capture failed: %s", hipGetErrorString(e));
+  HIP_TRY(hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0));
+  *out = g.release();
+  return HMVM_OK;
+}
+
+int hmvm_graph_launch(hmvm_graph *graph, void *stream) {
+  if (!graph || !graph->exec) return set_error(HMVM_ERR_ARG, "NULL graph");
+  HIP_TRY(hipGraphLaunch(graph->exec, (hipStream_t)stream));
+  return HMVM_OK;
+}
+
+void hmvm_graph_destroy(hmvm_graph *graph) { delete graph; }
+
+int hmvm_plan_time_graph(hmvm_plan *plan, int count, int warmup, int iters, hmvm_timing *tm) {
+  if (!plan || !tm || iters < 1 || warmup < 0 || count < 1) return set_error(HMVM_ERR_ARG, "bad argument");
+  std::memset(tm, 0, sizeof(*tm));
+  const int nd = plan->plan.nd;
+  {
+    std::vector<double> h((size_t)nd);
+    for (int i = 0; i < nd; i++) h[i] = std::sin((double)(i + 1));
+    HIP_TRY(hipMemcpy(plan->d_x, h.data(), sizeof(double) * nd, hipMemcpyHostToDevice));
+  }
+  hmvm_graph *g = nullptr;
+  int rc = hmvm_plan_graph_create(plan, plan->d_x, plan->d_y, 0, count, &g);
+  if (rc) return rc;
+  hipStream_t st = plan->stream;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  std::vector<double> t;
+  if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) rc = set_error(HMVM_ERR_HIP, "event");
+  for (int w = 0; w < warmup && !rc; w++) rc = hmvm_graph_launch(g, st);
+  for (int it = 0; it < iters && !rc; it++) {
+    hipEventRecord(e0, st);
+    rc = hmvm_graph_launch(g, st);
+    hipEventRecord(e1, st);
+    if (!rc && hipEventSynchronize(e1) != hipSuccess) rc = set_error(HMVM_ERR_HIP, "event sync failed");
+    float ms = 0;
+    if (!rc) hipEventElapsedTime(&ms, e0, e1);
+    t.push_back(ms / count);
+  }
+  if (!rc) {
+    hipEventRecord(e0, st);
+    for (int it = 0; it < iters && !rc; it++) rc = hmvm_graph_launch(g, st);
+    hipEventRecord(e1, st);
+    float ms = 0;
+    if (!rc && hipEventSynchronize(e1) == hipSuccess) hipEventElapsedTime(&ms, e0, e1);
+    tm->ms_stream = ms / ((double)iters * count);
+  }
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  hmvm_graph_destroy(g);
+  if (rc) return rc;
+  std::vector<double> srt = t;
+  std::sort(srt.begin(), srt.end());
+  double sum = 0;
+  for (double v : t) sum += v;
+  tm->iters = iters;
+  tm->ms_avg = sum / iters;
+  tm->ms_min = srt.front();
+  tm->ms_median = srt[srt.size() / 2];
+  return HMVM_OK;
+}
+
+int hmvm_plan_profile_tiles(hmvm_plan *plan, int which, int iters, double *tile_us, double *start_us,
+                            double *launch_ms) {
+  if (!plan || (which != 0 && which != 1) || iters < 1 || !tile_us) return set_error(HMVM_ERR_ARG, "bad argument");
+  const Plan &P = plan->plan;
+  const size_t nt = which == 0 ? P.tiles_a.size() : P.tiles_b.size();
+  if (nt == 0) {
+    if (launch_ms) *launch_ms = 0;
+    return HMVM_OK;
+  }
+  int rate_khz = 0;
+  HIP_TRY(hipDeviceGetAttribute(&rate_khz, hipDeviceAttributeWallClockRate, plan->device));
+  if (rate_khz <= 0) return set_error(HMVM_ERR_UNSUPPORTED, "device wall clock rate unknown");
+  const double us_per_tick = 1e3 / rate_khz;
+  const int nd = plan->plan.nd;
+  {
+    std::vector<double> h((size_t)nd);
+    for (int i = 0; i < nd; i++) h[i] = std::sin((double)(i + 1));
+    HIP_TRY(hipMemcpy(plan->d_x, h.data(), sizeof(double) * nd, hipMemcpyHostToDevice));
+  }
+  long long *d_stamps = nullptr;
+  int rc = dalloc(&d_stamps, 2 * nt);
+  if (rc) return rc;
+  std::vector<long long> h(2 * nt);
+  std::vector<double> acc(nt, 0.0), acc0(nt, 0.0);
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  double lms = 0;
+  hipStream_t st = plan->stream;
+  if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) rc = set_error(HMVM_ERR_HIP, "event");
+  for (int it = 0; it <= iters && !rc; it++) {  // iteration 0 warms up
+    if (which == 1) {
+      launch_phase(plan, 0, plan->d_x, plan->d_y, 0, st);
+      launch_phase(plan, 1, plan->d_x, plan->d_y, 0, st);
+    }
+    hipEventRecord(e0, st);
+    launch_timed(plan, which, plan->d_x, d_stamps, st);
+    hipEventRecord(e1, st);
+    if (hipGetLastError() != hipSuccess || hipEventSynchronize(e1) != hipSuccess) {
+      rc = set_error(HMVM_ERR_HIP, "timed launch failed");
+      break;
+    }
+    if (hipMemcpy(h.data(), d_stamps, sizeof(long long) * 2 * nt, hipMemcpyDeviceToHost) != hipSuccess) {
+      rc = set_error(HMVM_ERR_HIP, "stamp copy failed");
+      break;
+    }
+    if (it == 0) continue;
+    float ms = 0;
+    hipEventElapsedTime(&ms, e0, e1);
+    lms += ms;
+    long long t_min = h[0];
+    for (size_t i = 0; i < nt; i++) t_min = std::min(t_min, h[2 * i]);
+    for (size_t i = 0; i < nt; i++) {
+      acc[i] += (double)(h[2 * i + 1] - h[2 * i]) * us_per_tick;
+      acc0[i] += (double)(h[2 * i] - t_min) * us_per_tick;
+    }
+  }
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  (void)hipFree(d_stamps);
+  if (rc) return rc;
+  for (size_t i = 0; i < nt; i++) {
+    tile_us[i] = acc[i] / iters;
+    if (start_us) start_us[i] = acc0[i] / iters;
+  }
+  if (launch_ms) *launch_ms = lms / iters;
+  return HMVM_OK;
+}
+
+int hmvm_plan_tile_features(const hmvm_plan *plan, int which, int nrhs, float *feat) {
+  if (!plan || (which != 0 && which != 1) || !feat) return set_error(HMVM_ERR_ARG, "bad argument");
+  const Plan &P = plan->plan;
+  const std::vector<TileDesc> &tl = which == 0 ? P.tiles_a : P.tiles_b;
+  const int V = P.lane_elems;
+  const float lb = std::log2((float)elem_bytes(plan->storage));
+  for (size_t i = 0; i < tl.size(); i++) {
+    const TileDesc &td = tl[i];
+    // stage-1 panels carry no factor scale (factor -1); dense panels do
+    const int kind = which == 1 ? 2 : (P.panels[td.panel0].factor < 0 ? 0 : 1);
+    int64_t cols = 0;
+    double nch = 0;
+    for (int q = 0; q < td.npanel; q++) {
+      const PanelDesc &pd = P.panels[td.panel0 + q];
+      cols += pd.ncols;
+      if (pd.vec_sel == kVecTP) nch += (double)(pd.vec_chunks >> 16) * pd.ncols;
+    }
+    const int shift = (int)(P.panels[td.panel0].val_off % V);
+    const int lpc = (td.nrows + shift + V - 1) / V;
+    float *f = feat + i * HMVM_NFEAT;
+    f[0] = kind == 0; f[1] = kind == 1; f[2] = kind == 2;
+    f[3] = std::log2((float)td.nrows);
+    f[4] = std::log2(1.0f + (float)cols);
+    f[5] = std::log2((float)td.npanel);
+    f[6] = std::log2((float)lpc);
+    f[7] = std::log2(1.0f + (float)lpc * V * (float)cols);
+    f[8] = std::log2(1.0f + (float)(cols ? nch / cols : 0.0));
+    f[9] = lb;
+    const int kern = which ? plan->kernel_b : plan->kernel;
+    const int bt = which ? plan->block_threads_b : plan->block_threads;
+    f[10] = (float)(kern & 1);
+    f[11] = (float)((kern >> 1) & 1);
+    f[12] = (float)((kern >> 2) & 1);
+    f[13] = std::log2((float)(bt / 64));
+    f[14] = std::log2((float)std::max(nrhs, 1));
+  }
+  return HMVM_OK;
+}
+
+int hmvm_plan_set_tile_order(hmvm_plan *plan, int which, const int *perm) {
+  if (!plan || (which != 0 && which != 1) || !perm) return set_error(HMVM_ERR_ARG, "bad argument");
+  Plan &P = plan->plan;
+  std::vector<TileDesc> &tl = which == 0 ? P.tiles_a : P.tiles_b;
+  const int n = (int)tl.size();
+  std::vector<char> seen((size_t)n, 0);
+  for (int i = 0; i < n; i++) {
+    if (perm[i] < 0 || perm[i] >= n || seen[perm[i]]) return set_error(HMVM_ERR_ARG, "not a permutation");
+    seen[perm[i]] = 1;
+  }
+  std::vector<TileDesc> nt((size_t)n);
+  bool mixed = false;
+  for (int i = 0; i < n; i++) {
+    nt[i] = tl[perm[i]];
+    if (which == 0 && (i < P.n_stage1_tiles) != (perm[i] < P.n_stage1_tiles)) mixed = true;
+  }
+  TileDesc *d = which == 0 ? plan->d_tiles_a : plan->d_tiles_b;
+  HIP_TRY(hipStreamSynchronize(plan->stream));
+  HIP_TRY(hipMemcpy(d, nt.data(), sizeof(TileDesc) * (size_t)n, hipMemcpyHostToDevice));
+  tl.swap(nt);
+  if (which == 0) plan->a_mixed = plan->a_mixed || mixed;
+  return HMVM_OK;
+}
+
 void hmvm_hip_1(double *v, const matrix *mat, const double *b) { run_variant(v, mat, nullptr, b, 0); }
 void hmvm_hip_1t(double *v, const matrix *mat, const double *b) { run_variant(v, mat, nullptr, b, 1); }
 void hmvm_hip_2(double *v, const matrix2 *mat2, const double *b) { run_variant(v, nullptr, mat2, b, 2); }
 void hmvm_hip_2t(double *v, const matrix2 *mat2, const double *b) { run_variant(v, nullptr, mat2, b, 3); }
+
+void hmvm_hip_2_64(double *v, const matrix2_64 *mat2, const double *b) {
+  clear_error();
+  if (!v || !b) { set_error(HMVM_ERR_ARG, "v or b is NULL"); return; }
+  hmvm_plan *p = plan_for64(mat2, 4);
+  if (p) hmvm_plan_apply_host(p, b, v, 1);
+}
+void hmvm_hip_2t_64(double *v, const matrix2_64 *mat2, const double *b) {
+  clear_error();
+  if (!v || !b) { set_error(HMVM_ERR_ARG, "v or b is NULL"); return; }
+  hmvm_plan *p = plan_for64(mat2, 5);
+  if (p) hmvm_plan_apply_host(p, b, v, 1);
+}
 
 void hmvm_hip_release(const void *mat) {
   std::lock_guard<std::mutex> lk(g_cache_mu);

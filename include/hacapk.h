@@ -91,6 +91,48 @@ int writeHmatrix2(const char *fname, const matrix2 *mat2);
 /* Text of an error code. */
 const char *hacapk_strerror(int code);
 
+/* ------------------------------------------------------------------------
+ * int64 extension for matrices with more than INT_MAX stored elements.
+ *
+ * The reference's matrix2 keeps len and the a1/a2 offsets in int
+ * (src_c/hacapk.h:22,26), and its format-2 header stores len as int32
+ * (m_HACApK_hdump.f90:119,143), so neither can describe a matrix of more
+ * than 2^31-1 elements (N = 2^20 with eps = 1e-6 holds about 5e9).
+ * matrix2_64 is matrix2 with int64 len and offsets; rowmat_t is optional.
+ *
+ * On disk, "format 2L" is format 2 with a 20-byte first record
+ * {nd, nlf, ktmax, int64 len}; every other record is as in format 2 (no
+ * single leaf record comes near 2 GiB).  The readers below accept formats 1,
+ * 2 and 2L; they tell 2 from 2L by the length of the first record.
+ * ------------------------------------------------------------------------ */
+typedef struct matrix2_64 {
+  int nd;
+  int nlf;
+  int ktmax;
+  long long len;
+  int *ltmtx, *kt, *ndl, *ndt, *nstrtl, *nstrtt;
+  long long *a1, *a2;        /* element offsets into rowmat / rowmat_t        */
+  double *rowmat, *rowmat_t; /* rowmat_t is NULL unless HACAPK_WITH_ROWMAT_T */
+} matrix2_64;
+
+enum { HACAPK_WITH_ROWMAT_T = 1 };
+
+/* Format 1, 2 or 2L -> matrix2_64 (format detected from the first record). */
+int loadHmatrix_64(const char *fname, matrix2_64 *mat2, int flags);
+/* Format 2L (always the 20-byte header) and format 1, from a matrix2_64. */
+int writeHmatrix2_64(const char *fname, const matrix2_64 *mat2);
+int writeHmatrix1_64(const char *fname, const matrix2_64 *mat2);
+/* Deep copy of a matrix2 into the int64 form (rowmat_t copied if present). */
+int hacapk_widen_matrix2(const matrix2 *src, matrix2_64 *dst);
+void freeHmatrix64(matrix2_64 *mat2);
+
+/* Leaf descriptors of a format 1/2/2L file without its values: fills the
+ * index arrays, ktmax, len and offsets; rowmat and rowmat_t stay NULL.
+ * *data_pos receives the file position of the first data record when the
+ * file is format 2/2L (-1 for format 1, whose data records are interleaved
+ * with the leaf records). */
+int hacapk_scan_file(const char *fname, matrix2_64 *mat2, long long *data_pos, int *format);
+
 #ifdef __cplusplus
 }
 #endif

@@ -47,6 +47,10 @@ void hgen_default_params(hgen_params *p, int n);
 /* Build the H-matrix in memory (matrix2 with rowmat and rowmat_t). */
 int hgen_generate(const hgen_params *p, matrix2 *mat2);
 
+/* Same construction into the int64 form (for matrices beyond 2^31-1
+ * elements); rowmat_t is built only with HACAPK_WITH_ROWMAT_T in flags. */
+int hgen_generate64(const hgen_params *p, matrix2_64 *mat2, int flags);
+
 /* Build the array-of-structs form (with a2t) from a packed matrix2, as the
  * format-1 loader does; the per-leaf arrays are copies. */
 int hgen_matrix_from_matrix2(const matrix2 *mat2, matrix *mat);
@@ -61,6 +65,7 @@ int hgen_kernel_block(const hgen_params *p, int i0, int nr, int j0, int nc, doub
 /* ||A - H||_F / ||A||_F over the leaves (every stride-th leaf when stride>1),
  * with the entries of A recomputed (HACApK_accuracy_leafmtx, :407-466). */
 double hgen_accuracy(const hgen_params *p, const matrix2 *mat2, int stride);
+double hgen_accuracy64(const hgen_params *p, const matrix2_64 *mat2, int stride);
 
 #ifdef __cplusplus
 }

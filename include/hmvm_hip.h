@@ -89,15 +89,17 @@ enum { HMVM_A2_ROWMAJOR = 0, HMVM_A2_TRANSPOSED = 1 };
 
 typedef struct hmvm_options {
   int storage;           /* HMVM_STORAGE_*: element type kept in HBM            */
-  int w1;                /* target elements per phase-1 tile (0: default)       */
-  int w2;                /* target elements per phase-2 tile (0: default)       */
+  int w1;                /* target elements per stage-1 tile (0: 256 KiB worth) */
+  int w2;                /* target elements per dense/stage-2 tile (0: 128 KiB) */
   int block_threads;     /* threads per workgroup, 64..256 step 64 (0: default) */
   int kernel;            /* tile-kernel variant 0..7: bit0 16 loads per batch,  */
                          /* bit1 software pipelining, bit2 non-temporal loads   */
   int order;             /* tile order in a launch: 0 memory, 1 largest first   */
   int mode;              /* 0: three launches (A, B, y sum); 1: one persistent  */
                          /* launch with a device-wide work queue                */
-  int reserved[1];
+  int t_mode;            /* stage-2 vector entries: 0 sum the stage-1 partial   */
+                         /* slots on the fly, 1 read a materialised t (one more */
+                         /* launch), 2 choose per leaf                          */
 } hmvm_options;
 
 typedef struct hmvm_plan_info {
@@ -155,6 +157,8 @@ int hmvm_plan_time(hmvm_plan *plan, int warmup, int iters, hmvm_timing *timing);
  * (used by the autotuner; results are identical across variants up to the
  * summation order inside a tile, which every variant keeps). */
 int hmvm_plan_set_launch(hmvm_plan *plan, int kernel, int block_threads);
+/* The same for one launch only: which = 0 (A: stage 1 + dense) or 1 (B). */
+int hmvm_plan_set_phase_launch(hmvm_plan *plan, int which, int kernel, int block_threads);
 
 /* Several right-hand sides on the matrix cores: Y (+)= A X with X, Y of
  * nd x nrhs doubles, row-major (X[i*nrhs + c]); nrhs = 16, 32 or 64.  FP64
@@ -181,6 +185,88 @@ int hmvm_bandwidth_probe(size_t bytes, int iters, double *read_gbs, double *copy
 /* Host-only check of the schedule a plan would use for this matrix (no GPU
  * access): every element streamed once, every access in bounds. */
 int hmvm_plan_check(const matrix2 *mat2, const hmvm_options *opt, hmvm_plan_info *info);
+
+/* ---- int64 form ---------------------------------------------------------------
+ * Matrices of more than 2^31-1 stored elements (matrix2_64, hacapk.h); the
+ * reference has no counterpart, its int32 len overflows there.  Semantics as
+ * hmvm_plan_create and hmvm_hip_2 / hmvm_hip_2t (accumulate, cached upload,
+ * hmvm_hip_release drops the device copy). */
+int hmvm_plan_create64(const matrix2_64 *mat2, int a2_source, const hmvm_options *opt, hmvm_plan **out);
+void hmvm_hip_2_64(double *v, const matrix2_64 *mat2, const double *b);
+void hmvm_hip_2t_64(double *v, const matrix2_64 *mat2, const double *b);
+
+/* Plan straight from a format 1, 2 or 2L file: the leaf descriptors are read
+ * first, then the values stream through two pinned host buffers into HBM and
+ * are repacked there window by window; the whole matrix is never held in host
+ * memory (replaces loadHmatrix/loadHmatrix2 + upload, src_c/loadmatrix.c:8-232). */
+int hmvm_plan_create_file(const char *fname, const hmvm_options *opt, hmvm_plan **out);
+
+/* One plan over `count` independent matrices of any sizes: they are placed on
+ * the diagonal of one block-diagonal matrix (matrix q owns rows and columns
+ * row_offsets[q] .. row_offsets[q+1]-1 of x and y), so a single apply (one
+ * set of launches, or one graph replay) multiplies all of them.
+ * row_offsets (count+1 ints) may be NULL. */
+int hmvm_plan_create_batch(const matrix2 *const *mats, int count, int a2_source, const hmvm_options *opt,
+                           hmvm_plan **out, int *row_offsets);
+
+/* ---- hipGraph form -------------------------------------------------------------
+ * `count` applies d_y (+)= A d_x captured once and replayed by
+ * hmvm_graph_launch on any stream; d_x/d_y are fixed at capture. */
+typedef struct hmvm_graph hmvm_graph;
+int hmvm_plan_graph_create(hmvm_plan *plan, const double *d_x, double *d_y, int accumulate, int count,
+                           hmvm_graph **out);
+int hmvm_graph_launch(hmvm_graph *graph, void *stream);
+void hmvm_graph_destroy(hmvm_graph *graph);
+/* Times graph replays of `count` applies on the plan's internal vectors;
+ * results per apply (ms_avg/min/median per replay / count; ms_stream with
+ * replays back to back). */
+int hmvm_plan_time_graph(hmvm_plan *plan, int count, int warmup, int iters, hmvm_timing *timing);
+
+/* ---- tile-level measurement and scheduling (autotuner, cost model) ------------
+ * No reference counterpart: the reference chooses its kernel variant by a
+ * human reading timing CSVs (src_cpp/hmvm1.cpp:82-96).
+ *
+ * A tile is the work of one wavefront.  hmvm_plan_tile_features writes
+ * HMVM_NFEAT floats per tile of launch A (which = 0) or B (which = 1):
+ *   [0..2] one-hot kind: stage-1 / dense / stage-2   [3] log2 rows
+ *   [4] log2(1+columns)  [5] log2 panels  [6] log2 lanes per column
+ *   [7] log2(1+elements read incl. lane padding)  [8] log2(1+mean partial
+ *   slots summed per vector entry)  [9] log2 bytes per element
+ *   [10..12] kernel variant bits  [13] log2(block_threads/64)  [14] log2 nrhs
+ * hmvm_plan_profile_tiles runs the launch with a clock-stamping copy of the
+ * tile kernel (same results) and returns each tile's mean duration and start
+ * offset in microseconds, plus the mean launch time.
+ * hmvm_plan_set_tile_order permutes the dispatch order of a launch (results
+ * are unchanged: tiles write disjoint slots); an order that mixes stage-1 and
+ * dense tiles in launch A turns the persistent mode off for the plan. */
+enum { HMVM_NFEAT = 15 };
+int hmvm_plan_tile_features(const hmvm_plan *plan, int which, int nrhs, float *feat);
+int hmvm_plan_profile_tiles(hmvm_plan *plan, int which, int iters, double *tile_us, double *start_us,
+                            double *launch_ms);
+int hmvm_plan_set_tile_order(hmvm_plan *plan, int which, const int *perm);
+
+/* ---- the learned cost model: a small MLP on the GPU ---------------------------
+ * nin -> nh -> nh -> 1, ReLU, FP32; inputs are standardised with the mean and
+ * standard deviation stored in the model.  Forward, backward and Adam steps
+ * are HIP kernels; the host only moves data.  Parameters are laid out as
+ * W1 (nh x nin, row-major), b1 (nh), W2 (nh x nh), b2 (nh), w3 (nh), b3 (1). */
+typedef struct hmvm_mlp hmvm_mlp;
+int hmvm_mlp_create(int nin, int nh, unsigned long long seed, hmvm_mlp **out);
+void hmvm_mlp_destroy(hmvm_mlp *m);
+int hmvm_mlp_num_params(const hmvm_mlp *m);
+int hmvm_mlp_get_params(const hmvm_mlp *m, float *params, float *mean, float *stdev);
+int hmvm_mlp_set_params(hmvm_mlp *m, const float *params, const float *mean, const float *stdev);
+/* Sets mean/stdev from the columns of X (n x nin). */
+int hmvm_mlp_fit_scaler(hmvm_mlp *m, const float *X, int n);
+/* y[i] = f(X[i]) for n rows (host arrays). */
+int hmvm_mlp_predict(hmvm_mlp *m, const float *X, int n, float *y);
+/* Loss 0.5/n * sum (f(X_i) - y_i)^2 and its gradient wrt the parameters
+ * (computed on the GPU, deterministic order). */
+int hmvm_mlp_loss_grad(hmvm_mlp *m, const float *X, const float *y, int n, float *loss, float *grad);
+/* Adam (lr, beta1 0.9, beta2 0.999) over shuffled minibatches; returns the
+ * final full-data loss. */
+int hmvm_mlp_train(hmvm_mlp *m, const float *X, const float *y, int n, int epochs, int batch, float lr,
+                   unsigned long long seed, float *final_loss);
 
 /* ---- Krylov driver -----------------------------------------------------------
  * BiCGSTAB for A u = b with this plan as A, iterating as the reference's

@@ -1,0 +1,80 @@
+"""The learned cost model: HIP forward/backward/Adam kernels against the
+float64 numpy oracle (oracle/cost_mlp.py), and the oracle against finite
+differences of its own loss."""
+import numpy as np
+import pytest
+
+from oracle import cost_mlp
+
+NIN, NH = 15, 64
+
+
+def rand_problem(n=700, seed=0):
+    rng = np.random.default_rng(seed)
+    X = rng.normal(size=(n, NIN)).astype(np.float32) * rng.uniform(0.5, 3, NIN).astype(np.float32) + 2.0
+    w = rng.normal(size=NIN)
+    y = (np.tanh(X @ w / 6.0) + 0.1 * (X[:, 0] > 2)).astype(np.float32)
+    return X, y
+
+
+def rand_params(seed=1):
+    rng = np.random.default_rng(seed)
+    n = NH * NIN + NH + NH * NH + NH + NH + 1
+    return (rng.normal(size=n) * 0.2).astype(np.float32)
+
+
+def test_oracle_gradient_matches_finite_differences():
+    X, y = rand_problem(60, seed=3)
+    X = X.astype(np.float64)
+    p = rand_params(4).astype(np.float64)
+    mu, sd = X.mean(axis=0), X.std(axis=0)
+    loss, g = cost_mlp.loss_grad(p, mu, sd, X, y.astype(np.float64), NIN, NH)
+    rng = np.random.default_rng(5)
+    idx = np.concatenate([rng.choice(NH * NIN, 20, replace=False),                 # W1
+                          NH * NIN + rng.choice(NH, 5, replace=False),             # b1
+                          NH * NIN + NH + rng.choice(NH * NH, 20, replace=False),  # W2
+                          [p.size - 1 - NH - NH + 3, p.size - 1 - 5, p.size - 1]])  # b2, w3, b3
+    h = 1e-6
+    for i in idx:
+        e = np.zeros_like(p)
+        e[i] = h
+        lp, _ = cost_mlp.loss_grad(p + e, mu, sd, X, y, NIN, NH)
+        lm, _ = cost_mlp.loss_grad(p - e, mu, sd, X, y, NIN, NH)
+        fd = (lp - lm) / (2 * h)
+        assert abs(fd - g[i]) <= 1e-6 * max(1.0, abs(fd)), i
+
+
+@pytest.mark.gpu
+def test_hip_forward_and_gradient_match_oracle(gpu, H):
+    X, y = rand_problem(700, seed=6)   # 700 samples: three partial chunks of 256
+    m = H.CostMLP(NIN, NH, seed=9)
+    p = rand_params(7)
+    m.fit_scaler(X)
+    _, mu, sd = m.get()
+    assert np.allclose(mu, X.astype(np.float64).mean(axis=0), rtol=1e-5, atol=1e-5)
+    m.set(params=p)
+    pred = m.predict(X)
+    ref, _ = cost_mlp.forward(p, mu, sd, X, NIN, NH)
+    assert np.max(np.abs(pred - ref)) <= 1e-4 * max(1.0, np.max(np.abs(ref)))
+    loss, g = m.loss_grad(X, y)
+    loss_r, g_r = cost_mlp.loss_grad(p, mu, sd, X, y, NIN, NH)
+    assert abs(loss - loss_r) <= 1e-5 * max(loss_r, 1e-6)
+    assert np.linalg.norm(g - g_r) <= 1e-4 * np.linalg.norm(g_r)
+    # deterministic: the same gradient bit for bit
+    _, g2 = m.loss_grad(X, y)
+    assert np.array_equal(g, g2)
+    m.close()
+
+
+@pytest.mark.gpu
+def test_hip_training_fits(gpu, H):
+    X, y = rand_problem(4000, seed=8)
+    m = H.CostMLP(NIN, NH, seed=2)
+    m.fit_scaler(X)
+    l0, _ = m.loss_grad(X, y)
+    lf = m.train(X, y, epochs=60, batch=256, lr=2e-3, seed=3)
+    assert lf < 0.05 * l0
+    p, mu, sd = m.get()
+    ref, _ = cost_mlp.forward(p, mu, sd, X, NIN, NH)
+    assert np.max(np.abs(m.predict(X) - ref)) <= 1e-4 * max(1.0, np.max(np.abs(ref)))
+    m.close()

@@ -55,7 +55,9 @@ def test_bicgstab_graph_equals_eager_and_cpu(gpu, H, O, mode):
     assert r_e["iters"] == r_g["iters"] == 8 and r_g["graph_used"] == 1 and r_g["hmvms"] == 17
     assert np.array_equal(u_e, u_g)
     u_c, rel_c = cpu_bicgstab(O, h, b, np.zeros(h.nd), 8)
-    assert np.linalg.norm(u_e - u_c) <= 1e-9 * np.linalg.norm(u_c)
+    # eight BiCGSTAB steps amplify the ~1e-16 HMVM summation-order difference
+    # (observed 2.5e-9 on MI355X); a wrong step of the recurrence gives O(1)
+    assert np.linalg.norm(u_e - u_c) <= 1e-7 * np.linalg.norm(u_c)
     assert abs(r_e["relres"] - rel_c) <= 1e-6 * max(rel_c, 1e-30) + 1e-14
     p.close()
 
