@@ -1,16 +1,30 @@
-"""HMVM benchmark: ms per H-matrix-vector multiply at N=101,250 on one MI355X.
+"""HMVM benchmark on one MI355X.
 
-Workload (one "step" = one HMVM y = A x of one right-hand side): the H-matrix of
-1/(4 pi r) on a jittered grid of the unit-cube surface, N = 101,250
-(6 faces x 135 x 125), leaves of <= 32 points, eta = 2, ACA eps = 1e-4; x ~ U[-1,1].
-The matrix is generated, written as hmatrix_1.bin, read back through
-loadHmatrix (the drop-in path) and uploaded once; the timed region is the GPU
-launches of K back-to-back HMVMs with x and y resident in HBM.
+Default workload (one "step" = one HMVM y = A x of one right-hand side): the
+H-matrix of 1/(4 pi r) on a jittered grid of the unit-cube surface, N = 101,250
+(6 faces x 135 x 125), leaves of <= 32 points, eta = 2, ACA eps = 1e-4;
+x ~ U[-1,1].  The matrix is generated, written as hmatrix_1.bin, read back
+through loadHmatrix (the drop-in path) and uploaded once; the timed region is
+the GPU launches of K back-to-back HMVMs with x and y resident in HBM.
 
-Reported: FP64 storage (the headline `value`, ms/HMVM), FP32 storage (ms and
-relative l2 error vs the FP64 oracle), roofline of the dominant kernel from
-HIP events, and the CPU baseline: the oracle's OpenMP HMVM (restating
-src_c/hmvm_omp.c) on the host cores, plus the sequential oracle.
+Reported: FP64 storage (the headline `value`, ms/HMVM), FP32 and BF16 storage
+(ms and relative l2 error vs the FP64 oracle), the roofline of the dominant
+kernel from HIP events on the plan's stream, HBM traffic of that kernel from
+the committed rocprofv3 counter summary when it matches this workload, and the
+CPU baseline: the oracle's OpenMP HMVM (restating src_c/hmvm_omp.c) on the
+host cores, plus the sequential oracle.
+
+Other workloads (--config), one JSON line each:
+  c1    N=4,096 sphere: the four drop-in variants 1/1t/2/2t on the GPU and on
+        the CPU (sequential and OpenMP)
+  c3    N=101,250 cube, 32 right-hand sides on the matrix cores: FP64, FP32 and
+        BF16 storage, relative error vs the FP64 oracle, TFLOP/s
+  c4    64 independent H-matrices (N log-uniform in [2,000, 50,000]) in one
+        batched plan, FP32, replayed as one hipGraph; schedule from the
+        exhaustive autotuner vs the learned cost model
+  c5    N=1,048,576 sphere, eps 1e-6, leaf 64 (int64 offsets): a 100-HMVM
+        BiCGSTAB loop captured as a hipGraph, FP64 vs FP32 vs BF16 storage
+  tune  N=101,250 cube: exhaustive autotuner vs learned cost model
 """
 import argparse
 import json
@@ -25,6 +39,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+MFMA_PEAK_TFLOPS = {"f64": 78.6, "f32": 157.3, "bf16": 2500.0}
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_c2.json")
 
 
 def log(*a):
@@ -42,11 +58,344 @@ def cpu_model():
     return platform.processor()
 
 
+def rel(a, b):
+    import numpy as np
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def base_line(metric, value, unit, args, dtype, config, higher=False):
+    return {"metric": metric, "value": value, "unit": unit, "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": value if unit.startswith("ms") else None,
+            "higher_is_better": higher, "scaling": "weak", "vs_baseline": None, "dtype": dtype,
+            "data": "synthetic: generated H-matrices (1/(4 pi r) point kernel), vectors ~ U[-1,1]",
+            "config": config}
+
+
+def omp_time(fn, h, x, runs=10):
+    times = []
+    for _ in range(runs):  # hmvm_omp_bench protocol: 10 runs, the last 5 averaged
+        t0 = time.perf_counter()
+        fn(h, x)
+        times.append(time.perf_counter() - t0)
+    return 1e3 * sum(times[runs // 2:]) / (runs - runs // 2)
+
+
+# ---- C2: the headline ---------------------------------------------------------------
+def run_c2(args, H, np):
+    probe_read, probe_copy = H.bandwidth_probe(1 << 30, 5)
+    log("bandwidth probe: read %.0f GB/s, copy %.0f GB/s" % (probe_read, probe_copy))
+    t0 = time.time()
+    g = H.generate(args.n, geometry=args.geometry, eps=args.eps, leaf=args.leaf)
+    log("generated N=%d nlf=%d len=%d ktmax=%d in %.1fs" % (g.nd, g.mat2.nlf, g.mat2.len, g.mat2.ktmax,
+                                                              time.time() - t0))
+    if args.in_memory:
+        h = g
+    else:
+        tmpd = tempfile.mkdtemp(prefix="hmvm_bench_")
+        fname = os.path.join(tmpd, "hmatrix_1.bin")
+        t0 = time.time()
+        H.writeHmatrix1(fname, g)
+        g.free()
+        h = H.loadHmatrix(fname)
+        os.remove(fname)
+        log("hmatrix_1.bin written and loaded in %.1fs" % (time.time() - t0))
+    nd, length = h.nd, h.mat2.len
+    x = np.random.default_rng(20261202).uniform(-1.0, 1.0, nd)
+
+    results, ys = {}, {}
+    for storage in ("f64", "f32", "bf16"):
+        t0 = time.time()
+        plan = H.Plan(h, storage=storage, w1=args.w1, w2=args.w2)
+        info = plan.info()
+        ys[storage] = plan.apply_host(x)
+        tm = plan.time(warmup=args.warmup, iters=args.steps)
+        log("%s: plan %.1fs tiles=%d+%d panels=%d ms/HMVM stream=%.4f evented=%.4f (A %.4f tsum %.4f B %.4f "
+            "ysum %.4f)" % (storage, time.time() - t0, info["ntiles_a"], info["ntiles_b"], info["npanels"],
+                            tm["ms_stream"], tm["ms_avg"], tm["ms_a"], tm["ms_tsum"], tm["ms_b"], tm["ms_ysum"]))
+        results[storage] = (info, tm)
+        plan.close()
+
+    info64, tm64 = results["f64"]
+    ms = tm64["ms_stream"]
+    whole_gbs = info64["algorithmic_bytes"] / (ms * 1e-3) / 1e9
+    # dominant kernel: the longer of the two tile launches; algorithmic bytes =
+    # 8 B per stored element it streams (elements per launch from the plan)
+    if tm64["ms_b"] >= tm64["ms_a"]:
+        kname, kms, kbytes = "tile_kernel launch B (low-rank stage 2)", tm64["ms_b"], 8 * info64["elems_b"]
+    else:
+        kname, kms, kbytes = "tile_kernel launch A (low-rank stage 1 + dense)", tm64["ms_a"], 8 * info64["elems_a"]
+    k_gbs = kbytes / (kms * 1e-3) / 1e9
+    traffic = None
+    try:
+        with open(TRAFFIC_FILE) as f:
+            tr = json.load(f)
+        if tr.get("N") == nd and tr.get("len") == length and tr.get("launch") == ("B" if "B" in kname else "A"):
+            traffic = tr["hbm_bytes_per_launch"]
+    except (OSError, ValueError, KeyError):
+        pass
+
+    def form(st):
+        info, tm = results[st]
+        return {"ms": round(tm["ms_stream"], 5), "GBps": round(info["algorithmic_bytes"] / tm["ms_stream"] / 1e6, 1),
+                "speedup_vs_fp64": round(ms / tm["ms_stream"], 3), "rel_err_vs_fp64_hip": rel(ys[st], ys["f64"]),
+                "launch_ms": {"tiles_a": tm["ms_a"], "tsum": tm["ms_tsum"], "tiles_b": tm["ms_b"],
+                              "ysum": tm["ms_ysum"]}}
+
+    forms = {st: form(st) for st in ("f64", "f32", "bf16")}
+    cpu = None
+    if not args.no_cpu_baseline:
+        import oracle as O  # CPU baseline leg: the oracle, timed on the host cores
+        t0 = time.time()
+        y_or = O.seq_2(h, x)
+        seq_s = time.time() - t0
+        for st in forms:
+            forms[st]["rel_err_vs_oracle"] = rel(ys[st], y_or)
+        omp_ms = omp_time(O.omp_2, h, x)
+        cpu = {"value": round(omp_ms, 3), "unit": "ms/HMVM", "cores": O.omp_threads(), "kind": "port",
+               "sample": "full workload (N=%d, len=%d): oracle restatement of hmvm_omp_2, 10 runs, mean of the last "
+                         "5; sequential oracle hmvm_seq_2 %.1f ms; host %s" % (nd, length, 1e3 * seq_s, cpu_model())}
+        log("cpu: omp %.2f ms (%d threads), seq %.1f ms; rel err f64 %.2e f32 %.2e bf16 %.2e" %
+            (omp_ms, O.omp_threads(), 1e3 * seq_s, forms["f64"]["rel_err_vs_oracle"],
+             forms["f32"]["rel_err_vs_oracle"], forms["bf16"]["rel_err_vs_oracle"]))
+
+    line = base_line("ms/HMVM & % HBM peak at N=101,250: FP64 plain vs FP32-stored, rel err vs FP64",
+                     round(ms, 5), "ms/HMVM", args, "f64",
+                     {"workload": "HMVM y=A*x, N=%d %s surface, leaf<=%d, eta=2, ACA eps=%g, 1 RHS" %
+                                  (nd, args.geometry, args.leaf, args.eps),
+                      "N": nd, "nlf": info64["nlf"], "len": length, "ktmax": h.mat2.ktmax,
+                      "bytes_fp64": info64["algorithmic_bytes"], "parallelism": "single GPU"})
+    line["data"] = "synthetic: generated H-matrix (1/(4 pi r), cube surface), x ~ U[-1,1] seed 20261202"
+    line.update({
+        "hbm_frac_whole_hmvm": round(whole_gbs / HBM_PEAK_GBS, 4),
+        "fp64": forms["f64"], "fp32": forms["f32"], "bf16": forms["bf16"],
+        "roofline": {"bound": "hbm", "kernel": kname, "achieved": round(k_gbs, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(k_gbs / HBM_PEAK_GBS, 4), "traffic": traffic},
+        "achievable_GBps": {"read_stream": round(probe_read, 1), "copy": round(probe_copy, 1)},
+        "cpu_baseline": cpu,
+    })
+    return line
+
+
+# ---- C1: the reference's own CPU-runnable case -----------------------------------------
+def run_c1(args, H, np):
+    import oracle as O
+    g = H.generate(4096, geometry="sphere", eps=1e-4, leaf=32)
+    tmpd = tempfile.mkdtemp(prefix="hmvm_c1_")
+    f1, f2 = os.path.join(tmpd, "hmatrix_1.bin"), os.path.join(tmpd, "hmatrix_2.bin")
+    H.writeHmatrix1(f1, g)
+    H.writeHmatrix2(f2, g)
+    h = H.loadHmatrix(f1)
+    h2 = H.loadHmatrix2(f2)
+    x = np.random.default_rng(20261203).uniform(-1.0, 1.0, h.nd)
+    forms = {}
+    for var, src, seq, omp in (("1", "a2", O.seq_1, O.omp_1), ("1t", "a2t", O.seq_1t, O.omp_1t),
+                               ("2", "rowmat", O.seq_2, O.omp_2), ("2t", "rowmat_t", O.seq_2t, O.omp_2t)):
+        p = H.Plan(h if var in ("1", "1t") else h2, source=src)
+        y = p.apply_host(x)
+        tm = p.time(args.warmup, args.steps)
+        t0 = time.perf_counter()
+        y_or = seq(h if var in ("1", "1t") else h2, x)
+        seq_ms = 1e3 * (time.perf_counter() - t0)
+        omp_ms = omp_time(omp, h if var in ("1", "1t") else h2, x)
+        forms[var] = {"hip_ms": round(tm["ms_stream"], 5), "hip_ms_graph": round(p.time_graph(1, 3, 20)["ms_stream"], 5),
+                      "cpu_seq_ms": round(seq_ms, 3), "cpu_omp_ms": round(omp_ms, 3),
+                      "rel_err_vs_oracle": rel(y, y_or),
+                      "effective_GBps": round(p.info()["algorithmic_bytes"] / tm["ms_stream"] / 1e6, 1)}
+        p.close()
+        log("c1 %s: %s" % (var, forms[var]))
+    line = base_line("ms/HMVM at N=4,096 (sphere): HIP variants 1/1t/2/2t vs CPU seq/omp", forms["2"]["hip_ms"],
+                     "ms/HMVM", args, "f64",
+                     {"workload": "HMVM y=A*x, N=4096 Fibonacci sphere, leaf<=32, eta=2, ACA eps=1e-4, 1 RHS",
+                      "len": h2.mat2.len, "nlf": h2.mat2.nlf})
+    line["variants"] = forms
+    line["note"] = "matrix (%.1f MB) fits in the 256 MB Infinity Cache: effective GB/s, not HBM" % (
+        8e-6 * h2.mat2.len)
+    line["cpu_baseline"] = {"value": forms["2"]["cpu_omp_ms"], "unit": "ms/HMVM", "cores": O.omp_threads(),
+                            "kind": "port", "sample": "full workload, oracle hmvm_omp_2, mean of the last 5 of 10"}
+    return line
+
+
+# ---- C3: 32 right-hand sides on the matrix cores ---------------------------------------
+def run_c3(args, H, np):
+    import oracle as O
+    nrhs = 32
+    h = H.generate(101250, geometry="cube", eps=1e-4)
+    X = np.random.default_rng(20261204).uniform(-1.0, 1.0, (h.nd, nrhs))
+    t0 = time.time()
+    Y_or = np.stack([O.omp_2(h, np.ascontiguousarray(X[:, c])) for c in range(nrhs)], axis=1)
+    log("c3: FP64 oracle (OpenMP) for %d columns in %.1fs" % (nrhs, time.time() - t0))
+    forms = {}
+    for st in ("f64", "f32", "bf16"):
+        p = H.Plan(h, storage=st)
+        info = p.info()
+        Y = p.apply_multi_host(X)
+        tm = p.time_multi(nrhs, args.warmup, args.steps)
+        ms = tm["ms_stream"]
+        tflops = 2.0 * info["len"] * nrhs / (ms * 1e-3) / 1e12
+        b_alg = info["elem_bytes"] * info["len"] + 16 * h.nd * nrhs
+        forms[st] = {"ms": round(ms, 4), "rel_err_vs_fp64_oracle": rel(Y, Y_or), "TFLOPs": round(tflops, 2),
+                     "mfma_frac_of_dense_peak": round(tflops / MFMA_PEAK_TFLOPS[st], 4),
+                     "hbm_frac": round(b_alg / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                     "launch_ms": {"tiles_a": tm["ms_a"], "tsum": tm["ms_tsum"], "tiles_b": tm["ms_b"],
+                                   "ysum": tm["ms_ysum"]}}
+        log("c3 %s: %s" % (st, forms[st]))
+        p.close()
+    line = base_line("ms per 32-RHS HMVM at N=101,250: FP64 / FP32 (MFMA) / BF16 (MFMA) storage, rel err vs FP64",
+                     forms["f64"]["ms"], "ms/HMVM(32 RHS)", args, "f64",
+                     {"workload": "HMVM Y=A*X, X N x 32, N=101250 cube, ACA eps=1e-4", "len": h.mat2.len})
+    line["forms"] = forms
+    bound = max(8.0 * h.mat2.len / (HBM_PEAK_GBS * 1e9), 2.0 * h.mat2.len * nrhs / (MFMA_PEAK_TFLOPS["f64"] * 1e12))
+    line["roofline"] = {"bound": "hbm", "achieved": round(bound * 1e3 / forms["f64"]["ms"] * HBM_PEAK_GBS, 1),
+                        "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(bound * 1e3 / forms["f64"]["ms"], 4),
+                        "traffic": None}
+    return line
+
+
+# ---- autotuning helpers -------------------------------------------------------------------
+def train_cost_model(H, np, storages, seed=0):
+    """Trains the MLP on tile timings of generated matrices that are not part of any benchmark."""
+    from hacapk_hmvm_amd import autotune as AT
+    Xs, ys, recs = [], [], []
+    specs = [(20000, "sphere", 1e-4), (36000, "cube", 1e-6), (28000, "cube", 1e-4), (44000, "sphere", 1e-6)]
+    for i, (n, geo, eps) in enumerate(specs):
+        m = H.generate(n, geometry=geo, eps=eps, seed=1000 + i)
+        for st in storages:
+            X, y, r = AT.collect(m, st, w_scales=((1.0, 1.0), (0.5, 2.0), (2.0, 0.5)), iters=2, max_tiles=6000,
+                                 seed=seed + i)
+            Xs.append(X)
+            ys.append(y)
+            recs += r
+        m.free()
+    X, y = np.concatenate(Xs), np.concatenate(ys)
+    model = AT.CostModel(seed=seed + 5)
+    t0 = time.time()
+    loss = model.fit(X, y, recs, epochs=60)
+    log("cost model: %d tile samples, training loss %.4f (log2 us)^2, %.1fs on the GPU" % (len(y), loss,
+                                                                                         time.time() - t0))
+    return model, X, y
+
+
+def tune_report(H, np, h, storage, model, args, X_train=None, y_train=None):
+    from hacapk_hmvm_amd import autotune as AT
+    p0 = AT.default_plan(h, storage)
+    ms_default = p0.time(args.warmup, args.steps)["ms_stream"]
+    p0.close()
+    best, results, t_ex = AT.exhaustive(h, storage, warmup=2, iters=max(5, args.steps // 4))
+    p_ex = H.Plan(h, storage=storage, w1=best["w1"], w2=best["w2"])
+    AT._apply_choice(p_ex, best)
+    ms_ex = p_ex.time(args.warmup, args.steps)["ms_stream"]
+    p_ex.close()
+    choice, p_l, t_l = AT.learned(h, storage, model)
+    ms_l = p_l.time(args.warmup, args.steps)["ms_stream"]
+    out = {"ms_default": round(ms_default, 5), "ms_exhaustive": round(ms_ex, 5), "ms_learned": round(ms_l, 5),
+           "learned_over_exhaustive": round(ms_l / ms_ex, 4), "gain_over_default": round(ms_default / ms_l, 4),
+           "search_s_exhaustive": round(t_ex, 2), "search_s_learned": round(t_l, 2),
+           "candidates_timed_exhaustive": len(results), "exhaustive_choice": best,
+           "learned_choice": {k: v for k, v in choice.items()}}
+    if X_train is not None:
+        Xt, yt, _ = AT.collect(h, storage, iters=2, max_tiles=20000, seed=77)
+        out["heldout_abs_log2_err"] = {"mlp": round(float(np.mean(np.abs(model.mlp.predict(Xt) - yt))), 4),
+                                       "linear": round(float(np.mean(np.abs(
+                                           AT.linear_baseline(X_train, y_train)(Xt) - yt))), 4)}
+    p_l.close()
+    return out
+
+
+def run_tune(args, H, np):
+    model, X, y = train_cost_model(H, np, ("f64", "f32"))
+    h = H.generate(101250, geometry="cube", eps=1e-4)
+    rep = {st: tune_report(H, np, h, st, model, args, X, y) for st in ("f64", "f32")}
+    for st in rep:
+        log("tune %s: %s" % (st, rep[st]))
+    line = base_line("learned schedule / exhaustive schedule, ms/HMVM at N=101,250", rep["f64"]["ms_learned"],
+                     "ms/HMVM", args, "f64", {"workload": "HMVM y=A*x, N=101250 cube, ACA eps=1e-4, 1 RHS"})
+    line["tuning"] = rep
+    return line
+
+
+# ---- C4: 64 batched instances ---------------------------------------------------------------
+def run_c4(args, H, np):
+    import oracle as O
+    rng = np.random.default_rng(20261205)
+    ns = np.exp(rng.uniform(np.log(2000), np.log(50000), 64)).astype(int)
+    eps = rng.choice([1e-4, 1e-6], 64)
+    t0 = time.time()
+    mats = [H.generate(int(n), geometry=("sphere", "cube")[i % 2], eps=float(e), seed=3000 + i)
+            for i, (n, e) in enumerate(zip(ns, eps))]
+    total_len = sum(m.mat2.len for m in mats)
+    log("c4: 64 matrices, N %d..%d, sum N %d, sum len %d, generated in %.1fs" % (ns.min(), ns.max(), ns.sum(),
+                                                                                 total_len, time.time() - t0))
+    storage = "f32"
+    p = H.Plan(mats, storage=storage)
+    offs = p.row_offsets
+    x = np.random.default_rng(20261206).uniform(-1.0, 1.0, p.nd)
+    y = p.apply_host(x)
+    errs = []
+    for q, m in enumerate(mats):
+        xs = np.ascontiguousarray(x[offs[q]:offs[q + 1]])
+        errs.append(rel(y[offs[q]:offs[q + 1]], O.omp_2(m, xs)))
+    tm = p.time(args.warmup, args.steps)
+    tg = p.time_graph(1, args.warmup, args.steps)
+    info = p.info()
+    p.close()
+    model, X, yy = train_cost_model(H, np, (storage,))
+    tuning = tune_report(H, np, mats, storage, model, args, X, yy)
+    line = base_line("ms per batched HMVM of 64 H-matrices (one launch set, hipGraph), FP32 storage",
+                     round(tg["ms_stream"], 5), "ms/batch", args, "f32",
+                     {"workload": "64 independent HMVMs, N log-uniform [2000, 50000], sphere/cube alternating, "
+                                  "eps in {1e-4, 1e-6}, FP32 storage, 1 RHS each",
+                      "sum_N": int(ns.sum()), "sum_len": int(total_len)})
+    line.update({"ms_launches": round(tm["ms_stream"], 5), "ms_graph": round(tg["ms_stream"], 5),
+                 "hbm_frac": round(info["algorithmic_bytes"] / (tg["ms_stream"] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                 "rel_err_vs_fp64_oracle_max": max(errs), "rel_err_vs_fp64_oracle_median": float(np.median(errs)),
+                 "tuning": tuning})
+    return line
+
+
+# ---- C5: N = 2^20 with int64 offsets, BiCGSTAB graph loop ------------------------------------
+def run_c5(args, H, np):
+    import oracle as O
+    n = args.n if args.n != 101250 else 1048576
+    t0 = time.time()
+    h = H.generate64(n, geometry="sphere", eps=1e-6, leaf=64)
+    log("c5: N=%d nlf=%d len=%d (%.1f GB FP64) ktmax=%d generated in %.1fs" % (
+        h.nd, h.mat2.nlf, h.mat2.len, 8e-9 * h.mat2.len, h.mat2.ktmax, time.time() - t0))
+    x = np.random.default_rng(20261207).uniform(-1.0, 1.0, h.nd)
+    t0 = time.time()
+    y_or = O.omp_2_64(h, x)
+    log("c5: FP64 oracle (OpenMP) in %.1fs" % (time.time() - t0))
+    b = np.random.default_rng(20261208).uniform(-1.0, 1.0, h.nd)
+    forms = {}
+    for st in ("f64", "f32", "bf16"):
+        t0 = time.time()
+        p = H.Plan(h, storage=st)
+        info = p.info()
+        t_plan = time.time() - t0
+        y = p.apply_host(x)
+        tm = p.time(3, 10)
+        u, res = p.bicgstab(b, max_iter=50, tol=0.0, use_graph=True, check_every=0)
+        forms[st] = {"ms_hmvm": round(tm["ms_stream"], 4), "rel_err_vs_fp64_oracle": rel(y, y_or),
+                     "hbm_frac": round(info["algorithmic_bytes"] / (tm["ms_stream"] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                     "bicgstab": {"iters": res["iters"], "hmvms": res["hmvms"], "ms_loop": round(res["ms_loop"], 3),
+                                  "ms_per_hmvm_in_loop": round(res["ms_loop"] / (2 * res["iters"]), 4),
+                                  "relres": res["relres"], "graph": res["graph_used"]},
+                     "plan_s": round(t_plan, 1), "device_GB": round(info["device_bytes"] / 1e9, 2)}
+        log("c5 %s: %s" % (st, forms[st]))
+        p.close()
+    line = base_line("ms/HMVM in a 100-HMVM hipGraph BiCGSTAB loop at N=1,048,576 (eps 1e-6): FP64 vs FP32 vs BF16",
+                     forms["f64"]["bicgstab"]["ms_per_hmvm_in_loop"], "ms/HMVM", args, "f64",
+                     {"workload": "BiCGSTAB 50 iterations (2 HMVMs each), N=%d Fibonacci sphere, leaf<=64, eta=2, "
+                                  "ACA eps=1e-6, int64 offsets" % h.nd, "len": int(h.mat2.len)})
+    line["forms"] = forms
+    return line
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--config", default="c2", choices=["c1", "c2", "c3", "c4", "c5", "tune"])
     ap.add_argument("--n", type=int, default=101250)
     ap.add_argument("--geometry", default="cube")
     ap.add_argument("--eps", type=float, default=1e-4)
@@ -69,111 +418,8 @@ def main():
 
     if H.device_count() < 1:
         raise SystemExit("bench.py: no HIP device visible")
-
-    # achievable bandwidth of plain streaming kernels on this device (1 GiB, read and copy)
-    probe_read, probe_copy = H.bandwidth_probe(1 << 30, 5)
-    log("bandwidth probe: read %.0f GB/s, copy %.0f GB/s" % (probe_read, probe_copy))
-
-    t0 = time.time()
-    g = H.generate(args.n, geometry=args.geometry, eps=args.eps, leaf=args.leaf)
-    log("generated N=%d nlf=%d len=%d ktmax=%d in %.1fs" % (g.nd, g.mat2.nlf, g.mat2.len, g.mat2.ktmax,
-                                                              time.time() - t0))
-    if args.in_memory:
-        h = g
-    else:
-        tmpd = tempfile.mkdtemp(prefix="hmvm_bench_")
-        fname = os.path.join(tmpd, "hmatrix_1.bin")
-        t0 = time.time()
-        H.writeHmatrix1(fname, g)
-        g.free()
-        h = H.loadHmatrix(fname)
-        os.remove(fname)
-        log("hmatrix_1.bin written and loaded in %.1fs" % (time.time() - t0))
-    nd, length = h.nd, h.mat2.len
-    x = np.random.default_rng(20261202).uniform(-1.0, 1.0, nd)
-
-    results = {}
-    ys = {}
-    for storage in ("f64", "f32"):
-        t0 = time.time()
-        plan = H.Plan(h, storage=storage, w1=args.w1, w2=args.w2)
-        info = plan.info()
-        ys[storage] = plan.apply_host(x)
-        tm = plan.time(warmup=args.warmup, iters=args.steps)
-        log("%s: plan %.1fs tiles=%d+%d panels=%d ms/HMVM stream=%.4f evented=%.4f (p1 %.4f tsum %.4f p2 %.4f "
-            "ysum %.4f)" % (storage, time.time() - t0, info["ntiles_a"], info["ntiles_b"], info["npanels"],
-                            tm["ms_stream"], tm["ms_avg"], tm["ms_a"], tm["ms_tsum"], tm["ms_b"],
-                            tm["ms_ysum"]))
-        results[storage] = (info, tm)
-        plan.close()
-
-    info64, tm64 = results["f64"]
-    info32, tm32 = results["f32"]
-    eb = 8
-    ms = tm64["ms_stream"]
-    b_alg = info64["algorithmic_bytes"]
-    whole_gbs = b_alg / (ms * 1e-3) / 1e9
-    # dominant kernel: the larger of the two tile launches
-    if tm64["ms_b"] >= tm64["ms_a"]:
-        kname, kms, kbytes = "tile_kernel(launch B: low-rank stage 2)", tm64["ms_b"], eb * info64["elems_b"]
-    else:
-        kname, kms, kbytes = "tile_kernel(launch A: low-rank stage 1 + dense)", tm64["ms_a"], eb * info64["elems_a"]
-    k_gbs = kbytes / (kms * 1e-3) / 1e9
-    f32_ms = tm32["ms_stream"]
-    f32_gbs = info32["algorithmic_bytes"] / (f32_ms * 1e-3) / 1e9
-    rel_f32_vs_f64hip = float(np.linalg.norm(ys["f32"] - ys["f64"]) / np.linalg.norm(ys["f64"]))
-
-    cpu = None
-    rel64 = rel32 = None
-    if not args.no_cpu_baseline:
-        import oracle as O  # CPU baseline leg: the oracle, timed on the host cores
-        t0 = time.time()
-        y_or = O.seq_2(h, x)
-        seq_s = time.time() - t0
-        rel64 = float(np.linalg.norm(ys["f64"] - y_or) / np.linalg.norm(y_or))
-        rel32 = float(np.linalg.norm(ys["f32"] - y_or) / np.linalg.norm(y_or))
-        times = []
-        for _ in range(10):  # hmvm_omp_bench protocol: 10 runs, the last 5 averaged
-            t0 = time.perf_counter()
-            O.omp_2(h, x)
-            times.append(time.perf_counter() - t0)
-        omp_ms = 1e3 * sum(times[5:]) / 5
-        cpu = {"value": round(omp_ms, 3), "unit": "ms/HMVM", "cores": O.omp_threads(), "kind": "port",
-               "sample": "full workload (N=%d, len=%d), oracle hmvm_omp_2 restatement, 10 runs, mean of last 5; "
-                         "sequential oracle hmvm_seq_2 %.1f ms; host %s" % (nd, length, 1e3 * seq_s, cpu_model())}
-        log("cpu: omp %.2f ms (%d threads), seq %.1f ms; rel err f64 %.2e f32 %.2e" %
-            (omp_ms, O.omp_threads(), 1e3 * seq_s, rel64, rel32))
-
-    line = {
-        "metric": "ms/HMVM & % HBM peak at N=101,250: FP64 plain vs FP32-stored, rel err vs FP64",
-        "value": round(ms, 5),
-        "unit": "ms/HMVM",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": round(ms, 5),
-        "higher_is_better": False,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f64",
-        "data": "synthetic: generated H-matrix (1/(4 pi r), cube surface), x ~ U[-1,1] seed 20261202",
-        "config": {"workload": "HMVM y=A*x, N=%d %s surface, leaf<=%d, eta=2, ACA eps=%g, 1 RHS" %
-                               (nd, args.geometry, args.leaf, args.eps),
-                   "N": nd, "nlf": info64["nlf"], "len": length, "ktmax": h.mat2.ktmax,
-                   "bytes_fp64": info64["algorithmic_bytes"], "parallelism": "single GPU"},
-        "hbm_frac_whole_hmvm": round(whole_gbs / HBM_PEAK_GBS, 4),
-        "fp64": {"ms": round(ms, 5), "ms_evented": round(tm64["ms_avg"], 5), "GBps": round(whole_gbs, 1),
-                 "rel_err_vs_oracle": rel64,
-                 "launch_ms": {"tiles_a": tm64["ms_a"], "tsum": tm64["ms_tsum"], "tiles_b": tm64["ms_b"],
-                              "ysum": tm64["ms_ysum"]}},
-        "fp32": {"ms": round(f32_ms, 5), "GBps": round(f32_gbs, 1), "speedup_vs_fp64": round(ms / f32_ms, 3),
-                 "rel_err_vs_oracle": rel32, "rel_err_vs_fp64_hip": rel_f32_vs_f64hip},
-        "roofline": {"bound": "hbm", "kernel": kname, "achieved": round(k_gbs, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(k_gbs / HBM_PEAK_GBS, 4), "traffic": None},
-        "achievable_GBps": {"read_stream": round(probe_read, 1), "copy": round(probe_copy, 1)},
-        "cpu_baseline": cpu,
-    }
-    print(json.dumps(line), flush=True)
+    fn = {"c1": run_c1, "c2": run_c2, "c3": run_c3, "c4": run_c4, "c5": run_c5, "tune": run_tune}[args.config]
+    print(json.dumps(fn(args, H, np)), flush=True)
 
 
 if __name__ == "__main__":

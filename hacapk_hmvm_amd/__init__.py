@@ -193,6 +193,9 @@ def _load():
         "hmvm_plan_profile_tiles": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, c_double_p,
                                                    c_double_p, c_double_p]),
         "hmvm_plan_set_tile_order": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, c_int_p]),
+        "hmvm_tile_occupancy": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, c_int_p]),
+        "hmvm_schedule_export_batch": (ctypes.c_int, [P(P(matrix2)), ctypes.c_int, ctypes.c_int, P(hmvm_options),
+                                                      P(hmvm_schedule)]),
         "hmvm_mlp_create": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_ulonglong, P(ctypes.c_void_p)]),
         "hmvm_mlp_destroy": (None, [ctypes.c_void_p]),
         "hmvm_mlp_num_params": (ctypes.c_int, [ctypes.c_void_p]),
@@ -438,6 +441,30 @@ class HMatrix64:
         return out
 
 
+def leaf_subset(h, idx):
+    """An HMatrix64 view of the leaves ``idx`` of ``h`` (HMatrix or HMatrix64):
+    same matrix order and value buffer, only the selected leaves.  The view
+    borrows ``h``'s values (keep ``h`` alive) and frees nothing itself."""
+    idx = np.asarray(idx, dtype=np.int64)
+    a = h.arrays()
+    v = HMatrix64()
+    v._keep = {k: np.ascontiguousarray(a[k][idx], dtype=np.int32)
+               for k in ("ltmtx", "kt", "ndl", "ndt", "nstrtl", "nstrtt")}
+    v._keep.update({k: np.ascontiguousarray(a[k][idx], dtype=np.int64) for k in ("a1", "a2")})
+    v._parent = h
+    m = v.mat2
+    m.nd, m.nlf, m.len = h.mat2.nd, len(idx), h.mat2.len
+    m.ktmax = int(v._keep["kt"].max()) if len(idx) else 0
+    for k in ("ltmtx", "kt", "ndl", "ndt", "nstrtl", "nstrtt"):
+        setattr(m, k, v._keep[k].ctypes.data_as(c_int_p))
+    for k in ("a1", "a2"):
+        setattr(m, k, v._keep[k].ctypes.data_as(ctypes.POINTER(ctypes.c_longlong)))
+    m.rowmat = h.mat2.rowmat
+    m.rowmat_t = h.mat2.rowmat_t
+    v.has_mat2 = False  # borrowed: never freed through this object
+    return v
+
+
 def loadHmatrix_64(fname, with_rowmat_t=False):
     """Format 1, 2 or 2L -> HMatrix64."""
     h = HMatrix64()
@@ -556,6 +583,16 @@ def device_count():
     return lib.hmvm_device_count()
 
 
+def tile_occupancy(storage, kernel, block_threads):
+    """Resident wavefronts of the tile kernel on the current device."""
+    w = ctypes.c_int()
+    rc = lib.hmvm_tile_occupancy(STORAGE[storage] if isinstance(storage, str) else storage, kernel, block_threads,
+                                 ctypes.byref(w))
+    if rc:
+        raise HMVMError("hmvm_tile_occupancy: %s" % lib.hmvm_hip_last_error_message().decode())
+    return w.value
+
+
 # ---- device-resident plan ---------------------------------------------------------------
 def bandwidth_probe(nbytes=2 << 30, iters=10):
     """Achievable (read, copy) GB/s of this device with plain streaming kernels."""
@@ -612,11 +649,17 @@ REPACK_DT = np.dtype([("src_off", "<i8"), ("dst_off", "<i8"), ("rows", "<i4"), (
 
 
 def schedule(h, source="rowmat", **kw):
-    """The tile schedule a plan would run for this matrix, as numpy arrays (host only)."""
+    """The tile schedule a plan would run for this matrix (or list of matrices,
+    as a batch), as numpy arrays (host only)."""
     s = hmvm_schedule()
     o = options(**kw)
-    rc = lib.hmvm_schedule_export(ctypes.byref(h.mat2), 1 if source == "rowmat_t" else 0, ctypes.byref(o),
-                                  ctypes.byref(s))
+    if isinstance(h, (list, tuple)):
+        arr = (ctypes.POINTER(matrix2) * len(h))(*[ctypes.pointer(m.mat2) for m in h])
+        rc = lib.hmvm_schedule_export_batch(arr, len(h), 1 if source == "rowmat_t" else 0, ctypes.byref(o),
+                                            ctypes.byref(s))
+    else:
+        rc = lib.hmvm_schedule_export(ctypes.byref(h.mat2), 1 if source == "rowmat_t" else 0, ctypes.byref(o),
+                                      ctypes.byref(s))
     if rc:
         raise HMVMError("hmvm_schedule_export: %s" % lib.hmvm_hip_last_error_message().decode())
     try:

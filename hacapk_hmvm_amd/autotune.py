@@ -1,0 +1,253 @@
+"""Autotuner and learned cost model for the HIP HMVM schedule.
+
+The reference picks its GPU kernel variant offline: a sweep over 96 variant
+ids prints timings (src_cpp/hmvm1.cpp:82-96, hmvm_cuda1.cu:236-302) and a human
+reads the CSV.  Here the schedule of a plan has these knobs:
+
+  w1, w2          tile widths (target elements per stage-1 / dense+stage-2 tile)
+  kernel_a/b      tile-kernel variant of launch A / launch B (0..7)
+  bt_a/b          workgroup size of launch A / launch B (64, 128, 256)
+  order           dispatch order of the tiles of each launch
+
+Two ways to choose them:
+
+exhaustive(...)  builds a plan for every (w1, w2), times every (kernel, bt)
+                 for each launch on the GPU and keeps the fastest of each;
+                 this is the ground truth and the fallback.
+learned(...)     builds the candidate schedules on the host only, predicts
+                 every tile's duration with the MLP (HIP inference kernel),
+                 replays the hardware's greedy dispatch of the tiles (longest
+                 predicted first) onto the kernel's resident wavefront slots
+                 and picks the candidate with the smallest predicted launch
+                 times.  Only the chosen plan is uploaded.
+
+The MLP is trained on tile durations measured in situ (a clock-stamping copy
+of the tile kernel) on generated H-matrices, by HIP forward/backward/Adam
+kernels.  Its target is log2 of the tile duration in microseconds.
+"""
+import heapq
+import itertools
+import time
+
+import numpy as np
+
+from . import CostMLP, HMVM_NFEAT, Plan, schedule, tile_occupancy
+
+W1_CHOICES = (0.5, 1.0, 2.0)     # multiples of the default 256 KiB stage-1 tile
+W2_CHOICES = (0.5, 1.0, 2.0)     # multiples of the default 128 KiB stage-2 tile
+KERNELS = (2, 3, 6, 7)           # pipelined variants, with and without batched vector entries
+BLOCK_THREADS = (64, 256)
+EB = {"f64": 8, "f32": 4, "bf16": 2}
+
+
+def widths(storage, a, b):
+    eb = EB[storage]
+    return int(a * 262144 // eb), int(b * 131072 // eb)
+
+
+# ---- features from an exported schedule (host only) ------------------------------
+def schedule_features(sched, which, storage, kernel, block_threads, nrhs=1):
+    """Tile features of launch A (0) or B (1) of an exported schedule, as
+    hmvm_plan_tile_features computes them for an uploaded plan."""
+    tiles = sched["tiles_a"] if which == 0 else sched["tiles_b"]
+    pan = sched["panels"]
+    V = 16 // EB[storage]
+    n = len(tiles)
+    f = np.zeros((n, HMVM_NFEAT), dtype=np.float32)
+    if n == 0:
+        return f
+    p0 = tiles["panel0"].astype(np.int64)
+    npan = tiles["npanel"].astype(np.int64)
+    # per-tile sums over panels with a prefix sum over the panel array
+    ncols = pan["ncols"].astype(np.float64)
+    nch = np.where(pan["vec_sel"] == 2, (pan["vec_chunks"] >> 16).astype(np.float64) * ncols, 0.0)
+    cc = np.concatenate([[0.0], np.cumsum(ncols)])
+    cn = np.concatenate([[0.0], np.cumsum(nch)])
+    cols = cc[p0 + npan] - cc[p0]
+    chs = cn[p0 + npan] - cn[p0]
+    shift = (pan["val_off"][p0] % V).astype(np.int64)
+    nrows = tiles["nrows"].astype(np.int64)
+    lpc = (nrows + shift + V - 1) // V
+    kind = np.where(which == 1, 2, np.where(pan["factor"][p0] < 0, 0, 1))
+    f[np.arange(n), kind] = 1.0
+    f[:, 3] = np.log2(nrows)
+    f[:, 4] = np.log2(1.0 + cols)
+    f[:, 5] = np.log2(npan)
+    f[:, 6] = np.log2(lpc)
+    f[:, 7] = np.log2(1.0 + lpc * V * cols)
+    f[:, 8] = np.log2(1.0 + np.where(cols > 0, chs / np.maximum(cols, 1), 0.0))
+    f[:, 9] = np.log2(EB[storage])
+    f[:, 10] = kernel & 1
+    f[:, 11] = (kernel >> 1) & 1
+    f[:, 12] = (kernel >> 2) & 1
+    f[:, 13] = np.log2(block_threads // 64)
+    f[:, 14] = np.log2(max(nrhs, 1))
+    return f
+
+
+# ---- training data -------------------------------------------------------------------
+def collect(h, storage, variants=None, w_scales=((1.0, 1.0),), iters=2, max_tiles=20000, seed=0):
+    """Measured tile durations of one matrix: (features, log2 us, per-launch
+    records).  Each record holds the variant, the launch's features in
+    dispatch order and its measured time, against which the launch-time model
+    is calibrated."""
+    rng = np.random.default_rng(seed)
+    variants = variants or [(k, bt) for k in KERNELS for bt in BLOCK_THREADS]
+    Xs, ys, recs = [], [], []
+    for a, b in w_scales:
+        w1, w2 = widths(storage, a, b)
+        p = Plan(h, storage=storage, w1=w1, w2=w2)
+        for k, bt in variants:
+            p.set_launch(k, bt)
+            for which in (0, 1):
+                d, _, lms = p.profile_tiles(which, iters=iters)
+                if len(d) == 0:
+                    continue
+                f = p.tile_features(which)
+                recs.append({"storage": storage, "which": which, "kernel": k, "bt": bt, "w1": w1, "w2": w2,
+                             "sum_us": float(d.sum()), "launch_ms": lms, "ntiles": len(d), "X": f})
+                sel = rng.choice(len(d), min(len(d), max_tiles), replace=False)
+                Xs.append(f[sel])
+                ys.append(np.log2(np.maximum(d[sel], 1e-2)).astype(np.float32))
+        p.close()
+    return np.concatenate(Xs), np.concatenate(ys), recs
+
+
+def makespan(d, slots):
+    """Greedy list schedule of durations d (dispatch order) onto `slots`
+    identical wavefront slots: each tile starts on the slot that frees first."""
+    d = np.asarray(d, dtype=np.float64)
+    if len(d) == 0:
+        return 0.0
+    if len(d) <= slots:
+        return float(d.max())
+    h = list(d[:slots])
+    heapq.heapify(h)
+    for x in d[slots:]:
+        heapq.heapreplace(h, h[0] + x)
+    return float(max(h))
+
+
+_OCC = {}
+
+
+def occupancy(storage, kernel, bt):
+    key = (storage, kernel, bt)
+    if key not in _OCC:
+        _OCC[key] = tile_occupancy(storage, kernel, bt)
+    return _OCC[key]
+
+
+class CostModel:
+    """MLP on tile features -> log2(tile duration us), and a launch-time model:
+    the greedy dispatch of the predicted durations onto the kernel's resident
+    wavefront slots, calibrated (ms = a * makespan + b per storage) against the
+    measured launch times of the training runs."""
+
+    def __init__(self, nh=64, seed=1):
+        self.mlp = CostMLP(HMVM_NFEAT, nh, seed)
+        self.calib = {}
+
+    def fit(self, X, y, recs, epochs=80, batch=512, lr=2e-3, seed=3):
+        self.mlp.fit_scaler(X)
+        loss = self.mlp.train(X, y, epochs=epochs, batch=batch, lr=lr, seed=seed)
+        by = {}
+        for r in recs:
+            sim = self.simulate_ms(r["X"], r["storage"], r["kernel"], r["bt"], reorder=False)
+            by.setdefault(r["storage"], []).append((sim, r["launch_ms"]))
+        for st, pts in by.items():
+            a = np.array(pts)
+            A = np.stack([a[:, 0], np.ones(len(a))], axis=1)
+            coef, *_ = np.linalg.lstsq(A, a[:, 1], rcond=None)
+            self.calib[st] = (float(coef[0]), float(coef[1]))
+        return loss
+
+    def tile_us(self, X):
+        return np.exp2(self.mlp.predict(X).astype(np.float64))
+
+    def simulate_ms(self, X, storage, kernel, bt, reorder=True):
+        d = self.tile_us(X)
+        if reorder:
+            d = np.sort(d)[::-1]
+        return makespan(d, occupancy(storage, kernel, bt)) / 1e3
+
+    def launch_ms(self, X, storage, which, kernel, bt):
+        a, b = self.calib.get(storage, (1.0, 0.0))
+        return a * self.simulate_ms(X, storage, kernel, bt) + b
+
+
+def linear_baseline(X, y):
+    """The method without the network: log-duration linear in the features
+    (least squares), i.e. a roofline-style 'cost grows with elements' model."""
+    A = np.hstack([X.astype(np.float64), np.ones((X.shape[0], 1))])
+    coef, *_ = np.linalg.lstsq(A, y.astype(np.float64), rcond=None)
+    return lambda Z: np.hstack([Z.astype(np.float64), np.ones((Z.shape[0], 1))]) @ coef
+
+
+# ---- schedule search ---------------------------------------------------------------
+def _apply_choice(p, choice):
+    p.set_phase_launch(0, choice["kernel_a"], choice["bt_a"])
+    p.set_phase_launch(1, choice["kernel_b"], choice["bt_b"])
+
+
+def exhaustive(h, storage, warmup=2, iters=8, w_choices=None, kernels=KERNELS, bts=BLOCK_THREADS):
+    """Times every candidate on the GPU; returns (best choice, all results, seconds spent)."""
+    t0 = time.time()
+    w_choices = w_choices or list(itertools.product(W1_CHOICES, W2_CHOICES))
+    results = []
+    best = None
+    for a, b in w_choices:
+        w1, w2 = widths(storage, a, b)
+        p = Plan(h, storage=storage, w1=w1, w2=w2)
+        per = {}
+        for k in kernels:
+            for bt in bts:
+                p.set_launch(k, bt)
+                t = p.time(warmup, iters)
+                per[(k, bt)] = t
+                results.append({"w1": w1, "w2": w2, "kernel": k, "bt": bt, "ms_a": t["ms_a"], "ms_b": t["ms_b"],
+                                "ms_other": t["ms_tsum"] + t["ms_ysum"], "ms_stream": t["ms_stream"]})
+        ka = min(per, key=lambda v: per[v]["ms_a"])
+        kb = min(per, key=lambda v: per[v]["ms_b"])
+        choice = {"w1": w1, "w2": w2, "kernel_a": ka[0], "bt_a": ka[1], "kernel_b": kb[0], "bt_b": kb[1]}
+        _apply_choice(p, choice)
+        choice["ms"] = p.time(warmup, iters)["ms_stream"]
+        if best is None or choice["ms"] < best["ms"]:
+            best = choice
+        p.close()
+    return best, results, time.time() - t0
+
+
+def learned(h, storage, model, w_choices=None, kernels=KERNELS, bts=BLOCK_THREADS, reorder=True):
+    """Chooses the schedule from predictions only; returns (choice, plan, seconds spent)."""
+    t0 = time.time()
+    w_choices = w_choices or list(itertools.product(W1_CHOICES, W2_CHOICES))
+    best = None
+    for a, b in w_choices:
+        w1, w2 = widths(storage, a, b)
+        sched = schedule(h, storage=storage, w1=w1, w2=w2)
+        pred = {}
+        for which in (0, 1):
+            opts = []
+            for k in kernels:
+                for bt in bts:
+                    X = schedule_features(sched, which, storage, k, bt)
+                    opts.append((model.launch_ms(X, storage, which, k, bt), k, bt))
+            pred[which] = min(opts)
+        total = pred[0][0] + pred[1][0]
+        if best is None or total < best["pred_ms"]:
+            best = {"w1": w1, "w2": w2, "kernel_a": pred[0][1], "bt_a": pred[0][2], "kernel_b": pred[1][1],
+                    "bt_b": pred[1][2], "pred_ms": total}
+    p = Plan(h, storage=storage, w1=best["w1"], w2=best["w2"])
+    _apply_choice(p, best)
+    if reorder:
+        for which, (k, bt) in ((0, (best["kernel_a"], best["bt_a"])), (1, (best["kernel_b"], best["bt_b"]))):
+            d = model.tile_us(p.tile_features(which))
+            if len(d):
+                p.set_tile_order(which, np.argsort(-d, kind="stable"))
+    return best, p, time.time() - t0
+
+
+def default_plan(h, storage):
+    """The schedule without any tuning (library defaults)."""
+    return Plan(h, storage=storage)

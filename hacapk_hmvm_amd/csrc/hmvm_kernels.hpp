@@ -64,9 +64,11 @@ __device__ __forceinline__ auto load_stream(const S *p) {
   }
 }
 
-// Kernel variants (bit field): bit0 16 loads in flight per batch instead of 8,
-// bit1 software pipelining (next batch issued before the current one is
-// consumed), bit2 non-temporal matrix loads.
+// Kernel variants (bit field): bit0 vector entries of wide tiles fetched 64
+// columns at a time (one lane each) and broadcast with readlane instead of one
+// wave-uniform load per column, bit1 software pipelining (next batch issued
+// before the current one is consumed), bit2 non-temporal matrix loads.
+// Every variant keeps 8 loads in flight per batch.
 constexpr int kNumKernelVariants = 8;
 
 // Vector entry j of a panel.  kCoherent: the stage-1 partial slots were
@@ -90,10 +92,11 @@ __device__ __forceinline__ double vec_entry(const PanelDesc &pd, const double *_
 }
 
 // Scale of a panel's product under BF16 storage: its own factor, times the
-// a1 factor of the same leaf when the panel sums unscaled stage-1 partials;
-// stage-1 panels (no factor) stay unscaled.
+// a1 factor of the same leaf when the panel sums unscaled stage-1 partials.
+// Stage-1 panels (no factor) stay unscaled, and panels reading the
+// materialised t need none: the t sum applies both factors of the leaf.
 __device__ __forceinline__ float panel_scale(const PanelDesc &pd, const float *__restrict__ scales) {
-  if (pd.factor < 0) return 1.0f;  // stage-1 partials: scaled later, with t
+  if (pd.factor < 0 || pd.vec_sel == kVecT) return 1.0f;
   const float s = scales[pd.factor];
   return pd.vec_sel == kVecTP ? s * scales[pd.factor - 1] : s;
 }
@@ -142,7 +145,21 @@ __device__ __forceinline__ typename Wide<S>::T load_wide(const S *p) {
 
 // lanes_per_col > 32: lane owns V-row groups lane + 64q (q < R); the vector
 // entry of a column is wave-uniform.
-template <typename S, int R, int kLoads, bool kPipe, bool kNT, bool kCoh>
+// Lane-held vector entries: lane l holds entry jb + l of the panel (0 past
+// its end); bcast() hands entry jb + src to the whole wave.
+template <bool kCoh>
+__device__ __forceinline__ double lane_entry(const PanelDesc &pd, const double *__restrict__ x,
+                                             const double *__restrict__ t, const double *tp, int j) {
+  return j < pd.ncols ? vec_entry<kCoh>(pd, x, t, tp, j) : 0.0;
+}
+__device__ __forceinline__ double bcast(double v, int src) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)b, src);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), src);
+  return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+
+template <typename S, int R, int kLoads, bool kPipe, bool kNT, bool kCoh, bool kBV>
 __device__ __forceinline__ void wide_panel(const S *__restrict__ vals, const PanelDesc &pd, int lane, int shift,
                                            int lpc, const double *__restrict__ x, const double *__restrict__ t,
                                            const double *tp, typename AccOf<S>::T (&accp)[R][Wide<S>::V]) {
@@ -163,13 +180,32 @@ __device__ __forceinline__ void wide_panel(const S *__restrict__ vals, const Pan
 #pragma unroll
       for (int v = 0; v < V; ++v) accp[q][v] += (A)W::get(a[q], v) * b;
   };
+  // vector entry of column jj (jj wave-uniform, non-decreasing over the calls)
+  double vb = 0, vn = 0;
+  int jb = 0;
+  if constexpr (kBV) {
+    vb = lane_entry<kCoh>(pd, x, t, tp, lane);
+    vn = lane_entry<kCoh>(pd, x, t, tp, 64 + lane);
+  }
+  auto entry = [&](int jj) -> A {
+    if constexpr (kBV) {
+      if (jj >= jb + 64) {
+        jb += 64;
+        vb = vn;
+        vn = lane_entry<kCoh>(pd, x, t, tp, jb + 64 + lane);
+      }
+      return (A)bcast(vb, jj - jb);
+    } else {
+      return (A)vec_entry<kCoh>(pd, x, t, tp, jj);
+    }
+  };
   int j = 0;
   if (kPipe && nc >= 2 * U) {
     typename W::T a0[U][R];
     A b0[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-      b0[u] = (A)vec_entry<kCoh>(pd, x, t, tp, u);
+      b0[u] = entry(u);
 #pragma unroll
       for (int q = 0; q < R; ++q) a0[u][q] = load_wide<S, kNT>(base[q] + (int64_t)u * ld);
     }
@@ -178,7 +214,7 @@ __device__ __forceinline__ void wide_panel(const S *__restrict__ vals, const Pan
       A b1[U];
 #pragma unroll
       for (int u = 0; u < U; ++u) {
-        b1[u] = (A)vec_entry<kCoh>(pd, x, t, tp, j + u);
+        b1[u] = entry(j + u);
 #pragma unroll
         for (int q = 0; q < R; ++q) a1[u][q] = load_wide<S, kNT>(base[q] + (int64_t)(j + u) * ld);
       }
@@ -199,7 +235,7 @@ __device__ __forceinline__ void wide_panel(const S *__restrict__ vals, const Pan
       A b[U];
 #pragma unroll
       for (int u = 0; u < U; ++u) {
-        b[u] = (A)vec_entry<kCoh>(pd, x, t, tp, j + u);
+        b[u] = entry(j + u);
 #pragma unroll
         for (int q = 0; q < R; ++q) a[u][q] = load_wide<S, kNT>(base[q] + (int64_t)(j + u) * ld);
       }
@@ -211,11 +247,11 @@ __device__ __forceinline__ void wide_panel(const S *__restrict__ vals, const Pan
     typename W::T a[R];
 #pragma unroll
     for (int q = 0; q < R; ++q) a[q] = load_wide<S, kNT>(base[q] + (int64_t)j * ld);
-    consume(a, (A)vec_entry<kCoh>(pd, x, t, tp, j));
+    consume(a, entry(j));
   }
 }
 
-template <typename S, int R, int kLoads, bool kPipe, bool kNT, bool kScaled, bool kCoh>
+template <typename S, int R, int kLoads, bool kPipe, bool kNT, bool kScaled, bool kCoh, bool kBV>
 __device__ __forceinline__ void wide_tile(const TileDesc &td, int shift, int lpc, const PanelDesc *__restrict__ panels,
                                           const S *__restrict__ vals, const double *__restrict__ x,
                                           const double *__restrict__ t, const double *tp,
@@ -234,7 +270,7 @@ __device__ __forceinline__ void wide_tile(const TileDesc &td, int shift, int lpc
     for (int q = 0; q < R; ++q)
 #pragma unroll
       for (int v = 0; v < V; ++v) accp[q][v] = 0;
-    wide_panel<S, R, kLoads, kPipe, kNT, kCoh>(vals, pd, lane, shift, lpc, x, t, tp, accp);
+    wide_panel<S, R, kLoads, kPipe, kNT, kCoh, kBV>(vals, pd, lane, shift, lpc, x, t, tp, accp);
     const A sc = kScaled ? (A)panel_scale(pd, scales) : (A)1;
 #pragma unroll
     for (int q = 0; q < R; ++q)
@@ -313,7 +349,7 @@ __device__ __forceinline__ void grouped_tile(const TileDesc &td, int shift, int 
   }
 }
 
-template <typename S, bool kScaled, int kLoads, bool kPipe, bool kNT, bool kCoh>
+template <typename S, bool kScaled, int kLoads, bool kPipe, bool kNT, bool kCoh, bool kBV>
 __device__ __forceinline__ void run_tile(const TileDesc &td, const PanelDesc *__restrict__ panels,
                                          const S *__restrict__ vals, const double *__restrict__ x,
                                          const double *__restrict__ t, const double *tp,
@@ -324,18 +360,18 @@ __device__ __forceinline__ void run_tile(const TileDesc &td, const PanelDesc *__
   if (lpc <= 32) {
     grouped_tile<S, kLoads, kNT, kScaled, kCoh>(td, shift, lpc, panels, vals, x, t, tp, scales, out, lane);
   } else if (lpc <= 64) {
-    wide_tile<S, 1, kLoads, kPipe, kNT, kScaled, kCoh>(td, shift, lpc, panels, vals, x, t, tp, scales, out, lane);
+    wide_tile<S, 1, kLoads, kPipe, kNT, kScaled, kCoh, kBV>(td, shift, lpc, panels, vals, x, t, tp, scales, out, lane);
   } else if (lpc <= 128) {
     if constexpr (V <= 4)
-      wide_tile<S, 2, kLoads, kPipe, kNT, kScaled, kCoh>(td, shift, lpc, panels, vals, x, t, tp, scales, out, lane);
+      wide_tile<S, 2, kLoads, kPipe, kNT, kScaled, kCoh, kBV>(td, shift, lpc, panels, vals, x, t, tp, scales, out, lane);
   } else {
     if constexpr (V <= 2)
-      wide_tile<S, 3, kLoads, kPipe, kNT, kScaled, kCoh>(td, shift, lpc, panels, vals, x, t, tp, scales, out, lane);
+      wide_tile<S, 3, kLoads, kPipe, kNT, kScaled, kCoh, kBV>(td, shift, lpc, panels, vals, x, t, tp, scales, out, lane);
   }
 }
 
 // One wavefront per tile (see hmvm_plan.hpp for the tile model).
-template <typename S, bool kScaled, int kLoads, bool kPipe, bool kNT>
+template <typename S, bool kScaled, int kLoads, bool kPipe, bool kNT, bool kBV>
 __global__ __launch_bounds__(256) void tile_kernel(const TileDesc *__restrict__ tiles, int ntiles,
                                                    const PanelDesc *__restrict__ panels,
                                                    const S *__restrict__ vals,
@@ -349,14 +385,14 @@ __global__ __launch_bounds__(256) void tile_kernel(const TileDesc *__restrict__ 
   const int tid = (int)blockIdx.x * (int)(blockDim.x >> 6) + wave;
   if (tid >= ntiles) return;
   const TileDesc td = tiles[tid];
-  run_tile<S, kScaled, kLoads, kPipe, kNT, false>(td, panels, vals, x, t, tp, scales, out, lane);
+  run_tile<S, kScaled, kLoads, kPipe, kNT, false, kBV>(td, panels, vals, x, t, tp, scales, out, lane);
 }
 
 // tile_kernel with lane 0 of every wave recording the device wall clock
 // (constant rate, read with s_memrealtime) before and after its tile: the
 // in-situ per-tile durations the cost model learns from.  Same results as
 // tile_kernel.
-template <typename S, bool kScaled, int kLoads, bool kPipe, bool kNT>
+template <typename S, bool kScaled, int kLoads, bool kPipe, bool kNT, bool kBV>
 __global__ __launch_bounds__(256) void tile_kernel_timed(const TileDesc *__restrict__ tiles, int ntiles,
                                                          const PanelDesc *__restrict__ panels,
                                                          const S *__restrict__ vals,
@@ -372,7 +408,7 @@ __global__ __launch_bounds__(256) void tile_kernel_timed(const TileDesc *__restr
   if (tid >= ntiles) return;
   const long long t0 = (long long)wall_clock64();
   const TileDesc td = tiles[tid];
-  run_tile<S, kScaled, kLoads, kPipe, kNT, false>(td, panels, vals, x, t, tp, scales, out, lane);
+  run_tile<S, kScaled, kLoads, kPipe, kNT, false, kBV>(td, panels, vals, x, t, tp, scales, out, lane);
   const long long t1 = (long long)wall_clock64();
   if (lane == 0) {
     stamps[2 * (int64_t)tid] = t0;
@@ -428,7 +464,7 @@ __device__ __forceinline__ void wave_wait(int *ctr, int target, int *fault, int 
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
 }
 
-template <typename S, bool kScaled, int kLoads, bool kPipe, bool kNT>
+template <typename S, bool kScaled, int kLoads, bool kPipe, bool kNT, bool kBV>
 __global__ __launch_bounds__(256) void persistent_kernel(
     const TileDesc *__restrict__ tiles_a, int n1, int na, const TileDesc *__restrict__ tiles_b, int nt,
     int nitems, int nwaves, const PanelDesc *__restrict__ panels, const S *__restrict__ vals,
@@ -442,7 +478,7 @@ __global__ __launch_bounds__(256) void persistent_kernel(
     if (item >= nitems) break;
     if (item < na) {
       const TileDesc td = tiles_a[item];
-      run_tile<S, kScaled, kLoads, kPipe, kNT, true>(td, panels, vals, x, t, slots, scales, slots, lane);
+      run_tile<S, kScaled, kLoads, kPipe, kNT, true, kBV>(td, panels, vals, x, t, slots, scales, slots, lane);
       if (item < n1) wave_signal(&ctr->s1_done, lane);
       wave_signal(&ctr->tiles_done, lane);
     } else if (item < nt) {
@@ -451,7 +487,7 @@ __global__ __launch_bounds__(256) void persistent_kernel(
         s1_seen = true;
       }
       const TileDesc td = tiles_b[item - na];
-      run_tile<S, kScaled, kLoads, kPipe, kNT, true>(td, panels, vals, x, t, slots, scales, slots, lane);
+      run_tile<S, kScaled, kLoads, kPipe, kNT, true, kBV>(td, panels, vals, x, t, slots, scales, slots, lane);
       wave_signal(&ctr->tiles_done, lane);
     } else {
       if (!all_seen) {
@@ -493,8 +529,9 @@ __global__ __launch_bounds__(256) void persistent_kernel(
 //   BF16 storage: v_mfma_f32_16x16x16_bf16 (K = 16, vector entries rounded to
 //                 BF16 after the factor scale is applied, FP32 accumulate)
 // Operand layouts (wave64): A lane l holds rows l%16 of K-slice l/16, B lane l
-// holds rhs l%16 of K-slice l/16, the 16x16 result gives lane l the four rows
-// 4*(l/16)+v of rhs l%16.  Every address is formed from (lane, v) with the
+// holds rhs l%16 of K-slice l/16; the 16x16 result gives lane l four rows of
+// rhs l%16: 4*(l/16)+v for the FP32 and BF16 instructions, l/16+4v for the
+// FP64 one (Mfma<S>::row).  Every address is formed from (lane, v) with the
 // row and column bounds applied, so a layout mismatch could only produce wrong
 // values, never an access outside the panel.
 typedef double mf_d4 __attribute__((ext_vector_type(4)));
@@ -508,18 +545,23 @@ struct Mfma<double> {
   static constexpr int K = 4;
   using Acc = mf_d4;
   using Op = double;
+  // result element v of lane l: row l/16 + 4v (measured on gfx950)
+  __device__ static __forceinline__ int row(int lane, int v) { return (lane >> 4) + 4 * v; }
 };
 template <>
 struct Mfma<float> {
   static constexpr int K = 4;
   using Acc = mf_f4;
   using Op = float;
+  // result element v of lane l: row 4(l/16) + v
+  __device__ static __forceinline__ int row(int lane, int v) { return 4 * (lane >> 4) + v; }
 };
 template <>
 struct Mfma<uint16_t> {
   static constexpr int K = 16;
   using Acc = mf_f4;
   using Op = mf_s4;
+  __device__ static __forceinline__ int row(int lane, int v) { return 4 * (lane >> 4) + v; }
 };
 
 __device__ __forceinline__ mf_d4 mfma_step(double a, double b, mf_d4 c) {
@@ -642,7 +684,7 @@ __global__ __launch_bounds__(256) void tile_multi_kernel(const TileDesc *__restr
       for (int cb = 0; cb < NB; ++cb)
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
-          const int row = rc + 16 * rb + 4 * (lane >> 4) + v;
+          const int row = rc + 16 * rb + M::row(lane, v);
           if (row < td.nrows) out[(int64_t)(td.out_off + row) * nrhs + 16 * cb + (lane & 15)] = (double)acc[rb][cb][v];
         }
     }
@@ -664,8 +706,19 @@ __global__ __launch_bounds__(256) void segsum_multi_kernel(int n, int nrhs, cons
   const int s = seg[i];
   const int e1 = ptr[s + 1];
   double a = 0;
-  for (int e = ptr[s]; e < e1; ++e) a += buf[(list[e] + i) * nrhs + c];
-  if (kScaled) a *= (double)scales[factor_of[i]];
+  int e = ptr[s];
+  for (; e + 8 <= e1; e += 8) {
+    int64_t l[8];
+    double v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) l[u] = list[e + u];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) v[u] = buf[(l[u] + i) * nrhs + c];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) a += v[u];
+  }
+  for (; e < e1; ++e) a += buf[(list[e] + i) * nrhs + c];
+  if (kScaled) a *= (double)scales[factor_of[i]] * (double)scales[factor_of[i] + 1];
   out[g] = accumulate ? out[g] + a : a;
 }
 
@@ -687,7 +740,8 @@ __global__ __launch_bounds__(256) void probe_copy_kernel(const double4 *__restri
     dst[i] = src[i];
 }
 
-// out[i] = (accumulate ? out[i] : 0) + scale(i) * sum_e buf[list[e] + i]
+// out[i] = (accumulate ? out[i] : 0) + scale(i) * sum_e buf[list[e] + i],
+// scale(i) = product of the two factor scales of entry i's leaf (t sum only)
 template <bool kScaled>
 __global__ __launch_bounds__(256) void segsum_kernel(int n, const int32_t *__restrict__ seg,
                                                      const int32_t *__restrict__ ptr,
@@ -701,8 +755,21 @@ __global__ __launch_bounds__(256) void segsum_kernel(int n, const int32_t *__res
   const int s = seg[i];
   const int e1 = ptr[s + 1];
   double a = 0;
-  for (int e = ptr[s]; e < e1; ++e) a += buf[list[e] + i];
-  if (kScaled) a *= (double)scales[factor_of[i]];
+  int e = ptr[s];
+  // eight independent slot loads in flight, summed in list order
+  for (; e + 8 <= e1; e += 8) {
+    int64_t l[8];
+    double v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) l[u] = list[e + u];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) v[u] = buf[l[u] + i];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) a += v[u];
+  }
+  for (; e < e1; ++e) a += buf[list[e] + i];
+  // t entry of leaf b: both factor scales of b (a1 = factor_of, a2 = the next id)
+  if (kScaled) a *= (double)scales[factor_of[i]] * (double)scales[factor_of[i] + 1];
   out[i] = accumulate ? out[i] + a : a;
 }
 

@@ -105,6 +105,10 @@ std::string build_plan(const LeafArrays &la, const PlanOptions &opt, Plan *plan)
   P.lane_elems = V;
   if (opt.align % V) return "alignment must be a multiple of the lane width";
   std::vector<int64_t> t_idx(nlf, -1);     // index of t_b[0] in the materialised t
+  // stage-1 rows of every piece of a leaf (a leaf of more than kMaxTileRows
+  // ranks has two or more pieces): the t sum adds its partial slots
+  struct PieceRec { int leaf, k0, n; int64_t slot0; int nr, nch, kk0; };
+  std::vector<PieceRec> pieces;
   std::vector<int64_t> tp_off(nlf, -1);    // partial slot of t_b[0], chunk 0
   std::vector<int32_t> tp_chunks(nlf, 0);  // (nch << 16) | stride
   std::vector<char> forced(nlf, 0);        // stage 2 of this leaf must read the materialised t
@@ -144,8 +148,6 @@ std::string build_plan(const LeafArrays &la, const PlanOptions &opt, Plan *plan)
       r.dst_cs = Kp;
       r.factor = 2 * i;
       P.repack.push_back(r);
-      t_idx[i] = tpos + kofs;
-      for (int k = 0; k < kt; k++) P.factor_of_trow.push_back(2 * i);
       int done = 0;
       while (done < kt) {
         if (gsize.back() == kMaxTileRows || (done == 0 && gsize.back() + kt > kMaxTileRows && gsize.back() > 0 &&
@@ -167,8 +169,6 @@ std::string build_plan(const LeafArrays &la, const PlanOptions &opt, Plan *plan)
       const int C = chunk_cols(nr, opt.w1);
       const int nch = (ndt + C - 1) / C;
       const int64_t slot0 = P.n_tp;
-      const int seg = (int)P.tsum.ptr.size();
-      P.tsum.ptr.push_back((int)P.tsum.list.size());
       for (int c = 0; c < nch; c++) {
         const int c0 = c * C, nc = std::min(C, ndt - c0);
         PanelDesc pd;
@@ -186,19 +186,18 @@ std::string build_plan(const LeafArrays &la, const PlanOptions &opt, Plan *plan)
         tdsc.out_off = (int)P.n_tp;
         P.panels.push_back(pd);
         t1.push_back(tdsc);
-        P.tsum.list.push_back(P.n_tp - (tpos + k0));
         P.n_tp += nr;
         P.elems_stage1 += (int64_t)nr * nc;
         P.vec_a += nc;
       }
-      for (int k = 0; k < nr; k++) P.tsum.seg.push_back(seg);
       for (const Piece &pc : groups[gi]) {
+        int kk0 = -1;  // stack row of the piece's first rank, relative to the group
+        for (size_t q = 0; q < ls.size(); q++)
+          if (ls[q] == pc.leaf) kk0 = kofs_of[q] + pc.k0 - k0;
+        pieces.push_back({pc.leaf, pc.k0, pc.n, slot0, nr, nch, kk0});
         if (pc.k0 != 0 || pc.n != la.kt[pc.leaf]) {
           forced[pc.leaf] = 1;  // rank rows of this leaf straddle two row groups
         } else {
-          int kk0 = -1;
-          for (size_t q = 0; q < ls.size(); q++)
-            if (ls[q] == pc.leaf) kk0 = kofs_of[q] - k0;
           tp_off[pc.leaf] = slot0 + kk0;
           tp_chunks[pc.leaf] = (nch << 16) | nr;
           if (nch >= (1 << 15) || nr >= (1 << 16)) forced[pc.leaf] = 1;
@@ -208,10 +207,58 @@ std::string build_plan(const LeafArrays &la, const PlanOptions &opt, Plan *plan)
     }
     tpos += K;
   }
-  P.tsum.ptr.push_back((int)P.tsum.list.size());
-  P.n_t = tpos;
-  P.tsum.n = (int)tpos;
-  if (P.n_tp > INT32_MAX || P.n_t > INT32_MAX) return "stage-1 slots exceed int32";
+  (void)tpos;
+  if (P.n_tp > INT32_MAX) return "stage-1 slots exceed int32";
+
+  // ---- which stage-2 factors read a materialised t -----------------------------
+  //   t_mode 0: none, unless some leaf cannot be summed on the fly; then all
+  //   t_mode 1: all
+  //   t_mode 2: the leaves that cannot, and those whose partial sums would be
+  //             repeated by several row tiles or span more than two chunks
+  bool any_forced = false;
+  for (int i = 0; i < nlf; i++) any_forced = any_forced || forced[i];
+  std::vector<char> mat(nlf, 0);
+  for (int i = 0; i < nlf; i++) {
+    if (la.ltmtx[i] != 1 || la.kt[i] == 0) continue;
+    if (opt.t_mode == 1 || (opt.t_mode == 0 && any_forced) || forced[i])
+      mat[i] = 1;
+    else if (opt.t_mode == 2)
+      mat[i] = la.ndl[i] > kMaxTileRows || (tp_chunks[i] >> 16) > 2;
+  }
+  // t holds the materialised leaves in row-cluster order, ranks contiguous, so
+  // that the a2 factors of consecutive leaves of a row cluster (stored back to
+  // back) multiply one contiguous slice of t and merge into one panel
+  {
+    int64_t tp = 0;
+    for (auto &g : rowg)
+      for (int i : g.second)
+        if (mat[i]) {
+          t_idx[i] = tp;
+          tp += la.kt[i];
+        }
+    P.n_t = tp;
+    if (P.n_t > INT32_MAX) return "t exceeds int32";
+    P.factor_of_trow.assign((size_t)P.n_t, 0);
+    for (int i = 0; i < nlf; i++)
+      for (int k = 0; mat[i] && k < la.kt[i]; k++) P.factor_of_trow[t_idx[i] + k] = 2 * i;
+    // t segments: one per piece, t[t_idx + k0 + r] = sum_c slot(c, kk0 + r)
+    std::sort(pieces.begin(), pieces.end(), [&](const PieceRec &a, const PieceRec &b) {
+      const int64_t ta = t_idx[a.leaf] + a.k0, tb = t_idx[b.leaf] + b.k0;
+      return ta < tb;
+    });
+    P.tsum.n = (int)P.n_t;
+    P.tsum.seg.assign((size_t)P.n_t, -1);
+    for (const PieceRec &pc : pieces) {
+      if (!mat[pc.leaf]) continue;
+      const int64_t t0 = t_idx[pc.leaf] + pc.k0;
+      const int seg = (int)P.tsum.ptr.size();
+      P.tsum.ptr.push_back((int)P.tsum.list.size());
+      for (int c = 0; c < pc.nch; c++) P.tsum.list.push_back(pc.slot0 + (int64_t)c * pc.nr + pc.kk0 - t0);
+      for (int r = 0; r < pc.n; r++) P.tsum.seg[t0 + r] = seg;
+    }
+    P.tsum.ptr.push_back((int)P.tsum.list.size());
+  }
+  P.t_fused = P.n_t == 0;
 
   // ---- dense / stage 2 layout + tiles ------------------------------------------------
   struct Slot { int64_t row0; int nr; int64_t z; };
@@ -220,36 +267,39 @@ std::string build_plan(const LeafArrays &la, const PlanOptions &opt, Plan *plan)
     const int nstrtl = g.first.first, ndl = g.first.second;
     const std::vector<int> &ls = g.second;
     std::vector<int> dense, lowrank;
+    for (int i : ls) (la.ltmtx[i] == 2 ? dense : lowrank).push_back(i);
     const int ldp = (int)align_up(ndl, V);  // padded column stride of the cluster's factors
-    for (int i : ls) {
-      const int64_t base = align_up(off, opt.align);
-      dbase[i] = base;
-      RepackDesc r;
-      r.dst_off = base;
-      r.dst_cs = ldp;
-      r.factor = 2 * i + 1;
-      r.rows = ndl;  // il
-      if (la.ltmtx[i] == 2) {
-        r.src_off = la.src_a1[i];
-        r.cols = la.ndt[i];  // it
-        r.src_rs = la.ndt[i];
-        r.src_cs = 1;
-        off = base + (int64_t)ldp * la.ndt[i];
-        dense.push_back(i);
-      } else {
-        r.src_off = la.src_a2[i];
-        r.cols = la.kt[i];  // k
-        if (la.a2_source == kA2RowMajor) {
-          r.src_rs = 1;
-          r.src_cs = ndl;
-        } else {
-          r.src_rs = la.kt[i];
+    // the dense leaves of the cluster back to back, then its a2 factors back
+    // to back (each block aligned; column strides keep every factor on a
+    // 16-byte boundary inside a block)
+    for (int kind = 0; kind < 2; kind++) {
+      off = align_up(off, opt.align);
+      for (int i : kind == 0 ? dense : lowrank) {
+        dbase[i] = off;
+        RepackDesc r;
+        r.dst_off = off;
+        r.dst_cs = ldp;
+        r.factor = 2 * i + 1;
+        r.rows = ndl;  // il
+        if (kind == 0) {
+          r.src_off = la.src_a1[i];
+          r.cols = la.ndt[i];  // it
+          r.src_rs = la.ndt[i];
           r.src_cs = 1;
+        } else {
+          r.src_off = la.src_a2[i];
+          r.cols = la.kt[i];  // k
+          if (la.a2_source == kA2RowMajor) {
+            r.src_rs = 1;
+            r.src_cs = ndl;
+          } else {
+            r.src_rs = la.kt[i];
+            r.src_cs = 1;
+          }
         }
-        off = base + (int64_t)ldp * la.kt[i];
-        lowrank.push_back(i);
+        off += (int64_t)ldp * r.cols;
+        P.repack.push_back(r);
       }
-      P.repack.push_back(r);
     }
     for (auto rc : row_chunks(ndl)) {
       const int r0 = rc.first, nr = rc.second;
@@ -276,13 +326,28 @@ std::string build_plan(const LeafArrays &la, const PlanOptions &opt, Plan *plan)
               pd.vec_off = la.nstrtt[i] - 1;
               pd.vec_sel = kVecX;
               pd.vec_chunks = 0;
+            } else if (mat[i]) {
+              pd.vec_off = (int32_t)t_idx[i];
+              pd.vec_sel = kVecT;
+              pd.vec_chunks = 0;
             } else {
               pd.vec_off = (int32_t)tp_off[i];
               pd.vec_sel = kVecTP;
               pd.vec_chunks = tp_chunks[i];
             }
-            P.panels.push_back(pd);
-            tdsc.npanel++;
+            // a factor that continues the previous panel in memory and in its
+            // vector extends it (BF16 dense factors keep their own scale)
+            PanelDesc *prev = tdsc.npanel > 0 ? &P.panels.back() : nullptr;
+            const bool merge = prev && pd.vec_sel != kVecTP && prev->vec_sel == pd.vec_sel && prev->ld == pd.ld &&
+                               prev->val_off + (int64_t)prev->ncols * prev->ld == pd.val_off &&
+                               prev->vec_off + prev->ncols == pd.vec_off &&
+                               !(opt.storage == kBF16 && pd.vec_sel == kVecX);
+            if (merge) {
+              prev->ncols += nc;
+            } else {
+              P.panels.push_back(pd);
+              tdsc.npanel++;
+            }
             work += (int64_t)nr * nc;
             if (kind == 0) {
               P.elems_dense += (int64_t)nr * nc;
@@ -303,30 +368,6 @@ std::string build_plan(const LeafArrays &la, const PlanOptions &opt, Plan *plan)
   if (P.n_tp + P.n_z > INT32_MAX) return "output slots exceed int32";
   if (P.panels.size() > (size_t)INT32_MAX) return "too many panels";
   P.nvals = align_up(off, opt.align);
-  // Which stage-2 panels read the materialised t (computed by the t segment
-  // sum) instead of summing the stage-1 partial slots themselves:
-  //   t_mode 0: none, unless some leaf cannot be summed on the fly; then all
-  //   t_mode 1: all
-  //   t_mode 2: the leaves that cannot, and those whose partial sums would be
-  //             repeated by several row tiles or span more than two chunks
-  bool any_forced = false;
-  for (int i = 0; i < nlf; i++) any_forced = any_forced || forced[i];
-  std::vector<char> mat(nlf, 0);
-  for (int i = 0; i < nlf; i++) {
-    if (la.ltmtx[i] != 1) continue;
-    if (opt.t_mode == 1 || (opt.t_mode == 0 && any_forced) || forced[i])
-      mat[i] = 1;
-    else if (opt.t_mode == 2)
-      mat[i] = la.ndl[i] > kMaxTileRows || (tp_chunks[i] >> 16) > 2;
-  }
-  for (PanelDesc &pd : P.panels)
-    if (pd.vec_sel == kVecTP && mat[pd.factor / 2]) {
-      const int i = pd.factor / 2;
-      pd.vec_sel = kVecT;
-      pd.vec_off = (int32_t)t_idx[i];
-      pd.vec_chunks = 0;
-      P.t_fused = false;
-    }
   P.elems_a = P.elems_stage1 + P.elems_dense;
   P.elems_b = P.elems_stage2;
 
@@ -390,6 +431,14 @@ std::string verify_plan(const Plan &P, const LeafArrays &la) {
   if (copied != P.len) return "repack does not copy len elements";
 
   // 2. tiles: shapes, panels, value and vector ranges, output slots
+  // Device extent of every a2 / dense factor (from the repack records) so that
+  // a panel spanning several factors stored back to back can be walked.
+  struct Ext { int64_t off, end; int leaf, cols, cs; };
+  std::vector<Ext> ext;
+  for (const RepackDesc &r : P.repack)
+    if (r.factor % 2 == 1 && r.cols > 0) ext.push_back({r.dst_off, r.dst_off + (int64_t)r.cols * r.dst_cs,
+                                                        r.factor / 2, r.cols, r.dst_cs});
+  std::sort(ext.begin(), ext.end(), [](const Ext &a, const Ext &b) { return a.off < b.off; });
   const int64_t npan = (int64_t)P.panels.size();
   std::vector<int64_t> seen2(P.nlf, 0);
   int64_t seen1 = 0;
@@ -438,9 +487,34 @@ std::string verify_plan(const Plan &P, const LeafArrays &la) {
         } else {
           if (d.factor < 0 || d.factor >= P.nfactors || d.factor % 2 != 1) return "factor id out of range";
           if (t.out_off < P.n_tp || (int64_t)t.out_off + t.nrows > P.n_tp + P.n_z) return "output slot out of range";
-          if ((d.vec_sel == kVecX) != (la.ltmtx[d.factor / 2] == 2)) return "panel kind disagrees with leaf";
           if ((list == 1) != (d.vec_sel != kVecX)) return "panel in the wrong launch";
-          seen2[d.factor / 2] += (int64_t)t.nrows * d.ncols;
+          // walk the factors the panel's columns lie in: each must start where
+          // the previous one ended, share the column stride, and be the kind
+          // of leaf the vector selector says
+          auto it = std::upper_bound(ext.begin(), ext.end(), d.val_off,
+                                     [](int64_t v, const Ext &e) { return v < e.off; });
+          if (it == ext.begin()) return "panel starts outside the factors";
+          --it;
+          const int64_t r0 = d.val_off - it->off;
+          if (r0 < 0 || r0 + t.nrows > it->cs || it->leaf != d.factor / 2) return "panel rows outside its factor";
+          int done = 0;
+          while (done < d.ncols) {
+            if (it == ext.end() || it->cs != d.ld) return "merged panel crosses a factor of another stride";
+            const int leaf = it->leaf;
+            if ((d.vec_sel == kVecX) != (la.ltmtx[leaf] == 2)) return "panel kind disagrees with leaf";
+            if (r0 + t.nrows > la.ndl[leaf]) return "panel rows outside the leaf";
+            const int take = std::min(d.ncols - done, it->cols);
+            if (d.vec_sel == kVecX && (int64_t)d.vec_off + done != la.nstrtt[leaf] - 1)
+              return "merged dense panel: x slice does not follow the leaf";
+            if (d.vec_sel == kVecTP && d.ncols != la.kt[leaf]) return "partial-slot panel spans more than its leaf";
+            seen2[leaf] += (int64_t)t.nrows * take;
+            done += take;
+            if (done < d.ncols) {
+              const int64_t end = it->end;
+              ++it;
+              if (it == ext.end() || it->off != end) return "merged panel is not contiguous";
+            }
+          }
         }
       }
     }

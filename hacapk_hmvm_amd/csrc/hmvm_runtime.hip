@@ -356,7 +356,11 @@ PlanOptions plan_options(const hmvm_options *o) {
   po.w1 = o->w1 > 0 ? o->w1 : kStage1TileBytes / eb;
   po.w2 = o->w2 > 0 ? o->w2 : kStage2TileBytes / eb;
   po.order = o->order;
-  po.t_mode = o->t_mode;
+  // t_mode < 0: by storage.  FP64 tiles are wide enough that summing the
+  // stage-1 partials on the fly beats one more launch; the narrow types
+  // (measured, N = 101,250) are faster reading a materialised t for the
+  // leaves whose partial sums would be repeated or long.
+  po.t_mode = o->t_mode >= 0 ? o->t_mode : (po.storage == kF64 ? 0 : 2);
   return po;
 }
 
@@ -659,9 +663,10 @@ int create_plan(Source &src, const hmvm_options *opt, hmvm_plan **out) {
 template <typename S, bool kScaled, int kVar>
 void launch_tiles_v(hmvm_plan *p, dim3 grid, dim3 block, hipStream_t st, const TileDesc *d_tiles, int ntiles,
                     const double *x, const double *t, const double *tp, double *out) {
-  constexpr int L = (kVar & 1) ? 16 : 8;
+  constexpr int L = 8;
+  constexpr bool kBV = (kVar & 1) != 0;
   constexpr bool kPipe = (kVar & 2) != 0, kNT = (kVar & 4) != 0;
-  hipLaunchKernelGGL((tile_kernel<S, kScaled, L, kPipe, kNT>), grid, block, 0, st, d_tiles, ntiles, p->d_panels,
+  hipLaunchKernelGGL((tile_kernel<S, kScaled, L, kPipe, kNT, kBV>), grid, block, 0, st, d_tiles, ntiles, p->d_panels,
                      (const S *)p->d_vals, x, t, tp, (const float *)p->d_scales, out);
 }
 
@@ -708,10 +713,11 @@ void launch_tiles_both(hmvm_plan *p, int which, const double *x, hipStream_t st)
 template <typename S, bool kScaled, int kVar>
 void launch_timed_v(hmvm_plan *p, int bt, const TileDesc *d_tiles, int ntiles, const double *x, const double *t,
                     const double *tp, double *out, long long *stamps, hipStream_t st) {
-  constexpr int L = (kVar & 1) ? 16 : 8;
+  constexpr int L = 8;
+  constexpr bool kBV = (kVar & 1) != 0;
   constexpr bool kPipe = (kVar & 2) != 0, kNT = (kVar & 4) != 0;
   const int wpb = bt / 64;
-  hipLaunchKernelGGL((tile_kernel_timed<S, kScaled, L, kPipe, kNT>), dim3((ntiles + wpb - 1) / wpb),
+  hipLaunchKernelGGL((tile_kernel_timed<S, kScaled, L, kPipe, kNT, kBV>), dim3((ntiles + wpb - 1) / wpb),
                      dim3(bt), 0, st, d_tiles, ntiles, p->d_panels, (const S *)p->d_vals, x, t, tp,
                      (const float *)p->d_scales, out, stamps);
 }
@@ -855,10 +861,11 @@ int apply_multi(hmvm_plan *p, const double *X, double *Y, int nrhs, int accumula
 
 template <typename S, bool kScaled, int kVar>
 int launch_persistent_v(hmvm_plan *p, const double *x, double *y, int accumulate, hipStream_t st) {
-  constexpr int L = (kVar & 1) ? 16 : 8;
+  constexpr int L = 8;
+  constexpr bool kBV = (kVar & 1) != 0;
   constexpr bool kPipe = (kVar & 2) != 0, kNT = (kVar & 4) != 0;
   const Plan &P = p->plan;
-  auto kfn = persistent_kernel<S, kScaled, L, kPipe, kNT>;
+  auto kfn = persistent_kernel<S, kScaled, L, kPipe, kNT, kBV>;
   const int key = kVar * 1024 + p->block_threads;
   if (p->persist_key != key) {  // resident workgroups per CU, queried once per launch shape
     int per_cu = 0;
@@ -1053,7 +1060,7 @@ void hmvm_options_default(hmvm_options *opt) {
   opt->kernel = 6;
   opt->order = 1;
   opt->mode = 0;
-  opt->t_mode = 0;
+  opt->t_mode = -1;  // by storage
 }
 
 int hmvm_plan_create(const matrix2 *mat2, int a2_source, const hmvm_options *opt, hmvm_plan **out) {
@@ -1330,14 +1337,10 @@ int hmvm_plan_check(const matrix2 *mat2, const hmvm_options *opt, hmvm_plan_info
   return HMVM_OK;
 }
 
-int hmvm_schedule_export(const matrix2 *mat2, int a2_source, const hmvm_options *opt, hmvm_schedule *out) {
-  clear_error();
-  if (!out) return set_error(HMVM_ERR_ARG, "out is NULL");
-  std::memset(out, 0, sizeof(*out));
-  if (a2_source != kA2RowMajor && a2_source != kA2Transposed) return set_error(HMVM_ERR_ARG, "bad a2_source");
-  Source s;
-  int rc = source_from_matrix2(mat2, a2_source, &s);
-  if (rc) return rc;
+}  // extern "C"
+
+namespace {
+int export_schedule(const Source &s, const hmvm_options *opt, hmvm_schedule *out) {
   Plan P;
   std::string err = build_plan(s.la, plan_options(opt), &P);
   if (!err.empty()) return set_error(HMVM_ERR_MATRIX, "%s", err.c_str());
@@ -1363,6 +1366,34 @@ int hmvm_schedule_export(const matrix2 *mat2, int a2_source, const hmvm_options 
     return set_error(HMVM_ERR_NOMEM, "schedule export");
   }
   return HMVM_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int hmvm_schedule_export(const matrix2 *mat2, int a2_source, const hmvm_options *opt, hmvm_schedule *out) {
+  clear_error();
+  if (!out) return set_error(HMVM_ERR_ARG, "out is NULL");
+  std::memset(out, 0, sizeof(*out));
+  if (a2_source != kA2RowMajor && a2_source != kA2Transposed) return set_error(HMVM_ERR_ARG, "bad a2_source");
+  Source s;
+  int rc = source_from_matrix2(mat2, a2_source, &s);
+  if (rc) return rc;
+  return export_schedule(s, opt, out);
+}
+
+int hmvm_schedule_export_batch(const matrix2 *const *mats, int count, int a2_source, const hmvm_options *opt,
+                               hmvm_schedule *out) {
+  clear_error();
+  if (!out) return set_error(HMVM_ERR_ARG, "out is NULL");
+  std::memset(out, 0, sizeof(*out));
+  if (a2_source != kA2RowMajor && a2_source != kA2Transposed) return set_error(HMVM_ERR_ARG, "bad a2_source");
+  Source s;
+  ConcatFeeder feed;
+  std::vector<int> row0;
+  int rc = source_from_batch(mats, count, a2_source, &s, &feed, &row0);
+  if (rc) return rc;
+  return export_schedule(s, opt, out);
 }
 
 void hmvm_schedule_release(hmvm_schedule *s) {
@@ -1632,6 +1663,49 @@ int hmvm_plan_profile_tiles(hmvm_plan *plan, int which, int iters, double *tile_
     if (start_us) start_us[i] = acc0[i] / iters;
   }
   if (launch_ms) *launch_ms = lms / iters;
+  return HMVM_OK;
+}
+
+}  // extern "C"
+
+namespace {
+template <typename S, bool kScaled, int kVar>
+const void *tile_fn() {
+  constexpr int L = 8;
+  constexpr bool kBV = (kVar & 1) != 0;
+  constexpr bool kPipe = (kVar & 2) != 0, kNT = (kVar & 4) != 0;
+  return (const void *)tile_kernel<S, kScaled, L, kPipe, kNT, kBV>;
+}
+template <typename S, bool kScaled>
+const void *tile_fn_v(int kernel) {
+  switch (kernel) {
+    case 0: return tile_fn<S, kScaled, 0>();
+    case 1: return tile_fn<S, kScaled, 1>();
+    case 2: return tile_fn<S, kScaled, 2>();
+    case 3: return tile_fn<S, kScaled, 3>();
+    case 4: return tile_fn<S, kScaled, 4>();
+    case 5: return tile_fn<S, kScaled, 5>();
+    case 6: return tile_fn<S, kScaled, 6>();
+    default: return tile_fn<S, kScaled, 7>();
+  }
+}
+}  // namespace
+
+extern "C" {
+
+int hmvm_tile_occupancy(int storage, int kernel, int block_threads, int *waves) {
+  if (!waves || storage < kF64 || storage > kBF16 || kernel < 0 || kernel >= kNumKernelVariants ||
+      block_threads % 64 || block_threads < 64 || block_threads > 256)
+    return set_error(HMVM_ERR_ARG, "bad argument");
+  const void *fn = storage == kF64   ? tile_fn_v<double, false>(kernel)
+                   : storage == kF32 ? tile_fn_v<float, false>(kernel)
+                                     : tile_fn_v<uint16_t, true>(kernel);
+  int per_cu = 0, dev = 0;
+  HIP_TRY(hipGetDevice(&dev));
+  hipDeviceProp_t prop;
+  HIP_TRY(hipGetDeviceProperties(&prop, dev));
+  HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, block_threads, 0));
+  *waves = per_cu * (block_threads / 64) * prop.multiProcessorCount;
   return HMVM_OK;
 }
 

@@ -92,14 +92,16 @@ typedef struct hmvm_options {
   int w1;                /* target elements per stage-1 tile (0: 256 KiB worth) */
   int w2;                /* target elements per dense/stage-2 tile (0: 128 KiB) */
   int block_threads;     /* threads per workgroup, 64..256 step 64 (0: default) */
-  int kernel;            /* tile-kernel variant 0..7: bit0 16 loads per batch,  */
+  int kernel;            /* tile-kernel variant 0..7: bit0 vector entries read  */
+                         /* 64 columns at a time and broadcast (readlane),      */
                          /* bit1 software pipelining, bit2 non-temporal loads   */
   int order;             /* tile order in a launch: 0 memory, 1 largest first   */
   int mode;              /* 0: three launches (A, B, y sum); 1: one persistent  */
                          /* launch with a device-wide work queue                */
   int t_mode;            /* stage-2 vector entries: 0 sum the stage-1 partial   */
                          /* slots on the fly, 1 read a materialised t (one more */
-                         /* launch), 2 choose per leaf                          */
+                         /* launch), 2 choose per leaf, -1 by storage (default: */
+                         /* 0 for FP64, 2 for FP32/BF16)                         */
 } hmvm_options;
 
 typedef struct hmvm_plan_info {
@@ -244,6 +246,9 @@ int hmvm_plan_tile_features(const hmvm_plan *plan, int which, int nrhs, float *f
 int hmvm_plan_profile_tiles(hmvm_plan *plan, int which, int iters, double *tile_us, double *start_us,
                             double *launch_ms);
 int hmvm_plan_set_tile_order(hmvm_plan *plan, int which, const int *perm);
+/* Wavefronts of the tile kernel (storage, variant, workgroup size) that can be
+ * resident on the current device at once. */
+int hmvm_tile_occupancy(int storage, int kernel, int block_threads, int *waves);
 
 /* ---- the learned cost model: a small MLP on the GPU ---------------------------
  * nin -> nh -> nh -> 1, ReLU, FP32; inputs are standardised with the mean and
@@ -320,6 +325,9 @@ typedef struct hmvm_schedule {
 } hmvm_schedule;
 
 int hmvm_schedule_export(const matrix2 *mat2, int a2_source, const hmvm_options *opt, hmvm_schedule *out);
+/* The schedule of hmvm_plan_create_batch for the same matrices. */
+int hmvm_schedule_export_batch(const matrix2 *const *mats, int count, int a2_source, const hmvm_options *opt,
+                               hmvm_schedule *out);
 void hmvm_schedule_release(hmvm_schedule *s);
 
 /* ---- small device helpers for callers without a HIP runtime of their own -- */

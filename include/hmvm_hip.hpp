@@ -7,7 +7,7 @@
 // kept as FP32 in HBM with FP32 accumulation inside a tile and FP64 for the
 // cross-tile sums of y.
 // kernel: -1 = the default (tuned) launch; 0..7 = a tile-kernel variant
-//   (bit0 16 loads per batch, bit1 software pipelining, bit2 non-temporal).
+//   (bit0 batched vector entries, bit1 software pipelining, bit2 non-temporal).
 // dump_result != 0: computes v = A b once and writes
 //   result_hip_k<kernel>_<d|f>.txt, one "%.3E" per line (src_cpp/hmvm_cuda1.cu:273-279).
 // nbench > 0: 5 warm-up + nbench timed HMVMs (kernels only, b already on the

@@ -34,7 +34,8 @@ def lib():
         for name in ("oracle_hmvm_seq_1", "oracle_hmvm_seq_1t", "oracle_hmvm_omp_1", "oracle_hmvm_omp_1t"):
             getattr(L, name).argtypes = [c_double_p, ctypes.c_void_p, c_double_p]
             getattr(L, name).restype = None
-        for name in ("oracle_hmvm_seq_2", "oracle_hmvm_seq_2t", "oracle_hmvm_omp_2", "oracle_hmvm_omp_2t"):
+        for name in ("oracle_hmvm_seq_2", "oracle_hmvm_seq_2t", "oracle_hmvm_omp_2", "oracle_hmvm_omp_2t",
+                     "oracle_hmvm_seq_2_64", "oracle_hmvm_omp_2_64"):
             getattr(L, name).argtypes = [c_double_p, ctypes.c_void_p, c_double_p]
             getattr(L, name).restype = None
         L.oracle_hmvm_seq_2_f32.argtypes = [c_float_p, ctypes.c_void_p, c_float_p, c_float_p]
@@ -92,6 +93,15 @@ def omp_2(h, b):
 
 def omp_2t(h, b):
     return _run("oracle_hmvm_omp_2t", h, b, True)
+
+
+def seq_2_64(h, b, v=None):
+    """oracle_hmvm_seq_2 over an HMatrix64 (int64 offsets)."""
+    return _run("oracle_hmvm_seq_2_64", h, b, True, v)
+
+
+def omp_2_64(h, b):
+    return _run("oracle_hmvm_omp_2_64", h, b, True)
 
 
 def seq_2_f32(h, b32, rowmat32):

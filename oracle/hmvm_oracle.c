@@ -220,3 +220,31 @@ void oracle_convert_d2f(const double *src, float *dst, long long n) {
 }
 
 int oracle_omp_threads(void) { return omp_get_max_threads(); }
+
+/* The same two kernels over the int64 form (matrix2_64): only the offset type
+ * differs from oracle_hmvm_seq_2 / oracle_hmvm_omp_2. */
+void oracle_hmvm_seq_2_64(double *v, const matrix2_64 *mat, const double *b) {
+  double *t = (double *)malloc(sizeof(double) * (mat->ktmax > 0 ? mat->ktmax : 1));
+  for (int i = 0; i < mat->nlf; i++) {
+    double *vr = v + mat->nstrtl[i] - 1;
+    const double *bc = b + mat->nstrtt[i] - 1;
+    if (mat->ltmtx[i] == 1) {
+      lr_stage1(t, mat->rowmat + mat->a1[i], mat->kt[i], mat->ndt[i], bc);
+      lr_stage2_a2(vr, mat->rowmat + mat->a2[i], mat->kt[i], mat->ndl[i], t);
+    } else {
+      dense_leaf(vr, mat->rowmat + mat->a1[i], mat->ndl[i], mat->ndt[i], bc);
+    }
+  }
+  free(t);
+}
+
+static leaf_view view_2_64(const void *m, int i) {
+  const matrix2_64 *s = (const matrix2_64 *)m;
+  leaf_view l = {s->ltmtx[i], s->kt[i], s->ndl[i], s->ndt[i], s->nstrtl[i], s->nstrtt[i],
+                 s->rowmat + s->a1[i], s->rowmat + s->a2[i], 0};
+  return l;
+}
+
+void oracle_hmvm_omp_2_64(double *zau, const matrix2_64 *m, const double *zu) {
+  omp_body(zau, m->nd, m->nlf, m->ktmax, view_2_64, m, zu);
+}

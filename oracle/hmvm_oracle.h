@@ -53,6 +53,11 @@ void oracle_hmvm_seq_2_f32(float *v, const matrix2 *mat, const float *rowmat_f, 
 /* Round-to-nearest FP64 -> FP32 copy (src_cpp/loadmatrix.cpp:252-342). */
 void oracle_convert_d2f(const double *src, float *dst, long long n);
 
+/* oracle_hmvm_seq_2 and oracle_hmvm_omp_2 over the int64 form (matrix2_64);
+ * pinned by equality with the int32 functions on the same matrix. */
+void oracle_hmvm_seq_2_64(double *v, const matrix2_64 *mat, const double *b);
+void oracle_hmvm_omp_2_64(double *zau, const matrix2_64 *mat, const double *zu);
+
 /* Number of OpenMP threads the omp kernels will use. */
 int oracle_omp_threads(void);
 

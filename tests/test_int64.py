@@ -166,3 +166,11 @@ def test_drop_in_64(gpu, H, small):
     assert np.array_equal(v32t, v64t)
     H.release(small)
     w.free()
+
+
+def test_oracle_64_equals_oracle(H, O, small):
+    w = H.widen(small)
+    x = np.random.default_rng(6).uniform(-1, 1, small.nd)
+    assert np.array_equal(O.seq_2_64(w, x), O.seq_2(small, x))
+    ref = O.seq_2(small, x)
+    assert np.linalg.norm(O.omp_2_64(w, x) - ref) <= 1e-14 * np.linalg.norm(ref)

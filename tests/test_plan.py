@@ -41,7 +41,12 @@ def test_schedule_check_generated(H, w):
     a = h.arrays()
     lr = a["ltmtx"] == 1
     assert info["nlowrank"] == lr.sum() and info["ndense"] == (~lr).sum()
-    assert info["n_t"] == a["kt"][lr].sum()
+    # t holds only the leaves whose stage 2 reads it; none when every leaf sums
+    # its stage-1 partials on the fly, all with t_mode 1
+    assert bool(info["t_fused"]) == (info["n_t"] == 0)
+    info1 = H.plan_check(h, w1=w, w2=w, t_mode=1)
+    assert info1["n_t"] == a["kt"][lr].sum() and not info1["t_fused"]
+    assert info1["elems_a"] + info1["elems_b"] == h.mat2.len
 
 
 def test_schedule_check_dense_grid(H):
