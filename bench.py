@@ -129,8 +129,8 @@ def run_c2(args, H, np):
     try:
         with open(TRAFFIC_FILE) as f:
             tr = json.load(f)
-        if tr.get("N") == nd and tr.get("len") == length and tr.get("launch") == ("B" if "B" in kname else "A"):
-            traffic = tr["hbm_bytes_per_launch"]
+        if tr.get("N") == nd and tr.get("len") == length:
+            traffic = tr.get("hbm_bytes_" + ("B" if "launch B" in kname else "A"))
     except (OSError, ValueError, KeyError):
         pass
 
@@ -256,7 +256,8 @@ def train_cost_model(H, np, storages, seed=0):
     """Trains the MLP on tile timings of generated matrices that are not part of any benchmark."""
     from hacapk_hmvm_amd import autotune as AT
     Xs, ys, recs = [], [], []
-    specs = [(20000, "sphere", 1e-4), (36000, "cube", 1e-6), (28000, "cube", 1e-4), (44000, "sphere", 1e-6)]
+    specs = [(20000, "sphere", 1e-4), (36000, "cube", 1e-6), (28000, "cube", 1e-4), (44000, "sphere", 1e-6),
+             (70000, "sphere", 1e-4), (130000, "cube", 1e-4)]
     for i, (n, geo, eps) in enumerate(specs):
         m = H.generate(n, geometry=geo, eps=eps, seed=1000 + i)
         for st in storages:

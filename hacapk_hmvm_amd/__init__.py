@@ -138,7 +138,7 @@ HMVM_STORAGE_F64, HMVM_STORAGE_F32, HMVM_STORAGE_BF16 = 0, 1, 2
 HMVM_A2_ROWMAJOR, HMVM_A2_TRANSPOSED = 0, 1
 STORAGE = {"f64": 0, "f32": 1, "bf16": 2}
 HACAPK_WITH_ROWMAT_T = 1
-HMVM_NFEAT = 15
+HMVM_NFEAT = 17
 
 
 class HMVMError(RuntimeError):

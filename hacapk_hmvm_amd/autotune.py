@@ -33,16 +33,15 @@ import numpy as np
 
 from . import CostMLP, HMVM_NFEAT, Plan, schedule, tile_occupancy
 
-W1_CHOICES = (0.5, 1.0, 2.0)     # multiples of the default 256 KiB stage-1 tile
-W2_CHOICES = (0.5, 1.0, 2.0)     # multiples of the default 128 KiB stage-2 tile
-KERNELS = (2, 3, 6, 7)           # pipelined variants, with and without batched vector entries
+W1_CHOICES = (0.5, 1.0, 2.0)     # multiples of the default 32768-element stage-1 tile
+W2_CHOICES = (0.5, 1.0, 2.0)     # multiples of the default 16384-element stage-2 tile
+KERNELS = (2, 4, 6, 7)           # pipelined / non-temporal / batched-entry variants
 BLOCK_THREADS = (64, 256)
 EB = {"f64": 8, "f32": 4, "bf16": 2}
 
 
 def widths(storage, a, b):
-    eb = EB[storage]
-    return int(a * 262144 // eb), int(b * 131072 // eb)
+    return int(a * 32768), int(b * 16384)
 
 
 # ---- features from an exported schedule (host only) ------------------------------
@@ -82,6 +81,8 @@ def schedule_features(sched, which, storage, kernel, block_threads, nrhs=1):
     f[:, 12] = (kernel >> 2) & 1
     f[:, 13] = np.log2(block_threads // 64)
     f[:, 14] = np.log2(max(nrhs, 1))
+    f[:, 15] = np.log2(n)
+    f[:, 16] = np.log2(1.0 + float((nrows * cols).sum()))  # elements the launch streams
     return f
 
 

@@ -286,11 +286,19 @@ __device__ __forceinline__ void wide_tile(const TileDesc &td, int shift, int lpc
     }
 }
 
+// Orders this wave's LDS stores before its later LDS loads (LDS requests of a
+// wave complete in issue order; the fences keep the compiler from moving them).
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // lanes_per_col <= 32: the wave holds g = 64/lpc column groups; lane (c, jg)
 // sums columns jg, jg+g, ... so for a contiguous panel each load instruction
-// reads g*lpc*V consecutive elements.  The g partial sums of each row meet in
-// a fixed-order shuffle reduction at the end.
-template <typename S, int kLoads, bool kNT, bool kScaled, bool kCoh>
+// reads g*lpc*V consecutive elements.  The g partial sums of each row are
+// added in a fixed group order at the end.
+template <typename S, int kLoads, bool kPipe, bool kNT, bool kScaled, bool kCoh>
 __device__ __forceinline__ void grouped_tile(const TileDesc &td, int shift, int lpc,
                                              const PanelDesc *__restrict__ panels, const S *__restrict__ vals,
                                              const double *__restrict__ x, const double *__restrict__ t,
@@ -298,54 +306,85 @@ __device__ __forceinline__ void grouped_tile(const TileDesc &td, int shift, int 
                                              double *__restrict__ out, int lane) {
   using A = typename AccOf<S>::T;
   using W = Wide<S>;
+  using T = typename W::T;
   constexpr int V = W::V;
   const int g = 64 / lpc;
   const int jg = lane / lpc;
   const int c = lane - jg * lpc;
+  const bool live = jg < g;  // lanes past g*lpc idle
   A acc[V];
 #pragma unroll
   for (int v = 0; v < V; ++v) acc[v] = 0;
   for (int p = 0; p < td.npanel; ++p) {
     const PanelDesc pd = panels[td.panel0 + p];
     const int ld = pd.ld, nc = pd.ncols;
-    const int64_t gs = (int64_t)g * ld;
-    int j = jg < g ? jg : nc;
-    const S *__restrict__ pj = vals + (pd.val_off - shift) + (int64_t)V * c + (int64_t)j * ld;
+    const S *__restrict__ base = vals + (pd.val_off - shift) + (int64_t)V * c;
+    // batch s, slot u: column s*step + u*g + jg of this lane's group; the batch
+    // count is wave-uniform, columns past the panel load nothing and add 0
+    const int step = kLoads * g;
+    const int nb = (nc + step - 1) / step;
+    auto load_batch = [&](int s, T (&a)[kLoads], A (&b)[kLoads]) {
+#pragma unroll
+      for (int u = 0; u < kLoads; ++u) {
+        const int j = s * step + u * g + jg;
+        if (live && j < nc) {
+          a[u] = load_wide<S, kNT>(base + (int64_t)j * ld);
+          b[u] = (A)vec_entry<kCoh>(pd, x, t, tp, j);
+        } else {
+          a[u] = T{};
+          b[u] = 0;
+        }
+      }
+    };
     A accp[V];
 #pragma unroll
     for (int v = 0; v < V; ++v) accp[v] = 0;
-    for (; j + (kLoads - 1) * g < nc; j += kLoads * g) {
-      typename W::T a[kLoads];
-      A b[kLoads];
-#pragma unroll
-      for (int u = 0; u < kLoads; ++u) {
-        a[u] = load_wide<S, kNT>(pj + u * gs);
-        b[u] = (A)vec_entry<kCoh>(pd, x, t, tp, j + u * g);
-      }
+    auto consume = [&](const T (&a)[kLoads], const A (&b)[kLoads]) {
 #pragma unroll
       for (int u = 0; u < kLoads; ++u)
 #pragma unroll
         for (int v = 0; v < V; ++v) accp[v] += (A)W::get(a[u], v) * b[u];
-      pj += kLoads * gs;
-    }
-    for (; j < nc; j += g) {
-      const typename W::T a = load_wide<S, kNT>(pj);
-      const A b = (A)vec_entry<kCoh>(pd, x, t, tp, j);
+    };
+    if (kPipe && nb >= 2) {
+      T a0[kLoads], a1[kLoads];
+      A b0[kLoads], b1[kLoads];
+      load_batch(0, a0, b0);
+      for (int sb = 1; sb < nb; ++sb) {
+        load_batch(sb, a1, b1);  // in flight while batch sb-1 is consumed
+        consume(a0, b0);
 #pragma unroll
-      for (int v = 0; v < V; ++v) accp[v] += (A)W::get(a, v) * b;
-      pj += gs;
+        for (int u = 0; u < kLoads; ++u) {
+          a0[u] = a1[u];
+          b0[u] = b1[u];
+        }
+      }
+      consume(a0, b0);
+    } else {
+      for (int sb = 0; sb < nb; ++sb) {
+        T a[kLoads];
+        A b[kLoads];
+        load_batch(sb, a, b);
+        consume(a, b);
+      }
     }
     const A sc = kScaled ? (A)panel_scale(pd, scales) : (A)1;
 #pragma unroll
     for (int v = 0; v < V; ++v) acc[v] += kScaled ? accp[v] * sc : accp[v];
   }
+  // The g partial sums of every row meet in LDS (a region per wave): lane
+  // (jg, c) stores its V sums at jg*lpc*V + c*V + v, then lane o sums output
+  // row o = c*V + v over the groups in group order, so the stores to the slot
+  // are contiguous across lanes.
+  __shared__ double red[4 * 64 * V];  // <= 4 waves per workgroup (launch bound 256)
+  double *r = red + (threadIdx.x >> 6) * (64 * V);
 #pragma unroll
-  for (int v = 0; v < V; ++v) {
-    const double mine = (double)acc[v];
+  for (int v = 0; v < V; ++v) r[lane * V + v] = (double)acc[v];
+  wave_lds_sync();
+  for (int o = lane; o < lpc * V; o += 64) {
     double s = 0;
-    for (int q = 0; q < g; ++q) s += __shfl(mine, c + q * lpc, 64);
-    const int row = V * c + v - shift;
-    if (lane < lpc && row >= 0 && row < td.nrows) out[td.out_off + row] = s;
+    for (int q = 0; q < g; ++q) s += r[q * lpc * V + o];
+    const int row = o - shift;
+    if (row >= 0 && row < td.nrows) out[td.out_off + row] = s;
   }
 }
 
@@ -358,7 +397,7 @@ __device__ __forceinline__ void run_tile(const TileDesc &td, const PanelDesc *__
   const int shift = (int)(panels[td.panel0].val_off % V);
   const int lpc = (td.nrows + shift + V - 1) / V;  // lanes per column
   if (lpc <= 32) {
-    grouped_tile<S, kLoads, kNT, kScaled, kCoh>(td, shift, lpc, panels, vals, x, t, tp, scales, out, lane);
+    grouped_tile<S, kLoads, kPipe, kNT, kScaled, kCoh>(td, shift, lpc, panels, vals, x, t, tp, scales, out, lane);
   } else if (lpc <= 64) {
     wide_tile<S, 1, kLoads, kPipe, kNT, kScaled, kCoh, kBV>(td, shift, lpc, panels, vals, x, t, tp, scales, out, lane);
   } else if (lpc <= 128) {

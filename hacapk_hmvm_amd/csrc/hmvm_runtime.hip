@@ -89,7 +89,9 @@ struct hmvm_plan {
   int block_threads = 256;
   int kernel = 0;  // tile-kernel variant, 0 .. kNumKernelVariants-1 (launch A, persistent, multi)
   int kernel_b = 0, block_threads_b = 256;  // launch B
-  int mode = 0;    // 0: launches A, [t sum], B, y sum; 1: one persistent launch
+  int mode = 0;    // 0: launches on one stream; 1: one persistent launch; 2: dense tiles on a second stream
+  hipStream_t stream2 = nullptr;       // dense tiles, concurrent with stage 1 / t sum / stage 2
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   bool a_mixed = false;  // launch A order mixes stage-1 and dense tiles (no persistent mode)
   int persist_key = -1, persist_blocks = 0;  // occupancy cache (kernel variant, block size)
   // several right-hand sides: buffers sized for multi_cap columns
@@ -117,6 +119,9 @@ struct hmvm_plan {
                     d_ctr,   d_mslots,  d_mt,      d_mX,     d_mY};
     for (void *p : ptrs)
       if (p) (void)hipFree(p);
+    if (ev_fork) (void)hipEventDestroy(ev_fork);
+    if (ev_join) (void)hipEventDestroy(ev_join);
+    if (stream2) (void)hipStreamDestroy(stream2);
     if (stream) (void)hipStreamDestroy(stream);
   }
 };
@@ -339,10 +344,10 @@ int source_from_batch(const matrix2 *const *mats, int count, int a2_source, Sour
   return HMVM_OK;
 }
 
-// Tile widths default to a fixed number of bytes per tile (256 KiB for a
-// stage-1 tile, 128 KiB for a dense or stage-2 tile), i.e. twice the elements
-// for FP32 and four times for BF16.
-constexpr int kStage1TileBytes = 262144, kStage2TileBytes = 131072;
+// Default tile widths in elements (the same for every storage: the tile
+// duration is set more by columns and rows than by bytes; measured best for
+// FP64 and FP32 at N = 101,250).
+constexpr int kStage1TileElems = 32768, kStage2TileElems = 16384;
 
 PlanOptions plan_options(const hmvm_options *o) {
   hmvm_options d;
@@ -352,15 +357,14 @@ PlanOptions plan_options(const hmvm_options *o) {
   }
   PlanOptions po;
   po.storage = o->storage;
-  const int eb = (po.storage >= kF64 && po.storage <= kBF16) ? elem_bytes(po.storage) : 8;
-  po.w1 = o->w1 > 0 ? o->w1 : kStage1TileBytes / eb;
-  po.w2 = o->w2 > 0 ? o->w2 : kStage2TileBytes / eb;
+  po.w1 = o->w1 > 0 ? o->w1 : kStage1TileElems;
+  po.w2 = o->w2 > 0 ? o->w2 : kStage2TileElems;
   po.order = o->order;
-  // t_mode < 0: by storage.  FP64 tiles are wide enough that summing the
-  // stage-1 partials on the fly beats one more launch; the narrow types
-  // (measured, N = 101,250) are faster reading a materialised t for the
-  // leaves whose partial sums would be repeated or long.
-  po.t_mode = o->t_mode >= 0 ? o->t_mode : (po.storage == kF64 ? 0 : 2);
+  // t_mode < 0: materialise t for every leaf.  With t in row-cluster order
+  // the stage-2 factors of a cluster merge into one panel, which beats summing
+  // the stage-1 partials on the fly in all three storages (measured at
+  // N = 101,250) despite the extra launch.
+  po.t_mode = o->t_mode >= 0 ? o->t_mode : 1;
   return po;
 }
 
@@ -602,7 +606,7 @@ int create_plan(Source &src, const hmvm_options *opt, hmvm_plan **out) {
       return set_error(HMVM_ERR_ARG, "kernel variant must be in [0,%d)", kNumKernelVariants);
     p->kernel = opt->kernel;
     p->kernel_b = opt->kernel;
-    if (opt->mode < 0 || opt->mode > 1) return set_error(HMVM_ERR_ARG, "mode must be 0 or 1");
+    if (opt->mode < 0 || opt->mode > 2) return set_error(HMVM_ERR_ARG, "mode must be 0, 1 or 2");
     p->mode = opt->mode;
   }
   std::string err = build_plan(src.la, po, &p->plan);
@@ -612,6 +616,9 @@ int create_plan(Source &src, const hmvm_options *opt, hmvm_plan **out) {
   Plan &P = p->plan;
   HIP_TRY(hipGetDevice(&p->device));
   HIP_TRY(hipStreamCreate(&p->stream));
+  HIP_TRY(hipStreamCreate(&p->stream2));
+  HIP_TRY(hipEventCreateWithFlags(&p->ev_fork, hipEventDisableTiming));
+  HIP_TRY(hipEventCreateWithFlags(&p->ev_join, hipEventDisableTiming));
   {
     hipDeviceProp_t prop;
     HIP_TRY(hipGetDeviceProperties(&prop, p->device));
@@ -699,14 +706,29 @@ void launch_tiles(hmvm_plan *p, int which, const TileDesc *d_tiles, int ntiles, 
     launch_tiles_s<S, false>(p, kern, grid, block, st, d_tiles, ntiles, x, t, tp, out);
 }
 
+// which: 0 launch A (all), 1 launch B, 2 stage-1 tiles of A only, 3 dense tiles of A only
 template <typename S>
 void launch_tiles_both(hmvm_plan *p, int which, const double *x, hipStream_t st) {
   const Plan &P = p->plan;
   const bool bf = p->storage == kBF16;
+  const int na = (int)P.tiles_a.size(), n1 = P.n_stage1_tiles;
   if (which == 0)
-    launch_tiles<S>(p, 0, p->d_tiles_a, (int)P.tiles_a.size(), x, nullptr, nullptr, p->d_slots, bf, st);
+    launch_tiles<S>(p, 0, p->d_tiles_a, na, x, nullptr, nullptr, p->d_slots, bf, st);
+  else if (which == 2)
+    launch_tiles<S>(p, 0, p->d_tiles_a, n1, x, nullptr, nullptr, p->d_slots, bf, st);
+  else if (which == 3)
+    launch_tiles<S>(p, 0, p->d_tiles_a + n1, na - n1, x, nullptr, nullptr, p->d_slots, bf, st);
   else
     launch_tiles<S>(p, 1, p->d_tiles_b, (int)P.tiles_b.size(), x, p->d_t, p->d_slots, p->d_slots, bf, st);
+}
+
+void launch_tiles_part(hmvm_plan *p, int which, const double *x, hipStream_t st) {
+  if (p->storage == kF64)
+    launch_tiles_both<double>(p, which, x, st);
+  else if (p->storage == kF32)
+    launch_tiles_both<float>(p, which, x, st);
+  else
+    launch_tiles_both<uint16_t>(p, which, x, st);
 }
 
 // ---- per-tile timing (cost-model training data) -----------------------------
@@ -910,9 +932,30 @@ int launch_persistent(hmvm_plan *p, const double *x, double *y, int accumulate, 
 // partial slots directly (no materialised t).
 bool persistent_ok(const hmvm_plan *p) { return p->mode == 1 && p->plan.t_fused && !p->a_mixed; }
 
+// Dense tiles depend only on x and write their own output slots, so in mode 2
+// they run on a second stream, concurrently with the stage-1 tiles, the t sum
+// and the stage-2 tiles; the y sum waits for both.  The fork waits for
+// everything earlier on the caller's stream (including a previous HMVM's y
+// sum, which reads the same slots).  Measured on MI355X it does not beat one
+// stream (the launches already fill the device), so it is not the default.
+bool overlap_ok(const hmvm_plan *p) {
+  const Plan &P = p->plan;
+  return p->mode == 2 && !p->a_mixed && P.n_stage1_tiles > 0 && (int)P.tiles_a.size() > P.n_stage1_tiles;
+}
+
 int apply(hmvm_plan *p, const double *d_x, double *d_y, int accumulate, hipStream_t st) {
   if (persistent_ok(p)) {
     launch_persistent(p, d_x, d_y, accumulate, st);
+  } else if (overlap_ok(p)) {
+    HIP_TRY(hipEventRecord(p->ev_fork, st));
+    HIP_TRY(hipStreamWaitEvent(p->stream2, p->ev_fork, 0));
+    launch_tiles_part(p, 3, d_x, p->stream2);
+    HIP_TRY(hipEventRecord(p->ev_join, p->stream2));
+    launch_tiles_part(p, 2, d_x, st);
+    launch_phase(p, 1, d_x, d_y, accumulate, st);
+    launch_phase(p, 2, d_x, d_y, accumulate, st);
+    HIP_TRY(hipStreamWaitEvent(st, p->ev_join, 0));
+    launch_phase(p, 3, d_x, d_y, accumulate, st);
   } else {
     for (int ph = 0; ph < 4; ph++) launch_phase(p, ph, d_x, d_y, accumulate, st);
   }
@@ -1054,8 +1097,8 @@ void hmvm_options_default(hmvm_options *opt) {
   // measured best on MI355X for N = 101,250 (cube, eps 1e-4) in all three
   // storages: pipelined non-temporal loads, largest tiles first
   opt->storage = kF64;
-  opt->w1 = 0;  // by storage: 256 KiB per stage-1 tile
-  opt->w2 = 0;  // by storage: 128 KiB per dense / stage-2 tile
+  opt->w1 = 0;  // 32768 elements per stage-1 tile
+  opt->w2 = 0;  // 16384 elements per dense / stage-2 tile
   opt->block_threads = 256;
   opt->kernel = 6;
   opt->order = 1;
@@ -1257,7 +1300,7 @@ int hmvm_plan_time_multi(hmvm_plan *plan, int nrhs, int warmup, int iters, hmvm_
 
 int hmvm_plan_set_mode(hmvm_plan *plan, int mode) {
   if (!plan) return set_error(HMVM_ERR_ARG, "NULL plan");
-  if (mode < 0 || mode > 1) return set_error(HMVM_ERR_ARG, "mode must be 0 or 1");
+  if (mode < 0 || mode > 2) return set_error(HMVM_ERR_ARG, "mode must be 0, 1 or 2");
   plan->mode = mode;
   return HMVM_OK;
 }
@@ -1744,6 +1787,8 @@ int hmvm_plan_tile_features(const hmvm_plan *plan, int which, int nrhs, float *f
     f[12] = (float)((kern >> 2) & 1);
     f[13] = std::log2((float)(bt / 64));
     f[14] = std::log2((float)std::max(nrhs, 1));
+    f[15] = std::log2((float)tl.size());
+    f[16] = std::log2(1.0f + (float)(which ? P.elems_b : P.elems_a));
   }
   return HMVM_OK;
 }

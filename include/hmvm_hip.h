@@ -89,19 +89,19 @@ enum { HMVM_A2_ROWMAJOR = 0, HMVM_A2_TRANSPOSED = 1 };
 
 typedef struct hmvm_options {
   int storage;           /* HMVM_STORAGE_*: element type kept in HBM            */
-  int w1;                /* target elements per stage-1 tile (0: 256 KiB worth) */
-  int w2;                /* target elements per dense/stage-2 tile (0: 128 KiB) */
+  int w1;                /* target elements per stage-1 tile (0: 32768)         */
+  int w2;                /* target elements per dense/stage-2 tile (0: 16384)   */
   int block_threads;     /* threads per workgroup, 64..256 step 64 (0: default) */
   int kernel;            /* tile-kernel variant 0..7: bit0 vector entries read  */
                          /* 64 columns at a time and broadcast (readlane),      */
                          /* bit1 software pipelining, bit2 non-temporal loads   */
   int order;             /* tile order in a launch: 0 memory, 1 largest first   */
-  int mode;              /* 0: three launches (A, B, y sum); 1: one persistent  */
-                         /* launch with a device-wide work queue                */
+  int mode;              /* 0: launches on one stream; 1: one persistent launch */
+                         /* with a device work queue; 2: launches, dense tiles  */
+                         /* on a second stream concurrent with stage 1/2        */
   int t_mode;            /* stage-2 vector entries: 0 sum the stage-1 partial   */
                          /* slots on the fly, 1 read a materialised t (one more */
-                         /* launch), 2 choose per leaf, -1 by storage (default: */
-                         /* 0 for FP64, 2 for FP32/BF16)                         */
+                         /* launch), 2 choose per leaf, -1 default (1)          */
 } hmvm_options;
 
 typedef struct hmvm_plan_info {
@@ -235,13 +235,14 @@ int hmvm_plan_time_graph(hmvm_plan *plan, int count, int warmup, int iters, hmvm
  *   [7] log2(1+elements read incl. lane padding)  [8] log2(1+mean partial
  *   slots summed per vector entry)  [9] log2 bytes per element
  *   [10..12] kernel variant bits  [13] log2(block_threads/64)  [14] log2 nrhs
+ *   [15] log2 tiles in the launch  [16] log2(1+elements of the launch)
  * hmvm_plan_profile_tiles runs the launch with a clock-stamping copy of the
  * tile kernel (same results) and returns each tile's mean duration and start
  * offset in microseconds, plus the mean launch time.
  * hmvm_plan_set_tile_order permutes the dispatch order of a launch (results
  * are unchanged: tiles write disjoint slots); an order that mixes stage-1 and
  * dense tiles in launch A turns the persistent mode off for the plan. */
-enum { HMVM_NFEAT = 15 };
+enum { HMVM_NFEAT = 17 };
 int hmvm_plan_tile_features(const hmvm_plan *plan, int which, int nrhs, float *feat);
 int hmvm_plan_profile_tiles(hmvm_plan *plan, int which, int iters, double *tile_us, double *start_us,
                             double *launch_ms);

@@ -174,3 +174,31 @@ def test_oracle_64_equals_oracle(H, O, small):
     assert np.array_equal(O.seq_2_64(w, x), O.seq_2(small, x))
     ref = O.seq_2(small, x)
     assert np.linalg.norm(O.omp_2_64(w, x) - ref) <= 1e-14 * np.linalg.norm(ref)
+
+
+def test_leaf_subset_view(H, O, small):
+    a = small.arrays()
+    rows = a["nstrtl"] <= small.nd // 2          # a row band, as one GPU's shard
+    band = H.leaf_subset(small, np.nonzero(rows)[0])
+    rest = H.leaf_subset(small, np.nonzero(~rows)[0])
+    assert band.mat2.nlf + rest.mat2.nlf == small.mat2.nlf
+    x = np.random.default_rng(7).uniform(-1, 1, small.nd)
+    yb, yr = O.seq_2_64(band, x), O.seq_2_64(rest, x)
+    ref = O.seq_2(small, x)
+    assert np.linalg.norm(yb + yr - ref) <= 1e-14 * np.linalg.norm(ref)
+    # the band only touches its rows
+    top = a["nstrtl"][rows] + a["ndl"][rows] - 1
+    assert np.all(yb[top.max():] == 0)
+
+
+@pytest.mark.gpu
+def test_plan_on_leaf_subset(gpu, H, O, small):
+    a = small.arrays()
+    rows = a["nstrtl"] <= small.nd // 2
+    band = H.leaf_subset(small, np.nonzero(rows)[0])
+    x = np.random.default_rng(8).uniform(-1, 1, small.nd)
+    p = H.Plan(band)
+    y = p.apply_host(x)
+    ref = O.seq_2_64(band, x)
+    assert np.linalg.norm(y - ref) <= 1e-13 * np.linalg.norm(ref)
+    p.close()

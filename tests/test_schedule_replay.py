@@ -61,11 +61,15 @@ def rel(a, b):
 @pytest.mark.parametrize("kw", [{}, {"w1": 64, "w2": 64}, {"w1": 500, "w2": 3000, "order": 1},
                                 {"w1": 1 << 20, "w2": 1 << 20}])
 @pytest.mark.parametrize("source", ["rowmat", "rowmat_t"])
-def test_replay_matches_oracle(H, O, kw, source):
+@pytest.mark.parametrize("t_mode", [0, 1, 2])
+def test_replay_matches_oracle(H, O, kw, source, t_mode):
     h = H.generate(1500, geometry="cube", eps=1e-6)
     x = np.random.default_rng(3).uniform(-1, 1, h.nd)
-    y, s = replay(H, h, x, source, **kw)
-    assert s["t_fused"]
+    y, s = replay(H, h, x, source, t_mode=t_mode, **kw)
+    if t_mode == 0:
+        assert s["t_fused"]   # every stage-2 panel sums its partial slots
+    if t_mode == 1:
+        assert not s["t_fused"] and s["n_t"] == int(h.arrays()["kt"][h.arrays()["ltmtx"] == 1].sum())
     assert rel(y, O.seq_2(h, x)) <= 1e-13
 
 
