@@ -383,11 +383,22 @@ def run_c5(args, H, np):
                      "plan_s": round(t_plan, 1), "device_GB": round(info["device_bytes"] / 1e9, 2)}
         log("c5 %s: %s" % (st, forms[st]))
         p.close()
+    # schedule check on a sampled block subset: the leaves of the first row
+    # band of 1/16 of the rows (one GPU's share under row-band sharding)
+    tuning = None
+    if not args.no_tune:
+        a = h.arrays()
+        band = H.leaf_subset(h, np.nonzero(a["nstrtl"] <= h.nd // 16)[0])
+        model, X, yy = train_cost_model(H, np, ("f64",))
+        tuning = tune_report(H, np, band, "f64", model, args, X, yy)
+        tuning["band"] = {"rows": h.nd // 16, "nlf": int(band.mat2.nlf)}
+        log("c5 band tuning: %s" % tuning)
     line = base_line("ms/HMVM in a 100-HMVM hipGraph BiCGSTAB loop at N=1,048,576 (eps 1e-6): FP64 vs FP32 vs BF16",
                      forms["f64"]["bicgstab"]["ms_per_hmvm_in_loop"], "ms/HMVM", args, "f64",
                      {"workload": "BiCGSTAB 50 iterations (2 HMVMs each), N=%d Fibonacci sphere, leaf<=64, eta=2, "
                                   "ACA eps=1e-6, int64 offsets" % h.nd, "len": int(h.mat2.len)})
     line["forms"] = forms
+    line["tuning_band"] = tuning
     return line
 
 
@@ -405,6 +416,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--w1", type=int, default=0)
     ap.add_argument("--w2", type=int, default=0)
+    ap.add_argument("--no-tune", action="store_true", help="c5: skip the schedule check on a row band")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:

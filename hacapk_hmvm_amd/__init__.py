@@ -196,6 +196,8 @@ def _load():
         "hmvm_tile_occupancy": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, c_int_p]),
         "hmvm_schedule_export_batch": (ctypes.c_int, [P(P(matrix2)), ctypes.c_int, ctypes.c_int, P(hmvm_options),
                                                       P(hmvm_schedule)]),
+        "hmvm_schedule_export64": (ctypes.c_int, [P(matrix2_64), ctypes.c_int, P(hmvm_options), P(hmvm_schedule)]),
+        "hmvm_plan_reset_launch": (ctypes.c_int, [ctypes.c_void_p]),
         "hmvm_mlp_create": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_ulonglong, P(ctypes.c_void_p)]),
         "hmvm_mlp_destroy": (None, [ctypes.c_void_p]),
         "hmvm_mlp_num_params": (ctypes.c_int, [ctypes.c_void_p]),
@@ -657,6 +659,9 @@ def schedule(h, source="rowmat", **kw):
         arr = (ctypes.POINTER(matrix2) * len(h))(*[ctypes.pointer(m.mat2) for m in h])
         rc = lib.hmvm_schedule_export_batch(arr, len(h), 1 if source == "rowmat_t" else 0, ctypes.byref(o),
                                             ctypes.byref(s))
+    elif isinstance(h, HMatrix64):
+        rc = lib.hmvm_schedule_export64(ctypes.byref(h.mat2), 1 if source == "rowmat_t" else 0, ctypes.byref(o),
+                                        ctypes.byref(s))
     else:
         rc = lib.hmvm_schedule_export(ctypes.byref(h.mat2), 1 if source == "rowmat_t" else 0, ctypes.byref(o),
                                       ctypes.byref(s))

@@ -26,6 +26,8 @@ int hmvm_plan_create(const void *mat2, int a2_source, const void *opt, hmvm_plan
 void hmvm_plan_destroy(hmvm_plan *plan);
 int hmvm_plan_apply_host(hmvm_plan *plan, const double *x, double *y, int accumulate);
 int hmvm_plan_set_launch(hmvm_plan *plan, int kernel, int block_threads);
+int hmvm_plan_reset_launch(hmvm_plan *plan);
+void hmvm_options_default(void *opt);
 int hmvm_plan_bench_apply(hmvm_plan *plan, const double *x, int iters, double *seconds);
 int hmvm_hip_last_error(void);
 }
@@ -41,9 +43,10 @@ struct CMatrix2 {
   double *rowmat, *rowmat_t;
 };
 struct COptions {
-  int storage, w1, w2, block_threads, kernel, order, reserved[2];
+  int storage, w1, w2, block_threads, kernel, order, mode, t_mode;
 };
 static_assert(sizeof(CMatrix2) == 104, "C matrix2 layout");
+static_assert(sizeof(COptions) == 8 * sizeof(int), "hmvm_options layout");
 static_assert(offsetof(matrix2<double>, rowmat_t) == offsetof(CMatrix2, rowmat_t), "matrix2<double> prefix");
 
 template <class T>
@@ -87,9 +90,8 @@ hmvm_plan *plan_for(matrix2<T> *m) {
   }
   c.rowmat_t = nullptr;
   COptions o;
-  std::memset(&o, 0, sizeof(o));
+  hmvm_options_default(&o);
   o.storage = is_float ? 1 : 0;
-  o.block_threads = 256;
   hmvm_plan *p = nullptr;
   if (hmvm_plan_create(&c, 0, &o, &p)) return nullptr;
   Entry e;
@@ -197,8 +199,8 @@ void hmvm_hip(matrix2<T> *mat2, T *b, int kernel, int dump_result, int nbench) {
   if (!mat2 || !b) return;
   hmvm_plan *p = plan_for(mat2);
   if (!p) return;
-  const int variant = kernel < 0 ? 6 : kernel;
-  if (hmvm_plan_set_launch(p, variant, 256)) return;
+  // kernel < 0: the measured per-storage launch defaults; 0..7: that variant
+  if (kernel < 0 ? hmvm_plan_reset_launch(p) : hmvm_plan_set_launch(p, kernel, 256)) return;
   const int nd = mat2->nd;
   const char tc = sizeof(T) == 4 ? 'f' : 'd';
   std::vector<double> x((size_t)nd), y((size_t)nd, 0.0);

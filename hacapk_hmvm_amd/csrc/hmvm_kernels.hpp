@@ -298,7 +298,7 @@ __device__ __forceinline__ void wave_lds_sync() {
 // sums columns jg, jg+g, ... so for a contiguous panel each load instruction
 // reads g*lpc*V consecutive elements.  The g partial sums of each row are
 // added in a fixed group order at the end.
-template <typename S, int kLoads, bool kPipe, bool kNT, bool kScaled, bool kCoh>
+template <typename S, int kLoads, bool kNT, bool kScaled, bool kCoh>
 __device__ __forceinline__ void grouped_tile(const TileDesc &td, int shift, int lpc,
                                              const PanelDesc *__restrict__ panels, const S *__restrict__ vals,
                                              const double *__restrict__ x, const double *__restrict__ t,
@@ -345,27 +345,13 @@ __device__ __forceinline__ void grouped_tile(const TileDesc &td, int shift, int 
 #pragma unroll
         for (int v = 0; v < V; ++v) accp[v] += (A)W::get(a[u], v) * b[u];
     };
-    if (kPipe && nb >= 2) {
-      T a0[kLoads], a1[kLoads];
-      A b0[kLoads], b1[kLoads];
-      load_batch(0, a0, b0);
-      for (int sb = 1; sb < nb; ++sb) {
-        load_batch(sb, a1, b1);  // in flight while batch sb-1 is consumed
-        consume(a0, b0);
-#pragma unroll
-        for (int u = 0; u < kLoads; ++u) {
-          a0[u] = a1[u];
-          b0[u] = b1[u];
-        }
-      }
-      consume(a0, b0);
-    } else {
-      for (int sb = 0; sb < nb; ++sb) {
-        T a[kLoads];
-        A b[kLoads];
-        load_batch(sb, a, b);
-        consume(a, b);
-      }
+    // (no software pipelining here: a second batch of registers costs more
+    // occupancy than it gains, measured; the wide path pipelines)
+    for (int sb = 0; sb < nb; ++sb) {
+      T a[kLoads];
+      A b[kLoads];
+      load_batch(sb, a, b);
+      consume(a, b);
     }
     const A sc = kScaled ? (A)panel_scale(pd, scales) : (A)1;
 #pragma unroll
@@ -397,7 +383,7 @@ __device__ __forceinline__ void run_tile(const TileDesc &td, const PanelDesc *__
   const int shift = (int)(panels[td.panel0].val_off % V);
   const int lpc = (td.nrows + shift + V - 1) / V;  // lanes per column
   if (lpc <= 32) {
-    grouped_tile<S, kLoads, kPipe, kNT, kScaled, kCoh>(td, shift, lpc, panels, vals, x, t, tp, scales, out, lane);
+    grouped_tile<S, kLoads, kNT, kScaled, kCoh>(td, shift, lpc, panels, vals, x, t, tp, scales, out, lane);
   } else if (lpc <= 64) {
     wide_tile<S, 1, kLoads, kPipe, kNT, kScaled, kCoh, kBV>(td, shift, lpc, panels, vals, x, t, tp, scales, out, lane);
   } else if (lpc <= 128) {

@@ -344,10 +344,21 @@ int source_from_batch(const matrix2 *const *mats, int count, int a2_source, Sour
   return HMVM_OK;
 }
 
-// Default tile widths in elements (the same for every storage: the tile
-// duration is set more by columns and rows than by bytes; measured best for
-// FP64 and FP32 at N = 101,250).
-constexpr int kStage1TileElems = 32768, kStage2TileElems = 16384;
+// Launch defaults per storage, measured best on MI355X at N = 101,250 (cube,
+// eps 1e-4): tile widths in elements, then kernel variant and workgroup size
+// of launch A and launch B.  Options with 0 (widths, workgroup size) or -1
+// (kernel) take these.
+struct LaunchDefaults {
+  int w1, w2, kernel_a, bt_a, kernel_b, bt_b;
+};
+const LaunchDefaults &launch_defaults(int storage) {
+  static const LaunchDefaults d[3] = {
+      {32768, 16384, 6, 64, 6, 256},   // FP64
+      {16384, 16384, 4, 256, 4, 256},  // FP32
+      {32768, 16384, 4, 256, 4, 256},  // BF16
+  };
+  return d[storage >= kF64 && storage <= kBF16 ? storage : 0];
+}
 
 PlanOptions plan_options(const hmvm_options *o) {
   hmvm_options d;
@@ -357,8 +368,8 @@ PlanOptions plan_options(const hmvm_options *o) {
   }
   PlanOptions po;
   po.storage = o->storage;
-  po.w1 = o->w1 > 0 ? o->w1 : kStage1TileElems;
-  po.w2 = o->w2 > 0 ? o->w2 : kStage2TileElems;
+  po.w1 = o->w1 > 0 ? o->w1 : launch_defaults(po.storage).w1;
+  po.w2 = o->w2 > 0 ? o->w2 : launch_defaults(po.storage).w2;
   po.order = o->order;
   // t_mode < 0: materialise t for every leaf.  With t in row-cluster order
   // the stage-2 factors of a cluster merge into one panel, which beats summing
@@ -595,20 +606,25 @@ int create_plan(Source &src, const hmvm_options *opt, hmvm_plan **out) {
   if (po.storage < kF64 || po.storage > kBF16) return set_error(HMVM_ERR_ARG, "unknown storage %d", po.storage);
   std::unique_ptr<hmvm_plan> p(new hmvm_plan());
   p->storage = po.storage;
-  if (opt && opt->block_threads > 0) {
+  const LaunchDefaults &ld = launch_defaults(po.storage);
+  p->kernel = ld.kernel_a;
+  p->kernel_b = ld.kernel_b;
+  p->block_threads = ld.bt_a;
+  p->block_threads_b = ld.bt_b;
+  if (opt->block_threads > 0) {
     if (opt->block_threads % 64 || opt->block_threads > 256 || opt->block_threads < 64)
       return set_error(HMVM_ERR_ARG, "block_threads must be 64, 128, 192 or 256");
     p->block_threads = opt->block_threads;
     p->block_threads_b = opt->block_threads;
   }
-  if (opt) {
-    if (opt->kernel < 0 || opt->kernel >= kNumKernelVariants)
+  if (opt->kernel >= 0) {
+    if (opt->kernel >= kNumKernelVariants)
       return set_error(HMVM_ERR_ARG, "kernel variant must be in [0,%d)", kNumKernelVariants);
     p->kernel = opt->kernel;
     p->kernel_b = opt->kernel;
-    if (opt->mode < 0 || opt->mode > 2) return set_error(HMVM_ERR_ARG, "mode must be 0, 1 or 2");
-    p->mode = opt->mode;
   }
+  if (opt->mode < 0 || opt->mode > 2) return set_error(HMVM_ERR_ARG, "mode must be 0, 1 or 2");
+  p->mode = opt->mode;
   std::string err = build_plan(src.la, po, &p->plan);
   if (!err.empty()) return set_error(HMVM_ERR_MATRIX, "%s", err.c_str());
   err = verify_plan(p->plan, src.la);
@@ -1097,10 +1113,10 @@ void hmvm_options_default(hmvm_options *opt) {
   // measured best on MI355X for N = 101,250 (cube, eps 1e-4) in all three
   // storages: pipelined non-temporal loads, largest tiles first
   opt->storage = kF64;
-  opt->w1 = 0;  // 32768 elements per stage-1 tile
-  opt->w2 = 0;  // 16384 elements per dense / stage-2 tile
-  opt->block_threads = 256;
-  opt->kernel = 6;
+  opt->w1 = 0;             // by storage (launch_defaults)
+  opt->w2 = 0;
+  opt->block_threads = 0;  // by storage, per launch
+  opt->kernel = -1;        // by storage, per launch
   opt->order = 1;
   opt->mode = 0;
   opt->t_mode = -1;  // by storage
@@ -1200,6 +1216,16 @@ int hmvm_plan_set_launch(hmvm_plan *plan, int kernel, int block_threads) {
     return set_error(HMVM_ERR_ARG, "block_threads must be 64, 128, 192 or 256");
   plan->kernel = plan->kernel_b = kernel;
   plan->block_threads = plan->block_threads_b = block_threads;
+  return HMVM_OK;
+}
+
+int hmvm_plan_reset_launch(hmvm_plan *plan) {
+  if (!plan) return set_error(HMVM_ERR_ARG, "NULL plan");
+  const LaunchDefaults &ld = launch_defaults(plan->storage);
+  plan->kernel = ld.kernel_a;
+  plan->kernel_b = ld.kernel_b;
+  plan->block_threads = ld.bt_a;
+  plan->block_threads_b = ld.bt_b;
   return HMVM_OK;
 }
 
@@ -1421,6 +1447,17 @@ int hmvm_schedule_export(const matrix2 *mat2, int a2_source, const hmvm_options 
   if (a2_source != kA2RowMajor && a2_source != kA2Transposed) return set_error(HMVM_ERR_ARG, "bad a2_source");
   Source s;
   int rc = source_from_matrix2(mat2, a2_source, &s);
+  if (rc) return rc;
+  return export_schedule(s, opt, out);
+}
+
+int hmvm_schedule_export64(const matrix2_64 *mat2, int a2_source, const hmvm_options *opt, hmvm_schedule *out) {
+  clear_error();
+  if (!out) return set_error(HMVM_ERR_ARG, "out is NULL");
+  std::memset(out, 0, sizeof(*out));
+  if (a2_source != kA2RowMajor && a2_source != kA2Transposed) return set_error(HMVM_ERR_ARG, "bad a2_source");
+  Source s;
+  int rc = source_from_matrix2_64(mat2, a2_source, &s);
   if (rc) return rc;
   return export_schedule(s, opt, out);
 }

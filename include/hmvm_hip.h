@@ -89,10 +89,12 @@ enum { HMVM_A2_ROWMAJOR = 0, HMVM_A2_TRANSPOSED = 1 };
 
 typedef struct hmvm_options {
   int storage;           /* HMVM_STORAGE_*: element type kept in HBM            */
-  int w1;                /* target elements per stage-1 tile (0: 32768)         */
-  int w2;                /* target elements per dense/stage-2 tile (0: 16384)   */
-  int block_threads;     /* threads per workgroup, 64..256 step 64 (0: default) */
-  int kernel;            /* tile-kernel variant 0..7: bit0 vector entries read  */
+  int w1;                /* target elements per stage-1 tile (0: by storage)    */
+  int w2;                /* target elements per dense/stage-2 tile (0: ditto)   */
+  int block_threads;     /* threads per workgroup, 64..256 step 64 (0: by       */
+                         /* storage, per launch)                                */
+  int kernel;            /* tile-kernel variant 0..7 (-1: by storage, per       */
+                         /* launch): bit0 vector entries read                   */
                          /* 64 columns at a time and broadcast (readlane),      */
                          /* bit1 software pipelining, bit2 non-temporal loads   */
   int order;             /* tile order in a launch: 0 memory, 1 largest first   */
@@ -161,6 +163,8 @@ int hmvm_plan_time(hmvm_plan *plan, int warmup, int iters, hmvm_timing *timing);
 int hmvm_plan_set_launch(hmvm_plan *plan, int kernel, int block_threads);
 /* The same for one launch only: which = 0 (A: stage 1 + dense) or 1 (B). */
 int hmvm_plan_set_phase_launch(hmvm_plan *plan, int which, int kernel, int block_threads);
+/* Back to the measured per-storage defaults of both launches. */
+int hmvm_plan_reset_launch(hmvm_plan *plan);
 
 /* Several right-hand sides on the matrix cores: Y (+)= A X with X, Y of
  * nd x nrhs doubles, row-major (X[i*nrhs + c]); nrhs = 16, 32 or 64.  FP64
@@ -326,6 +330,8 @@ typedef struct hmvm_schedule {
 } hmvm_schedule;
 
 int hmvm_schedule_export(const matrix2 *mat2, int a2_source, const hmvm_options *opt, hmvm_schedule *out);
+/* The schedule of hmvm_plan_create64 for the same matrix. */
+int hmvm_schedule_export64(const matrix2_64 *mat2, int a2_source, const hmvm_options *opt, hmvm_schedule *out);
 /* The schedule of hmvm_plan_create_batch for the same matrices. */
 int hmvm_schedule_export_batch(const matrix2 *const *mats, int count, int a2_source, const hmvm_options *opt,
                                hmvm_schedule *out);
