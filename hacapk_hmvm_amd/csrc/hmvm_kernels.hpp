@@ -387,11 +387,9 @@ __device__ __forceinline__ void run_tile(const TileDesc &td, const PanelDesc *__
   } else if (lpc <= 64) {
     wide_tile<S, 1, kLoads, kPipe, kNT, kScaled, kCoh, kBV>(td, shift, lpc, panels, vals, x, t, tp, scales, out, lane);
   } else if (lpc <= 128) {
-    if constexpr (V <= 4)
-      wide_tile<S, 2, kLoads, kPipe, kNT, kScaled, kCoh, kBV>(td, shift, lpc, panels, vals, x, t, tp, scales, out, lane);
-  } else {
-    if constexpr (V <= 2)
-      wide_tile<S, 3, kLoads, kPipe, kNT, kScaled, kCoh, kBV>(td, shift, lpc, panels, vals, x, t, tp, scales, out, lane);
+    wide_tile<S, 2, kLoads, kPipe, kNT, kScaled, kCoh, kBV>(td, shift, lpc, panels, vals, x, t, tp, scales, out, lane);
+  } else {  // lpc <= 192: the schedule check bounds the rows of every tile
+    wide_tile<S, 3, kLoads, kPipe, kNT, kScaled, kCoh, kBV>(td, shift, lpc, panels, vals, x, t, tp, scales, out, lane);
   }
 }
 

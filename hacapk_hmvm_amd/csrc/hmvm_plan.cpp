@@ -15,10 +15,10 @@ namespace {
 
 inline int64_t align_up(int64_t v, int64_t a) { return (v + a - 1) / a * a; }
 
-// Splits n rows into ceil(n/kMaxTileRows) near-equal chunks.
-std::vector<std::pair<int, int>> row_chunks(int n) {
+// Splits n rows into ceil(n/mr) near-equal chunks.
+std::vector<std::pair<int, int>> row_chunks(int n, int mr) {
   std::vector<std::pair<int, int>> out;
-  const int nc = (n + kMaxTileRows - 1) / kMaxTileRows;
+  const int nc = (n + mr - 1) / mr;
   int r0 = 0;
   for (int c = 0; c < nc; c++) {
     const int sz = n / nc + (c < n % nc ? 1 : 0);
@@ -66,6 +66,10 @@ std::string build_plan(const LeafArrays &la, const PlanOptions &opt, Plan *plan)
   P.nlf = la.nlf;
   if (la.nd < 1 || la.nlf < 1) return "empty matrix";
   if (opt.w1 < 64 || opt.w2 < 64 || opt.align < 1 || opt.t_mode < 0 || opt.t_mode > 2) return "bad plan options";
+  const int MR = opt.max_rows;
+  if (MR < 1 || MR > max_tile_rows_limit(lane_elems(opt.storage)) || MR >= (1 << 16))
+    return "max_rows out of range for this storage";
+  P.max_rows = MR;
   if (la.nd > (1 << 30)) return "matrix order too large";
   const int nlf = la.nlf;
 
@@ -105,7 +109,7 @@ std::string build_plan(const LeafArrays &la, const PlanOptions &opt, Plan *plan)
   P.lane_elems = V;
   if (opt.align % V) return "alignment must be a multiple of the lane width";
   std::vector<int64_t> t_idx(nlf, -1);     // index of t_b[0] in the materialised t
-  // stage-1 rows of every piece of a leaf (a leaf of more than kMaxTileRows
+  // stage-1 rows of every piece of a leaf (a leaf of more than max_rows
   // ranks has two or more pieces): the t sum adds its partial slots
   struct PieceRec { int leaf, k0, n; int64_t slot0; int nr, nch, kk0; };
   std::vector<PieceRec> pieces;
@@ -128,8 +132,8 @@ std::string build_plan(const LeafArrays &la, const PlanOptions &opt, Plan *plan)
     const int Kp = (int)align_up(K, V);  // padded stack height = column stride
     const int64_t base = align_up(off, opt.align);
     off = base + (int64_t)ndt * Kp;
-    // row groups of <= kMaxTileRows rank rows, cut at leaf boundaries when a
-    // leaf fits; a leaf of more than kMaxTileRows ranks is cut inside
+    // row groups of <= max_rows rank rows, cut at leaf boundaries when a leaf
+    // fits; a leaf of more than max_rows ranks is cut inside
     struct Piece { int leaf, k0, n; };  // leaf rows [k0, k0+n) at stack row kofs
     std::vector<std::vector<Piece>> groups(1);
     std::vector<int> gsize(1, 0);
@@ -150,12 +154,11 @@ std::string build_plan(const LeafArrays &la, const PlanOptions &opt, Plan *plan)
       P.repack.push_back(r);
       int done = 0;
       while (done < kt) {
-        if (gsize.back() == kMaxTileRows || (done == 0 && gsize.back() + kt > kMaxTileRows && gsize.back() > 0 &&
-                                              kt <= kMaxTileRows)) {
+        if (gsize.back() == MR || (done == 0 && gsize.back() + kt > MR && gsize.back() > 0 && kt <= MR)) {
           groups.emplace_back();
           gsize.push_back(0);
         }
-        const int take = std::min(kt - done, kMaxTileRows - gsize.back());
+        const int take = std::min(kt - done, MR - gsize.back());
         groups.back().push_back({i, done, take});
         gsize.back() += take;
         done += take;
@@ -223,7 +226,7 @@ std::string build_plan(const LeafArrays &la, const PlanOptions &opt, Plan *plan)
     if (opt.t_mode == 1 || (opt.t_mode == 0 && any_forced) || forced[i])
       mat[i] = 1;
     else if (opt.t_mode == 2)
-      mat[i] = la.ndl[i] > kMaxTileRows || (tp_chunks[i] >> 16) > 2;
+      mat[i] = la.ndl[i] > MR || (tp_chunks[i] >> 16) > 2;
   }
   // t holds the materialised leaves in row-cluster order, ranks contiguous, so
   // that the a2 factors of consecutive leaves of a row cluster (stored back to
@@ -301,7 +304,7 @@ std::string build_plan(const LeafArrays &la, const PlanOptions &opt, Plan *plan)
         P.repack.push_back(r);
       }
     }
-    for (auto rc : row_chunks(ndl)) {
+    for (auto rc : row_chunks(ndl, MR)) {
       const int r0 = rc.first, nr = rc.second;
       for (int kind = 0; kind < 2; kind++) {
         const std::vector<int> &lst = kind == 0 ? dense : lowrank;
@@ -445,7 +448,8 @@ std::string verify_plan(const Plan &P, const LeafArrays &la) {
   for (int list = 0; list < 2; list++) {
     const std::vector<TileDesc> &tl = list == 0 ? P.tiles_a : P.tiles_b;
     for (const TileDesc &t : tl) {
-      if (t.nrows < 1 || t.nrows > kMaxTileRows) return "tile rows out of range";
+      if (t.nrows < 1 || t.nrows > P.max_rows || P.max_rows > max_tile_rows_limit(P.lane_elems))
+        return "tile rows out of range";
       if (t.panel0 < 0 || t.npanel < 1 || (int64_t)t.panel0 + t.npanel > npan) return "tile panel range";
       if (t.out_off < 0) return "negative output slot";
       const int64_t shift0 = P.panels[t.panel0].val_off % P.lane_elems;

@@ -47,7 +47,10 @@ enum VecSel : int {
   kVecTP = 2   // sum_{c < nch} tp[vec_off + c*stride + j]  (stage-1 partial slots)
 };
 
+// Rows of one tile: at most kMaxTileRows (the FP64 default); the kernels
+// handle up to 192 lanes x V rows (192*V - V + 1 with the row shift).
 constexpr int kMaxTileRows = 256;
+inline int max_tile_rows_limit(int lane_elems) { return 192 * lane_elems - lane_elems + 1; }
 
 struct PanelDesc {        // 32 bytes, read with scalar loads
   int64_t val_off;        // element offset of (row 0 of the tile, column 0)
@@ -60,7 +63,7 @@ struct PanelDesc {        // 32 bytes, read with scalar loads
 };
 
 struct TileDesc {         // 16 bytes
-  int32_t nrows;          // 1..kMaxTileRows
+  int32_t nrows;          // 1..Plan::max_rows
   int32_t panel0;
   int32_t npanel;
   int32_t out_off;        // first element of the tile's output slot
@@ -90,6 +93,7 @@ struct PlanOptions {
   int align = 16;         // factor start alignment in elements
   int order = 0;          // 0: memory order, 1: largest tiles first
   int t_mode = 0;         // stage-2 vector: 0 partial sums on the fly, 1 materialised t, 2 per leaf
+  int max_rows = kMaxTileRows;  // rows per tile (rank rows per stage-1 group, leaf rows per chunk)
 };
 
 struct LeafArrays {       // the caller's descriptors (matrix2 or matrix)
@@ -108,6 +112,7 @@ int lane_elems(int storage);
 struct Plan {
   int nd = 0, nlf = 0;
   int lane_elems = 2;                // column strides and factor bases are multiples of this
+  int max_rows = kMaxTileRows;       // rows per tile
   int64_t nvals = 0;                 // device value buffer (elements)
   int64_t len = 0;                   // reference element count
   int nfactors = 0;

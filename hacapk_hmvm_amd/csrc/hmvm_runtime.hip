@@ -346,16 +346,18 @@ int source_from_batch(const matrix2 *const *mats, int count, int a2_source, Sour
 
 // Launch defaults per storage, measured best on MI355X at N = 101,250 (cube,
 // eps 1e-4): tile widths in elements, then kernel variant and workgroup size
-// of launch A and launch B.  Options with 0 (widths, workgroup size) or -1
-// (kernel) take these.
+// of launch A and launch B, and the most rows one tile (one wavefront) may
+// cover.  Taller tiles only pay off for BF16, whose 8-element lane loads
+// leave short stage-2 tiles with few lanes busy.  Options with 0 (widths,
+// workgroup size, rows) or -1 (kernel) take these.
 struct LaunchDefaults {
-  int w1, w2, kernel_a, bt_a, kernel_b, bt_b;
+  int w1, w2, kernel_a, bt_a, kernel_b, bt_b, max_rows;
 };
 const LaunchDefaults &launch_defaults(int storage) {
   static const LaunchDefaults d[3] = {
-      {32768, 16384, 6, 64, 6, 256},   // FP64
-      {16384, 16384, 4, 256, 4, 256},  // FP32
-      {32768, 16384, 4, 256, 4, 256},  // BF16
+      {32768, 16384, 6, 64, 6, 256, 256},   // FP64
+      {16384, 16384, 4, 256, 4, 256, 256},  // FP32
+      {32768, 16384, 6, 256, 6, 256, 512},  // BF16
   };
   return d[storage >= kF64 && storage <= kBF16 ? storage : 0];
 }
@@ -370,6 +372,7 @@ PlanOptions plan_options(const hmvm_options *o) {
   po.storage = o->storage;
   po.w1 = o->w1 > 0 ? o->w1 : launch_defaults(po.storage).w1;
   po.w2 = o->w2 > 0 ? o->w2 : launch_defaults(po.storage).w2;
+  po.max_rows = o->max_rows > 0 ? o->max_rows : launch_defaults(po.storage).max_rows;
   po.order = o->order;
   // t_mode < 0: materialise t for every leaf.  With t in row-cluster order
   // the stage-2 factors of a cluster merge into one panel, which beats summing
@@ -1120,6 +1123,7 @@ void hmvm_options_default(hmvm_options *opt) {
   opt->order = 1;
   opt->mode = 0;
   opt->t_mode = -1;  // by storage
+  opt->max_rows = 0;  // by storage
 }
 
 int hmvm_plan_create(const matrix2 *mat2, int a2_source, const hmvm_options *opt, hmvm_plan **out) {

@@ -104,6 +104,8 @@ typedef struct hmvm_options {
   int t_mode;            /* stage-2 vector entries: 0 sum the stage-1 partial   */
                          /* slots on the fly, 1 read a materialised t (one more */
                          /* launch), 2 choose per leaf, -1 default (1)          */
+  int max_rows;          /* rows per tile (0: by storage); at most 192*V-V+1,  */
+                         /* V = 16 bytes / element size                         */
 } hmvm_options;
 
 typedef struct hmvm_plan_info {

@@ -276,3 +276,29 @@ def test_two_stream_overlap_matches_one_stream(gpu, H, sphere4096, storage):
     assert t["ms_stream"] > 0 and tg["ms_stream"] > 0
     assert np.array_equal(p.apply_host(b), y0)
     p.close()
+
+
+_TALL = []
+
+
+def tall_matrix(H):
+    if not _TALL:
+        _TALL.append(H.generate(12000, geometry="sphere", eps=1e-4))
+    return _TALL[0]
+
+
+@pytest.mark.parametrize("storage,max_rows,tol", [("f64", 383, 1e-13), ("f32", 765, 1e-5), ("bf16", 1529, 3e-2),
+                                                  ("f32", 512, 1e-5), ("bf16", 1024, 3e-2)])
+def test_tall_tiles(gpu, H, O, storage, max_rows, tol):
+    """Tiles of up to 192 lanes x 16 bytes of rows (three row groups per lane)."""
+    h = tall_matrix(H)  # row clusters of 753 rows: 2 or 3 row groups per lane
+    x = np.random.default_rng(16).uniform(-1, 1, h.nd)
+    p = H.Plan(h, storage=storage, max_rows=max_rows, w1=1 << 18, w2=1 << 18)
+    y = p.apply_host(x)
+    s = H.schedule(h, storage=storage, max_rows=max_rows, w1=1 << 18, w2=1 << 18)
+    assert max(s["tiles_a"]["nrows"].max(), s["tiles_b"]["nrows"].max()) > 256
+    assert rel(y, O.seq_2(h, x)) <= tol
+    for k in (0, 6, 7):
+        p.set_launch(k, 256)
+        assert np.array_equal(p.apply_host(x), y)
+    p.close()
