@@ -256,10 +256,11 @@ def train_cost_model(H, np, storages, seed=0):
     """Trains the MLP on tile timings of generated matrices that are not part of any benchmark."""
     from hacapk_hmvm_amd import autotune as AT
     Xs, ys, recs = [], [], []
-    specs = [(20000, "sphere", 1e-4), (36000, "cube", 1e-6), (28000, "cube", 1e-4), (44000, "sphere", 1e-6),
-             (70000, "sphere", 1e-4), (130000, "cube", 1e-4)]
-    for i, (n, geo, eps) in enumerate(specs):
-        m = H.generate(n, geometry=geo, eps=eps, seed=1000 + i)
+    specs = [(20000, "sphere", 1e-4, 32), (36000, "cube", 1e-6, 32), (28000, "cube", 1e-4, 32),
+             (44000, "sphere", 1e-6, 32), (70000, "sphere", 1e-4, 32), (130000, "cube", 1e-4, 32),
+             (40000, "sphere", 1e-6, 64), (60000, "cube", 1e-4, 64)]
+    for i, (n, geo, eps, leaf) in enumerate(specs):
+        m = H.generate(n, geometry=geo, eps=eps, leaf=leaf, seed=1000 + i)
         for st in storages:
             X, y, r = AT.collect(m, st, w_scales=((1.0, 1.0), (0.5, 2.0), (2.0, 0.5)), iters=2, max_tiles=6000,
                                  seed=seed + i)
@@ -295,7 +296,7 @@ def tune_report(H, np, h, storage, model, args, X_train=None, y_train=None):
            "learned_choice": {k: v for k, v in choice.items()}}
     if X_train is not None:
         Xt, yt, _ = AT.collect(h, storage, iters=2, max_tiles=20000, seed=77)
-        out["heldout_abs_log2_err"] = {"mlp": round(float(np.mean(np.abs(model.mlp.predict(Xt) - yt))), 4),
+        out["heldout_abs_log2_err"] = {"mlp": round(float(np.mean(np.abs(model.predict_log2(Xt) - yt))), 4),
                                        "linear": round(float(np.mean(np.abs(
                                            AT.linear_baseline(X_train, y_train)(Xt) - yt))), 4)}
     p_l.close()

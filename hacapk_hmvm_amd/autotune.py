@@ -81,8 +81,9 @@ def schedule_features(sched, which, storage, kernel, block_threads, nrhs=1):
     f[:, 12] = (kernel >> 2) & 1
     f[:, 13] = np.log2(block_threads // 64)
     f[:, 14] = np.log2(max(nrhs, 1))
-    f[:, 15] = np.log2(n)
-    f[:, 16] = np.log2(1.0 + float((nrows * cols).sum()))  # elements the launch streams
+    slots = max(occupancy(storage, kernel, block_threads), 1)
+    f[:, 15] = np.log2(min(n, slots))  # saturation of the launch
+    f[:, 16] = np.log2(slots)
     return f
 
 
@@ -148,8 +149,19 @@ class CostModel:
     def __init__(self, nh=64, seed=1):
         self.mlp = CostMLP(HMVM_NFEAT, nh, seed)
         self.calib = {}
+        self.lo = self.hi = None
+
+    def clip(self, X):
+        """Features clamped to the range seen in training.  The launch-level
+        features (tiles and elements per launch) of a large batched plan lie
+        far beyond any training launch; past saturation of the chip they no
+        longer change a tile's duration, and an MLP extrapolates them badly."""
+        if self.lo is None:
+            return X
+        return np.clip(X, self.lo, self.hi).astype(np.float32)
 
     def fit(self, X, y, recs, epochs=80, batch=512, lr=2e-3, seed=3):
+        self.lo, self.hi = X.min(axis=0), X.max(axis=0)
         self.mlp.fit_scaler(X)
         loss = self.mlp.train(X, y, epochs=epochs, batch=batch, lr=lr, seed=seed)
         by = {}
@@ -163,8 +175,11 @@ class CostModel:
             self.calib[st] = (float(coef[0]), float(coef[1]))
         return loss
 
+    def predict_log2(self, X):
+        return self.mlp.predict(self.clip(X))
+
     def tile_us(self, X):
-        return np.exp2(self.mlp.predict(X).astype(np.float64))
+        return np.exp2(self.predict_log2(X).astype(np.float64))
 
     def simulate_ms(self, X, storage, kernel, bt, reorder=True):
         d = self.tile_us(X)
@@ -179,10 +194,12 @@ class CostModel:
 
 def linear_baseline(X, y):
     """The method without the network: log-duration linear in the features
-    (least squares), i.e. a roofline-style 'cost grows with elements' model."""
+    (least squares), i.e. a roofline-style 'cost grows with elements' model.
+    Inputs are clamped to the training range like the MLP's."""
+    lo, hi = X.min(axis=0), X.max(axis=0)
     A = np.hstack([X.astype(np.float64), np.ones((X.shape[0], 1))])
     coef, *_ = np.linalg.lstsq(A, y.astype(np.float64), rcond=None)
-    return lambda Z: np.hstack([Z.astype(np.float64), np.ones((Z.shape[0], 1))]) @ coef
+    return lambda Z: np.hstack([np.clip(Z, lo, hi).astype(np.float64), np.ones((Z.shape[0], 1))]) @ coef
 
 
 # ---- schedule search ---------------------------------------------------------------

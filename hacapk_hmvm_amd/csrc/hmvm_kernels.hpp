@@ -66,9 +66,9 @@ __device__ __forceinline__ auto load_stream(const S *p) {
 
 // Kernel variants (bit field): bit0 vector entries of wide tiles fetched 64
 // columns at a time (one lane each) and broadcast with readlane instead of one
-// wave-uniform load per column, bit1 software pipelining (next batch issued
-// before the current one is consumed), bit2 non-temporal matrix loads.
-// Every variant keeps 8 loads in flight per batch.
+// wave-uniform load per column, bit1 software pipelining of wide tiles (next
+// batch issued before the current one is consumed), bit2 non-temporal matrix
+// loads.  Every variant keeps 8 loads in flight per batch.
 constexpr int kNumKernelVariants = 8;
 
 // Vector entry j of a panel.  kCoherent: the stage-1 partial slots were
@@ -294,16 +294,31 @@ __device__ __forceinline__ void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// LDS of one wavefront, in doubles: a chunk of kStage staged vector entries
+// of the grouped path, reused for its group reduction (64*V).  Workgroups get
+// (waves per workgroup) regions as dynamic shared memory, so one-wave
+// workgroups hold no LDS for waves they do not have.
+constexpr int kStage = 512;
+template <typename S>
+__host__ __device__ constexpr int lds_wave_doubles() {
+  return kStage > 64 * Wide<S>::V ? kStage : 64 * Wide<S>::V;
+}
+
 // lanes_per_col <= 32: the wave holds g = 64/lpc column groups; lane (c, jg)
 // sums columns jg, jg+g, ... so for a contiguous panel each load instruction
 // reads g*lpc*V consecutive elements.  The g partial sums of each row are
-// added in a fixed group order at the end.
+// added in a fixed group order at the end.  The vector entries of kStage
+// columns at a time are fetched once (one coalesced load per lane per 64
+// columns) into the wave's LDS region, and every lane reads its columns'
+// entries there: a lane needs a different column's entry for every matrix
+// load, and a global load per matrix load halves the rate of these tiles
+// (measured; the wide path, whose entries are wave-uniform, does not stage).
 template <typename S, int kLoads, bool kNT, bool kScaled, bool kCoh>
 __device__ __forceinline__ void grouped_tile(const TileDesc &td, int shift, int lpc,
                                              const PanelDesc *__restrict__ panels, const S *__restrict__ vals,
                                              const double *__restrict__ x, const double *__restrict__ t,
                                              const double *tp, const float *__restrict__ scales,
-                                             double *__restrict__ out, int lane) {
+                                             double *__restrict__ out, int lane, double *lds) {
   using A = typename AccOf<S>::T;
   using W = Wide<S>;
   using T = typename W::T;
@@ -322,20 +337,6 @@ __device__ __forceinline__ void grouped_tile(const TileDesc &td, int shift, int 
     // batch s, slot u: column s*step + u*g + jg of this lane's group; the batch
     // count is wave-uniform, columns past the panel load nothing and add 0
     const int step = kLoads * g;
-    const int nb = (nc + step - 1) / step;
-    auto load_batch = [&](int s, T (&a)[kLoads], A (&b)[kLoads]) {
-#pragma unroll
-      for (int u = 0; u < kLoads; ++u) {
-        const int j = s * step + u * g + jg;
-        if (live && j < nc) {
-          a[u] = load_wide<S, kNT>(base + (int64_t)j * ld);
-          b[u] = (A)vec_entry<kCoh>(pd, x, t, tp, j);
-        } else {
-          a[u] = T{};
-          b[u] = 0;
-        }
-      }
-    };
     A accp[V];
 #pragma unroll
     for (int v = 0; v < V; ++v) accp[v] = 0;
@@ -347,22 +348,54 @@ __device__ __forceinline__ void grouped_tile(const TileDesc &td, int shift, int 
     };
     // (no software pipelining here: a second batch of registers costs more
     // occupancy than it gains, measured; the wide path pipelines)
-    for (int sb = 0; sb < nb; ++sb) {
-      T a[kLoads];
-      A b[kLoads];
-      load_batch(sb, a, b);
-      consume(a, b);
+    {
+      // columns [c0, c0 + cn) per chunk; the chunk is the largest multiple
+      // of step not above kStage (step <= 8 * 64 = kStage), so every batch
+      // finds its entries in the staged chunk
+      const int chunk = (kStage / step) * step;
+      for (int c0 = 0; c0 < nc; c0 += chunk) {
+        const int cn = nc - c0 < chunk ? nc - c0 : chunk;
+        wave_lds_sync();  // the previous chunk's LDS reads are done
+        for (int q0 = 0; q0 < kStage / 64 && q0 * 64 < cn; q0 += 4) {
+          double e[4];
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            const int k = (q0 + q) * 64 + lane;
+            e[q] = k < cn ? vec_entry<kCoh>(pd, x, t, tp, c0 + k) : 0.0;
+          }
+#pragma unroll
+          for (int q = 0; q < 4; ++q) lds[(q0 + q) * 64 + lane] = e[q];
+        }
+        wave_lds_sync();
+        const int nb = (cn + step - 1) / step;
+        for (int sb = 0; sb < nb; ++sb) {
+          T a[kLoads];
+          A b[kLoads];
+#pragma unroll
+          for (int u = 0; u < kLoads; ++u) {
+            const int j = sb * step + u * g + jg;
+            if (live && j < cn) {
+              a[u] = load_wide<S, kNT>(base + (int64_t)(c0 + j) * ld);
+              b[u] = (A)lds[j];
+            } else {
+              a[u] = T{};
+              b[u] = 0;
+            }
+          }
+          consume(a, b);
+        }
+      }
     }
     const A sc = kScaled ? (A)panel_scale(pd, scales) : (A)1;
 #pragma unroll
     for (int v = 0; v < V; ++v) acc[v] += kScaled ? accp[v] * sc : accp[v];
   }
-  // The g partial sums of every row meet in LDS (a region per wave): lane
+  // The g partial sums of every row meet in the wave's LDS region: lane
   // (jg, c) stores its V sums at jg*lpc*V + c*V + v, then lane o sums output
   // row o = c*V + v over the groups in group order, so the stores to the slot
   // are contiguous across lanes.
-  __shared__ double red[4 * 64 * V];  // <= 4 waves per workgroup (launch bound 256)
-  double *r = red + (threadIdx.x >> 6) * (64 * V);
+  double *r = lds;
+  wave_lds_sync();  // staged entries read before they are overwritten
 #pragma unroll
   for (int v = 0; v < V; ++v) r[lane * V + v] = (double)acc[v];
   wave_lds_sync();
@@ -378,12 +411,13 @@ template <typename S, bool kScaled, int kLoads, bool kPipe, bool kNT, bool kCoh,
 __device__ __forceinline__ void run_tile(const TileDesc &td, const PanelDesc *__restrict__ panels,
                                          const S *__restrict__ vals, const double *__restrict__ x,
                                          const double *__restrict__ t, const double *tp,
-                                         const float *__restrict__ scales, double *out, int lane) {
+                                         const float *__restrict__ scales, double *out, int lane,
+                                         double *lds) {
   constexpr int V = Wide<S>::V;
   const int shift = (int)(panels[td.panel0].val_off % V);
   const int lpc = (td.nrows + shift + V - 1) / V;  // lanes per column
   if (lpc <= 32) {
-    grouped_tile<S, kLoads, kNT, kScaled, kCoh>(td, shift, lpc, panels, vals, x, t, tp, scales, out, lane);
+    grouped_tile<S, kLoads, kNT, kScaled, kCoh>(td, shift, lpc, panels, vals, x, t, tp, scales, out, lane, lds);
   } else if (lpc <= 64) {
     wide_tile<S, 1, kLoads, kPipe, kNT, kScaled, kCoh, kBV>(td, shift, lpc, panels, vals, x, t, tp, scales, out, lane);
   } else if (lpc <= 128) {
@@ -407,8 +441,10 @@ __global__ __launch_bounds__(256) void tile_kernel(const TileDesc *__restrict__ 
   const int lane = (int)(threadIdx.x & 63);
   const int tid = (int)blockIdx.x * (int)(blockDim.x >> 6) + wave;
   if (tid >= ntiles) return;
+  extern __shared__ double tile_lds[];
   const TileDesc td = tiles[tid];
-  run_tile<S, kScaled, kLoads, kPipe, kNT, false, kBV>(td, panels, vals, x, t, tp, scales, out, lane);
+  run_tile<S, kScaled, kLoads, kPipe, kNT, false, kBV>(td, panels, vals, x, t, tp, scales, out, lane,
+                                                       tile_lds + wave * lds_wave_doubles<S>());
 }
 
 // tile_kernel with lane 0 of every wave recording the device wall clock
@@ -430,8 +466,10 @@ __global__ __launch_bounds__(256) void tile_kernel_timed(const TileDesc *__restr
   const int tid = (int)blockIdx.x * (int)(blockDim.x >> 6) + wave;
   if (tid >= ntiles) return;
   const long long t0 = (long long)wall_clock64();
+  extern __shared__ double tile_lds[];
   const TileDesc td = tiles[tid];
-  run_tile<S, kScaled, kLoads, kPipe, kNT, false, kBV>(td, panels, vals, x, t, tp, scales, out, lane);
+  run_tile<S, kScaled, kLoads, kPipe, kNT, false, kBV>(td, panels, vals, x, t, tp, scales, out, lane,
+                                                       tile_lds + wave * lds_wave_doubles<S>());
   const long long t1 = (long long)wall_clock64();
   if (lane == 0) {
     stamps[2 * (int64_t)tid] = t0;
@@ -495,13 +533,15 @@ __global__ __launch_bounds__(256) void persistent_kernel(
     const int32_t *__restrict__ yseg, const int32_t *__restrict__ yptr, const int64_t *__restrict__ ylist,
     int nd, double *__restrict__ y, int accumulate, PersistCounters *ctr) {
   const int lane = (int)(threadIdx.x & 63);
+  extern __shared__ double tile_lds[];
+  double *lds = tile_lds + (threadIdx.x >> 6) * lds_wave_doubles<S>();
   bool s1_seen = false, all_seen = false;
   for (;;) {
     const int item = wave_take(&ctr->head, lane);
     if (item >= nitems) break;
     if (item < na) {
       const TileDesc td = tiles_a[item];
-      run_tile<S, kScaled, kLoads, kPipe, kNT, true, kBV>(td, panels, vals, x, t, slots, scales, slots, lane);
+      run_tile<S, kScaled, kLoads, kPipe, kNT, true, kBV>(td, panels, vals, x, t, slots, scales, slots, lane, lds);
       if (item < n1) wave_signal(&ctr->s1_done, lane);
       wave_signal(&ctr->tiles_done, lane);
     } else if (item < nt) {
@@ -510,7 +550,7 @@ __global__ __launch_bounds__(256) void persistent_kernel(
         s1_seen = true;
       }
       const TileDesc td = tiles_b[item - na];
-      run_tile<S, kScaled, kLoads, kPipe, kNT, true, kBV>(td, panels, vals, x, t, slots, scales, slots, lane);
+      run_tile<S, kScaled, kLoads, kPipe, kNT, true, kBV>(td, panels, vals, x, t, slots, scales, slots, lane, lds);
       wave_signal(&ctr->tiles_done, lane);
     } else {
       if (!all_seen) {
@@ -541,22 +581,27 @@ __global__ __launch_bounds__(256) void persistent_kernel(
 }
 
 // ---- several right-hand sides: MFMA tiles --------------------------------
-// Y (nd x nrhs, row-major) += A X.  A tile's output is nrows x nrhs; the wave
-// walks the tile in chunks of 64 rows (four 16-row MFMA blocks) and the
-// right-hand sides in 16-wide blocks.  Per panel and per K step the wave
-// loads one A operand per row block (16 rows x K columns of the column-major
-// panel) and one B operand per rhs block (K entries of the panel's vector for
-// 16 right-hand sides) and issues one MFMA per (row block, rhs block):
+// Y (nd x nrhs, row-major) += A X.  A tile's output is nrows x nrhs; the
+// right-hand sides go in 16-wide blocks (NB = nrhs / 16):
 //   FP64 storage: v_mfma_f64_16x16x4_f64   (K = 4, FP64 accumulate)
 //   FP32 storage: v_mfma_f32_16x16x4_f32   (K = 4, FP32 accumulate)
 //   BF16 storage: v_mfma_f32_16x16x16_bf16 (K = 16, vector entries rounded to
 //                 BF16 after the factor scale is applied, FP32 accumulate)
-// Operand layouts (wave64): A lane l holds rows l%16 of K-slice l/16, B lane l
-// holds rhs l%16 of K-slice l/16; the 16x16 result gives lane l four rows of
-// rhs l%16: 4*(l/16)+v for the FP32 and BF16 instructions, l/16+4v for the
-// FP64 one (Mfma<S>::row).  Every address is formed from (lane, v) with the
-// row and column bounds applied, so a layout mismatch could only produce wrong
-// values, never an access outside the panel.
+// Operand layouts (wave64): A lane l holds row l%16 of K-slice l/16 (one
+// column for K = 4, four consecutive columns for K = 16), B lane l holds rhs
+// l%16 of the same K-slice; the 16x16 result gives lane l four rows of rhs
+// l%16: 4*(l/16)+v for the FP32 and BF16 instructions, l/16+4v for the FP64
+// one (Mfma<S>::row).
+// Matrix loads are the 16-byte lane loads of the one-vector path (V = 2, 4
+// or 8 consecutive rows of one column, aligned as there): lane l loads rows
+// V*(l%16) .. V*(l%16)+V-1 of a chunk of 16V rows, so one load feeds V MFMAs,
+// MFMA m taking row V*i+m of the chunk as its row i.  The wave walks the tile
+// in passes of CH chunks, and per pass the panels' columns in batches of KB
+// K-steps: all loads of a batch (matrix, and the vector entries shared by the
+// CH chunks) are issued before its MFMAs.  Loads are bounded like the
+// one-vector path's (lane groups < lanes per column, columns < ncols), so no
+// access leaves the panel; rows outside the tile only feed result rows that
+// are not stored.
 typedef double mf_d4 __attribute__((ext_vector_type(4)));
 typedef float mf_f4 __attribute__((ext_vector_type(4)));
 typedef short mf_s4 __attribute__((ext_vector_type(4)));
@@ -614,48 +659,153 @@ __device__ __forceinline__ double vec_entry_multi(const PanelDesc &pd, const dou
   return s;
 }
 
-// A operand of row block rb at K step k0 (zero outside the panel).
-template <typename S>
-__device__ __forceinline__ typename Mfma<S>::Op a_operand(const S *__restrict__ vals, const PanelDesc &pd, int row,
-                                                          int nrows, int k0, int lane) {
-  if constexpr (sizeof(S) == 2) {
-    mf_s4 a;
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const int col = k0 + 4 * (lane >> 4) + q;
-      const bool ok = row < nrows && col < pd.ncols;
-      const int64_t off = pd.val_off + (ok ? row + (int64_t)col * pd.ld : 0);
-      a[q] = ok ? (short)vals[off] : (short)0;
-    }
-    return a;
-  } else {
-    const int col = k0 + (lane >> 4);
-    const bool ok = row < nrows && col < pd.ncols;
-    const int64_t off = pd.val_off + (ok ? row + (int64_t)col * pd.ld : 0);
-    const S v = vals[off];
-    return ok ? v : (S)0;
-  }
-}
-
-// B operand of rhs block cb at K step k0, times the panel scale.
+// Vector operand of rhs block cb at K-step k0 (times the panel scale): FP64
+// and FP32 one entry (column k0 + l/16), BF16 four (columns k0 + 4(l/16) + q).
 template <typename S>
 __device__ __forceinline__ typename Mfma<S>::Op b_operand(const PanelDesc &pd, const double *__restrict__ X,
                                                           const double *__restrict__ T, const double *__restrict__ TP,
                                                           int rhs, int nrhs, int k0, int lane, float scale) {
   if constexpr (sizeof(S) == 2) {
-    mf_s4 b;
+    double v[4];
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
       const int col = k0 + 4 * (lane >> 4) + q;
-      const double v = col < pd.ncols ? vec_entry_multi(pd, X, T, TP, col, rhs, nrhs) : 0.0;
-      b[q] = bf16_bits((float)v * scale);
+      v[q] = col < pd.ncols ? vec_entry_multi(pd, X, T, TP, col, rhs, nrhs) : 0.0;
     }
+    mf_s4 b;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) b[q] = bf16_bits((float)v[q] * scale);
     return b;
   } else {
     const int col = k0 + (lane >> 4);
     const double v = col < pd.ncols ? vec_entry_multi(pd, X, T, TP, col, rhs, nrhs) : 0.0;
     return (S)v;
   }
+}
+
+// Matrix words of one K-step for one chunk: FP64/FP32 one 16-byte word
+// (column k0 + l/16), BF16 four (columns k0 + 4(l/16) + q); zero where the
+// lane group or the column is outside the tile.
+template <typename S>
+struct MultiWords {
+  static constexpr int NW = sizeof(S) == 2 ? 4 : 1;
+  typename Wide<S>::T w[NW];
+};
+template <typename S>
+__device__ __forceinline__ void load_words(MultiWords<S> &mw, const S *__restrict__ base, const PanelDesc &pd,
+                                           bool lane_live, int k0, int lane) {
+  constexpr int NW = MultiWords<S>::NW;
+#pragma unroll
+  for (int q = 0; q < NW; ++q) {
+    const int col = NW == 1 ? k0 + (lane >> 4) : k0 + 4 * (lane >> 4) + q;
+    if (lane_live && col < pd.ncols)
+      mw.w[q] = load_wide<S, false>(base + (int64_t)col * pd.ld);
+    else
+      mw.w[q] = typename Wide<S>::T{};
+  }
+}
+// A operand of MFMA m (row V*i+m of the chunk as row i) from a chunk's words.
+template <typename S>
+__device__ __forceinline__ typename Mfma<S>::Op a_of(const MultiWords<S> &mw, int m) {
+  if constexpr (sizeof(S) == 2) {
+    mf_s4 a;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const unsigned int u = mw.w[q][m >> 1];
+      a[q] = (short)(unsigned short)((m & 1) ? (u >> 16) : (u & 0xFFFFu));
+    }
+    return a;
+  } else {
+    return mw.w[0][m];
+  }
+}
+
+template <typename S, int NB>
+struct MultiShape {
+  static constexpr int V = Wide<S>::V;
+  static constexpr int ACCW = sizeof(typename Mfma<S>::Acc) / 4;  // VGPRs per accumulator
+  static constexpr int CH = V * NB * ACCW <= 32 ? 2 : 1;          // chunks per pass
+  static constexpr int KB = sizeof(S) == 2 ? 2 : 4;                // K-steps per batch
+};
+
+// A operand of one K-step for a short tile: row `row` of the tile as MFMA
+// row l%16 (FP64/FP32 one element of column k0 + l/16, BF16 four of columns
+// k0 + 4(l/16) + q); zero outside the tile.
+template <typename S>
+__device__ __forceinline__ typename Mfma<S>::Op a_narrow(const S *__restrict__ vals, const PanelDesc &pd, int row,
+                                                         int nrows, int k0, int lane) {
+  if constexpr (sizeof(S) == 2) {
+    mf_s4 a;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int col = k0 + 4 * (lane >> 4) + q;
+      a[q] = row < nrows && col < pd.ncols ? (short)vals[pd.val_off + row + (int64_t)col * pd.ld] : (short)0;
+    }
+    return a;
+  } else {
+    const int col = k0 + (lane >> 4);
+    return row < nrows && col < pd.ncols ? vals[pd.val_off + row + (int64_t)col * pd.ld] : (S)0;
+  }
+}
+
+// Short FP32 tiles (at most 32 rows): MFMA row i is tile row 16*ch + i, so a
+// tile of r rows runs ceil(r/16) MFMAs per K-step and right-hand-side block
+// instead of 4, with 4-byte element loads.  Measured at N=101,250 with 32
+// right-hand sides: 7% faster for FP32; slower for FP64 (8-byte loads, only
+// two MFMAs saved) and BF16 (four strided 2-byte loads per lane), which keep
+// the 16-byte path.
+template <typename S, bool kScaled, int NB>
+__device__ __forceinline__ void multi_tile_narrow(const TileDesc &td, const PanelDesc *__restrict__ panels,
+                                                  const S *__restrict__ vals, const double *__restrict__ X,
+                                                  const double *__restrict__ T, const double *__restrict__ TP,
+                                                  const float *__restrict__ scales, double *__restrict__ out,
+                                                  int lane) {
+  using M = Mfma<S>;
+  using Acc = typename M::Acc;
+  constexpr int V = Wide<S>::V, CH = V / 2 > 0 ? V / 2 : 1, K = M::K;
+  constexpr int KB = MultiShape<S, NB>::KB;
+  constexpr int nrhs = 16 * NB;
+  const int nlive = (td.nrows + 15) / 16 < CH ? (td.nrows + 15) / 16 : CH;
+  Acc acc[CH][NB];
+#pragma unroll
+  for (int ch = 0; ch < CH; ++ch)
+#pragma unroll
+    for (int cb = 0; cb < NB; ++cb) acc[ch][cb] = Acc{0, 0, 0, 0};
+  for (int p = 0; p < td.npanel; ++p) {
+    const PanelDesc pd = panels[td.panel0 + p];
+    float scale = 1.0f;
+    if constexpr (kScaled) scale = panel_scale(pd, scales);
+    for (int k0 = 0; k0 < pd.ncols; k0 += KB * K) {
+      typename M::Op a[KB][CH], b[KB][NB];
+#pragma unroll
+      for (int kb = 0; kb < KB; ++kb) {
+#pragma unroll
+        for (int ch = 0; ch < CH; ++ch)
+          if (ch < nlive) a[kb][ch] = a_narrow<S>(vals, pd, 16 * ch + (lane & 15), td.nrows, k0 + kb * K, lane);
+#pragma unroll
+        for (int cb = 0; cb < NB; ++cb)
+          b[kb][cb] = b_operand<S>(pd, X, T, TP, 16 * cb + (lane & 15), nrhs, k0 + kb * K, lane, scale);
+      }
+#pragma unroll
+      for (int kb = 0; kb < KB; ++kb)
+#pragma unroll
+        for (int ch = 0; ch < CH; ++ch)
+          if (ch < nlive)
+#pragma unroll
+            for (int cb = 0; cb < NB; ++cb) acc[ch][cb] = mfma_step(a[kb][ch], b[kb][cb], acc[ch][cb]);
+    }
+  }
+#pragma unroll
+  for (int ch = 0; ch < CH; ++ch)
+    if (ch < nlive)
+#pragma unroll
+      for (int cb = 0; cb < NB; ++cb)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int row = 16 * ch + M::row(lane, v);
+          if (row < td.nrows)
+            out[(int64_t)(td.out_off + row) * nrhs + 16 * cb + (lane & 15)] = (double)acc[ch][cb][v];
+        }
 }
 
 // nrhs = 16*NB; one wave per tile.
@@ -669,48 +819,83 @@ __global__ __launch_bounds__(256) void tile_multi_kernel(const TileDesc *__restr
                                                          double *__restrict__ out) {
   using M = Mfma<S>;
   using Acc = typename M::Acc;
+  using Sh = MultiShape<S, NB>;
+  constexpr int V = Sh::V, CH = Sh::CH, KB = Sh::KB, K = M::K;
   constexpr int nrhs = 16 * NB;
   const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
   const int lane = (int)(threadIdx.x & 63);
   const int tid = (int)blockIdx.x * (int)(blockDim.x >> 6) + wave;
   if (tid >= ntiles) return;
   const TileDesc td = tiles[tid];
-  for (int rc = 0; rc < td.nrows; rc += 64) {
-    const int nrb = (td.nrows - rc + 15) >> 4 < 4 ? (td.nrows - rc + 15) >> 4 : 4;  // live row blocks
-    Acc acc[4][NB];
+  if constexpr (sizeof(S) == 4) {
+    if (td.nrows <= 8 * V) {
+      multi_tile_narrow<S, kScaled, NB>(td, panels, vals, X, T, TP, scales, out, lane);
+      return;
+    }
+  }
+  const int shift = (int)(panels[td.panel0].val_off % V);
+  const int lpc = (td.nrows + shift + V - 1) / V;  // lane groups (V rows each) of the tile
+  for (int g0 = 0; g0 < lpc; g0 += 16 * CH) {      // pass: CH chunks of 16 lane groups
+    Acc acc[CH][V][NB];
 #pragma unroll
-    for (int rb = 0; rb < 4; ++rb)
+    for (int ch = 0; ch < CH; ++ch)
 #pragma unroll
-      for (int cb = 0; cb < NB; ++cb) acc[rb][cb] = Acc{0, 0, 0, 0};
+      for (int m = 0; m < V; ++m)
+#pragma unroll
+        for (int cb = 0; cb < NB; ++cb) acc[ch][m][cb] = Acc{0, 0, 0, 0};
+    // chunks of this pass that hold rows of the tile (wave-uniform): a short
+    // tile runs no MFMAs for chunks past its last row
+    const int nlive = (lpc - g0 + 15) / 16 < CH ? (lpc - g0 + 15) / 16 : CH;
+    bool live[CH];
+#pragma unroll
+    for (int ch = 0; ch < CH; ++ch) live[ch] = g0 + 16 * ch + (lane & 15) < lpc;
     for (int p = 0; p < td.npanel; ++p) {
       const PanelDesc pd = panels[td.panel0 + p];
       float scale = 1.0f;
       if constexpr (kScaled) scale = panel_scale(pd, scales);
-      for (int k0 = 0; k0 < pd.ncols; k0 += M::K) {
-        typename M::Op a[4], b[NB];
+      const S *__restrict__ base[CH];
 #pragma unroll
-        for (int rb = 0; rb < 4; ++rb)
-          if (rb < nrb) a[rb] = a_operand<S>(vals, pd, rc + 16 * rb + (lane & 15), td.nrows, k0, lane);
+      for (int ch = 0; ch < CH; ++ch)
+        base[ch] = vals + (pd.val_off - shift) + (int64_t)V * (live[ch] ? g0 + 16 * ch + (lane & 15) : 0);
+      for (int k0 = 0; k0 < pd.ncols; k0 += KB * K) {
+        MultiWords<S> a[KB][CH];
+        typename M::Op b[KB][NB];
 #pragma unroll
-        for (int cb = 0; cb < NB; ++cb) b[cb] = b_operand<S>(pd, X, T, TP, 16 * cb + (lane & 15), nrhs, k0, lane, scale);
+        for (int kb = 0; kb < KB; ++kb) {
 #pragma unroll
-        for (int rb = 0; rb < 4; ++rb)
-          if (rb < nrb)
+          for (int ch = 0; ch < CH; ++ch)
+            if (ch < nlive) load_words<S>(a[kb][ch], base[ch], pd, live[ch], k0 + kb * K, lane);
 #pragma unroll
-            for (int cb = 0; cb < NB; ++cb) acc[rb][cb] = mfma_step(a[rb], b[cb], acc[rb][cb]);
+          for (int cb = 0; cb < NB; ++cb)
+            b[kb][cb] = b_operand<S>(pd, X, T, TP, 16 * cb + (lane & 15), nrhs, k0 + kb * K, lane, scale);
+        }
+#pragma unroll
+        for (int kb = 0; kb < KB; ++kb)
+#pragma unroll
+          for (int ch = 0; ch < CH; ++ch)
+            if (ch < nlive)
+#pragma unroll
+            for (int m = 0; m < V; ++m) {
+              const typename M::Op am = a_of<S>(a[kb][ch], m);
+#pragma unroll
+              for (int cb = 0; cb < NB; ++cb) acc[ch][m][cb] = mfma_step(am, b[kb][cb], acc[ch][m][cb]);
+            }
       }
     }
 #pragma unroll
-    for (int rb = 0; rb < 4; ++rb) {
-      if (rb >= nrb) continue;
+    for (int ch = 0; ch < CH; ++ch)
+      if (ch < nlive)
 #pragma unroll
-      for (int cb = 0; cb < NB; ++cb)
+      for (int m = 0; m < V; ++m)
 #pragma unroll
-        for (int v = 0; v < 4; ++v) {
-          const int row = rc + 16 * rb + M::row(lane, v);
-          if (row < td.nrows) out[(int64_t)(td.out_off + row) * nrhs + 16 * cb + (lane & 15)] = (double)acc[rb][cb][v];
-        }
-    }
+        for (int cb = 0; cb < NB; ++cb)
+#pragma unroll
+          for (int v = 0; v < 4; ++v) {
+            // result row i of MFMA m: row V*i+m of the chunk, minus the shift
+            const int row = V * (16 * ch + g0 + M::row(lane, v)) + m - shift;
+            if (row >= 0 && row < td.nrows)
+              out[(int64_t)(td.out_off + row) * nrhs + 16 * cb + (lane & 15)] = (double)acc[ch][m][cb][v];
+          }
   }
 }
 
@@ -764,10 +949,11 @@ __global__ __launch_bounds__(256) void probe_copy_kernel(const double4 *__restri
 }
 
 // out[i] = (accumulate ? out[i] : 0) + scale(i) * sum_e buf[list[e] + i],
+// e over [rng[i].x, rng[i].y) (entry i's segment, resolved per entry on the
+// host so that the chain of dependent loads is range -> list -> slots),
 // scale(i) = product of the two factor scales of entry i's leaf (t sum only)
 template <bool kScaled>
-__global__ __launch_bounds__(256) void segsum_kernel(int n, const int32_t *__restrict__ seg,
-                                                     const int32_t *__restrict__ ptr,
+__global__ __launch_bounds__(256) void segsum_kernel(int n, const int2 *__restrict__ rng,
                                                      const int64_t *__restrict__ list,
                                                      const double *__restrict__ buf,
                                                      const int32_t *__restrict__ factor_of,
@@ -775,10 +961,10 @@ __global__ __launch_bounds__(256) void segsum_kernel(int n, const int32_t *__res
                                                      double *__restrict__ out, int accumulate) {
   const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
   if (i >= n) return;
-  const int s = seg[i];
-  const int e1 = ptr[s + 1];
+  const int2 r = rng[i];
+  const int e1 = r.y;
   double a = 0;
-  int e = ptr[s];
+  int e = r.x;
   // eight independent slot loads in flight, summed in list order
   for (; e + 8 <= e1; e += 8) {
     int64_t l[8];

@@ -11,6 +11,7 @@
 #include <memory>
 #include <mutex>
 #include <string>
+#include <tuple>
 #include <vector>
 
 #include "hacapk.h"
@@ -77,6 +78,13 @@ int upload(T **d, const std::vector<T> &h) {
   return h2d(*d, h.data(), h.size());
 }
 
+// Per-entry [begin, end) list ranges of a segment sum.
+std::vector<int2> seg_ranges(const SegSum &g) {
+  std::vector<int2> r(g.seg.size());
+  for (size_t i = 0; i < g.seg.size(); i++) r[i] = make_int2(g.ptr[g.seg[i]], g.ptr[g.seg[i] + 1]);
+  return r;
+}
+
 int elem_bytes(int storage) { return storage == kF64 ? 8 : storage == kF32 ? 4 : 2; }
 
 }  // namespace
@@ -109,6 +117,7 @@ struct hmvm_plan {
   int64_t *d_tlist = nullptr;
   int32_t *d_yseg = nullptr, *d_yptr = nullptr;
   int64_t *d_ylist = nullptr;
+  int2 *d_trng = nullptr, *d_yrng = nullptr;  // per-entry list ranges of the segment sums
   float *d_scales = nullptr;
   double *d_x = nullptr, *d_y = nullptr;  // staging for host-pointer calls
   size_t device_bytes = 0;
@@ -116,6 +125,7 @@ struct hmvm_plan {
   ~hmvm_plan() {
     void *ptrs[] = {d_vals,  d_tiles_a, d_tiles_b, d_panels, d_t,      d_slots, d_tseg, d_tptr,
                     d_tfac,  d_tlist,   d_yseg,    d_yptr,   d_ylist,  d_scales, d_x,    d_y,
+                    d_trng,  d_yrng,
                     d_ctr,   d_mslots,  d_mt,      d_mX,     d_mY};
     for (void *p : ptrs)
       if (p) (void)hipFree(p);
@@ -355,9 +365,9 @@ struct LaunchDefaults {
 };
 const LaunchDefaults &launch_defaults(int storage) {
   static const LaunchDefaults d[3] = {
-      {32768, 16384, 6, 64, 6, 256, 256},   // FP64
-      {16384, 16384, 4, 256, 4, 256, 256},  // FP32
-      {32768, 16384, 6, 256, 6, 256, 512},  // BF16
+      {32768, 16384, 6, 256, 6, 256, 256},  // FP64
+      {32768, 16384, 6, 256, 6, 256, 256},  // FP32
+      {65536, 32768, 6, 64, 6, 64, 512},    // BF16
   };
   return d[storage >= kF64 && storage <= kBF16 ? storage : 0];
 }
@@ -653,7 +663,8 @@ int create_plan(Source &src, const hmvm_options *opt, hmvm_plan **out) {
       (rc = upload(&p->d_panels, P.panels)) || (rc = upload(&p->d_tseg, P.tsum.seg)) ||
       (rc = upload(&p->d_tptr, P.tsum.ptr)) || (rc = upload(&p->d_tlist, P.tsum.list)) ||
       (rc = upload(&p->d_yseg, P.ysum.seg)) || (rc = upload(&p->d_yptr, P.ysum.ptr)) ||
-      (rc = upload(&p->d_ylist, P.ysum.list)))
+      (rc = upload(&p->d_ylist, P.ysum.list)) || (rc = upload(&p->d_trng, seg_ranges(P.tsum))) ||
+      (rc = upload(&p->d_yrng, seg_ranges(P.ysum))))
     return rc;
   if (p->storage == kBF16) {
     if ((rc = upload(&p->d_tfac, P.factor_of_trow))) return rc;
@@ -675,7 +686,7 @@ int create_plan(Source &src, const hmvm_options *opt, hmvm_plan **out) {
   p->device_bytes = eb * (size_t)P.nvals + sizeof(TileDesc) * (P.tiles_a.size() + P.tiles_b.size()) +
                     sizeof(PanelDesc) * P.panels.size() + 8 * (size_t)(P.n_t + P.n_tp + P.n_z + 2 * P.nd) +
                     4 * (P.tsum.seg.size() + P.tsum.ptr.size() + P.ysum.seg.size() + P.ysum.ptr.size()) +
-                    8 * (P.tsum.list.size() + P.ysum.list.size());
+                    8 * (P.tsum.list.size() + P.ysum.list.size() + P.tsum.seg.size() + P.ysum.seg.size());
   if (verbose())
     std::fprintf(stderr,
                  "hmvm_hip: plan nd=%d nlf=%d len=%lld storage=%d tiles=%zu+%zu panels=%zu t=%lld tslots=%lld "
@@ -692,8 +703,9 @@ void launch_tiles_v(hmvm_plan *p, dim3 grid, dim3 block, hipStream_t st, const T
   constexpr int L = 8;
   constexpr bool kBV = (kVar & 1) != 0;
   constexpr bool kPipe = (kVar & 2) != 0, kNT = (kVar & 4) != 0;
-  hipLaunchKernelGGL((tile_kernel<S, kScaled, L, kPipe, kNT, kBV>), grid, block, 0, st, d_tiles, ntiles, p->d_panels,
-                     (const S *)p->d_vals, x, t, tp, (const float *)p->d_scales, out);
+  const size_t lds = sizeof(double) * (block.x / 64) * lds_wave_doubles<S>();
+  hipLaunchKernelGGL((tile_kernel<S, kScaled, L, kPipe, kNT, kBV>), grid, block, lds, st, d_tiles, ntiles,
+                     p->d_panels, (const S *)p->d_vals, x, t, tp, (const float *)p->d_scales, out);
 }
 
 template <typename S, bool kScaled>
@@ -758,8 +770,9 @@ void launch_timed_v(hmvm_plan *p, int bt, const TileDesc *d_tiles, int ntiles, c
   constexpr bool kBV = (kVar & 1) != 0;
   constexpr bool kPipe = (kVar & 2) != 0, kNT = (kVar & 4) != 0;
   const int wpb = bt / 64;
+  const size_t lds = sizeof(double) * wpb * lds_wave_doubles<S>();
   hipLaunchKernelGGL((tile_kernel_timed<S, kScaled, L, kPipe, kNT, kBV>), dim3((ntiles + wpb - 1) / wpb),
-                     dim3(bt), 0, st, d_tiles, ntiles, p->d_panels, (const S *)p->d_vals, x, t, tp,
+                     dim3(bt), lds, st, d_tiles, ntiles, p->d_panels, (const S *)p->d_vals, x, t, tp,
                      (const float *)p->d_scales, out, stamps);
 }
 
@@ -812,16 +825,14 @@ void launch_phase(hmvm_plan *p, int phase, const double *x, double *y, int accum
   } else if (phase == 1) {
     if (P.t_fused || P.tsum.n == 0) return;
     if (bf)
-      hipLaunchKernelGGL(segsum_kernel<true>, dim3((P.tsum.n + 255) / 256), dim3(256), 0, st, P.tsum.n, p->d_tseg,
-                         p->d_tptr, p->d_tlist, p->d_slots, p->d_tfac, p->d_scales, p->d_t, 0);
+      hipLaunchKernelGGL(segsum_kernel<true>, dim3((P.tsum.n + 255) / 256), dim3(256), 0, st, P.tsum.n, p->d_trng,
+                         p->d_tlist, p->d_slots, p->d_tfac, p->d_scales, p->d_t, 0);
     else
-      hipLaunchKernelGGL(segsum_kernel<false>, dim3((P.tsum.n + 255) / 256), dim3(256), 0, st, P.tsum.n, p->d_tseg,
-                         p->d_tptr, p->d_tlist, p->d_slots, (const int32_t *)nullptr, (const float *)nullptr,
-                         p->d_t, 0);
+      hipLaunchKernelGGL(segsum_kernel<false>, dim3((P.tsum.n + 255) / 256), dim3(256), 0, st, P.tsum.n, p->d_trng,
+                         p->d_tlist, p->d_slots, (const int32_t *)nullptr, (const float *)nullptr, p->d_t, 0);
   } else {
-    hipLaunchKernelGGL(segsum_kernel<false>, dim3((P.ysum.n + 255) / 256), dim3(256), 0, st, P.ysum.n, p->d_yseg,
-                       p->d_yptr, p->d_ylist, p->d_slots, (const int32_t *)nullptr, (const float *)nullptr, y,
-                       accumulate);
+    hipLaunchKernelGGL(segsum_kernel<false>, dim3((P.ysum.n + 255) / 256), dim3(256), 0, st, P.ysum.n, p->d_yrng,
+                       p->d_ylist, p->d_slots, (const int32_t *)nullptr, (const float *)nullptr, y, accumulate);
   }
 }
 
@@ -908,9 +919,11 @@ int launch_persistent_v(hmvm_plan *p, const double *x, double *y, int accumulate
   const Plan &P = p->plan;
   auto kfn = persistent_kernel<S, kScaled, L, kPipe, kNT, kBV>;
   const int key = kVar * 1024 + p->block_threads;
+  const size_t lds = sizeof(double) * (p->block_threads / 64) * lds_wave_doubles<S>();
   if (p->persist_key != key) {  // resident workgroups per CU, queried once per launch shape
     int per_cu = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kfn, p->block_threads, 0) != hipSuccess || per_cu < 1)
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kfn, p->block_threads, lds) != hipSuccess ||
+        per_cu < 1)
       per_cu = 1;
     p->persist_blocks = per_cu * std::max(p->ncu, 1);
     p->persist_key = key;
@@ -921,7 +934,7 @@ int launch_persistent_v(hmvm_plan *p, const double *x, double *y, int accumulate
   const int na = (int)P.tiles_a.size();
   const int nt = na + (int)P.tiles_b.size();
   const int nitems = nt + (P.nd + 63) / 64;
-  hipLaunchKernelGGL(kfn, dim3(blocks), dim3(p->block_threads), 0, st, p->d_tiles_a, ns1, na, p->d_tiles_b, nt,
+  hipLaunchKernelGGL(kfn, dim3(blocks), dim3(p->block_threads), lds, st, p->d_tiles_a, ns1, na, p->d_tiles_b, nt,
                      nitems, nwaves, p->d_panels, (const S *)p->d_vals, x, p->d_t, p->d_slots,
                      (const float *)p->d_scales, p->d_yseg, p->d_yptr, p->d_ylist, P.nd, y, accumulate, p->d_ctr);
   return HMVM_OK;
@@ -1781,15 +1794,36 @@ int hmvm_tile_occupancy(int storage, int kernel, int block_threads, int *waves) 
   if (!waves || storage < kF64 || storage > kBF16 || kernel < 0 || kernel >= kNumKernelVariants ||
       block_threads % 64 || block_threads < 64 || block_threads > 256)
     return set_error(HMVM_ERR_ARG, "bad argument");
+  // cached per (device, storage, variant, workgroup size): the features of
+  // every tile of a launch need it
+  static std::mutex mu;
+  static std::map<std::tuple<int, int, int, int>, int> cache;
+  int dev0 = 0;
+  HIP_TRY(hipGetDevice(&dev0));
+  const auto key = std::make_tuple(dev0, storage, kernel, block_threads);
+  {
+    std::lock_guard<std::mutex> g(mu);
+    auto it = cache.find(key);
+    if (it != cache.end()) {
+      *waves = it->second;
+      return HMVM_OK;
+    }
+  }
   const void *fn = storage == kF64   ? tile_fn_v<double, false>(kernel)
                    : storage == kF32 ? tile_fn_v<float, false>(kernel)
                                      : tile_fn_v<uint16_t, true>(kernel);
+  const int wave_doubles = storage == kF64   ? lds_wave_doubles<double>()
+                           : storage == kF32 ? lds_wave_doubles<float>()
+                                             : lds_wave_doubles<uint16_t>();
+  const size_t lds = sizeof(double) * (block_threads / 64) * (size_t)wave_doubles;
   int per_cu = 0, dev = 0;
   HIP_TRY(hipGetDevice(&dev));
   hipDeviceProp_t prop;
   HIP_TRY(hipGetDeviceProperties(&prop, dev));
-  HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, block_threads, 0));
+  HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, block_threads, lds));
   *waves = per_cu * (block_threads / 64) * prop.multiProcessorCount;
+  std::lock_guard<std::mutex> g(mu);
+  cache[key] = *waves;
   return HMVM_OK;
 }
 
@@ -1799,6 +1833,13 @@ int hmvm_plan_tile_features(const hmvm_plan *plan, int which, int nrhs, float *f
   const std::vector<TileDesc> &tl = which == 0 ? P.tiles_a : P.tiles_b;
   const int V = P.lane_elems;
   const float lb = std::log2((float)elem_bytes(plan->storage));
+  const int kern = which ? plan->kernel_b : plan->kernel;
+  const int bt = which ? plan->block_threads_b : plan->block_threads;
+  int slots = 0;
+  if (int rc = hmvm_tile_occupancy(plan->storage, kern, bt, &slots)) return rc;
+  slots = std::max(slots, 1);
+  const float f_sat = std::log2((float)std::min<size_t>(tl.size(), (size_t)slots));
+  const float f_slots = std::log2((float)slots);
   for (size_t i = 0; i < tl.size(); i++) {
     const TileDesc &td = tl[i];
     // stage-1 panels carry no factor scale (factor -1); dense panels do
@@ -1821,15 +1862,13 @@ int hmvm_plan_tile_features(const hmvm_plan *plan, int which, int nrhs, float *f
     f[7] = std::log2(1.0f + (float)lpc * V * (float)cols);
     f[8] = std::log2(1.0f + (float)(cols ? nch / cols : 0.0));
     f[9] = lb;
-    const int kern = which ? plan->kernel_b : plan->kernel;
-    const int bt = which ? plan->block_threads_b : plan->block_threads;
     f[10] = (float)(kern & 1);
     f[11] = (float)((kern >> 1) & 1);
     f[12] = (float)((kern >> 2) & 1);
     f[13] = std::log2((float)(bt / 64));
     f[14] = std::log2((float)std::max(nrhs, 1));
-    f[15] = std::log2((float)tl.size());
-    f[16] = std::log2(1.0f + (float)(which ? P.elems_b : P.elems_a));
+    f[15] = f_sat;
+    f[16] = f_slots;
   }
   return HMVM_OK;
 }

@@ -94,9 +94,10 @@ typedef struct hmvm_options {
   int block_threads;     /* threads per workgroup, 64..256 step 64 (0: by       */
                          /* storage, per launch)                                */
   int kernel;            /* tile-kernel variant 0..7 (-1: by storage, per       */
-                         /* launch): bit0 vector entries read                   */
+                         /* launch): bit0 vector entries of wide tiles read     */
                          /* 64 columns at a time and broadcast (readlane),      */
-                         /* bit1 software pipelining, bit2 non-temporal loads   */
+                         /* bit1 software pipelining of wide tiles, bit2        */
+                         /* non-temporal matrix loads                           */
   int order;             /* tile order in a launch: 0 memory, 1 largest first   */
   int mode;              /* 0: launches on one stream; 1: one persistent launch */
                          /* with a device work queue; 2: launches, dense tiles  */
@@ -241,7 +242,10 @@ int hmvm_plan_time_graph(hmvm_plan *plan, int count, int warmup, int iters, hmvm
  *   [7] log2(1+elements read incl. lane padding)  [8] log2(1+mean partial
  *   slots summed per vector entry)  [9] log2 bytes per element
  *   [10..12] kernel variant bits  [13] log2(block_threads/64)  [14] log2 nrhs
- *   [15] log2 tiles in the launch  [16] log2(1+elements of the launch)
+ *   [15] log2 min(tiles in the launch, W)  [16] log2 W, where W is the
+ *   number of wavefronts of the launch's variant resident at once on the
+ *   device (hmvm_tile_occupancy): how saturated the launch is, bounded so
+ *   that no launch size lies outside what a model has seen
  * hmvm_plan_profile_tiles runs the launch with a clock-stamping copy of the
  * tile kernel (same results) and returns each tile's mean duration and start
  * offset in microseconds, plus the mean launch time.

@@ -51,7 +51,7 @@ def test_learned_model_beats_baseline_and_tracks_exhaustive(gpu, H, train_mats, 
     model = AT.CostModel(seed=5)
     model.fit(X, y, recs, epochs=60)
     Xt, yt, _ = AT.collect(test_mat, storage, iters=2, max_tiles=8000, seed=7)
-    err_mlp = np.mean(np.abs(model.mlp.predict(Xt) - yt))
+    err_mlp = np.mean(np.abs(model.predict_log2(Xt) - yt))
     err_lin = np.mean(np.abs(AT.linear_baseline(X, y)(Xt) - yt))
     print("held-out mean |log2 error|: mlp %.3f  linear %.3f" % (err_mlp, err_lin))
     assert err_mlp < err_lin

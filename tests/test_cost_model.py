@@ -78,3 +78,23 @@ def test_hip_training_fits(gpu, H):
     ref, _ = cost_mlp.forward(p, mu, sd, X, NIN, NH)
     assert np.max(np.abs(m.predict(X) - ref)) <= 1e-4 * max(1.0, np.max(np.abs(ref)))
     m.close()
+
+
+def test_features_clamped_to_training_range(H):
+    """Launch-level features beyond every training launch (a large batched
+    plan) are clamped to the training range before prediction."""
+    from hacapk_hmvm_amd import autotune as AT
+    m = AT.CostModel.__new__(AT.CostModel)  # the clamp needs no network (nor a GPU)
+    X = np.array([[0.0] * H.HMVM_NFEAT, [1.0] * H.HMVM_NFEAT], dtype=np.float32)
+    X[1, 15] = 14.0
+    m.lo, m.hi = X.min(axis=0), X.max(axis=0)
+    Z = np.full((1, H.HMVM_NFEAT), 0.5, dtype=np.float32)
+    Z[0, 15] = 18.0
+    Z[0, 3] = -1.0
+    c = m.clip(Z)
+    assert c.dtype == np.float32
+    assert c[0, 15] == 14.0 and c[0, 3] == 0.0 and c[0, 4] == 0.5
+    lin = AT.linear_baseline(np.vstack([X, X + 0.25]).astype(np.float32), np.array([0.0, 1.0, 0.5, 1.5]))
+    z2 = c.copy()
+    z2[0, 15] = 14.25  # the linear fit's own training maximum of that feature
+    assert np.allclose(lin(Z), lin(z2))

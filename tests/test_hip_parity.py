@@ -201,19 +201,21 @@ def test_bad_arguments_report_errors(gpu, H):
         H.Plan(h, storage=7)
 
 
-def test_kernel_variants_bit_identical(gpu, H, cube3000):
-    """Every tile-kernel variant keeps the per-lane summation order, so all of
-    them (and both workgroup sizes and tile orders) give the same bits."""
+@pytest.mark.parametrize("storage", ["f64", "f32", "bf16"])
+def test_kernel_variants_bit_identical(gpu, H, cube3000, storage):
+    """Every tile-kernel variant keeps the per-lane summation order (the
+    staged vector entries of the grouped path included), so all of them (and
+    every workgroup size and both tile orders) give the same bits."""
     h = cube3000
     b = np.random.default_rng(12).uniform(-1, 1, h.nd)
-    p = H.Plan(h)
+    p = H.Plan(h, storage=storage)
     y0 = p.apply_host(b)
     for k in range(8):
-        for bt in (64, 256):
+        for bt in (64, 128, 256):
             p.set_launch(k, bt)
             assert np.array_equal(p.apply_host(b), y0), (k, bt)
     p.close()
-    q = H.Plan(h, order=1)
+    q = H.Plan(h, storage=storage, order=1)
     assert np.array_equal(q.apply_host(b), y0)
     q.close()
 
