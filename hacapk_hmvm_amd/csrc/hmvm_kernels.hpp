@@ -295,24 +295,50 @@ __device__ __forceinline__ void wave_lds_sync() {
 }
 
 // LDS of one wavefront, in doubles: a chunk of kStage staged vector entries
-// of the grouped path, reused for its group reduction (64*V).  Workgroups get
-// (waves per workgroup) regions as dynamic shared memory, so one-wave
-// workgroups hold no LDS for waves they do not have.
+// of the grouped path, reused for its group reduction (64*V), then a table
+// of up to kPanTab panels (first column, vector offset, vec_chunks, vec_sel,
+// scale: 20 bytes each).  Workgroups get (waves per workgroup) regions as
+// dynamic shared memory, so one-wave workgroups hold no LDS for waves they
+// do not have.
 constexpr int kStage = 512;
+constexpr int kPanTab = 64;
 template <typename S>
 __host__ __device__ constexpr int lds_wave_doubles() {
-  return kStage > 64 * Wide<S>::V ? kStage : 64 * Wide<S>::V;
+  return (kStage > 64 * Wide<S>::V ? kStage : 64 * Wide<S>::V) + (kPanTab * 20) / 8;
+}
+
+// Vector entry j of a panel given its fields (vec_entry's rule).
+template <bool kCoh>
+__device__ __forceinline__ double entry_of(int sel, int off, int chunks, const double *__restrict__ x,
+                                           const double *__restrict__ t, const double *tp, int j) {
+  if (sel == kVecX) return x[off + j];
+  if (sel == kVecT) return t[off + j];
+  const int nch = chunks >> 16, stride = chunks & 0xFFFF;
+  double s = 0;
+  for (int c = 0; c < nch; ++c) {
+    const double *q = tp + off + c * stride + j;
+    if constexpr (kCoh)
+      s += __hip_atomic_load(q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    else
+      s += *q;
+  }
+  return s;
 }
 
 // lanes_per_col <= 32: the wave holds g = 64/lpc column groups; lane (c, jg)
-// sums columns jg, jg+g, ... so for a contiguous panel each load instruction
-// reads g*lpc*V consecutive elements.  The g partial sums of each row are
-// added in a fixed group order at the end.  The vector entries of kStage
-// columns at a time are fetched once (one coalesced load per lane per 64
-// columns) into the wave's LDS region, and every lane reads its columns'
-// entries there: a lane needs a different column's entry for every matrix
-// load, and a global load per matrix load halves the rate of these tiles
-// (measured; the wide path, whose entries are wave-uniform, does not stage).
+// sums columns jg, jg+g, ... so each load instruction reads g*lpc*V
+// consecutive elements.  The g partial sums of each row are added in a fixed
+// group order at the end.
+// The tile's panels lie back to back in memory with one column stride (the
+// plan builds every tile so, and checks it), so the wave walks them as one
+// column space; the vector entries of kStage columns at a time are fetched
+// once (one coalesced load per lane per 64 columns, each column's panel found
+// in an LDS table of the tile's panels) and multiplied by their panel's
+// scale into the wave's LDS region, and every lane reads its columns' entries
+// there.  Per-panel batches would leave most lanes idle on short panels (a
+// BF16 dense tile has ~10 of ~30 columns), and a global load of the entry per
+// matrix load halves the rate of these tiles (both measured).  Tiles of more
+// than kPanTab panels walk their panels one by one the same way.
 template <typename S, int kLoads, bool kNT, bool kScaled, bool kCoh>
 __device__ __forceinline__ void grouped_tile(const TileDesc &td, int shift, int lpc,
                                              const PanelDesc *__restrict__ panels, const S *__restrict__ vals,
@@ -327,68 +353,101 @@ __device__ __forceinline__ void grouped_tile(const TileDesc &td, int shift, int 
   const int jg = lane / lpc;
   const int c = lane - jg * lpc;
   const bool live = jg < g;  // lanes past g*lpc idle
+  // batch s, slot u: column s*step + u*g + jg of this lane's group; the batch
+  // count is wave-uniform, columns past the end load nothing and add 0
+  const int step = kLoads * g;
+  // a chunk is the largest multiple of step not above kStage (step <= 8 * 64
+  // = kStage), so every batch finds its entries in the staged chunk
+  const int chunk = (kStage / step) * step;
   A acc[V];
 #pragma unroll
   for (int v = 0; v < V; ++v) acc[v] = 0;
-  for (int p = 0; p < td.npanel; ++p) {
-    const PanelDesc pd = panels[td.panel0 + p];
-    const int ld = pd.ld, nc = pd.ncols;
-    const S *__restrict__ base = vals + (pd.val_off - shift) + (int64_t)V * c;
-    // batch s, slot u: column s*step + u*g + jg of this lane's group; the batch
-    // count is wave-uniform, columns past the panel load nothing and add 0
-    const int step = kLoads * g;
-    A accp[V];
+  // columns [0, nc) at base (column stride ld); entry(k) gives the scaled
+  // vector entry of column k
+  auto run_columns = [&](const S *__restrict__ base, int ld, int nc, auto entry) {
+    for (int c0 = 0; c0 < nc; c0 += chunk) {
+      const int cn = nc - c0 < chunk ? nc - c0 : chunk;
+      wave_lds_sync();  // the previous chunk's LDS reads are done
+      for (int q0 = 0; q0 < kStage / 64 && q0 * 64 < cn; q0 += 4) {
+        double e[4];
 #pragma unroll
-    for (int v = 0; v < V; ++v) accp[v] = 0;
-    auto consume = [&](const T (&a)[kLoads], const A (&b)[kLoads]) {
-#pragma unroll
-      for (int u = 0; u < kLoads; ++u)
-#pragma unroll
-        for (int v = 0; v < V; ++v) accp[v] += (A)W::get(a[u], v) * b[u];
-    };
-    // (no software pipelining here: a second batch of registers costs more
-    // occupancy than it gains, measured; the wide path pipelines)
-    {
-      // columns [c0, c0 + cn) per chunk; the chunk is the largest multiple
-      // of step not above kStage (step <= 8 * 64 = kStage), so every batch
-      // finds its entries in the staged chunk
-      const int chunk = (kStage / step) * step;
-      for (int c0 = 0; c0 < nc; c0 += chunk) {
-        const int cn = nc - c0 < chunk ? nc - c0 : chunk;
-        wave_lds_sync();  // the previous chunk's LDS reads are done
-        for (int q0 = 0; q0 < kStage / 64 && q0 * 64 < cn; q0 += 4) {
-          double e[4];
-#pragma unroll
-          for (int q = 0; q < 4; ++q) {
-            const int k = (q0 + q) * 64 + lane;
-            e[q] = k < cn ? vec_entry<kCoh>(pd, x, t, tp, c0 + k) : 0.0;
-          }
-#pragma unroll
-          for (int q = 0; q < 4; ++q) lds[(q0 + q) * 64 + lane] = e[q];
+        for (int q = 0; q < 4; ++q) {
+          const int k = (q0 + q) * 64 + lane;
+          e[q] = k < cn ? entry(c0 + k) : 0.0;
         }
-        wave_lds_sync();
-        const int nb = (cn + step - 1) / step;
-        for (int sb = 0; sb < nb; ++sb) {
-          T a[kLoads];
-          A b[kLoads];
 #pragma unroll
-          for (int u = 0; u < kLoads; ++u) {
-            const int j = sb * step + u * g + jg;
-            if (live && j < cn) {
-              a[u] = load_wide<S, kNT>(base + (int64_t)(c0 + j) * ld);
-              b[u] = (A)lds[j];
-            } else {
-              a[u] = T{};
-              b[u] = 0;
-            }
+        for (int q = 0; q < 4; ++q) lds[(q0 + q) * 64 + lane] = e[q];
+      }
+      wave_lds_sync();
+      const int nb = (cn + step - 1) / step;
+      for (int sb = 0; sb < nb; ++sb) {
+        T a[kLoads];
+        A b[kLoads];
+#pragma unroll
+        for (int u = 0; u < kLoads; ++u) {
+          const int j = sb * step + u * g + jg;
+          if (live && j < cn) {
+            a[u] = load_wide<S, kNT>(base + (int64_t)(c0 + j) * ld);
+            b[u] = (A)lds[j];
+          } else {
+            a[u] = T{};
+            b[u] = 0;
           }
-          consume(a, b);
         }
+        // (no software pipelining here: a second batch of registers costs
+        // more occupancy than it gains, measured; the wide path pipelines)
+#pragma unroll
+        for (int u = 0; u < kLoads; ++u)
+#pragma unroll
+          for (int v = 0; v < V; ++v) acc[v] += (A)W::get(a[u], v) * b[u];
       }
     }
-    const A sc = kScaled ? (A)panel_scale(pd, scales) : (A)1;
+  };
+  const int np = td.npanel;
+  if (np <= kPanTab) {
+    int *ptab = (int *)(lds + (lds_wave_doubles<S>() - (kPanTab * 20) / 8));  // 4 ints per panel
+    float *pscale = (float *)(ptab + 4 * kPanTab);
+    const PanelDesc pd0 = panels[td.panel0];
+    int ncl = 0;
+    if (lane < np) {
+      const PanelDesc pd = panels[td.panel0 + lane];
+      ncl = pd.ncols;
+      ptab[4 * lane + 1] = pd.vec_off;
+      ptab[4 * lane + 2] = pd.vec_chunks;
+      ptab[4 * lane + 3] = pd.vec_sel;
+      pscale[lane] = kScaled ? panel_scale(pd, scales) : 1.0f;
+    }
+    int incl = ncl;  // inclusive prefix sum of the panels' columns
 #pragma unroll
-    for (int v = 0; v < V; ++v) acc[v] += kScaled ? accp[v] * sc : accp[v];
+    for (int o = 1; o < 64; o <<= 1) {
+      const int v = __shfl_up(incl, o, 64);
+      if (lane >= o) incl += v;
+    }
+    if (lane < np) ptab[4 * lane] = incl - ncl;
+    const int nc = __shfl(incl, 63, 64);
+    wave_lds_sync();
+    run_columns(vals + (pd0.val_off - shift) + (int64_t)V * c, pd0.ld, nc, [&](int col) -> double {
+      int lo = 0, hi = np - 1;  // the last panel starting at or before col
+      while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (ptab[4 * mid] <= col)
+          lo = mid;
+        else
+          hi = mid - 1;
+      }
+      const double e = entry_of<kCoh>(ptab[4 * lo + 3], ptab[4 * lo + 1], ptab[4 * lo + 2], x, t, tp,
+                                      col - ptab[4 * lo]);
+      return kScaled ? e * (double)pscale[lo] : e;
+    });
+  } else {
+    for (int p = 0; p < np; ++p) {
+      const PanelDesc pd = panels[td.panel0 + p];
+      const double sc = kScaled ? (double)panel_scale(pd, scales) : 1.0;
+      run_columns(vals + (pd.val_off - shift) + (int64_t)V * c, pd.ld, pd.ncols, [&](int col) -> double {
+        const double e = entry_of<kCoh>(pd.vec_sel, pd.vec_off, pd.vec_chunks, x, t, tp, col);
+        return kScaled ? e * sc : e;
+      });
+    }
   }
   // The g partial sums of every row meet in the wave's LDS region: lane
   // (jg, c) stores its V sums at jg*lpc*V + c*V + v, then lane o sums output
@@ -964,19 +1023,20 @@ __global__ __launch_bounds__(256) void segsum_kernel(int n, const int2 *__restri
   const int2 r = rng[i];
   const int e1 = r.y;
   double a = 0;
-  int e = r.x;
-  // eight independent slot loads in flight, summed in list order
-  for (; e + 8 <= e1; e += 8) {
-    int64_t l[8];
-    double v[8];
+  // up to 24 list entries, then their slots, in flight at once (the y sum
+  // adds ~18 slots per row: one round trip for the list and one for the
+  // slots instead of one pair per 8), added in list order
+  for (int e = r.x; e < e1; e += 24) {
+    int64_t l[24];
+    double v[24];
 #pragma unroll
-    for (int u = 0; u < 8; ++u) l[u] = list[e + u];
+    for (int u = 0; u < 24; ++u) l[u] = e + u < e1 ? list[e + u] : 0;
 #pragma unroll
-    for (int u = 0; u < 8; ++u) v[u] = buf[l[u] + i];
+    for (int u = 0; u < 24; ++u) v[u] = e + u < e1 ? buf[l[u] + i] : 0.0;
 #pragma unroll
-    for (int u = 0; u < 8; ++u) a += v[u];
+    for (int u = 0; u < 24; ++u)
+      if (e + u < e1) a += v[u];
   }
-  for (; e < e1; ++e) a += buf[list[e] + i];
   // t entry of leaf b: both factor scales of b (a1 = factor_of, a2 = the next id)
   if (kScaled) a *= (double)scales[factor_of[i]] * (double)scales[factor_of[i] + 1];
   out[i] = accumulate ? out[i] + a : a;

@@ -455,6 +455,10 @@ std::string verify_plan(const Plan &P, const LeafArrays &la) {
       const int64_t shift0 = P.panels[t.panel0].val_off % P.lane_elems;
       for (int p = t.panel0; p < t.panel0 + t.npanel; p++) {
         const PanelDesc &d = P.panels[p];
+        // the grouped tile path walks a tile's panels as one column space
+        if (p > t.panel0 && (d.ld != P.panels[p - 1].ld ||
+                             d.val_off != P.panels[p - 1].val_off + (int64_t)P.panels[p - 1].ncols * d.ld))
+          return "panels of a tile are not back to back in memory";
         if (d.ncols < 1 || d.ld < t.nrows) return "panel shape";
         if (d.ld % P.lane_elems) return "column stride not a multiple of the lane width";
         // the kernel reads whole lane groups: [val_off - shift, val_off - shift + V*lpc) per column

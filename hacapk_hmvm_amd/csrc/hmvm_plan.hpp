@@ -15,18 +15,22 @@
 // low-rank stage 2: the a2 factors (element (k, il) at k*ndl + il, the
 //          reference order) of all low-rank leaves of tau times their t_b.
 // A tile is one wavefront and every lane loads 16 bytes (V consecutive rows
-// of one column; column strides are padded to multiples of V).  When a column
-// of the tile needs at most 32 lanes, the wave splits the columns over
-// 64/lanes-per-column groups; otherwise each lane owns up to three V-row
-// groups and reads every column whole, with the vector entry a wave-uniform
-// scalar load.  No lane-crossing reduction happens inside the streaming loop.
+// of one column; column strides are padded to multiples of V).  The panels of
+// a tile lie back to back in memory with one column stride (factors of a
+// cluster are stored back to back), which the grouped kernel path relies on.
+// When a column of the tile needs at most 32 lanes, the wave splits the
+// columns over 64/lanes-per-column groups; otherwise each lane owns up to
+// three V-row groups and reads every column whole, with the vector entry a
+// wave-uniform scalar load.  No lane-crossing reduction happens inside the
+// streaming loop.
 //
 // Launch A runs the stage-1 tiles and the dense tiles (independent of each
-// other); launch B runs the stage-2 tiles, which sum the stage-1 partial slots
-// of their leaf on the fly (or read a materialised t when a leaf's rank rows
-// straddle two stage-1 row groups); a final segment sum adds the private
-// output slots of all tiles into y in a fixed order: y is accumulated without
-// atomics and the result is bit-reproducible run to run.
+// other); a segment sum adds the stage-1 partial slots into t (t_mode 1, or
+// for the leaves t_mode 2 picks); launch B runs the stage-2 tiles, which read
+// t or sum the partial slots of their leaf on the fly (t_mode 0); a final
+// segment sum adds the private output slots of all tiles into y in a fixed
+// order: y is accumulated without atomics and the result is bit-reproducible
+// run to run.
 #pragma once
 
 #include <cstdint>
