@@ -367,7 +367,7 @@ const LaunchDefaults &launch_defaults(int storage) {
   static const LaunchDefaults d[3] = {
       {32768, 16384, 6, 256, 6, 256, 256},  // FP64
       {32768, 16384, 6, 256, 6, 256, 256},  // FP32
-      {65536, 32768, 6, 64, 6, 64, 512},    // BF16
+      {32768, 32768, 6, 256, 6, 256, 512},  // BF16
   };
   return d[storage >= kF64 && storage <= kBF16 ? storage : 0];
 }
