@@ -779,12 +779,17 @@ __device__ __forceinline__ typename Mfma<S>::Op a_of(const MultiWords<S> &mw, in
   }
 }
 
+// Chunks per pass and K-steps per batch, measured at N=101,250 for 16, 32
+// and 64 right-hand sides: two chunks when the accumulators fit 32 VGPRs
+// (one for BF16 with 16 right-hand sides); FP64 batches 8 K-steps for 16
+// right-hand sides and 4 otherwise; FP32 batches 2 (its occupancy drops to
+// two waves per SIMD with 4); BF16 batches 2 (32 columns).
 template <typename S, int NB>
 struct MultiShape {
   static constexpr int V = Wide<S>::V;
   static constexpr int ACCW = sizeof(typename Mfma<S>::Acc) / 4;  // VGPRs per accumulator
-  static constexpr int CH = V * NB * ACCW <= 32 ? 2 : 1;          // chunks per pass
-  static constexpr int KB = sizeof(S) == 2 ? 2 : 4;                // K-steps per batch
+  static constexpr int CH = (sizeof(S) == 2 && NB == 1) ? 1 : (V * NB * ACCW <= 32 ? 2 : 1);
+  static constexpr int KB = sizeof(S) == 8 ? (NB == 1 ? 8 : 4) : 2;
 };
 
 // A operand of one K-step for a short tile: row `row` of the tile as MFMA
@@ -958,10 +963,10 @@ __global__ __launch_bounds__(256) void tile_multi_kernel(const TileDesc *__restr
   }
 }
 
-// out[i*nrhs+c] = (accumulate ? out : 0) + scale(i) * sum_e buf[(list[e]+i)*nrhs + c]
+// out[i*nrhs+c] = (accumulate ? out : 0) + scale(i) * sum_e buf[(list[e]+i)*nrhs + c],
+// e over entry i's list range (as segsum_kernel: up to 24 in flight)
 template <bool kScaled>
-__global__ __launch_bounds__(256) void segsum_multi_kernel(int n, int nrhs, const int32_t *__restrict__ seg,
-                                                           const int32_t *__restrict__ ptr,
+__global__ __launch_bounds__(256) void segsum_multi_kernel(int n, int nrhs, const int2 *__restrict__ rng,
                                                            const int64_t *__restrict__ list,
                                                            const double *__restrict__ buf,
                                                            const int32_t *__restrict__ factor_of,
@@ -970,21 +975,20 @@ __global__ __launch_bounds__(256) void segsum_multi_kernel(int n, int nrhs, cons
   const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= (int64_t)n * nrhs) return;
   const int i = (int)(g / nrhs), c = (int)(g - (int64_t)i * nrhs);
-  const int s = seg[i];
-  const int e1 = ptr[s + 1];
+  const int2 r = rng[i];
+  const int e1 = r.y;
   double a = 0;
-  int e = ptr[s];
-  for (; e + 8 <= e1; e += 8) {
-    int64_t l[8];
-    double v[8];
+  for (int e = r.x; e < e1; e += 24) {
+    int64_t l[24];
+    double v[24];
 #pragma unroll
-    for (int u = 0; u < 8; ++u) l[u] = list[e + u];
+    for (int u = 0; u < 24; ++u) l[u] = e + u < e1 ? list[e + u] : 0;
 #pragma unroll
-    for (int u = 0; u < 8; ++u) v[u] = buf[(l[u] + i) * nrhs + c];
+    for (int u = 0; u < 24; ++u) v[u] = e + u < e1 ? buf[(l[u] + i) * nrhs + c] : 0.0;
 #pragma unroll
-    for (int u = 0; u < 8; ++u) a += v[u];
+    for (int u = 0; u < 24; ++u)
+      if (e + u < e1) a += v[u];
   }
-  for (; e < e1; ++e) a += buf[(list[e] + i) * nrhs + c];
   if (kScaled) a *= (double)scales[factor_of[i]] * (double)scales[factor_of[i] + 1];
   out[g] = accumulate ? out[g] + a : a;
 }

@@ -113,7 +113,7 @@ struct hmvm_plan {
   TileDesc *d_tiles_a = nullptr, *d_tiles_b = nullptr;
   PanelDesc *d_panels = nullptr;
   double *d_t = nullptr, *d_slots = nullptr;  // stage-1 partial slots, then output slots
-  int32_t *d_tseg = nullptr, *d_tptr = nullptr, *d_tfac = nullptr;
+  int32_t *d_tfac = nullptr;
   int64_t *d_tlist = nullptr;
   int32_t *d_yseg = nullptr, *d_yptr = nullptr;
   int64_t *d_ylist = nullptr;
@@ -123,7 +123,7 @@ struct hmvm_plan {
   size_t device_bytes = 0;
 
   ~hmvm_plan() {
-    void *ptrs[] = {d_vals,  d_tiles_a, d_tiles_b, d_panels, d_t,      d_slots, d_tseg, d_tptr,
+    void *ptrs[] = {d_vals,  d_tiles_a, d_tiles_b, d_panels, d_t,      d_slots,
                     d_tfac,  d_tlist,   d_yseg,    d_yptr,   d_ylist,  d_scales, d_x,    d_y,
                     d_trng,  d_yrng,
                     d_ctr,   d_mslots,  d_mt,      d_mX,     d_mY};
@@ -660,8 +660,7 @@ int create_plan(Source &src, const hmvm_options *opt, hmvm_plan **out) {
   const size_t eb = (size_t)elem_bytes(p->storage);
   // schedule
   if ((rc = upload(&p->d_tiles_a, P.tiles_a)) || (rc = upload(&p->d_tiles_b, P.tiles_b)) ||
-      (rc = upload(&p->d_panels, P.panels)) || (rc = upload(&p->d_tseg, P.tsum.seg)) ||
-      (rc = upload(&p->d_tptr, P.tsum.ptr)) || (rc = upload(&p->d_tlist, P.tsum.list)) ||
+      (rc = upload(&p->d_panels, P.panels)) || (rc = upload(&p->d_tlist, P.tsum.list)) ||
       (rc = upload(&p->d_yseg, P.ysum.seg)) || (rc = upload(&p->d_yptr, P.ysum.ptr)) ||
       (rc = upload(&p->d_ylist, P.ysum.list)) || (rc = upload(&p->d_trng, seg_ranges(P.tsum))) ||
       (rc = upload(&p->d_yrng, seg_ranges(P.ysum))))
@@ -685,7 +684,7 @@ int create_plan(Source &src, const hmvm_options *opt, hmvm_plan **out) {
 
   p->device_bytes = eb * (size_t)P.nvals + sizeof(TileDesc) * (P.tiles_a.size() + P.tiles_b.size()) +
                     sizeof(PanelDesc) * P.panels.size() + 8 * (size_t)(P.n_t + P.n_tp + P.n_z + 2 * P.nd) +
-                    4 * (P.tsum.seg.size() + P.tsum.ptr.size() + P.ysum.seg.size() + P.ysum.ptr.size()) +
+                    4 * (P.ysum.seg.size() + P.ysum.ptr.size()) + 4 * (size_t)P.n_t +
                     8 * (P.tsum.list.size() + P.ysum.list.size() + P.tsum.seg.size() + P.ysum.seg.size());
   if (verbose())
     std::fprintf(stderr,
@@ -891,16 +890,16 @@ void launch_multi_phase(hmvm_plan *p, int phase, const double *X, double *Y, int
     const int64_t tot = (int64_t)P.tsum.n * nrhs;
     if (bf)
       hipLaunchKernelGGL(segsum_multi_kernel<true>, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, P.tsum.n,
-                         nrhs, p->d_tseg, p->d_tptr, p->d_tlist, p->d_mslots, p->d_tfac, p->d_scales, p->d_mt, 0);
+                         nrhs, p->d_trng, p->d_tlist, p->d_mslots, p->d_tfac, p->d_scales, p->d_mt, 0);
     else
       hipLaunchKernelGGL(segsum_multi_kernel<false>, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, P.tsum.n,
-                         nrhs, p->d_tseg, p->d_tptr, p->d_tlist, p->d_mslots, (const int32_t *)nullptr,
-                         (const float *)nullptr, p->d_mt, 0);
+                         nrhs, p->d_trng, p->d_tlist, p->d_mslots, (const int32_t *)nullptr, (const float *)nullptr,
+                         p->d_mt, 0);
   } else {
     const int64_t tot = (int64_t)P.ysum.n * nrhs;
     hipLaunchKernelGGL(segsum_multi_kernel<false>, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, P.ysum.n,
-                       nrhs, p->d_yseg, p->d_yptr, p->d_ylist, p->d_mslots, (const int32_t *)nullptr,
-                       (const float *)nullptr, Y, accumulate);
+                       nrhs, p->d_yrng, p->d_ylist, p->d_mslots, (const int32_t *)nullptr, (const float *)nullptr,
+                       Y, accumulate);
   }
 }
 
