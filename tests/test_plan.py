@@ -75,3 +75,23 @@ def test_storage_option_validated(H):
     h = H.generate(400, geometry="sphere", leaf=16)
     assert H.plan_check(h, storage="bf16")["elem_bytes"] == 2
     assert H.plan_check(h, storage="f32")["algorithmic_bytes"] == 4 * h.mat2.len + 16 * h.nd
+
+
+@pytest.mark.parametrize("storage", ["f64", "f32", "bf16"])
+@pytest.mark.parametrize("t_mode", [0, 1, 2])
+def test_tile_panels_back_to_back(H, storage, t_mode):
+    """The grouped kernel path walks a tile's panels as one column space: in
+    every exported schedule the panels of a tile follow each other in memory
+    with one column stride (the plan check enforces it as well)."""
+    h = H.generate(6000, geometry="sphere", eps=1e-6, leaf=48)
+    s = H.schedule(h, storage=storage, t_mode=t_mode)
+    pan = s["panels"]
+    multi = 0
+    for key in ("tiles_a", "tiles_b"):
+        for p0, n in zip(s[key]["panel0"], s[key]["npanel"]):
+            multi += n > 1
+            for q in range(p0 + 1, p0 + n):
+                assert pan["ld"][q] == pan["ld"][q - 1]
+                assert pan["val_off"][q] == pan["val_off"][q - 1] + pan["ncols"][q - 1] * pan["ld"][q - 1]
+    assert multi > 0
+    h.free()
