@@ -1,13 +1,16 @@
 // hmvm_kernels.hpp -- gfx950 device code of the HMVM (included by the runtime).
 //
-// Kernel roles (one HMVM = three launches on one stream, four when a leaf's
-// rank rows straddle two stage-1 row groups):
+// Kernel roles (one HMVM = four launches on one stream; three when no leaf
+// reads a materialised t, t_mode 0):
 //   tile_kernel (launch A)  stage-1 slots = stacked a1 panels x x-slices,
 //                           dense slots   = transposed dense leaves x x-slices
 //   [segsum_kernel (t)      t = sum of stage-1 slots            (fixed order)]
-//   tile_kernel (launch B)  stage-2 slots = a2 panels x t (summed from the
-//                           stage-1 slots on the fly, or the materialised t)
+//   tile_kernel (launch B)  stage-2 slots = a2 panels x t (the materialised t,
+//                           or summed from the stage-1 slots on the fly)
 //   segsum_kernel (y)       y = y + sum of dense and stage-2 slots (fixed order)
+// Separate launches are the synchronisation: a device-scope release after
+// every tile (to let the last tile of a group do the sum) costs more than a
+// launch boundary on gfx950, where it writes back the XCD's L2.
 // plus the one-time repack kernels that move the caller's rowmat layout into
 // the plan's layout (and, for BF16 storage, compute per-factor scales).
 #pragma once
