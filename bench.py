@@ -35,6 +35,11 @@ import sys
 import tempfile
 import time
 
+# CPU baseline protocol: OpenMP threads pinned one per core, spread over the
+# cores this process may use (set before any OpenMP runtime is loaded)
+os.environ.setdefault("OMP_PLACES", "cores")
+os.environ.setdefault("OMP_PROC_BIND", "spread")
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
@@ -109,13 +114,17 @@ def run_c2(args, H, np):
         info = plan.info()
         ys[storage] = plan.apply_host(x)
         tm = plan.time(warmup=args.warmup, iters=args.steps)
+        # per-apply times with L2 and the 256 MB Infinity Cache flushed first
+        # (the BF16 matrix, 0.48 GB, is under twice the cache)
+        tf = plan.time_flushed(iters=max(10, args.steps // 2), flush_mb=1024)
         log("%s: plan %.1fs tiles=%d+%d panels=%d ms/HMVM stream=%.4f evented=%.4f (A %.4f tsum %.4f B %.4f "
-            "ysum %.4f)" % (storage, time.time() - t0, info["ntiles_a"], info["ntiles_b"], info["npanels"],
-                            tm["ms_stream"], tm["ms_avg"], tm["ms_a"], tm["ms_tsum"], tm["ms_b"], tm["ms_ysum"]))
-        results[storage] = (info, tm)
+            "ysum %.4f) flushed median %.4f min %.4f" % (
+                storage, time.time() - t0, info["ntiles_a"], info["ntiles_b"], info["npanels"], tm["ms_stream"],
+                tm["ms_avg"], tm["ms_a"], tm["ms_tsum"], tm["ms_b"], tm["ms_ysum"], tf["ms_median"], tf["ms_min"]))
+        results[storage] = (info, tm, tf)
         plan.close()
 
-    info64, tm64 = results["f64"]
+    info64, tm64, _ = results["f64"]
     ms = tm64["ms_stream"]
     whole_gbs = info64["algorithmic_bytes"] / (ms * 1e-3) / 1e9
     # dominant kernel: the longer of the two tile launches; algorithmic bytes =
@@ -135,9 +144,11 @@ def run_c2(args, H, np):
         pass
 
     def form(st):
-        info, tm = results[st]
+        info, tm, tf = results[st]
         return {"ms": round(tm["ms_stream"], 5), "GBps": round(info["algorithmic_bytes"] / tm["ms_stream"] / 1e6, 1),
                 "speedup_vs_fp64": round(ms / tm["ms_stream"], 3), "rel_err_vs_fp64_hip": rel(ys[st], ys["f64"]),
+                "ms_per_apply_median": round(tm["ms_median"], 5), "ms_per_apply_min": round(tm["ms_min"], 5),
+                "ms_flushed_median": round(tf["ms_median"], 5), "ms_flushed_min": round(tf["ms_min"], 5),
                 "launch_ms": {"tiles_a": tm["ms_a"], "tsum": tm["ms_tsum"], "tiles_b": tm["ms_b"],
                               "ysum": tm["ms_ysum"]}}
 
@@ -153,7 +164,10 @@ def run_c2(args, H, np):
         omp_ms = omp_time(O.omp_2, h, x)
         cpu = {"value": round(omp_ms, 3), "unit": "ms/HMVM", "cores": O.omp_threads(), "kind": "port",
                "sample": "full workload (N=%d, len=%d): oracle restatement of hmvm_omp_2, 10 runs, mean of the last "
-                         "5; sequential oracle hmvm_seq_2 %.1f ms; host %s" % (nd, length, 1e3 * seq_s, cpu_model())}
+                         "5; sequential oracle hmvm_seq_2 %.1f ms; host %s, %d CPUs usable by this process, "
+                         "OMP_PLACES=%s OMP_PROC_BIND=%s" % (nd, length, 1e3 * seq_s, cpu_model(),
+                                                             len(os.sched_getaffinity(0)), os.environ["OMP_PLACES"],
+                                                             os.environ["OMP_PROC_BIND"])}
         log("cpu: omp %.2f ms (%d threads), seq %.1f ms; rel err f64 %.2e f32 %.2e bf16 %.2e" %
             (omp_ms, O.omp_threads(), 1e3 * seq_s, forms["f64"]["rel_err_vs_oracle"],
              forms["f32"]["rel_err_vs_oracle"], forms["bf16"]["rel_err_vs_oracle"]))

@@ -232,6 +232,7 @@ def _load():
                                            ctypes.c_void_p]),
         "hmvm_plan_apply_host": (ctypes.c_int, [ctypes.c_void_p, c_double_p, c_double_p, ctypes.c_int]),
         "hmvm_plan_time": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, P(hmvm_timing)]),
+        "hmvm_plan_time_flushed": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_longlong, ctypes.c_int, P(hmvm_timing)]),
         "hmvm_plan_bench_apply": (ctypes.c_int, [ctypes.c_void_p, c_double_p, ctypes.c_int, c_double_p]),
         "hmvm_plan_reserve_multi": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
         "hmvm_plan_apply_multi": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
@@ -862,6 +863,15 @@ class Plan:
         rc = lib.hmvm_plan_time(self._p, warmup, iters, ctypes.byref(t))
         if rc:
             raise HMVMError("hmvm_plan_time: %s" % lib.hmvm_hip_last_error_message().decode())
+        return t.as_dict()
+
+    def time_flushed(self, iters=20, flush_mb=1024):
+        """Per-apply timing with L2 and the Infinity Cache flushed before each
+        apply (a kernel writes flush_mb MiB of scratch)."""
+        t = hmvm_timing()
+        rc = lib.hmvm_plan_time_flushed(self._p, int(flush_mb) << 20, iters, ctypes.byref(t))
+        if rc:
+            raise HMVMError("hmvm_plan_time_flushed: %s" % lib.hmvm_hip_last_error_message().decode())
         return t.as_dict()
 
 

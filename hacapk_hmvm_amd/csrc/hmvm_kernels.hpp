@@ -1014,6 +1014,13 @@ __global__ __launch_bounds__(256) void probe_copy_kernel(const double4 *__restri
     dst[i] = src[i];
 }
 
+// Writes n double4 of dst (grid-stride): evicts L2 and the Infinity Cache
+// between timed applies.
+__global__ __launch_bounds__(256) void flush_kernel(double4 *__restrict__ dst, size_t n, double v) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    dst[i] = double4{v, v, v, v};
+}
+
 // out[i] = (accumulate ? out[i] : 0) + scale(i) * sum_e buf[list[e] + i],
 // e over [rng[i].x, rng[i].y) (entry i's segment, resolved per entry on the
 // host so that the chain of dependent loads is range -> list -> slots),

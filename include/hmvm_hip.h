@@ -159,6 +159,13 @@ int hmvm_plan_bench_apply(hmvm_plan *plan, const double *x, int iters, double *s
 /* Times `iters` applies on the plan's internal device vectors after `warmup`
  * untimed ones, with HIP events around every launch. */
 int hmvm_plan_time(hmvm_plan *plan, int warmup, int iters, hmvm_timing *timing);
+/* Times `iters` single applies (after one untimed), each preceded by a kernel
+ * that writes flush_bytes of scratch memory so that no factor is still in L2
+ * or the Infinity Cache: ms_avg/min/median per apply (ms_stream = ms_avg; the
+ * per-launch fields are 0).  For matrices smaller than twice the 256 MB
+ * Infinity Cache, where back-to-back applies can hit in it.  No reference
+ * counterpart (the reference times without flushing, hmvm_cuda1.cu:236-302). */
+int hmvm_plan_time_flushed(hmvm_plan *plan, long long flush_bytes, int iters, hmvm_timing *timing);
 
 /* Switches the tile-kernel variant and workgroup size of an existing plan
  * (used by the autotuner; results are identical across variants up to the

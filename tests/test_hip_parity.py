@@ -304,3 +304,15 @@ def test_tall_tiles(gpu, H, O, storage, max_rows, tol):
         p.set_launch(k, 256)
         assert np.array_equal(p.apply_host(x), y)
     p.close()
+
+
+def test_time_flushed(gpu, H, cube3000):
+    """Per-apply timing with the caches flushed first: positive times, ordered
+    statistics, and the plan's results untouched."""
+    b = np.random.default_rng(14).uniform(-1, 1, cube3000.nd)
+    p = H.Plan(cube3000)
+    y0 = p.apply_host(b)
+    t = p.time_flushed(iters=5, flush_mb=64)
+    assert t["iters"] == 5 and 0 < t["ms_min"] <= t["ms_median"] and t["ms_avg"] > 0
+    assert np.array_equal(p.apply_host(b), y0)
+    p.close()
