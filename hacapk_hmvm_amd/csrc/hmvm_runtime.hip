@@ -87,6 +87,22 @@ std::vector<int2> seg_ranges(const SegSum &g) {
 
 int elem_bytes(int storage) { return storage == kF64 ? 8 : storage == kF32 ? 4 : 2; }
 
+// N events, destroyed on every return path of the timing functions.
+template <int N>
+struct EventSet {
+  hipEvent_t e[N] = {};
+  hipError_t create() {
+    for (int i = 0; i < N; i++)
+      if (hipError_t err = hipEventCreate(&e[i])) return err;
+    return hipSuccess;
+  }
+  hipEvent_t &operator[](int i) { return e[i]; }
+  ~EventSet() {
+    for (int i = 0; i < N; i++)
+      if (e[i]) (void)hipEventDestroy(e[i]);
+  }
+};
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -1299,8 +1315,8 @@ int hmvm_plan_time_multi(hmvm_plan *plan, int nrhs, int warmup, int iters, hmvm_
   }
   for (int w = 0; w < warmup; w++)
     if ((rc = apply_multi(plan, plan->d_mX, plan->d_mY, nrhs, 0, st))) return rc;
-  hipEvent_t ev[5];
-  for (int e = 0; e < 5; e++) HIP_TRY(hipEventCreate(&ev[e]));
+  EventSet<5> ev;
+  HIP_TRY(ev.create());
   double ph[4] = {0, 0, 0, 0}, total = 0, mn = 1e30;
   for (int it = 0; it < iters; it++) {
     hipEventRecord(ev[0], st);
@@ -1326,7 +1342,6 @@ int hmvm_plan_time_multi(hmvm_plan *plan, int nrhs, int warmup, int iters, hmvm_
     if (!rc && hipEventSynchronize(ev[1]) == hipSuccess) hipEventElapsedTime(&ms, ev[0], ev[1]);
     tm->ms_stream = ms / iters;
   }
-  for (int e = 0; e < 5; e++) (void)hipEventDestroy(ev[e]);
   if (rc) return rc;
   HIP_TRY(hipGetLastError());
   tm->iters = iters;
@@ -1555,8 +1570,8 @@ int hmvm_plan_time(hmvm_plan *plan, int warmup, int iters, hmvm_timing *tm) {
     int rc = apply(plan, plan->d_x, plan->d_y, 0, st);
     if (rc) return rc;
   }
-  hipEvent_t ev[5];
-  for (int e = 0; e < 5; e++) HIP_TRY(hipEventCreate(&ev[e]));
+  EventSet<5> ev;
+  HIP_TRY(ev.create());
   std::vector<double> total;
   double ph[4] = {0, 0, 0, 0};
   int rc = HMVM_OK;
@@ -1590,7 +1605,6 @@ int hmvm_plan_time(hmvm_plan *plan, int warmup, int iters, hmvm_timing *tm) {
     if (!rc) hipEventElapsedTime(&ms, ev[0], ev[1]);
     tm->ms_stream = ms / iters;
   }
-  for (int e = 0; e < 5; e++) (void)hipEventDestroy(ev[e]);
   if (rc) return rc;
   HIP_TRY(hipGetLastError());
   std::vector<double> s = total;
@@ -1619,11 +1633,16 @@ int hmvm_plan_time_flushed(hmvm_plan *plan, long long flush_bytes, int iters, hm
     HIP_TRY(hipMemcpy(plan->d_x, h.data(), sizeof(double) * nd, hipMemcpyHostToDevice));
   }
   const size_t n4 = (size_t)flush_bytes / sizeof(double4);
-  double4 *buf = nullptr;
-  if (n4) HIP_TRY(hipMalloc(&buf, n4 * sizeof(double4)));
-  hipEvent_t ev[2];
-  HIP_TRY(hipEventCreate(&ev[0]));
-  HIP_TRY(hipEventCreate(&ev[1]));
+  struct Buf {  // released on every return path
+    double4 *p = nullptr;
+    ~Buf() {
+      if (p) (void)hipFree(p);
+    }
+  } scratch;
+  if (n4) HIP_TRY(hipMalloc(&scratch.p, n4 * sizeof(double4)));
+  EventSet<2> ev;
+  HIP_TRY(ev.create());
+  double4 *buf = scratch.p;
   std::vector<double> total;
   int rc = apply(plan, plan->d_x, plan->d_y, 0, st);  // one untimed apply
   for (int it = 0; it < iters && !rc; it++) {
@@ -1637,9 +1656,6 @@ int hmvm_plan_time_flushed(hmvm_plan *plan, long long flush_bytes, int iters, hm
     if (!rc) hipEventElapsedTime(&ms, ev[0], ev[1]);
     total.push_back(ms);
   }
-  (void)hipEventDestroy(ev[0]);
-  (void)hipEventDestroy(ev[1]);
-  if (buf) (void)hipFree(buf);
   if (rc) return rc;
   HIP_TRY(hipGetLastError());
   std::vector<double> s = total;
