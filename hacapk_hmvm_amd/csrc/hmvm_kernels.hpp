@@ -352,6 +352,7 @@ __device__ __forceinline__ void grouped_tile(const TileDesc &td, int shift, int 
   using W = Wide<S>;
   using T = typename W::T;
   constexpr int V = W::V;
+  static_assert(kLoads * 64 <= kStage, "a batch of columns must fit the staged chunk");
   const int g = 64 / lpc;
   const int jg = lane / lpc;
   const int c = lane - jg * lpc;
