@@ -43,10 +43,10 @@ struct CMatrix2 {
   double *rowmat, *rowmat_t;
 };
 struct COptions {
-  int storage, w1, w2, block_threads, kernel, order, mode, t_mode, max_rows;
+  int storage, w1, w2, block_threads, kernel, order, mode, t_mode, max_rows, max_rows_b;
 };
 static_assert(sizeof(CMatrix2) == 104, "C matrix2 layout");
-static_assert(sizeof(COptions) == 9 * sizeof(int), "hmvm_options layout");
+static_assert(sizeof(COptions) == 10 * sizeof(int), "hmvm_options layout");
 static_assert(offsetof(matrix2<double>, rowmat_t) == offsetof(CMatrix2, rowmat_t), "matrix2<double> prefix");
 
 template <class T>

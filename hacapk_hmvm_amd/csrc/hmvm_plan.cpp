@@ -67,9 +67,10 @@ std::string build_plan(const LeafArrays &la, const PlanOptions &opt, Plan *plan)
   if (la.nd < 1 || la.nlf < 1) return "empty matrix";
   if (opt.w1 < 64 || opt.w2 < 64 || opt.align < 1 || opt.t_mode < 0 || opt.t_mode > 2) return "bad plan options";
   const int MR = opt.max_rows;
-  if (MR < 1 || MR > max_tile_rows_limit(lane_elems(opt.storage)) || MR >= (1 << 16))
-    return "max_rows out of range for this storage";
-  P.max_rows = MR;
+  const int MRb = opt.max_rows_b > 0 ? opt.max_rows_b : MR;  // stage-2 tiles
+  const int lim = max_tile_rows_limit(lane_elems(opt.storage));
+  if (MR < 1 || MR > lim || MR >= (1 << 16) || MRb > lim) return "max_rows out of range for this storage";
+  P.max_rows = std::max(MR, MRb);
   if (la.nd > (1 << 30)) return "matrix order too large";
   const int nlf = la.nlf;
 
@@ -304,9 +305,11 @@ std::string build_plan(const LeafArrays &la, const PlanOptions &opt, Plan *plan)
         P.repack.push_back(r);
       }
     }
-    for (auto rc : row_chunks(ndl, MR)) {
-      const int r0 = rc.first, nr = rc.second;
-      for (int kind = 0; kind < 2; kind++) {
+    // dense tiles (launch A) in row chunks of <= MR rows, stage-2 tiles
+    // (launch B) of <= MRb rows
+    for (int kind = 0; kind < 2; kind++)
+      for (auto rc : row_chunks(ndl, kind == 0 ? MR : MRb)) {
+        const int r0 = rc.first, nr = rc.second;
         const std::vector<int> &lst = kind == 0 ? dense : lowrank;
         size_t q = 0;
         while (q < lst.size()) {
@@ -366,7 +369,6 @@ std::string build_plan(const LeafArrays &la, const PlanOptions &opt, Plan *plan)
           P.n_z += nr;
         }
       }
-    }
   }
   if (P.n_tp + P.n_z > INT32_MAX) return "output slots exceed int32";
   if (P.panels.size() > (size_t)INT32_MAX) return "too many panels";

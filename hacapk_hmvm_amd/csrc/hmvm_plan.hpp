@@ -98,6 +98,7 @@ struct PlanOptions {
   int order = 0;          // 0: memory order, 1: largest tiles first
   int t_mode = 0;         // stage-2 vector: 0 partial sums on the fly, 1 materialised t, 2 per leaf
   int max_rows = kMaxTileRows;  // rows per tile (rank rows per stage-1 group, leaf rows per chunk)
+  int max_rows_b = 0;           // rows per stage-2 tile (0: max_rows)
 };
 
 struct LeafArrays {       // the caller's descriptors (matrix2 or matrix)

@@ -373,17 +373,19 @@ int source_from_batch(const matrix2 *const *mats, int count, int a2_source, Sour
 // Launch defaults per storage, measured best on MI355X at N = 101,250 (cube,
 // eps 1e-4): tile widths in elements, then kernel variant and workgroup size
 // of launch A and launch B, and the most rows one tile (one wavefront) may
-// cover.  Taller tiles only pay off for BF16, whose 8-element lane loads
-// leave short stage-2 tiles with few lanes busy.  Options with 0 (widths,
+// cover, for launch A and for the stage-2 tiles of launch B.  Taller tiles
+// only pay off for BF16, whose 8-element lane loads leave short stage-2 tiles
+// with few lanes busy; FP64 stage-2 tiles run best at 128 rows (one 16-byte
+// row group per lane).  Options with 0 (widths,
 // workgroup size, rows) or -1 (kernel) take these.
 struct LaunchDefaults {
-  int w1, w2, kernel_a, bt_a, kernel_b, bt_b, max_rows;
+  int w1, w2, kernel_a, bt_a, kernel_b, bt_b, max_rows, max_rows_b;
 };
 const LaunchDefaults &launch_defaults(int storage) {
   static const LaunchDefaults d[3] = {
-      {32768, 16384, 6, 256, 6, 256, 256},  // FP64
-      {32768, 16384, 6, 256, 6, 256, 256},  // FP32
-      {32768, 32768, 6, 256, 6, 256, 512},  // BF16
+      {32768, 16384, 6, 256, 6, 256, 256, 128},  // FP64
+      {32768, 16384, 6, 256, 6, 256, 256, 256},  // FP32
+      {32768, 32768, 6, 256, 6, 256, 512, 512},  // BF16
   };
   return d[storage >= kF64 && storage <= kBF16 ? storage : 0];
 }
@@ -398,7 +400,12 @@ PlanOptions plan_options(const hmvm_options *o) {
   po.storage = o->storage;
   po.w1 = o->w1 > 0 ? o->w1 : launch_defaults(po.storage).w1;
   po.w2 = o->w2 > 0 ? o->w2 : launch_defaults(po.storage).w2;
+  // an explicit max_rows sets both unless max_rows_b is given; by default
+  // stage-2 tiles have their own
   po.max_rows = o->max_rows > 0 ? o->max_rows : launch_defaults(po.storage).max_rows;
+  po.max_rows_b = o->max_rows_b > 0 ? o->max_rows_b
+                  : o->max_rows > 0 ? o->max_rows
+                                    : launch_defaults(po.storage).max_rows_b;
   po.order = o->order;
   // t_mode < 0: materialise t for every leaf.  With t in row-cluster order
   // the stage-2 factors of a cluster merge into one panel, which beats summing
@@ -1152,6 +1159,7 @@ void hmvm_options_default(hmvm_options *opt) {
   opt->mode = 0;
   opt->t_mode = -1;  // by storage
   opt->max_rows = 0;  // by storage
+  opt->max_rows_b = 0;
 }
 
 int hmvm_plan_create(const matrix2 *mat2, int a2_source, const hmvm_options *opt, hmvm_plan **out) {

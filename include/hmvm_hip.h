@@ -106,7 +106,10 @@ typedef struct hmvm_options {
                          /* slots on the fly, 1 read a materialised t (one more */
                          /* launch), 2 choose per leaf, -1 default (1)          */
   int max_rows;          /* rows per tile (0: by storage); at most 192*V-V+1,  */
-                         /* V = 16 bytes / element size                         */
+                         /* V = 16 bytes / element size; when set, also for the */
+                         /* stage-2 tiles unless max_rows_b is set              */
+  int max_rows_b;        /* rows per stage-2 tile of launch B (0: max_rows if   */
+                         /* set, else by storage)                               */
 } hmvm_options;
 
 typedef struct hmvm_plan_info {
