@@ -35,10 +35,10 @@ import sys
 import tempfile
 import time
 
-# CPU baseline protocol: OpenMP threads pinned one per core, spread over the
-# cores this process may use (set before any OpenMP runtime is loaded)
-os.environ.setdefault("OMP_PLACES", "cores")
-os.environ.setdefault("OMP_PROC_BIND", "spread")
+# CPUs this process may run on, read before any OpenMP runtime starts (the
+# CPU baseline runs one OpenMP thread per usable CPU, unpinned: binding would
+# pin the main thread, and every thread it starts later, to one core)
+USABLE_CPUS = len(os.sched_getaffinity(0))
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
@@ -164,10 +164,8 @@ def run_c2(args, H, np):
         omp_ms = omp_time(O.omp_2, h, x)
         cpu = {"value": round(omp_ms, 3), "unit": "ms/HMVM", "cores": O.omp_threads(), "kind": "port",
                "sample": "full workload (N=%d, len=%d): oracle restatement of hmvm_omp_2, 10 runs, mean of the last "
-                         "5; sequential oracle hmvm_seq_2 %.1f ms; host %s, %d CPUs usable by this process, "
-                         "OMP_PLACES=%s OMP_PROC_BIND=%s" % (nd, length, 1e3 * seq_s, cpu_model(),
-                                                             len(os.sched_getaffinity(0)), os.environ["OMP_PLACES"],
-                                                             os.environ["OMP_PROC_BIND"])}
+                         "5; sequential oracle hmvm_seq_2 %.1f ms; host %s, %d CPUs usable by this process" % (
+                             nd, length, 1e3 * seq_s, cpu_model(), USABLE_CPUS)}
         log("cpu: omp %.2f ms (%d threads), seq %.1f ms; rel err f64 %.2e f32 %.2e bf16 %.2e" %
             (omp_ms, O.omp_threads(), 1e3 * seq_s, forms["f64"]["rel_err_vs_oracle"],
              forms["f32"]["rel_err_vs_oracle"], forms["bf16"]["rel_err_vs_oracle"]))
