@@ -316,3 +316,17 @@ def test_time_flushed(gpu, H, cube3000):
     assert t["iters"] == 5 and 0 < t["ms_min"] <= t["ms_median"] and t["ms_avg"] > 0
     assert np.array_equal(p.apply_host(b), y0)
     p.close()
+
+
+@pytest.mark.parametrize("storage,max_rows,max_rows_b,tol", [("f64", 256, 64, 1e-13), ("f64", 128, 383, 1e-13),
+                                                             ("f32", 0, 96, 1e-5), ("bf16", 512, 200, 3e-2)])
+def test_stage2_tile_height(gpu, H, O, storage, max_rows, max_rows_b, tol):
+    """Stage-2 tiles cut to their own height (max_rows_b), independent of the
+    launch-A tiles."""
+    h = tall_matrix(H)
+    x = np.random.default_rng(17).uniform(-1, 1, h.nd)
+    p = H.Plan(h, storage=storage, max_rows=max_rows, max_rows_b=max_rows_b)
+    s = H.schedule(h, storage=storage, max_rows=max_rows, max_rows_b=max_rows_b)
+    assert s["tiles_b"]["nrows"].max() <= max_rows_b
+    assert rel(p.apply_host(x), O.seq_2(h, x)) <= tol
+    p.close()

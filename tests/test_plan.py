@@ -95,3 +95,15 @@ def test_tile_panels_back_to_back(H, storage, t_mode):
                 assert pan["val_off"][q] == pan["val_off"][q - 1] + pan["ncols"][q - 1] * pan["ld"][q - 1]
     assert multi > 0
     h.free()
+
+
+def test_stage2_tile_height_option(H):
+    """max_rows_b bounds the stage-2 tiles only; max_rows alone bounds both."""
+    h = H.generate(6000, geometry="sphere", eps=1e-6, leaf=48)
+    s = H.schedule(h, storage="f64", max_rows=256, max_rows_b=64)
+    assert s["tiles_b"]["nrows"].max() <= 64 < s["tiles_a"]["nrows"].max()
+    s = H.schedule(h, storage="f64", max_rows=96)
+    assert max(s["tiles_a"]["nrows"].max(), s["tiles_b"]["nrows"].max()) <= 96
+    with pytest.raises(H.HMVMError):
+        H.schedule(h, storage="f64", max_rows_b=400)  # above 192*2-1 for FP64
+    h.free()
