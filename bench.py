@@ -85,40 +85,6 @@ def omp_time(fn, h, x, runs=10):
     return 1e3 * sum(times[runs // 2:]) / (runs - runs // 2)
 
 
-_PINNED_CHILD = r"""
-import json, sys, time
-sys.path.insert(0, sys.argv[1])
-import numpy as np
-import hacapk_hmvm_amd as H
-import oracle as O
-h = H.loadHmatrix(sys.argv[2])
-x = np.load(sys.argv[3])
-t = []
-for _ in range(10):
-    t0 = time.perf_counter()
-    O.omp_2(h, x)
-    t.append(time.perf_counter() - t0)
-print(json.dumps({"ms": 1e3 * sum(t[5:]) / 5, "threads": O.omp_threads()}))
-"""
-
-
-def omp_time_pinned(fname, x):
-    """The OpenMP oracle on the dump file, in a child process whose OpenMP
-    threads are pinned one per core (pinning in this process would also pin
-    the threads the GPU runtime starts).  The child makes no GPU call."""
-    xf = fname + ".x.npy"
-    import numpy as np
-    np.save(xf, x)
-    env = dict(os.environ, OMP_PLACES="cores", OMP_PROC_BIND="spread")
-    try:
-        out = subprocess.run([sys.executable, "-c", _PINNED_CHILD, ROOT, fname, xf], env=env, check=True,
-                             stdout=subprocess.PIPE, timeout=600).stdout
-    finally:
-        os.remove(xf)
-    r = json.loads(out.decode().strip().splitlines()[-1])
-    return r["ms"], r["threads"]
-
-
 # ---- C2: the headline ---------------------------------------------------------------
 def run_c2(args, H, np):
     probe_read, probe_copy = H.bandwidth_probe(1 << 30, 5)
@@ -136,6 +102,7 @@ def run_c2(args, H, np):
         H.writeHmatrix1(fname, g)
         g.free()
         h = H.loadHmatrix(fname)
+        os.remove(fname)
         log("hmatrix_1.bin written and loaded in %.1fs" % (time.time() - t0))
     nd, length = h.nd, h.mat2.len
     x = np.random.default_rng(20261202).uniform(-1.0, 1.0, nd)
@@ -194,21 +161,14 @@ def run_c2(args, H, np):
         seq_s = time.time() - t0
         for st in forms:
             forms[st]["rel_err_vs_oracle"] = rel(ys[st], y_or)
-        if args.in_memory:
-            omp_ms, threads, pinned = omp_time(O.omp_2, h, x), O.omp_threads(), "unpinned"
-        else:
-            omp_ms, threads = omp_time_pinned(fname, x)
-            pinned = "OMP_PLACES=cores OMP_PROC_BIND=spread, in a child process that loads hmatrix_1.bin"
-        cpu = {"value": round(omp_ms, 3), "unit": "ms/HMVM", "cores": threads, "kind": "port",
+        omp_ms = omp_time(O.omp_2, h, x)
+        cpu = {"value": round(omp_ms, 3), "unit": "ms/HMVM", "cores": O.omp_threads(), "kind": "port",
                "sample": "full workload (N=%d, len=%d): oracle restatement of hmvm_omp_2, 10 runs, mean of the last "
-                         "5, threads %s; sequential oracle hmvm_seq_2 %.1f ms; host %s, %d CPUs usable by this "
-                         "process" % (nd, length, pinned, 1e3 * seq_s, cpu_model(), USABLE_CPUS)}
+                         "5; sequential oracle hmvm_seq_2 %.1f ms; host %s, %d CPUs usable by this process" % (
+                             nd, length, 1e3 * seq_s, cpu_model(), USABLE_CPUS)}
         log("cpu: omp %.2f ms (%d threads), seq %.1f ms; rel err f64 %.2e f32 %.2e bf16 %.2e" %
-            (omp_ms, threads, 1e3 * seq_s, forms["f64"]["rel_err_vs_oracle"],
+            (omp_ms, O.omp_threads(), 1e3 * seq_s, forms["f64"]["rel_err_vs_oracle"],
              forms["f32"]["rel_err_vs_oracle"], forms["bf16"]["rel_err_vs_oracle"]))
-    if not args.in_memory:
-        os.remove(fname)
-        os.rmdir(tmpd)
 
     line = base_line("ms/HMVM & % HBM peak at N=101,250: FP64 plain vs FP32-stored, rel err vs FP64",
                      round(ms, 5), "ms/HMVM", args, "f64",
