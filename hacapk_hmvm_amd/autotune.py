@@ -33,15 +33,28 @@ import numpy as np
 
 from . import CostMLP, HMVM_NFEAT, Plan, schedule, tile_occupancy
 
-W1_CHOICES = (0.5, 1.0, 2.0)     # multiples of the default 32768-element stage-1 tile
-W2_CHOICES = (0.5, 1.0, 2.0)     # multiples of the default 16384-element stage-2 tile
+W1_CHOICES = (0.5, 1.0, 2.0)     # multiples of the base stage-1 tile width (32768 elements)
+W2_CHOICES = (0.5, 1.0, 2.0)     # multiples of the base stage-2 tile width (16384 elements)
 KERNELS = (2, 4, 6, 7)           # pipelined / non-temporal / batched-entry variants
 BLOCK_THREADS = (64, 256)
 EB = {"f64": 8, "f32": 4, "bf16": 2}
 
 
-def widths(storage, a, b):
-    return int(a * 32768), int(b * 16384)
+def widths(storage, a, b, length=None):
+    """Tile widths (elements) for multiples a, b of the base widths.  Given the
+    matrix's element count, the bases shrink as the library's default widths do
+    for small matrices (to length / 4096, at least 1024), so that the candidates
+    stay around one tile per resident wavefront slot."""
+    b1, b2 = 32768, 16384
+    if length is not None:
+        fit = max(1024, int(length) // 4096)
+        b1, b2 = min(b1, fit), min(b2, fit)
+    return int(a * b1), int(b * b2)
+
+
+def _length(h):
+    m = getattr(h, "mat2", None)
+    return getattr(m, "len", None)
 
 
 # ---- features from an exported schedule (host only) ------------------------------
@@ -97,7 +110,7 @@ def collect(h, storage, variants=None, w_scales=((1.0, 1.0),), iters=2, max_tile
     variants = variants or [(k, bt) for k in KERNELS for bt in BLOCK_THREADS]
     Xs, ys, recs = [], [], []
     for a, b in w_scales:
-        w1, w2 = widths(storage, a, b)
+        w1, w2 = widths(storage, a, b, _length(h))
         p = Plan(h, storage=storage, w1=w1, w2=w2)
         for k, bt in variants:
             p.set_launch(k, bt)
@@ -215,7 +228,7 @@ def exhaustive(h, storage, warmup=2, iters=8, w_choices=None, kernels=KERNELS, b
     results = []
     best = None
     for a, b in w_choices:
-        w1, w2 = widths(storage, a, b)
+        w1, w2 = widths(storage, a, b, _length(h))
         p = Plan(h, storage=storage, w1=w1, w2=w2)
         per = {}
         for k in kernels:
@@ -242,7 +255,7 @@ def learned(h, storage, model, w_choices=None, kernels=KERNELS, bts=BLOCK_THREAD
     w_choices = w_choices or list(itertools.product(W1_CHOICES, W2_CHOICES))
     best = None
     for a, b in w_choices:
-        w1, w2 = widths(storage, a, b)
+        w1, w2 = widths(storage, a, b, _length(h))
         sched = schedule(h, storage=storage, w1=w1, w2=w2)
         pred = {}
         for which in (0, 1):

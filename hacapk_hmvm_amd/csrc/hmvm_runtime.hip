@@ -385,12 +385,16 @@ const LaunchDefaults &launch_defaults(int storage) {
   static const LaunchDefaults d[3] = {
       {32768, 16384, 6, 256, 6, 256, 256, 128},  // FP64
       {32768, 16384, 6, 256, 6, 256, 256, 256},  // FP32
-      {32768, 32768, 6, 256, 6, 256, 512, 512},  // BF16
+      {32768, 32768, 6, 256, 6, 256, 512, 256},  // BF16
   };
   return d[storage >= kF64 && storage <= kBF16 ? storage : 0];
 }
 
-PlanOptions plan_options(const hmvm_options *o) {
+// len: elements of the matrix (or of all matrices of a batch).  Default tile
+// widths shrink for small matrices so that a launch still has about one tile
+// per resident wavefront slot of the device (4096 on MI355X): at N = 4,096
+// the default widths would give ~650 tiles, a tenth of the slots.
+PlanOptions plan_options(const hmvm_options *o, int64_t len) {
   hmvm_options d;
   if (!o) {
     hmvm_options_default(&d);
@@ -398,8 +402,9 @@ PlanOptions plan_options(const hmvm_options *o) {
   }
   PlanOptions po;
   po.storage = o->storage;
-  po.w1 = o->w1 > 0 ? o->w1 : launch_defaults(po.storage).w1;
-  po.w2 = o->w2 > 0 ? o->w2 : launch_defaults(po.storage).w2;
+  const int64_t wfit = std::max<int64_t>(1024, len / 4096);
+  po.w1 = o->w1 > 0 ? o->w1 : (int)std::min<int64_t>(launch_defaults(po.storage).w1, wfit);
+  po.w2 = o->w2 > 0 ? o->w2 : (int)std::min<int64_t>(launch_defaults(po.storage).w2, wfit);
   // an explicit max_rows sets both unless max_rows_b is given; by default
   // stage-2 tiles have their own
   po.max_rows = o->max_rows > 0 ? o->max_rows : launch_defaults(po.storage).max_rows;
@@ -638,7 +643,7 @@ int create_plan(Source &src, const hmvm_options *opt, hmvm_plan **out) {
   }
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return set_error(HMVM_ERR_NODEVICE, "no HIP device visible");
-  const PlanOptions po = plan_options(opt);
+  const PlanOptions po = plan_options(opt, src.la.src_len);
   if (po.storage < kF64 || po.storage > kBF16) return set_error(HMVM_ERR_ARG, "unknown storage %d", po.storage);
   std::unique_ptr<hmvm_plan> p(new hmvm_plan());
   p->storage = po.storage;
@@ -1435,7 +1440,7 @@ int hmvm_plan_check(const matrix2 *mat2, const hmvm_options *opt, hmvm_plan_info
   Source s;
   int rc = source_from_matrix2(mat2, kA2RowMajor, &s);
   if (rc) return rc;
-  const PlanOptions po = plan_options(opt);
+  const PlanOptions po = plan_options(opt, s.la.src_len);
   Plan P;
   std::string err = build_plan(s.la, po, &P);
   if (!err.empty()) return set_error(HMVM_ERR_MATRIX, "%s", err.c_str());
@@ -1450,7 +1455,7 @@ int hmvm_plan_check(const matrix2 *mat2, const hmvm_options *opt, hmvm_plan_info
 namespace {
 int export_schedule(const Source &s, const hmvm_options *opt, hmvm_schedule *out) {
   Plan P;
-  std::string err = build_plan(s.la, plan_options(opt), &P);
+  std::string err = build_plan(s.la, plan_options(opt, s.la.src_len), &P);
   if (!err.empty()) return set_error(HMVM_ERR_MATRIX, "%s", err.c_str());
   err = verify_plan(P, s.la);
   if (!err.empty()) return set_error(HMVM_ERR_MATRIX, "schedule check failed: %s", err.c_str());

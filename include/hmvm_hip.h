@@ -89,7 +89,8 @@ enum { HMVM_A2_ROWMAJOR = 0, HMVM_A2_TRANSPOSED = 1 };
 
 typedef struct hmvm_options {
   int storage;           /* HMVM_STORAGE_*: element type kept in HBM            */
-  int w1;                /* target elements per stage-1 tile (0: by storage)    */
+  int w1;                /* target elements per stage-1 tile (0: by storage,    */
+                         /* capped at max(1024, len / 4096) for small matrices) */
   int w2;                /* target elements per dense/stage-2 tile (0: ditto)   */
   int block_threads;     /* threads per workgroup, 64..256 step 64 (0: by       */
                          /* storage, per launch)                                */
