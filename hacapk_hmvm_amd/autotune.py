@@ -110,7 +110,9 @@ def collect(h, storage, variants=None, w_scales=((1.0, 1.0),), iters=2, max_tile
     variants = variants or [(k, bt) for k in KERNELS for bt in BLOCK_THREADS]
     Xs, ys, recs = [], [], []
     for a, b in w_scales:
-        w1, w2 = widths(storage, a, b, _length(h))
+        # training tiles use the fixed base widths whatever the matrix size,
+        # so that small matrices still contribute full-size tiles
+        w1, w2 = widths(storage, a, b)
         p = Plan(h, storage=storage, w1=w1, w2=w2)
         for k, bt in variants:
             p.set_launch(k, bt)

@@ -710,61 +710,65 @@ __device__ __forceinline__ short bf16_bits(float f) {
   return (short)(uint16_t)((b + 0x7FFFu + ((b >> 16) & 1u)) >> 16);
 }
 
-// Vector entry (col, rhs) of a panel in the multi-RHS layout.
-__device__ __forceinline__ double vec_entry_multi(const PanelDesc &pd, const double *__restrict__ X,
-                                                  const double *__restrict__ T, const double *__restrict__ TP,
-                                                  int col, int rhs, int nrhs) {
-  if (pd.vec_sel == kVecX) return X[(int64_t)(pd.vec_off + col) * nrhs + rhs];
-  if (pd.vec_sel == kVecT) return T[(int64_t)(pd.vec_off + col) * nrhs + rhs];
-  const int nch = pd.vec_chunks >> 16, stride = pd.vec_chunks & 0xFFFF;
-  double s = 0;
-  for (int c = 0; c < nch; ++c) s += TP[(int64_t)(pd.vec_off + c * stride + col) * nrhs + rhs];
-  return s;
+// Vector entry (col, rhs) of a panel in the multi-RHS layout, zero past the
+// panel's last column.  The load address is clamped into the panel so that
+// the load is issued unconditionally: a K-step's loads then sit in one basic
+// block with the rest of the batch and are all in flight before the first
+// MFMA (a branch per load makes the compiler wait for each one).  X and T
+// panels read `src` (the caller's choice of X or T); kTP panels sum stage-1
+// partial slots.
+template <bool kTP>
+__device__ __forceinline__ double vec_entry_multi(const PanelDesc &pd, const double *__restrict__ src,
+                                                  const double *__restrict__ TP, int col, int rhs, int nrhs) {
+  const int c = col < pd.ncols ? col : pd.ncols - 1;
+  double v;
+  if constexpr (kTP) {
+    const int nch = pd.vec_chunks >> 16, stride = pd.vec_chunks & 0xFFFF;
+    v = 0;
+    for (int q = 0; q < nch; ++q) v += TP[(int64_t)(pd.vec_off + q * stride + c) * nrhs + rhs];
+  } else {
+    v = src[(int64_t)(pd.vec_off + c) * nrhs + rhs];
+  }
+  return col < pd.ncols ? v : 0.0;
 }
 
 // Vector operand of rhs block cb at K-step k0 (times the panel scale): FP64
 // and FP32 one entry (column k0 + l/16), BF16 four (columns k0 + 4(l/16) + q).
-template <typename S>
-__device__ __forceinline__ typename Mfma<S>::Op b_operand(const PanelDesc &pd, const double *__restrict__ X,
-                                                          const double *__restrict__ T, const double *__restrict__ TP,
-                                                          int rhs, int nrhs, int k0, int lane, float scale) {
+template <typename S, bool kTP>
+__device__ __forceinline__ typename Mfma<S>::Op b_operand(const PanelDesc &pd, const double *__restrict__ src,
+                                                          const double *__restrict__ TP, int rhs, int nrhs, int k0,
+                                                          int lane, float scale) {
   if constexpr (sizeof(S) == 2) {
     double v[4];
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const int col = k0 + 4 * (lane >> 4) + q;
-      v[q] = col < pd.ncols ? vec_entry_multi(pd, X, T, TP, col, rhs, nrhs) : 0.0;
-    }
+    for (int q = 0; q < 4; ++q) v[q] = vec_entry_multi<kTP>(pd, src, TP, k0 + 4 * (lane >> 4) + q, rhs, nrhs);
     mf_s4 b;
 #pragma unroll
     for (int q = 0; q < 4; ++q) b[q] = bf16_bits((float)v[q] * scale);
     return b;
   } else {
-    const int col = k0 + (lane >> 4);
-    const double v = col < pd.ncols ? vec_entry_multi(pd, X, T, TP, col, rhs, nrhs) : 0.0;
-    return (S)v;
+    return (S)vec_entry_multi<kTP>(pd, src, TP, k0 + (lane >> 4), rhs, nrhs);
   }
 }
 
 // Matrix words of one K-step for one chunk: FP64/FP32 one 16-byte word
-// (column k0 + l/16), BF16 four (columns k0 + 4(l/16) + q); zero where the
-// lane group or the column is outside the tile.
+// (column k0 + l/16), BF16 four (columns k0 + 4(l/16) + q).  Columns past the
+// panel's last one read its last column (their vector entries are zero), so
+// every load stays inside the panel and none is conditional.
 template <typename S>
 struct MultiWords {
   static constexpr int NW = sizeof(S) == 2 ? 4 : 1;
   typename Wide<S>::T w[NW];
 };
 template <typename S>
-__device__ __forceinline__ void load_words(MultiWords<S> &mw, const S *__restrict__ base, const PanelDesc &pd,
-                                           bool lane_live, int k0, int lane) {
+__device__ __forceinline__ void load_words(MultiWords<S> &mw, const S *__restrict__ base, const PanelDesc &pd, int k0,
+                                           int lane) {
   constexpr int NW = MultiWords<S>::NW;
 #pragma unroll
   for (int q = 0; q < NW; ++q) {
     const int col = NW == 1 ? k0 + (lane >> 4) : k0 + 4 * (lane >> 4) + q;
-    if (lane_live && col < pd.ncols)
-      mw.w[q] = load_wide<S, false>(base + (int64_t)col * pd.ld);
-    else
-      mw.w[q] = typename Wide<S>::T{};
+    const int c = col < pd.ncols ? col : pd.ncols - 1;
+    mw.w[q] = load_wide<S, false>(base + (int64_t)c * pd.ld);
   }
 }
 // A operand of MFMA m (row V*i+m of the chunk as row i) from a chunk's words.
@@ -784,56 +788,113 @@ __device__ __forceinline__ typename Mfma<S>::Op a_of(const MultiWords<S> &mw, in
 }
 
 // Chunks per pass and K-steps per batch, measured at N=101,250 for 16, 32
-// and 64 right-hand sides: two chunks when the accumulators fit 32 VGPRs
-// (one for BF16 with 16 right-hand sides); FP64 batches 8 K-steps for 16
-// right-hand sides and 4 otherwise; FP32 batches 2 (its occupancy drops to
-// two waves per SIMD with 4); BF16 batches 2 (32 columns).
+// and 64 right-hand sides (K-steps halved and doubled): two chunks when the
+// accumulators fit 32 VGPRs (one for BF16 with 16 right-hand sides); FP64
+// batches 8 K-steps for 16 right-hand sides and 4 otherwise; FP32 4 for 16
+// and 32 right-hand sides, 2 for 64; BF16 1 (16 columns) for 16 and 32, 2
+// for 64.
 template <typename S, int NB>
 struct MultiShape {
   static constexpr int V = Wide<S>::V;
   static constexpr int ACCW = sizeof(typename Mfma<S>::Acc) / 4;  // VGPRs per accumulator
   static constexpr int CH = (sizeof(S) == 2 && NB == 1) ? 1 : (V * NB * ACCW <= 32 ? 2 : 1);
-  static constexpr int KB = sizeof(S) == 8 ? (NB == 1 ? 8 : 4) : 2;
+  static constexpr int KB = sizeof(S) == 8   ? (NB == 1 ? 8 : 4)
+                            : sizeof(S) == 4 ? (NB <= 2 ? 4 : 2)
+                                             : (NB <= 2 ? 1 : 2);
 };
 
-// A operand of one K-step for a short tile: row `row` of the tile as MFMA
-// row l%16 (FP64/FP32 one element of column k0 + l/16, BF16 four of columns
-// k0 + 4(l/16) + q); zero outside the tile.
-template <typename S>
-__device__ __forceinline__ typename Mfma<S>::Op a_narrow(const S *__restrict__ vals, const PanelDesc &pd, int row,
-                                                         int nrows, int k0, int lane) {
-  if constexpr (sizeof(S) == 2) {
-    mf_s4 a;
+// One panel of one pass: nl (wave-uniform) live chunks; chunks past the
+// tile's last row run no loads and no MFMAs.  A batch of KB K-steps issues
+// all its loads, then its MFMAs.
+template <typename S, int NB, bool kTP, int CH>
+__device__ __forceinline__ void multi_panel(const PanelDesc &pd, const S *const *base, int nl,
+                                            const double *__restrict__ src, const double *__restrict__ TP,
+                                            float scale, int lane,
+                                            typename Mfma<S>::Acc (&acc)[CH][Wide<S>::V][NB]) {
+  using M = Mfma<S>;
+  constexpr int V = Wide<S>::V, K = M::K, KB = MultiShape<S, NB>::KB, nrhs = 16 * NB;
+  for (int k0 = 0; k0 < pd.ncols; k0 += KB * K) {
+    MultiWords<S> a[KB][CH];
+    typename M::Op b[KB][NB];
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const int col = k0 + 4 * (lane >> 4) + q;
-      a[q] = row < nrows && col < pd.ncols ? (short)vals[pd.val_off + row + (int64_t)col * pd.ld] : (short)0;
+    for (int kb = 0; kb < KB; ++kb) {
+#pragma unroll
+      for (int ch = 0; ch < CH; ++ch)
+        if (ch < nl) load_words<S>(a[kb][ch], base[ch], pd, k0 + kb * K, lane);
+#pragma unroll
+      for (int cb = 0; cb < NB; ++cb)
+        b[kb][cb] = b_operand<S, kTP>(pd, src, TP, 16 * cb + (lane & 15), nrhs, k0 + kb * K, lane, scale);
     }
-    return a;
-  } else {
-    const int col = k0 + (lane >> 4);
-    return row < nrows && col < pd.ncols ? vals[pd.val_off + row + (int64_t)col * pd.ld] : (S)0;
+#pragma unroll
+    for (int kb = 0; kb < KB; ++kb)
+#pragma unroll
+      for (int ch = 0; ch < CH; ++ch)
+        if (ch < nl)
+#pragma unroll
+        for (int m = 0; m < V; ++m) {
+          const typename M::Op am = a_of<S>(a[kb][ch], m);
+#pragma unroll
+          for (int cb = 0; cb < NB; ++cb) acc[ch][m][cb] = mfma_step(am, b[kb][cb], acc[ch][m][cb]);
+        }
+  }
+}
+
+// A operand of one K-step for a short tile: row `row` of the tile as MFMA
+// row l%16, one element of column k0 + l/16 (FP32 only).  Rows past the tile
+// and columns past the panel read the tile's last row / the panel's last
+// column: such rows are not stored, and such columns meet zero vector entries.
+template <typename S>
+__device__ __forceinline__ S a_narrow(const S *__restrict__ vals, const PanelDesc &pd, int row, int nrows, int k0,
+                                      int lane) {
+  const int col = k0 + (lane >> 4);
+  const int c = col < pd.ncols ? col : pd.ncols - 1;
+  const int r = row < nrows ? row : nrows - 1;
+  return vals[pd.val_off + r + (int64_t)c * pd.ld];
+}
+
+template <typename S, int NB, int NL, bool kTP, int CH>
+__device__ __forceinline__ void multi_panel_narrow(const PanelDesc &pd, const S *__restrict__ vals, int nrows,
+                                                   const double *__restrict__ src, const double *__restrict__ TP,
+                                                   float scale, int lane, typename Mfma<S>::Acc (&acc)[CH][NB]) {
+  using M = Mfma<S>;
+  constexpr int K = M::K, KB = MultiShape<S, NB>::KB, nrhs = 16 * NB;
+  for (int k0 = 0; k0 < pd.ncols; k0 += KB * K) {
+    typename M::Op a[KB][NL], b[KB][NB];
+#pragma unroll
+    for (int kb = 0; kb < KB; ++kb) {
+#pragma unroll
+      for (int ch = 0; ch < NL; ++ch) a[kb][ch] = a_narrow<S>(vals, pd, 16 * ch + (lane & 15), nrows, k0 + kb * K, lane);
+#pragma unroll
+      for (int cb = 0; cb < NB; ++cb)
+        b[kb][cb] = b_operand<S, kTP>(pd, src, TP, 16 * cb + (lane & 15), nrhs, k0 + kb * K, lane, scale);
+    }
+#pragma unroll
+    for (int kb = 0; kb < KB; ++kb)
+#pragma unroll
+      for (int ch = 0; ch < NL; ++ch)
+#pragma unroll
+        for (int cb = 0; cb < NB; ++cb) acc[ch][cb] = mfma_step(a[kb][ch], b[kb][cb], acc[ch][cb]);
   }
 }
 
 // Short FP32 tiles (at most 32 rows): MFMA row i is tile row 16*ch + i, so a
 // tile of r rows runs ceil(r/16) MFMAs per K-step and right-hand-side block
 // instead of 4, with 4-byte element loads.  Measured at N=101,250 with 32
-// right-hand sides: 7% faster for FP32; slower for FP64 (8-byte loads, only
+// right-hand sides: faster for FP32; slower for FP64 (8-byte loads, only
 // two MFMAs saved) and BF16 (four strided 2-byte loads per lane), which keep
 // the 16-byte path.
-template <typename S, bool kScaled, int NB>
+template <typename S, bool kScaled, int NB, bool kHasTP>
 __device__ __forceinline__ void multi_tile_narrow(const TileDesc &td, const PanelDesc *__restrict__ panels,
                                                   const S *__restrict__ vals, const double *__restrict__ X,
                                                   const double *__restrict__ T, const double *__restrict__ TP,
                                                   const float *__restrict__ scales, double *__restrict__ out,
                                                   int lane) {
+  static_assert(sizeof(S) == 4, "the narrow path is FP32 only");
   using M = Mfma<S>;
   using Acc = typename M::Acc;
-  constexpr int V = Wide<S>::V, CH = V / 2 > 0 ? V / 2 : 1, K = M::K;
-  constexpr int KB = MultiShape<S, NB>::KB;
+  constexpr int CH = 2;  // 32 rows
   constexpr int nrhs = 16 * NB;
-  const int nlive = (td.nrows + 15) / 16 < CH ? (td.nrows + 15) / 16 : CH;
+  const int nlive = td.nrows > 16 ? 2 : 1;
   Acc acc[CH][NB];
 #pragma unroll
   for (int ch = 0; ch < CH; ++ch)
@@ -843,24 +904,13 @@ __device__ __forceinline__ void multi_tile_narrow(const TileDesc &td, const Pane
     const PanelDesc pd = panels[td.panel0 + p];
     float scale = 1.0f;
     if constexpr (kScaled) scale = panel_scale(pd, scales);
-    for (int k0 = 0; k0 < pd.ncols; k0 += KB * K) {
-      typename M::Op a[KB][CH], b[KB][NB];
-#pragma unroll
-      for (int kb = 0; kb < KB; ++kb) {
-#pragma unroll
-        for (int ch = 0; ch < CH; ++ch)
-          if (ch < nlive) a[kb][ch] = a_narrow<S>(vals, pd, 16 * ch + (lane & 15), td.nrows, k0 + kb * K, lane);
-#pragma unroll
-        for (int cb = 0; cb < NB; ++cb)
-          b[kb][cb] = b_operand<S>(pd, X, T, TP, 16 * cb + (lane & 15), nrhs, k0 + kb * K, lane, scale);
-      }
-#pragma unroll
-      for (int kb = 0; kb < KB; ++kb)
-#pragma unroll
-        for (int ch = 0; ch < CH; ++ch)
-          if (ch < nlive)
-#pragma unroll
-            for (int cb = 0; cb < NB; ++cb) acc[ch][cb] = mfma_step(a[kb][ch], b[kb][cb], acc[ch][cb]);
+    const double *src = pd.vec_sel == kVecX ? X : T;
+    if (kHasTP && pd.vec_sel == kVecTP) {
+      if (nlive == 2) multi_panel_narrow<S, NB, 2, kHasTP>(pd, vals, td.nrows, src, TP, scale, lane, acc);
+      else multi_panel_narrow<S, NB, 1, kHasTP>(pd, vals, td.nrows, src, TP, scale, lane, acc);
+    } else {
+      if (nlive == 2) multi_panel_narrow<S, NB, 2, false>(pd, vals, td.nrows, src, TP, scale, lane, acc);
+      else multi_panel_narrow<S, NB, 1, false>(pd, vals, td.nrows, src, TP, scale, lane, acc);
     }
   }
 #pragma unroll
@@ -876,8 +926,9 @@ __device__ __forceinline__ void multi_tile_narrow(const TileDesc &td, const Pane
         }
 }
 
-// nrhs = 16*NB; one wave per tile.
-template <typename S, bool kScaled, int NB>
+// nrhs = 16*NB; one wave per tile.  kHasTP: the plan has panels that sum
+// stage-1 partial slots (the loop for those is compiled only then).
+template <typename S, bool kScaled, int NB, bool kHasTP>
 __global__ __launch_bounds__(256) void tile_multi_kernel(const TileDesc *__restrict__ tiles, int ntiles,
                                                          const PanelDesc *__restrict__ panels,
                                                          const S *__restrict__ vals, const double *__restrict__ X,
@@ -888,7 +939,7 @@ __global__ __launch_bounds__(256) void tile_multi_kernel(const TileDesc *__restr
   using M = Mfma<S>;
   using Acc = typename M::Acc;
   using Sh = MultiShape<S, NB>;
-  constexpr int V = Sh::V, CH = Sh::CH, KB = Sh::KB, K = M::K;
+  constexpr int V = Sh::V, CH = Sh::CH;
   constexpr int nrhs = 16 * NB;
   const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
   const int lane = (int)(threadIdx.x & 63);
@@ -897,7 +948,7 @@ __global__ __launch_bounds__(256) void tile_multi_kernel(const TileDesc *__restr
   const TileDesc td = tiles[tid];
   if constexpr (sizeof(S) == 4) {
     if (td.nrows <= 8 * V) {
-      multi_tile_narrow<S, kScaled, NB>(td, panels, vals, X, T, TP, scales, out, lane);
+      multi_tile_narrow<S, kScaled, NB, kHasTP>(td, panels, vals, X, T, TP, scales, out, lane);
       return;
     }
   }
@@ -911,44 +962,23 @@ __global__ __launch_bounds__(256) void tile_multi_kernel(const TileDesc *__restr
       for (int m = 0; m < V; ++m)
 #pragma unroll
         for (int cb = 0; cb < NB; ++cb) acc[ch][m][cb] = Acc{0, 0, 0, 0};
-    // chunks of this pass that hold rows of the tile (wave-uniform): a short
-    // tile runs no MFMAs for chunks past its last row
+    // chunks of this pass that hold rows of the tile (wave-uniform)
     const int nlive = (lpc - g0 + 15) / 16 < CH ? (lpc - g0 + 15) / 16 : CH;
-    bool live[CH];
+    int grp[CH];  // lane group each lane loads; lanes past the tile load group 0
 #pragma unroll
-    for (int ch = 0; ch < CH; ++ch) live[ch] = g0 + 16 * ch + (lane & 15) < lpc;
+    for (int ch = 0; ch < CH; ++ch) grp[ch] = g0 + 16 * ch + (lane & 15) < lpc ? g0 + 16 * ch + (lane & 15) : 0;
     for (int p = 0; p < td.npanel; ++p) {
       const PanelDesc pd = panels[td.panel0 + p];
       float scale = 1.0f;
       if constexpr (kScaled) scale = panel_scale(pd, scales);
-      const S *__restrict__ base[CH];
+      const S *base[CH];
 #pragma unroll
-      for (int ch = 0; ch < CH; ++ch)
-        base[ch] = vals + (pd.val_off - shift) + (int64_t)V * (live[ch] ? g0 + 16 * ch + (lane & 15) : 0);
-      for (int k0 = 0; k0 < pd.ncols; k0 += KB * K) {
-        MultiWords<S> a[KB][CH];
-        typename M::Op b[KB][NB];
-#pragma unroll
-        for (int kb = 0; kb < KB; ++kb) {
-#pragma unroll
-          for (int ch = 0; ch < CH; ++ch)
-            if (ch < nlive) load_words<S>(a[kb][ch], base[ch], pd, live[ch], k0 + kb * K, lane);
-#pragma unroll
-          for (int cb = 0; cb < NB; ++cb)
-            b[kb][cb] = b_operand<S>(pd, X, T, TP, 16 * cb + (lane & 15), nrhs, k0 + kb * K, lane, scale);
-        }
-#pragma unroll
-        for (int kb = 0; kb < KB; ++kb)
-#pragma unroll
-          for (int ch = 0; ch < CH; ++ch)
-            if (ch < nlive)
-#pragma unroll
-            for (int m = 0; m < V; ++m) {
-              const typename M::Op am = a_of<S>(a[kb][ch], m);
-#pragma unroll
-              for (int cb = 0; cb < NB; ++cb) acc[ch][m][cb] = mfma_step(am, b[kb][cb], acc[ch][m][cb]);
-            }
-      }
+      for (int ch = 0; ch < CH; ++ch) base[ch] = vals + (pd.val_off - shift) + (int64_t)V * grp[ch];
+      const double *src = pd.vec_sel == kVecX ? X : T;
+      if (kHasTP && pd.vec_sel == kVecTP)
+        multi_panel<S, NB, kHasTP, CH>(pd, base, nlive, src, TP, scale, lane, acc);
+      else
+        multi_panel<S, NB, false, CH>(pd, base, nlive, src, TP, scale, lane, acc);
     }
 #pragma unroll
     for (int ch = 0; ch < CH; ++ch)

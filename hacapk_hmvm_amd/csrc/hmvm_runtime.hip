@@ -117,6 +117,7 @@ struct hmvm_plan {
   hipStream_t stream2 = nullptr;       // dense tiles, concurrent with stage 1 / t sum / stage 2
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   bool a_mixed = false;  // launch A order mixes stage-1 and dense tiles (no persistent mode)
+  bool has_tp = false;   // some panel sums stage-1 partial slots (kVecTP)
   int persist_key = -1, persist_blocks = 0;  // occupancy cache (kernel variant, block size)
   // several right-hand sides: buffers sized for multi_cap columns
   int multi_cap = 0;
@@ -671,6 +672,7 @@ int create_plan(Source &src, const hmvm_options *opt, hmvm_plan **out) {
   err = verify_plan(p->plan, src.la);
   if (!err.empty()) return set_error(HMVM_ERR_MATRIX, "schedule check failed: %s", err.c_str());
   Plan &P = p->plan;
+  for (const PanelDesc &pd : P.panels) p->has_tp = p->has_tp || pd.vec_sel == kVecTP;
   HIP_TRY(hipGetDevice(&p->device));
   HIP_TRY(hipStreamCreate(&p->stream));
   HIP_TRY(hipStreamCreate(&p->stream2));
@@ -885,9 +887,14 @@ template <typename S, bool kScaled, int NB>
 void launch_multi_tiles(hmvm_plan *p, const TileDesc *d_tiles, int ntiles, const double *X, hipStream_t st) {
   if (ntiles == 0) return;
   const int wpb = p->block_threads / 64;
-  hipLaunchKernelGGL((tile_multi_kernel<S, kScaled, NB>), dim3((ntiles + wpb - 1) / wpb), dim3(p->block_threads), 0,
-                     st, d_tiles, ntiles, p->d_panels, (const S *)p->d_vals, X, p->d_mt, p->d_mslots,
-                     (const float *)p->d_scales, p->d_mslots);
+  if (p->has_tp)
+    hipLaunchKernelGGL((tile_multi_kernel<S, kScaled, NB, true>), dim3((ntiles + wpb - 1) / wpb),
+                       dim3(p->block_threads), 0, st, d_tiles, ntiles, p->d_panels, (const S *)p->d_vals, X, p->d_mt,
+                       p->d_mslots, (const float *)p->d_scales, p->d_mslots);
+  else
+    hipLaunchKernelGGL((tile_multi_kernel<S, kScaled, NB, false>), dim3((ntiles + wpb - 1) / wpb),
+                       dim3(p->block_threads), 0, st, d_tiles, ntiles, p->d_panels, (const S *)p->d_vals, X, p->d_mt,
+                       p->d_mslots, (const float *)p->d_scales, p->d_mslots);
 }
 
 template <typename S, bool kScaled>

@@ -65,3 +65,17 @@ def test_multi_rhs_rank_rows_straddle(gpu, H, O, tmp_path):
     for c in (0, 5, 15):
         assert rel(got[:, c], O.seq_2(h, np.ascontiguousarray(X[:, c]))) <= 1e-13
     p.close()
+
+
+@pytest.mark.parametrize("storage,tol", [("f64", 1e-13), ("f32", 2e-6), ("bf16", 2e-2)])
+@pytest.mark.parametrize("t_mode", [0, 2])
+def test_multi_rhs_partial_slot_panels(gpu, H, hm, XY, storage, tol, t_mode):
+    """Stage-2 panels that sum stage-1 partial slots (t_mode 0 and 2) take the
+    kernel instantiation with the summing loop."""
+    X, Y = XY
+    p = H.Plan(hm, storage=storage, t_mode=t_mode)
+    got = p.apply_multi_host(X)
+    assert max(rel(got[:, c], Y[:, c]) for c in range(32)) <= tol
+    got16 = p.apply_multi_host(np.ascontiguousarray(X[:, :16]))
+    assert max(rel(got16[:, c], Y[:, c]) for c in range(16)) <= tol
+    p.close()
