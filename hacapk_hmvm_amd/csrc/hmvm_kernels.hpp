@@ -18,6 +18,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 #include "hmvm_plan.hpp"
 
 namespace hmvm {
@@ -998,33 +1000,54 @@ __global__ __launch_bounds__(256) void tile_multi_kernel(const TileDesc *__restr
 }
 
 // out[i*nrhs+c] = (accumulate ? out : 0) + scale(i) * sum_e buf[(list[e]+i)*nrhs + c],
-// e over entry i's list range (as segsum_kernel: up to 24 in flight)
-template <bool kScaled>
+// e over entry i's list range (as segsum_kernel: up to 24 in flight).  A
+// thread sums W consecutive right-hand sides (W = 2: 16-byte loads and
+// stores, half the waves; buf and out 16-byte aligned, nrhs even).
+template <bool kScaled, int W>
 __global__ __launch_bounds__(256) void segsum_multi_kernel(int n, int nrhs, const int2 *__restrict__ rng,
                                                            const int64_t *__restrict__ list,
                                                            const double *__restrict__ buf,
                                                            const int32_t *__restrict__ factor_of,
                                                            const float *__restrict__ scales,
                                                            double *__restrict__ out, int accumulate) {
+  static_assert(W == 1 || W == 2, "W is 1 or 2");
+  using VT = typename std::conditional<W == 2, double2, double>::type;
+  const int nw = nrhs / W;
   const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= (int64_t)n * nrhs) return;
-  const int i = (int)(g / nrhs), c = (int)(g - (int64_t)i * nrhs);
+  if (g >= (int64_t)n * nw) return;
+  const int i = (int)(g / nw), c = (int)(g - (int64_t)i * nw) * W;
   const int2 r = rng[i];
   const int e1 = r.y;
-  double a = 0;
+  double a[W];
+#pragma unroll
+  for (int w = 0; w < W; ++w) a[w] = 0;
   for (int e = r.x; e < e1; e += 24) {
     int64_t l[24];
-    double v[24];
+    VT v[24];
 #pragma unroll
     for (int u = 0; u < 24; ++u) l[u] = e + u < e1 ? list[e + u] : 0;
 #pragma unroll
-    for (int u = 0; u < 24; ++u) v[u] = e + u < e1 ? buf[(l[u] + i) * nrhs + c] : 0.0;
+    for (int u = 0; u < 24; ++u)
+      v[u] = e + u < e1 ? *reinterpret_cast<const VT *>(buf + (l[u] + i) * nrhs + c) : VT{};
 #pragma unroll
     for (int u = 0; u < 24; ++u)
-      if (e + u < e1) a += v[u];
+      if (e + u < e1) {
+        if constexpr (W == 2) {
+          a[0] += v[u].x;
+          a[1] += v[u].y;
+        } else {
+          a[0] += v[u];
+        }
+      }
   }
-  if (kScaled) a *= (double)scales[factor_of[i]] * (double)scales[factor_of[i] + 1];
-  out[g] = accumulate ? out[g] + a : a;
+  double f = 1.0;
+  if (kScaled) f = (double)scales[factor_of[i]] * (double)scales[factor_of[i] + 1];
+  double *o = out + (int64_t)i * nrhs + c;
+#pragma unroll
+  for (int w = 0; w < W; ++w) {
+    if (kScaled) a[w] *= f;
+    o[w] = accumulate ? o[w] + a[w] : a[w];
+  }
 }
 
 // Achievable-bandwidth probes: a read-only stream (sum) and a copy, each

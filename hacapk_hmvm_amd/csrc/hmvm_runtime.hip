@@ -921,20 +921,30 @@ void launch_multi_phase(hmvm_plan *p, int phase, const double *X, double *Y, int
     else
       launch_multi_nb<uint16_t, true>(p, tl, nt, X, nrhs, st);
   } else if (phase == 1) {
+    // two columns per thread (measured 15-20% faster than one at N=101,250);
+    // slots and t are the plan's own allocations, nrhs is even
     if (P.t_fused || P.tsum.n == 0) return;
-    const int64_t tot = (int64_t)P.tsum.n * nrhs;
+    const int64_t tot = (int64_t)P.tsum.n * (nrhs / 2);
     if (bf)
-      hipLaunchKernelGGL(segsum_multi_kernel<true>, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, P.tsum.n,
-                         nrhs, p->d_trng, p->d_tlist, p->d_mslots, p->d_tfac, p->d_scales, p->d_mt, 0);
+      hipLaunchKernelGGL((segsum_multi_kernel<true, 2>), dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st,
+                         P.tsum.n, nrhs, p->d_trng, p->d_tlist, p->d_mslots, p->d_tfac, p->d_scales, p->d_mt, 0);
     else
-      hipLaunchKernelGGL(segsum_multi_kernel<false>, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, P.tsum.n,
-                         nrhs, p->d_trng, p->d_tlist, p->d_mslots, (const int32_t *)nullptr, (const float *)nullptr,
-                         p->d_mt, 0);
+      hipLaunchKernelGGL((segsum_multi_kernel<false, 2>), dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st,
+                         P.tsum.n, nrhs, p->d_trng, p->d_tlist, p->d_mslots, (const int32_t *)nullptr,
+                         (const float *)nullptr, p->d_mt, 0);
   } else {
-    const int64_t tot = (int64_t)P.ysum.n * nrhs;
-    hipLaunchKernelGGL(segsum_multi_kernel<false>, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, P.ysum.n,
-                       nrhs, p->d_yrng, p->d_ylist, p->d_mslots, (const int32_t *)nullptr, (const float *)nullptr,
-                       Y, accumulate);
+    // two columns per thread where the caller's Y is 16-byte aligned
+    if (((uintptr_t)Y & 15) == 0) {
+      const int64_t tot = (int64_t)P.ysum.n * (nrhs / 2);
+      hipLaunchKernelGGL((segsum_multi_kernel<false, 2>), dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st,
+                         P.ysum.n, nrhs, p->d_yrng, p->d_ylist, p->d_mslots, (const int32_t *)nullptr,
+                         (const float *)nullptr, Y, accumulate);
+    } else {
+      const int64_t tot = (int64_t)P.ysum.n * nrhs;
+      hipLaunchKernelGGL((segsum_multi_kernel<false, 1>), dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st,
+                         P.ysum.n, nrhs, p->d_yrng, p->d_ylist, p->d_mslots, (const int32_t *)nullptr,
+                         (const float *)nullptr, Y, accumulate);
+    }
   }
 }
 
