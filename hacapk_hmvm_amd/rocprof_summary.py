@@ -5,7 +5,8 @@ profiles/ and the traffic record bench.py reads.
     python -m hacapk_hmvm_amd.rocprof_summary --trace DIR [--fetch DIR --write DIR]
         --grid-a THREADS --grid-b THREADS [--md OUT.md] [--json OUT.json] [--cmd "..."]
 
-The two tile launches of one HMVM run the same kernel; they are told apart by
+The two tile launches of one HMVM run the same kernel (tile_kernel, or
+tile_multi_kernel for several right-hand sides); they are told apart by
 their grid size (threads = 256 * ceil(tiles / 4) with 256-thread workgroups).
 Counter values are KiB per dispatch; they are scaled by what the same counter
 reads for the bandwidth-probe kernels of bench.py, which read (probe_read) and
@@ -44,7 +45,7 @@ def trace_table(rows):
     by = {}
     for r in rows:
         k = short(r["Kernel_Name"])
-        key = (k, int(r["Grid_Size_X"]) if k.startswith("tile_kernel") else 0)
+        key = (k, int(r["Grid_Size_X"]) if k.startswith(("tile_kernel", "tile_multi_kernel")) else 0)
         by.setdefault(key, []).append((int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3)
     return by
 
