@@ -790,16 +790,19 @@ __device__ __forceinline__ typename Mfma<S>::Op a_of(const MultiWords<S> &mw, in
 }
 
 // Chunks per pass and K-steps per batch, measured at N=101,250 for 16, 32
-// and 64 right-hand sides (K-steps halved and doubled): two chunks when the
-// accumulators fit 32 VGPRs (one for BF16 with 16 right-hand sides); FP64
-// batches 8 K-steps for 16 right-hand sides and 4 otherwise; FP32 4 for 16
-// and 32 right-hand sides, 2 for 64; BF16 1 (16 columns) for 16 and 32, 2
-// for 64.
+// and 64 right-hand sides (chunks and K-steps halved and doubled): FP64 four
+// chunks up to 32 right-hand sides and two for 64 (fewer passes, fewer
+// re-reads of X); FP32 and BF16 two chunks when the accumulators fit 32
+// VGPRs (one for BF16 with 16 right-hand sides); FP64 batches 8 K-steps for
+// 16 right-hand sides and 4 otherwise; FP32 4 for 16 and 32 right-hand
+// sides, 2 for 64; BF16 1 (16 columns) for 16 and 32, 2 for 64.
 template <typename S, int NB>
 struct MultiShape {
   static constexpr int V = Wide<S>::V;
   static constexpr int ACCW = sizeof(typename Mfma<S>::Acc) / 4;  // VGPRs per accumulator
-  static constexpr int CH = (sizeof(S) == 2 && NB == 1) ? 1 : (V * NB * ACCW <= 32 ? 2 : 1);
+  static constexpr int CH = sizeof(S) == 8                ? (NB <= 2 ? 4 : 2)
+                            : (sizeof(S) == 2 && NB == 1) ? 1
+                                                          : (V * NB * ACCW <= 32 ? 2 : 1);
   static constexpr int KB = sizeof(S) == 8   ? (NB == 1 ? 8 : 4)
                             : sizeof(S) == 4 ? (NB <= 2 ? 4 : 2)
                                              : (NB <= 2 ? 1 : 2);
