@@ -186,7 +186,8 @@ int hmvm_plan_reset_launch(hmvm_plan *plan);
  * in a tile, FP64 across tiles), BF16 v_mfma_f32_16x16x16_bf16 with the
  * vector entries rounded to BF16.  hmvm_plan_reserve_multi allocates the
  * per-plan work buffers for up to nrhs columns (the apply itself allocates
- * nothing); the host form reserves by itself. */
+ * nothing); the host form reserves by itself.  d_X and d_Y need 8-byte
+ * alignment; a 16-byte-aligned d_Y gets 16-byte stores (same values). */
 int hmvm_plan_reserve_multi(hmvm_plan *plan, int nrhs);
 int hmvm_plan_apply_multi(hmvm_plan *plan, const double *d_X, double *d_Y, int nrhs, int accumulate, void *stream);
 int hmvm_plan_apply_multi_host(hmvm_plan *plan, const double *X, double *Y, int nrhs, int accumulate);
