@@ -389,6 +389,28 @@ std::string build_plan(const LeafArrays &la, const PlanOptions &opt, Plan *plan)
   P.n_stage1_tiles = (int)t1.size();
   P.tiles_a = t1;
   P.tiles_a.insert(P.tiles_a.end(), td.begin(), td.end());
+  P.a_stage1.assign(P.tiles_a.size(), 0);
+  std::fill(P.a_stage1.begin(), P.a_stage1.begin() + t1.size(), (uint8_t)1);
+  if (opt.order == 1) {
+    // launch A: stage-1 and dense tiles in one largest-first order (measured
+    // 1.9% faster for FP64 at N=101,250 than all stage-1 tiles first, whose
+    // smallest tiles then ran before the largest dense ones)
+    std::vector<int> ord(P.tiles_a.size());
+    for (size_t i = 0; i < ord.size(); i++) ord[i] = (int)i;
+    std::vector<int64_t> w(P.tiles_a.size());
+    for (size_t i = 0; i < w.size(); i++) w[i] = tile_work(P, P.tiles_a[i]);
+    std::sort(ord.begin(), ord.end(), [&](int a, int b) {
+      return w[a] != w[b] ? w[a] > w[b] : P.tiles_a[a].panel0 < P.tiles_a[b].panel0;
+    });
+    std::vector<TileDesc> ta(ord.size());
+    std::vector<uint8_t> s1(ord.size());
+    for (size_t i = 0; i < ord.size(); i++) {
+      ta[i] = P.tiles_a[ord[i]];
+      s1[i] = P.a_stage1[ord[i]];
+    }
+    P.tiles_a.swap(ta);
+    P.a_stage1.swap(s1);
+  }
   P.tiles_b = t2;
 
   // ---- y segment sum: strips of rows covered by the same set of slots ----------

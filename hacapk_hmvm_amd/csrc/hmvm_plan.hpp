@@ -124,8 +124,9 @@ struct Plan {
   bool t_fused = true;               // no stage-2 panel reads the materialised t (no t sum launch)
   std::vector<RepackDesc> repack;
   std::vector<int32_t> factor_of_trow;  // t index -> a1 factor id (BF16 t scaling)
-  std::vector<TileDesc> tiles_a, tiles_b;  // tiles_a: stage-1 tiles first, then dense
-  int n_stage1_tiles = 0;
+  std::vector<TileDesc> tiles_a, tiles_b;  // tiles_a: stage-1 and dense tiles
+  int n_stage1_tiles = 0;            // stage-1 tiles in tiles_a
+  std::vector<uint8_t> a_stage1;     // per tile of tiles_a (dispatch order): 1 stage-1, 0 dense
   std::vector<PanelDesc> panels;
   // slot buffer: stage-1 partial slots [0, n_tp), then the dense and stage-2
   // output slots [n_tp, n_tp + n_z)

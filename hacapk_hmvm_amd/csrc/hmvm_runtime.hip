@@ -116,7 +116,7 @@ struct hmvm_plan {
   int mode = 0;    // 0: launches on one stream; 1: one persistent launch; 2: dense tiles on a second stream
   hipStream_t stream2 = nullptr;       // dense tiles, concurrent with stage 1 / t sum / stage 2
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
-  bool a_mixed = false;  // launch A order mixes stage-1 and dense tiles (no persistent mode)
+  bool a_mixed = false;  // launch A order is not "stage-1 tiles, then dense" (modes 1 and 2 need that)
   bool has_tp = false;   // some panel sums stage-1 partial slots (kVecTP)
   int persist_key = -1, persist_blocks = 0;  // occupancy cache (kernel variant, block size)
   // several right-hand sides: buffers sized for multi_cap columns
@@ -635,6 +635,39 @@ int upload_values(hmvm_plan *p, const Source &src) {
   return rc;
 }
 
+// Whether launch A's order puts a dense tile before a stage-1 tile.
+bool order_mixed(const Plan &P) {
+  for (int i = 0; i < P.n_stage1_tiles; i++)
+    if (!P.a_stage1[i]) return true;
+  return false;
+}
+
+// Launch A as "stage-1 tiles, then dense", each part in its current order:
+// the layout the persistent and two-stream modes rely on.
+int partition_a(hmvm_plan *p) {
+  Plan &P = p->plan;
+  if (!order_mixed(P)) {
+    p->a_mixed = false;
+    return HMVM_OK;
+  }
+  std::vector<TileDesc> nt;
+  std::vector<uint8_t> ns;
+  nt.reserve(P.tiles_a.size());
+  ns.reserve(P.tiles_a.size());
+  for (int part = 1; part >= 0; part--)
+    for (size_t i = 0; i < P.tiles_a.size(); i++)
+      if (P.a_stage1[i] == part) {
+        nt.push_back(P.tiles_a[i]);
+        ns.push_back((uint8_t)part);
+      }
+  HIP_TRY(hipStreamSynchronize(p->stream));
+  HIP_TRY(hipMemcpy(p->d_tiles_a, nt.data(), sizeof(TileDesc) * nt.size(), hipMemcpyHostToDevice));
+  P.tiles_a.swap(nt);
+  P.a_stage1.swap(ns);
+  p->a_mixed = false;
+  return HMVM_OK;
+}
+
 int create_plan(Source &src, const hmvm_options *opt, hmvm_plan **out) {
   *out = nullptr;
   hmvm_options dflt;
@@ -673,6 +706,7 @@ int create_plan(Source &src, const hmvm_options *opt, hmvm_plan **out) {
   if (!err.empty()) return set_error(HMVM_ERR_MATRIX, "schedule check failed: %s", err.c_str());
   Plan &P = p->plan;
   for (const PanelDesc &pd : P.panels) p->has_tp = p->has_tp || pd.vec_sel == kVecTP;
+  p->a_mixed = order_mixed(P);
   HIP_TRY(hipGetDevice(&p->device));
   HIP_TRY(hipStreamCreate(&p->stream));
   HIP_TRY(hipStreamCreate(&p->stream2));
@@ -695,6 +729,7 @@ int create_plan(Source &src, const hmvm_options *opt, hmvm_plan **out) {
       (rc = upload(&p->d_ylist, P.ysum.list)) || (rc = upload(&p->d_trng, seg_ranges(P.tsum))) ||
       (rc = upload(&p->d_yrng, seg_ranges(P.ysum))))
     return rc;
+  if (p->mode != 0 && (rc = partition_a(p.get()))) return rc;
   if (p->storage == kBF16) {
     if ((rc = upload(&p->d_tfac, P.factor_of_trow))) return rc;
     if ((rc = dalloc(&p->d_scales, (size_t)P.nfactors))) return rc;
@@ -1388,6 +1423,10 @@ int hmvm_plan_time_multi(hmvm_plan *plan, int nrhs, int warmup, int iters, hmvm_
 int hmvm_plan_set_mode(hmvm_plan *plan, int mode) {
   if (!plan) return set_error(HMVM_ERR_ARG, "NULL plan");
   if (mode < 0 || mode > 2) return set_error(HMVM_ERR_ARG, "mode must be 0, 1 or 2");
+  if (mode != 0) {
+    const int rc = partition_a(plan);
+    if (rc) return rc;
+  }
   plan->mode = mode;
   return HMVM_OK;
 }
@@ -1975,16 +2014,17 @@ int hmvm_plan_set_tile_order(hmvm_plan *plan, int which, const int *perm) {
     seen[perm[i]] = 1;
   }
   std::vector<TileDesc> nt((size_t)n);
-  bool mixed = false;
-  for (int i = 0; i < n; i++) {
-    nt[i] = tl[perm[i]];
-    if (which == 0 && (i < P.n_stage1_tiles) != (perm[i] < P.n_stage1_tiles)) mixed = true;
-  }
+  for (int i = 0; i < n; i++) nt[i] = tl[perm[i]];
   TileDesc *d = which == 0 ? plan->d_tiles_a : plan->d_tiles_b;
   HIP_TRY(hipStreamSynchronize(plan->stream));
   HIP_TRY(hipMemcpy(d, nt.data(), sizeof(TileDesc) * (size_t)n, hipMemcpyHostToDevice));
   tl.swap(nt);
-  if (which == 0) plan->a_mixed = plan->a_mixed || mixed;
+  if (which == 0) {
+    std::vector<uint8_t> ns((size_t)n);
+    for (int i = 0; i < n; i++) ns[i] = P.a_stage1[perm[i]];
+    P.a_stage1.swap(ns);
+    plan->a_mixed = order_mixed(P);
+  }
   return HMVM_OK;
 }
 

@@ -100,6 +100,7 @@ typedef struct hmvm_options {
                          /* bit1 software pipelining of wide tiles, bit2        */
                          /* non-temporal matrix loads                           */
   int order;             /* tile order in a launch: 0 memory, 1 largest first   */
+                         /* (launch A: stage-1 and dense tiles in one order)    */
   int mode;              /* 0: launches on one stream; 1: one persistent launch */
                          /* with a device work queue; 2: launches, dense tiles  */
                          /* on a second stream concurrent with stage 1/2        */
@@ -262,8 +263,9 @@ int hmvm_plan_time_graph(hmvm_plan *plan, int count, int warmup, int iters, hmvm
  * tile kernel (same results) and returns each tile's mean duration and start
  * offset in microseconds, plus the mean launch time.
  * hmvm_plan_set_tile_order permutes the dispatch order of a launch (results
- * are unchanged: tiles write disjoint slots); an order that mixes stage-1 and
- * dense tiles in launch A turns the persistent mode off for the plan. */
+ * are unchanged: tiles write disjoint slots).  Modes 1 and 2 need launch A as
+ * "stage-1 tiles, then dense": selecting them regroups launch A so, keeping
+ * each part's order. */
 enum { HMVM_NFEAT = 17 };
 int hmvm_plan_tile_features(const hmvm_plan *plan, int which, int nrhs, float *feat);
 int hmvm_plan_profile_tiles(hmvm_plan *plan, int which, int iters, double *tile_us, double *start_us,
