@@ -261,6 +261,25 @@ def test_persistent_mode_matches_launches(gpu, H, sphere4096, storage):
     p.close()
 
 
+def test_modes_regroup_launch_a(gpu, H, sphere4096):
+    """The default launch A interleaves stage-1 and dense tiles (largest
+    first); selecting mode 1 or 2 regroups it as stage-1 tiles, then dense,
+    which those modes rely on, with the same bits."""
+    h = sphere4096
+    b = np.random.default_rng(16).uniform(-1, 1, h.nd)
+    for mode, t_mode in ((2, None), (1, 0)):
+        p = H.Plan(h, t_mode=t_mode)
+        k = p.tile_features(0)[:, 0] > 0.5  # stage-1 tiles
+        n1 = int(k.sum())
+        assert 0 < n1 < len(k) and not np.all(k[:n1])
+        y0 = p.apply_host(b)
+        p.set_mode(mode)
+        k = p.tile_features(0)[:, 0] > 0.5
+        assert np.all(k[:n1]) and not np.any(k[n1:])
+        assert np.array_equal(p.apply_host(b), y0)
+        p.close()
+
+
 @pytest.mark.parametrize("storage", ["f64", "bf16"])
 def test_two_stream_overlap_matches_one_stream(gpu, H, sphere4096, storage):
     """Mode 2 runs the dense tiles on a second stream; mode 0 runs every launch

@@ -107,3 +107,27 @@ def test_stage2_tile_height_option(H):
     with pytest.raises(H.HMVMError):
         H.schedule(h, storage="f64", max_rows_b=400)  # above 192*2-1 for FP64
     h.free()
+
+
+def test_launch_a_order(H):
+    """Largest first (the default): launch A's stage-1 and dense tiles in one
+    order of non-increasing elements, interleaved; memory order: all stage-1
+    tiles, then the dense ones.  Both orders hold the same tiles."""
+    h = H.generate(4096, geometry="sphere", eps=1e-4)
+
+    def kinds_and_work(s):
+        t, pan = s["tiles_a"], s["panels"]
+        cc = np.concatenate([[0], np.cumsum(pan["ncols"].astype(np.int64))])
+        p0, n = t["panel0"].astype(np.int64), t["npanel"].astype(np.int64)
+        return pan["factor"][p0] < 0, t["nrows"].astype(np.int64) * (cc[p0 + n] - cc[p0])
+
+    s1 = H.schedule(h)
+    k1, w1 = kinds_and_work(s1)
+    assert np.all(np.diff(w1) <= 0)
+    n1 = int(k1.sum())
+    assert 0 < n1 < len(k1) and not np.all(k1[:n1])  # interleaved
+    s0 = H.schedule(h, order=0)
+    k0, _ = kinds_and_work(s0)
+    assert np.all(k0[:n1]) and not np.any(k0[n1:])
+    assert sorted(s0["tiles_a"]["panel0"].tolist()) == sorted(s1["tiles_a"]["panel0"].tolist())
+    h.free()
