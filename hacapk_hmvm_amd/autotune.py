@@ -251,11 +251,20 @@ def exhaustive(h, storage, warmup=2, iters=8, w_choices=None, kernels=KERNELS, b
     return best, results, time.time() - t0
 
 
-def learned(h, storage, model, w_choices=None, kernels=KERNELS, bts=BLOCK_THREADS, reorder=True):
-    """Chooses the schedule from predictions only; returns (choice, plan, seconds spent)."""
+def learned(h, storage, model, w_choices=None, kernels=KERNELS, bts=BLOCK_THREADS, reorder=True, verify=3):
+    """Chooses the schedule from the cost model's predictions; returns (choice,
+    plan, seconds spent).
+
+    Every candidate (tile widths, then per launch the variant and workgroup
+    size with the smallest predicted launch time) is predicted on the host.
+    With verify > 0 the `verify` candidates predicted fastest and the library
+    defaults are built and timed (a few applies each, with and without the
+    predicted-duration tile order) and the fastest is kept: the model prunes
+    the search to a handful of runs instead of timing all of them.  verify=0
+    takes the best prediction as it is."""
     t0 = time.time()
     w_choices = w_choices or list(itertools.product(W1_CHOICES, W2_CHOICES))
-    best = None
+    cands = []
     for a, b in w_choices:
         w1, w2 = widths(storage, a, b, _length(h))
         sched = schedule(h, storage=storage, w1=w1, w2=w2)
@@ -267,18 +276,38 @@ def learned(h, storage, model, w_choices=None, kernels=KERNELS, bts=BLOCK_THREAD
                     X = schedule_features(sched, which, storage, k, bt)
                     opts.append((model.launch_ms(X, storage, which, k, bt), k, bt))
             pred[which] = min(opts)
-        total = pred[0][0] + pred[1][0]
-        if best is None or total < best["pred_ms"]:
-            best = {"w1": w1, "w2": w2, "kernel_a": pred[0][1], "bt_a": pred[0][2], "kernel_b": pred[1][1],
-                    "bt_b": pred[1][2], "pred_ms": total}
-    p = Plan(h, storage=storage, w1=best["w1"], w2=best["w2"])
-    _apply_choice(p, best)
-    if reorder:
-        for which, (k, bt) in ((0, (best["kernel_a"], best["bt_a"])), (1, (best["kernel_b"], best["bt_b"]))):
-            d = model.tile_us(p.tile_features(which))
-            if len(d):
-                p.set_tile_order(which, np.argsort(-d, kind="stable"))
-    return best, p, time.time() - t0
+        cands.append({"w1": w1, "w2": w2, "kernel_a": pred[0][1], "bt_a": pred[0][2], "kernel_b": pred[1][1],
+                      "bt_b": pred[1][2], "pred_ms": pred[0][0] + pred[1][0]})
+    cands.sort(key=lambda c: c["pred_ms"])
+
+    def build(c, ordered):
+        p = Plan(h, storage=storage) if c is None else Plan(h, storage=storage, w1=c["w1"], w2=c["w2"])
+        if c is not None:
+            _apply_choice(p, c)
+            if ordered:
+                for which in (0, 1):
+                    d = model.tile_us(p.tile_features(which))
+                    if len(d):
+                        p.set_tile_order(which, np.argsort(-d, kind="stable"))
+        return p
+
+    if verify <= 0:
+        best = dict(cands[0], reordered=bool(reorder))
+        return best, build(cands[0], reorder), time.time() - t0
+    trials = [(None, False)]
+    for c in cands[:verify]:
+        trials += [(c, False)] + ([(c, True)] if reorder else [])
+    best, best_ms = None, None
+    for c, ordered in trials:
+        p = build(c, ordered)
+        ms = p.time(2, 5)["ms_stream"]
+        p.close()
+        if best_ms is None or ms < best_ms:
+            best, best_ms = (c, ordered), ms
+    c, ordered = best
+    choice = {"defaults": True} if c is None else dict(c)
+    choice.update({"reordered": ordered, "timed_trials": len(trials)})
+    return choice, build(c, ordered), time.time() - t0
 
 
 def default_plan(h, storage):
