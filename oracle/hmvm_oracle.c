@@ -3,8 +3,8 @@
  *
  * CPU restatement of the reference HMVM.  The loop nests and the order of the
  * floating-point accumulations are the reference's, because bit-identical FP64
- * output is what lets this file stand in for the reference on the GPU box
- * (where the reference sources are not available).
+ * output is what lets this file stand in for the reference on machines
+ * where the reference sources are not available.
  */
 #include <stdlib.h>
 #include <string.h>

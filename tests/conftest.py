@@ -35,6 +35,6 @@ def gpu(H):
 
 def reference_path(*parts):
     """Path under the read-only reference checkout, or None when it is absent
-    (it exists only in the build container, never on the GPU box)."""
+    (machines that run the GPU tests do not have it)."""
     p = os.path.join(REFERENCE, *parts)
     return p if os.path.exists(p) else None
