@@ -4,6 +4,7 @@ profiles/ and the traffic record bench.py reads.
 
     python -m hacapk_hmvm_amd.rocprof_summary --trace DIR [--fetch DIR --write DIR]
         --grid-a THREADS --grid-b THREADS [--md OUT.md] [--json OUT.json] [--cmd "..."]
+    python -m hacapk_hmvm_amd.rocprof_summary --mfma DIR [--md OUT.md] [--cmd "..."]
 
 The two tile launches of one HMVM run the same kernel (tile_kernel, or
 tile_multi_kernel for several right-hand sides); they are told apart by
@@ -60,19 +61,51 @@ def counters(rows, counter):
     return by
 
 
+def mfma_table(d, nxcd=8, nsimd=1024):
+    """MFMA busy share and MFMA FLOPs of the tile_multi_kernel launches from a
+    --pmc run of SQ_VALU_MFMA_BUSY_CYCLES, SQ_INSTS_VALU_MFMA_MOPS_{F64,F32,
+    BF16} and GRBM_GUI_ACTIVE: busy % = MFMA busy cycles summed over the SIMDs
+    / (GRBM_GUI_ACTIVE per XCD x SIMDs); GRBM_GUI_ACTIVE is read summed over
+    the XCDs.  FLOPs = MOPS x 512."""
+    by = {}
+    for r in read_csv(d, "_counter_collection.csv"):
+        k = short(r["Kernel_Name"])
+        if not k.startswith("tile_multi_kernel"):
+            continue
+        by.setdefault((k, int(r["Grid_Size"])), {}).setdefault(r["Counter_Name"], []).append(float(r["Counter_Value"]))
+    lines = ["| kernel | grid | MFMA busy % | MFMA GFLOP per launch |", "|---|---|---|---|"]
+    for (k, g), c in sorted(by.items()):
+        m = {n: statistics.mean(v) for n, v in c.items()}
+        busy = 100.0 * m["SQ_VALU_MFMA_BUSY_CYCLES"] / (m["GRBM_GUI_ACTIVE"] / nxcd * nsimd)
+        mops = sum(m.get("SQ_INSTS_VALU_MFMA_MOPS_" + t, 0.0) for t in ("F64", "F32", "BF16"))
+        lines.append("| `%s` | %d | %.1f | %.2f |" % (k, g, busy, mops * 512 / 1e9))
+    return lines
+
+
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--trace", required=True)
+    ap.add_argument("--mfma", help="directory of a --pmc run with the MFMA counters (prints their table only)")
+    ap.add_argument("--trace")
     ap.add_argument("--fetch")
     ap.add_argument("--write")
-    ap.add_argument("--grid-a", type=int, required=True)
-    ap.add_argument("--grid-b", type=int, required=True)
+    ap.add_argument("--grid-a", type=int, default=0)
+    ap.add_argument("--grid-b", type=int, default=0)
     ap.add_argument("--storage-kernel", default="tile_kernel<double")
     ap.add_argument("--md")
     ap.add_argument("--json")
     ap.add_argument("--cmd", default="")
     ap.add_argument("--meta", default="{}", help="JSON merged into the traffic record (N, len, ...)")
     a = ap.parse_args()
+    if a.mfma:
+        lines = ["# rocprofv3 MFMA counters: %s" % a.cmd, ""] + mfma_table(a.mfma)
+        if a.md:
+            with open(a.md, "w") as f:
+                f.write("\n".join(lines) + "\n")
+        else:
+            print("\n".join(lines))
+        return
+    if not a.trace:
+        ap.error("--trace is required")
 
     tr = trace_table(read_csv(a.trace, "_kernel_trace.csv"))
     lines = ["# rocprofv3 kernel trace: %s" % a.cmd, "", "| kernel | grid | calls | avg us | min us | max us |",
