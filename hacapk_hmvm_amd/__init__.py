@@ -111,7 +111,8 @@ class hmvm_plan_info(ctypes.Structure):
                 ("elems_stage1", ctypes.c_longlong), ("elems_dense", ctypes.c_longlong),
                 ("elems_stage2", ctypes.c_longlong),
                 ("vec_reads_a", ctypes.c_longlong), ("vec_reads_b", ctypes.c_longlong),
-                ("algorithmic_bytes", ctypes.c_longlong), ("device_bytes", ctypes.c_longlong)]
+                ("algorithmic_bytes", ctypes.c_longlong), ("device_bytes", ctypes.c_longlong),
+                ("mode", ctypes.c_int), ("pad_", ctypes.c_int)]
 
     def as_dict(self):
         return {name: getattr(self, name) for name, _ in self._fields_}
@@ -142,8 +143,22 @@ HACAPK_WITH_ROWMAT_T = 1
 HMVM_NFEAT = 17
 
 
+HMVM_ERR_ARG, HMVM_ERR_MATRIX, HMVM_ERR_HIP, HMVM_ERR_NOMEM = -101, -102, -103, -104
+HMVM_ERR_NODEVICE, HMVM_ERR_UNSUPPORTED, HMVM_ERR_BREAKDOWN, HMVM_ERR_WAIT = -105, -106, -107, -108
+
+
 class HMVMError(RuntimeError):
     pass
+
+
+class KrylovBreakdown(HMVMError):
+    """BiCGSTAB met a zero divisor or a non-finite scalar; ``u`` and ``result``
+    hold the state at the stop."""
+
+    def __init__(self, msg, u, result):
+        super().__init__(msg)
+        self.u = u
+        self.result = result
 
 
 def _load():
@@ -857,6 +872,9 @@ class Plan:
         res = hmvm_krylov_result()
         rc = lib.hmvm_bicgstab(self._p, _dptr(b), _dptr(u), int(max_iter), float(tol), int(bool(use_graph)),
                                int(check_every), ctypes.byref(res))
+        if rc == HMVM_ERR_BREAKDOWN:
+            raise KrylovBreakdown("hmvm_bicgstab: breakdown after %d iterations (zero divisor or non-finite "
+                                  "scalar)" % res.iters, u, res.as_dict())
         if rc:
             raise HMVMError("hmvm_bicgstab failed (code %d): %s" % (rc, lib.hmvm_hip_last_error_message().decode()))
         return u, res.as_dict()

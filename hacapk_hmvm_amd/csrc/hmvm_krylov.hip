@@ -3,11 +3,20 @@
 // Restates the iteration of the reference's HACApK_bicgstab_lfmtx_hyp
 // (src_hdump/m_HACApK_solve.f90:306-380): two HMVMs per iteration, vector
 // updates and dot products between them.  Every scalar (alpha, zeta, beta,
-// the residual norm) stays in device memory, so one iteration is a fixed
-// sequence of launches with no host synchronisation; with use_graph the
-// iteration is captured once into a hipGraph and replayed.  Dot products are
-// two-pass (per-block partials, then one block) and therefore deterministic:
-// the eager and the graph-replayed solve give the same bits.
+// the residual norm, the stop state and the iteration count) stays in device
+// memory, so one iteration is a fixed sequence of launches with no host
+// synchronisation; with use_graph the iteration is captured once into a
+// hipGraph and replayed.  Dot products are two-pass (per-block partials, then
+// one block) and therefore deterministic: the eager and the graph-replayed
+// solve give the same bits.
+//
+// Stopping: the reference leaves its loop at the top of the first iteration
+// whose ||r||/||b|| is below eps (m_HACApK_solve.f90:342).  Here the kernel
+// that computes ||r|| sets a device stop word instead, and every later update
+// kernel and scalar update returns at once, so the host may launch whole
+// chunks of iterations between its read-backs and still get the reference's
+// iterate and iteration count.  A zero divisor or a non-finite scalar sets
+// the stop word to "breakdown" the same way.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -25,7 +34,10 @@ constexpr int kDotBlocks = 512;
 
 // device scalars
 enum {
-  kAlpha = 0, kBeta, kZeta, kZnomOld, kRnorm, kBnorm, kNumScalars = 8
+  kAlpha = 0, kBeta, kZeta, kZnomOld, kRnorm, kBnorm,
+  kStop,   // 0 running, 1 converged, 2 breakdown
+  kIters,  // iterations that updated u
+  kNumScalars
 };
 
 // partial[blockIdx] = (sum a*b, sum c*d) over a grid-stride slice
@@ -55,15 +67,23 @@ __global__ __launch_bounds__(kBlock) void dot2_partial(int n, const double *__re
   }
 }
 
+__device__ __forceinline__ bool below_tol(double rnorm, double bnorm, double tol) {
+  return rnorm / (bnorm > 0.0 ? bnorm : 1.0) < tol;
+}
+
 // Reduces the partials and updates the scalars:
 //   op 0 (start):  bnorm = sqrt(d0)               [d0 = (b,b)]
-//   op 1 (rnorm):  rnorm = sqrt(d0)               [d0 = (r,r)]
+//   op 1 (rnorm):  rnorm = sqrt(d0)               [d0 = (r,r)], stop if below tol
 //   op 2 (alpha):  alpha = d0/d1, znomold = d0    [(s,r), (s,Ap)]
 //   op 3 (zeta):   zeta = d0/d1                   [(At,t), (At,At)]
-//   op 4 (beta):   beta = alpha/zeta*d0/znomold, rnorm = sqrt(d1)   [(s,r), (r,r)]
+//   op 4 (beta):   beta = alpha/zeta*d0/znomold, rnorm = sqrt(d1), iters += 1,
+//                  stop if below tol              [(s,r), (r,r)]
+// Ops 2-4 do nothing once the stop word is set; a non-finite alpha, zeta,
+// beta or rnorm sets it to 2.
 __global__ __launch_bounds__(kBlock) void dot2_finish(int nblk, const double *__restrict__ part, int op,
-                                                      double *__restrict__ S) {
+                                                      double tol, double *__restrict__ S) {
   __shared__ double s0[kBlock], s1[kBlock];
+  if (op >= 2 && S[kStop] != 0.0) return;  // uniform over the block
   double x0 = 0, x1 = 0;
   for (int i = threadIdx.x; i < nblk; i += kBlock) {
     x0 += part[2 * i];
@@ -83,15 +103,43 @@ __global__ __launch_bounds__(kBlock) void dot2_finish(int nblk, const double *__
     const double d0 = s0[0], d1 = s1[0];
     switch (op) {
       case 0: S[kBnorm] = sqrt(d0); break;
-      case 1: S[kRnorm] = sqrt(d0); break;
-      case 2: S[kAlpha] = d0 / d1; S[kZnomOld] = d0; break;
-      case 3: S[kZeta] = d0 / d1; break;
-      default:
-        S[kBeta] = S[kAlpha] / S[kZeta] * d0 / S[kZnomOld];
-        S[kRnorm] = sqrt(d1);
+      case 1:
+        S[kRnorm] = sqrt(d0);
+        if (below_tol(S[kRnorm], S[kBnorm], tol)) S[kStop] = 1.0;
         break;
+      case 2: {
+        const double al = d0 / d1;
+        S[kAlpha] = al;
+        S[kZnomOld] = d0;
+        if (!isfinite(al)) S[kStop] = 2.0;
+        break;
+      }
+      case 3: {
+        const double ze = d0 / d1;
+        S[kZeta] = ze;
+        if (!isfinite(ze)) S[kStop] = 2.0;
+        break;
+      }
+      default: {
+        const double be = S[kAlpha] / S[kZeta] * d0 / S[kZnomOld];
+        const double rn = sqrt(d1);
+        S[kBeta] = be;
+        S[kRnorm] = rn;
+        S[kIters] += 1.0;
+        if (below_tol(rn, S[kBnorm], tol))
+          S[kStop] = 1.0;  // the reference leaves the loop before beta is used
+        else if (!isfinite(rn) || !isfinite(be))
+          S[kStop] = 2.0;
+        break;
+      }
     }
   }
+}
+
+// S = 0 (a kernel rather than a memset, so that no memset node enters a
+// captured graph)
+__global__ void bicg_scalars_zero(double *__restrict__ S) {
+  if (threadIdx.x < kNumScalars) S[threadIdx.x] = 0.0;
 }
 
 // r = b - r (r holds A u on entry), shadow = r, p = 0, Ap = 0
@@ -111,7 +159,7 @@ __global__ __launch_bounds__(kBlock) void bicg_init(int n, const double *__restr
 __global__ __launch_bounds__(kBlock) void bicg_p(int n, const double *__restrict__ r, double *__restrict__ p,
                                                  const double *__restrict__ ap, const double *__restrict__ S) {
   const int i = blockIdx.x * kBlock + threadIdx.x;
-  if (i >= n) return;
+  if (i >= n || S[kStop] != 0.0) return;
   p[i] = r[i] + S[kBeta] * (p[i] - S[kZeta] * ap[i]);
 }
 
@@ -119,7 +167,7 @@ __global__ __launch_bounds__(kBlock) void bicg_p(int n, const double *__restrict
 __global__ __launch_bounds__(kBlock) void bicg_t(int n, const double *__restrict__ r, const double *__restrict__ ap,
                                                  double *__restrict__ t, const double *__restrict__ S) {
   const int i = blockIdx.x * kBlock + threadIdx.x;
-  if (i >= n) return;
+  if (i >= n || S[kStop] != 0.0) return;
   t[i] = r[i] - S[kAlpha] * ap[i];
 }
 
@@ -128,24 +176,38 @@ __global__ __launch_bounds__(kBlock) void bicg_update(int n, double *__restrict_
                                                       const double *__restrict__ t, const double *__restrict__ at,
                                                       double *__restrict__ r, const double *__restrict__ S) {
   const int i = blockIdx.x * kBlock + threadIdx.x;
-  if (i >= n) return;
+  if (i >= n || S[kStop] != 0.0) return;
   const double al = S[kAlpha], ze = S[kZeta];
   u[i] = u[i] + al * p[i] + ze * t[i];
   r[i] = t[i] - ze * at[i];
 }
 
+// Device vectors and scalars of one solve: b and u are the caller's (or
+// copies of the caller's host vectors), the rest is work space.
 struct Solver {
   int n = 0;
+  double tol = 0;
   hmvm_plan *plan = nullptr;
   hipStream_t st = nullptr;
-  double *b = nullptr, *u = nullptr, *r = nullptr, *sh = nullptr, *p = nullptr, *ap = nullptr, *t = nullptr,
-         *at = nullptr, *part = nullptr, *S = nullptr;
+  const double *b = nullptr;
+  double *u = nullptr, *r = nullptr, *sh = nullptr, *p = nullptr, *ap = nullptr, *t = nullptr, *at = nullptr,
+         *part = nullptr, *S = nullptr;
   int nblk() const { return std::min(kDotBlocks, (n + kBlock - 1) / kBlock); }
   dim3 grid() const { return dim3((n + kBlock - 1) / kBlock); }
 
   void dots(const double *a, const double *bb, const double *c, const double *d, int op) {
     hipLaunchKernelGGL(dot2_partial, dim3(nblk()), dim3(kBlock), 0, st, n, a, bb, c, d, part);
-    hipLaunchKernelGGL(dot2_finish, dim3(1), dim3(kBlock), 0, st, nblk(), part, op, S);
+    hipLaunchKernelGGL(dot2_finish, dim3(1), dim3(kBlock), 0, st, nblk(), part, op, tol, S);
+  }
+  // r = b - A u ; shadow = r ; p = Ap = 0 ; norms (m_HACApK_solve.f90:326-339)
+  int start() {
+    hipLaunchKernelGGL(bicg_scalars_zero, dim3(1), dim3(64), 0, st, S);
+    dots(b, b, b, b, 0);
+    int rc = hmvm_plan_apply(plan, u, r, 0, st);
+    if (rc) return rc;
+    hipLaunchKernelGGL(bicg_init, grid(), dim3(kBlock), 0, st, n, b, r, sh, p, ap);
+    dots(r, r, r, r, 1);
+    return hipGetLastError() == hipSuccess ? HMVM_OK : HMVM_ERR_HIP;
   }
   // one BiCGSTAB iteration (m_HACApK_solve.f90:341-372)
   int iterate() {
@@ -161,54 +223,26 @@ struct Solver {
     dots(sh, r, r, r, 4);
     return hipGetLastError() == hipSuccess ? HMVM_OK : HMVM_ERR_HIP;
   }
-  ~Solver() {
-    double *bufs[] = {b, u, r, sh, p, ap, t, at, part, S};
-    for (double *q : bufs)
-      if (q) (void)hipFree(q);
-    if (st) (void)hipStreamDestroy(st);
+  int read_scalars(double *S_host) {
+    if (hipMemcpyAsync(S_host, S, sizeof(double) * kNumScalars, hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess)
+      return HMVM_ERR_HIP;
+    return HMVM_OK;
   }
 };
 
-}  // namespace
-
-extern "C" int hmvm_bicgstab(hmvm_plan *plan, const double *b_host, double *u_host, int max_iter, double tol,
-                             int use_graph, int check_every, hmvm_krylov_result *res) {
-  if (!plan || !b_host || !u_host || !res || max_iter < 0 || check_every < 0) return HMVM_ERR_ARG;
-  std::memset(res, 0, sizeof(*res));
-  hmvm_plan_info info;
-  int rc = hmvm_plan_get_info(plan, &info);
+// The solve loop shared by the entry points: start, then chunks of
+// check_every iterations (eager, or replays of one captured iteration) until
+// the stop word is set or max_iter iterations have been launched.  `timed`
+// brackets the loop with events.
+int solve(Solver &s, int max_iter, int use_graph, int check_every, bool timed, hmvm_krylov_result *res) {
+  int rc = s.start();
   if (rc) return rc;
-  Solver s;
-  s.n = info.nd;
-  s.plan = plan;
-  const size_t nb = sizeof(double) * (size_t)s.n;
-  if (hipStreamCreate(&s.st) != hipSuccess) return HMVM_ERR_HIP;
-  double **bufs[] = {&s.b, &s.u, &s.r, &s.sh, &s.p, &s.ap, &s.t, &s.at};
-  for (double **q : bufs)
-    if (hipMalloc(q, nb) != hipSuccess) return HMVM_ERR_HIP;
-  if (hipMalloc(&s.part, sizeof(double) * 2 * kDotBlocks) != hipSuccess) return HMVM_ERR_HIP;
-  if (hipMalloc(&s.S, sizeof(double) * kNumScalars) != hipSuccess) return HMVM_ERR_HIP;
-  if (hipMemcpyAsync(s.b, b_host, nb, hipMemcpyHostToDevice, s.st) != hipSuccess) return HMVM_ERR_HIP;
-  if (hipMemcpyAsync(s.u, u_host, nb, hipMemcpyHostToDevice, s.st) != hipSuccess) return HMVM_ERR_HIP;
-  {
-    const double zero[kNumScalars] = {0, 0, 0, 0, 0, 0, 0, 0};
-    if (hipMemcpyAsync(s.S, zero, sizeof(zero), hipMemcpyHostToDevice, s.st) != hipSuccess) return HMVM_ERR_HIP;
-  }
-  // r = b - A u ; shadow = r ; p = Ap = 0 ; norms (m_HACApK_solve.f90:326-339)
-  s.dots(s.b, s.b, s.b, s.b, 0);
-  if ((rc = hmvm_plan_apply(plan, s.u, s.r, 0, s.st))) return rc;
-  hipLaunchKernelGGL(bicg_init, s.grid(), dim3(kBlock), 0, s.st, s.n, s.b, s.r, s.sh, s.p, s.ap);
-  s.dots(s.r, s.r, s.r, s.r, 1);
   double S0[kNumScalars];
-  if (hipMemcpyAsync(S0, s.S, sizeof(S0), hipMemcpyDeviceToHost, s.st) != hipSuccess ||
-      hipStreamSynchronize(s.st) != hipSuccess)
-    return HMVM_ERR_HIP;
-  const double bnorm = S0[kBnorm] > 0 ? S0[kBnorm] : 1.0;
-  double relres = S0[kRnorm] / bnorm;
-
+  if ((rc = s.read_scalars(S0))) return rc;
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
-  if (use_graph) {
+  if (use_graph && S0[kStop] == 0.0 && max_iter > 0) {
     if (hipStreamBeginCapture(s.st, hipStreamCaptureModeThreadLocal) != hipSuccess) return HMVM_ERR_HIP;
     rc = s.iterate();
     hipError_t e = hipStreamEndCapture(s.st, &graph);
@@ -221,43 +255,91 @@ extern "C" int hmvm_bicgstab(hmvm_plan *plan, const double *b_host, double *u_ho
       return HMVM_ERR_HIP;
     }
   }
-  hipEvent_t e0, e1;
-  (void)hipEventCreate(&e0);
-  (void)hipEventCreate(&e1);
-  hipEventRecord(e0, s.st);
-  int it = 0;
-  while (it < max_iter && !(relres < tol)) {
-    const int chunk = check_every > 0 ? std::min(check_every, max_iter - it) : max_iter - it;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (timed) {
+    (void)hipEventCreate(&e0);
+    (void)hipEventCreate(&e1);
+    (void)hipEventRecord(e0, s.st);
+  }
+  int launched = 0;
+  while (launched < max_iter && S0[kStop] == 0.0) {
+    const int chunk = check_every > 0 ? std::min(check_every, max_iter - launched) : max_iter - launched;
     for (int c = 0; c < chunk && !rc; c++) {
-      if (use_graph)
+      if (gexec)
         rc = hipGraphLaunch(gexec, s.st) == hipSuccess ? HMVM_OK : HMVM_ERR_HIP;
       else
         rc = s.iterate();
     }
     if (rc) break;
-    it += chunk;
-    double rn = 0;
-    if (hipMemcpyAsync(&rn, s.S + kRnorm, sizeof(double), hipMemcpyDeviceToHost, s.st) != hipSuccess ||
-        hipStreamSynchronize(s.st) != hipSuccess) {
-      rc = HMVM_ERR_HIP;
-      break;
-    }
-    relres = rn / bnorm;
+    launched += chunk;
+    if ((rc = s.read_scalars(S0))) break;
   }
-  hipEventRecord(e1, s.st);
   float ms = 0;
-  if (hipEventSynchronize(e1) == hipSuccess) hipEventElapsedTime(&ms, e0, e1);
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
+  if (timed) {
+    (void)hipEventRecord(e1, s.st);
+    if (hipEventSynchronize(e1) == hipSuccess) (void)hipEventElapsedTime(&ms, e0, e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+  }
   if (gexec) (void)hipGraphExecDestroy(gexec);
   if (graph) (void)hipGraphDestroy(graph);
   if (rc) return rc;
-  if (hipMemcpy(u_host, s.u, nb, hipMemcpyDeviceToHost) != hipSuccess) return HMVM_ERR_HIP;
-  res->iters = it;
-  res->hmvms = 2 * it + 1;
-  res->relres = relres;
+  const double bnorm = S0[kBnorm] > 0 ? S0[kBnorm] : 1.0;
+  res->iters = (int)S0[kIters];
+  res->hmvms = 2 * res->iters + 1;
+  res->relres = S0[kRnorm] / bnorm;
   res->ms_loop = ms;
-  res->ms_per_iter = it > 0 ? ms / it : 0.0;
-  res->graph_used = use_graph ? 1 : 0;
-  return HMVM_OK;
+  res->ms_per_iter = launched > 0 ? ms / launched : 0.0;
+  res->graph_used = gexec ? 1 : 0;
+  return S0[kStop] == 2.0 ? HMVM_ERR_BREAKDOWN : HMVM_OK;
+}
+
+}  // namespace
+
+extern "C" int hmvm_bicgstab(hmvm_plan *plan, const double *b_host, double *u_host, int max_iter, double tol,
+                             int use_graph, int check_every, hmvm_krylov_result *res) {
+  if (!plan || !b_host || !u_host || !res || max_iter < 0 || check_every < 0) return HMVM_ERR_ARG;
+  std::memset(res, 0, sizeof(*res));
+  hmvm_plan_info info;
+  int rc = hmvm_plan_get_info(plan, &info);
+  if (rc) return rc;
+  Solver s;
+  s.n = info.nd;
+  s.tol = tol;
+  s.plan = plan;
+  const size_t nb = sizeof(double) * (size_t)s.n;
+  // b, u, the six work vectors, the dot partials and the scalars
+  struct Buffers {
+    double *d[10] = {};
+    hipStream_t st = nullptr;
+    ~Buffers() {
+      if (st) (void)hipStreamSynchronize(st);
+      for (double *q : d)
+        if (q) (void)hipFree(q);
+      if (st) (void)hipStreamDestroy(st);
+    }
+  } m;
+  if (hipStreamCreate(&m.st) != hipSuccess) return HMVM_ERR_HIP;
+  for (int i = 0; i < 10; i++) {
+    const size_t bytes = i < 8 ? nb : i == 8 ? sizeof(double) * 2 * kDotBlocks : sizeof(double) * kNumScalars;
+    if (hipMalloc(&m.d[i], bytes) != hipSuccess) return HMVM_ERR_HIP;
+  }
+  s.st = m.st;
+  s.b = m.d[0];
+  s.u = m.d[1];
+  s.r = m.d[2];
+  s.sh = m.d[3];
+  s.p = m.d[4];
+  s.ap = m.d[5];
+  s.t = m.d[6];
+  s.at = m.d[7];
+  s.part = m.d[8];
+  s.S = m.d[9];
+  if (hipMemcpyAsync(m.d[0], b_host, nb, hipMemcpyHostToDevice, s.st) != hipSuccess ||
+      hipMemcpyAsync(s.u, u_host, nb, hipMemcpyHostToDevice, s.st) != hipSuccess)
+    return HMVM_ERR_HIP;
+  rc = solve(s, max_iter, use_graph, check_every, true, res);
+  if ((rc == HMVM_OK || rc == HMVM_ERR_BREAKDOWN) && hipMemcpy(u_host, s.u, nb, hipMemcpyDeviceToHost) != hipSuccess)
+    rc = HMVM_ERR_HIP;
+  return rc;
 }

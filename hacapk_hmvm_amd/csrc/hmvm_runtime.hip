@@ -416,8 +416,9 @@ PlanOptions plan_options(const hmvm_options *o, int64_t len) {
   // t_mode < 0: materialise t for every leaf.  With t in row-cluster order
   // the stage-2 factors of a cluster merge into one panel, which beats summing
   // the stage-1 partials on the fly in all three storages (measured at
-  // N = 101,250) despite the extra launch.
-  po.t_mode = o->t_mode >= 0 ? o->t_mode : 1;
+  // N = 101,250) despite the extra launch.  The persistent mode has no launch
+  // boundary for the t sum, so there the stage-2 tiles sum the partials.
+  po.t_mode = o->t_mode >= 0 ? o->t_mode : o->mode == 1 ? 0 : 1;
   return po;
 }
 
@@ -705,6 +706,9 @@ int create_plan(Source &src, const hmvm_options *opt, hmvm_plan **out) {
   err = verify_plan(p->plan, src.la);
   if (!err.empty()) return set_error(HMVM_ERR_MATRIX, "schedule check failed: %s", err.c_str());
   Plan &P = p->plan;
+  if (p->mode == 1 && !P.t_fused)
+    return set_error(HMVM_ERR_UNSUPPORTED, "mode 1 (persistent launch) needs t_mode 0: stage-2 tiles of this plan "
+                                           "read a materialised t");
   for (const PanelDesc &pd : P.panels) p->has_tp = p->has_tp || pd.vec_sel == kVecTP;
   p->a_mixed = order_mixed(P);
   HIP_TRY(hipGetDevice(&p->device));
@@ -1102,6 +1106,33 @@ void fill_info(const hmvm_plan *p, const Plan &P, int storage, hmvm_plan_info *i
   info->vec_reads_b = P.vec_b;
   info->algorithmic_bytes = (long long)info->elem_bytes * P.len + 16LL * P.nd;
   info->device_bytes = p ? (long long)p->device_bytes : 0;
+  info->mode = !p ? 0 : persistent_ok(p) ? 1 : overlap_ok(p) ? 2 : 0;
+}
+
+// Reads and clears the fault word of the persistent launches (after the
+// plan's stream has drained).
+int take_fault(hmvm_plan *p, int *fault) {
+  *fault = 0;
+  HIP_TRY(hipStreamSynchronize(p->stream));
+  int f = 0;
+  HIP_TRY(hipMemcpy(&f, &p->d_ctr->fault, sizeof(int), hipMemcpyDeviceToHost));
+  if (f) {
+    const int zero = 0;
+    HIP_TRY(hipMemcpy(&p->d_ctr->fault, &zero, sizeof(int), hipMemcpyHostToDevice));
+  }
+  *fault = f;
+  return HMVM_OK;
+}
+
+// For the blocking entry points, after their synchronisation: an error when
+// a persistent launch of this plan gave up a bounded wait.
+int check_wait(hmvm_plan *p) {
+  if (!persistent_ok(p)) return HMVM_OK;
+  int f = 0;
+  const int rc = take_fault(p, &f);
+  if (rc) return rc;
+  return f ? set_error(HMVM_ERR_WAIT, "a bounded wait of the persistent launch gave up; the results are invalid")
+           : HMVM_OK;
 }
 
 // ---------------------------------------------------------------------------
@@ -1423,6 +1454,8 @@ int hmvm_plan_time_multi(hmvm_plan *plan, int nrhs, int warmup, int iters, hmvm_
 int hmvm_plan_set_mode(hmvm_plan *plan, int mode) {
   if (!plan) return set_error(HMVM_ERR_ARG, "NULL plan");
   if (mode < 0 || mode > 2) return set_error(HMVM_ERR_ARG, "mode must be 0, 1 or 2");
+  if (mode == 1 && !plan->plan.t_fused)
+    return set_error(HMVM_ERR_UNSUPPORTED, "mode 1 (persistent launch) needs a plan built with t_mode 0");
   if (mode != 0) {
     const int rc = partition_a(plan);
     if (rc) return rc;
@@ -1433,10 +1466,9 @@ int hmvm_plan_set_mode(hmvm_plan *plan, int mode) {
 
 int hmvm_plan_fault(hmvm_plan *plan) {
   if (!plan) return set_error(HMVM_ERR_ARG, "NULL plan");
-  PersistCounters c;
-  HIP_TRY(hipStreamSynchronize(plan->stream));
-  HIP_TRY(hipMemcpy(&c, plan->d_ctr, sizeof(c), hipMemcpyDeviceToHost));
-  return c.fault ? 1 : 0;
+  int f = 0;
+  const int rc = take_fault(plan, &f);
+  return rc ? rc : f;
 }
 
 int hmvm_bandwidth_probe(size_t bytes, int iters, double *read_gbs, double *copy_gbs) {
@@ -1599,7 +1631,7 @@ int hmvm_plan_apply_host(hmvm_plan *plan, const double *x, double *y, int accumu
   if (rc) return rc;
   HIP_TRY(hipMemcpyAsync(y, plan->d_y, nb, hipMemcpyDeviceToHost, plan->stream));
   HIP_TRY(hipStreamSynchronize(plan->stream));
-  return HMVM_OK;
+  return check_wait(plan);
 }
 
 int hmvm_plan_bench_apply(hmvm_plan *plan, const double *x, int iters, double *seconds) {
@@ -1622,7 +1654,7 @@ int hmvm_plan_bench_apply(hmvm_plan *plan, const double *x, int iters, double *s
   (void)hipEventDestroy(e1);
   if (rc) return rc;
   *seconds = ms * 1e-3 / iters;
-  return HMVM_OK;
+  return check_wait(plan);
 }
 
 int hmvm_plan_time(hmvm_plan *plan, int warmup, int iters, hmvm_timing *tm) {
@@ -1688,7 +1720,7 @@ int hmvm_plan_time(hmvm_plan *plan, int warmup, int iters, hmvm_timing *tm) {
   tm->ms_tsum = ph[1] / iters;
   tm->ms_b = ph[2] / iters;
   tm->ms_ysum = ph[3] / iters;
-  return HMVM_OK;
+  return check_wait(plan);
 }
 
 int hmvm_plan_time_flushed(hmvm_plan *plan, long long flush_bytes, int iters, hmvm_timing *tm) {
@@ -1736,7 +1768,7 @@ int hmvm_plan_time_flushed(hmvm_plan *plan, long long flush_bytes, int iters, hm
   tm->ms_min = s.front();
   tm->ms_median = s[s.size() / 2];
   tm->ms_stream = tm->ms_avg;
-  return HMVM_OK;
+  return check_wait(plan);
 }
 
 // ---- hipGraph form of the apply -------------------------------------------
@@ -1823,7 +1855,7 @@ int hmvm_plan_time_graph(hmvm_plan *plan, int count, int warmup, int iters, hmvm
   tm->ms_avg = sum / iters;
   tm->ms_min = srt.front();
   tm->ms_median = srt[srt.size() / 2];
-  return HMVM_OK;
+  return check_wait(plan);
 }
 
 int hmvm_plan_profile_tiles(hmvm_plan *plan, int which, int iters, double *tile_us, double *start_us,

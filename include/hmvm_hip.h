@@ -47,7 +47,12 @@ enum {
   HMVM_ERR_HIP = -103,      /* HIP runtime error (message has the details)  */
   HMVM_ERR_NOMEM = -104,    /* host allocation failed                       */
   HMVM_ERR_NODEVICE = -105, /* no HIP device is visible                     */
-  HMVM_ERR_UNSUPPORTED = -106
+  HMVM_ERR_UNSUPPORTED = -106,
+  HMVM_ERR_BREAKDOWN = -107, /* Krylov iteration met a zero divisor or a     */
+                             /* non-finite residual (u is the last finite    */
+                             /* iterate)                                     */
+  HMVM_ERR_WAIT = -108       /* a bounded wait of the persistent launch gave */
+                             /* up: that launch's results are invalid        */
 };
 
 /* ---- drop-in entry points ------------------------------------------------ */
@@ -102,11 +107,16 @@ typedef struct hmvm_options {
   int order;             /* tile order in a launch: 0 memory, 1 largest first   */
                          /* (launch A: stage-1 and dense tiles in one order)    */
   int mode;              /* 0: launches on one stream; 1: one persistent launch */
-                         /* with a device work queue; 2: launches, dense tiles  */
-                         /* on a second stream concurrent with stage 1/2        */
+                         /* with a device work queue (needs every stage-2 tile  */
+                         /* to sum its stage-1 partials itself: t_mode 0, which */
+                         /* t_mode -1 then selects; a plan with a materialised  */
+                         /* t is refused with HMVM_ERR_UNSUPPORTED); 2:         */
+                         /* launches, dense tiles on a second stream concurrent */
+                         /* with stage 1/2                                      */
   int t_mode;            /* stage-2 vector entries: 0 sum the stage-1 partial   */
                          /* slots on the fly, 1 read a materialised t (one more */
-                         /* launch), 2 choose per leaf, -1 default (1)          */
+                         /* launch), 2 choose per leaf, -1 default (1; 0 in     */
+                         /* mode 1)                                             */
   int max_rows;          /* rows per tile (0: by storage); at most 192*V-V+1,  */
                          /* V = 16 bytes / element size; when set, also for the */
                          /* stage-2 tiles unless max_rows_b is set              */
@@ -129,6 +139,8 @@ typedef struct hmvm_plan_info {
   long long vec_reads_a, vec_reads_b;
   long long algorithmic_bytes; /* elem_bytes*len + 16*nd (x read, y written)    */
   long long device_bytes;      /* HBM held by the plan                          */
+  int mode;                    /* execution mode an apply runs (hmvm_options)   */
+  int pad_;
 } hmvm_plan_info;
 
 typedef struct hmvm_timing {
@@ -194,10 +206,15 @@ int hmvm_plan_apply_multi(hmvm_plan *plan, const double *d_X, double *d_Y, int n
 int hmvm_plan_apply_multi_host(hmvm_plan *plan, const double *X, double *Y, int nrhs, int accumulate);
 int hmvm_plan_time_multi(hmvm_plan *plan, int nrhs, int warmup, int iters, hmvm_timing *timing);
 
-/* Selects the execution mode of an existing plan (see hmvm_options.mode). */
+/* Selects the execution mode of an existing plan (see hmvm_options.mode);
+ * mode 1 on a plan whose stage-2 tiles read a materialised t returns
+ * HMVM_ERR_UNSUPPORTED and leaves the mode unchanged. */
 int hmvm_plan_set_mode(hmvm_plan *plan, int mode);
-/* 1 when a bounded wait inside the persistent launch gave up (the results of
- * that launch are then invalid), 0 otherwise; synchronises the plan's stream. */
+/* 1 when a bounded wait inside a persistent launch gave up since the last
+ * report (the results of that launch are invalid), 0 otherwise; synchronises
+ * the plan's stream and clears the flag.  The blocking entry points
+ * (hmvm_plan_apply_host, the timing functions) check it themselves and return
+ * HMVM_ERR_WAIT. */
 int hmvm_plan_fault(hmvm_plan *plan);
 
 /* Achievable HBM bandwidth on this device: a read-only stream and a copy of
@@ -302,15 +319,20 @@ int hmvm_mlp_train(hmvm_mlp *m, const float *X, const float *y, int n, int epoch
  * BiCGSTAB for A u = b with this plan as A, iterating as the reference's
  * HACApK_bicgstab_lfmtx_hyp (src_hdump/m_HACApK_solve.f90:306-380): two HMVMs
  * per iteration, all scalars on the device.  use_graph != 0 captures one
- * iteration in a hipGraph and replays it.  The residual is read back every
- * check_every iterations (0: only at the end) and the loop stops when
- * ||r||/||b|| < tol (tol <= 0: run max_iter iterations).  u is the initial
- * guess on entry and the iterate on return (host pointers). */
+ * iteration in a hipGraph and replays it.  As in the reference, the solve
+ * stops at the first iteration whose ||r||/||b|| is below tol (tol <= 0: run
+ * max_iter iterations): the device records that iteration and turns the
+ * updates of later ones into no-ops, so u and iters are exactly those of the
+ * reference loop even though the host only reads the state back every
+ * check_every iterations (0: only at the end) and launches whole chunks.  A
+ * zero divisor or a non-finite scalar stops the updates the same way and the
+ * call returns HMVM_ERR_BREAKDOWN with u the last finite iterate.  u is the
+ * initial guess on entry and the iterate on return (host pointers). */
 typedef struct hmvm_krylov_result {
-  int iters;          /* iterations run                                        */
-  int hmvms;          /* HMVMs applied, 2 per iteration + 1 for the residual   */
+  int iters;          /* iterations that updated u (the reference's nstp)      */
+  int hmvms;          /* HMVMs the solve needed: 2 per iteration + 1           */
   double relres;      /* ||r||/||b|| of the recurrence at the end              */
-  double ms_loop;     /* device time of the iteration loop                     */
+  double ms_loop;     /* device time of the iteration loop (0 when not timed)  */
   double ms_per_iter;
   int graph_used;
 } hmvm_krylov_result;

@@ -248,6 +248,7 @@ def test_persistent_mode_matches_launches(gpu, H, sphere4096, storage):
     assert p.info()["t_fused"]
     y0 = p.apply_host(b)
     p.set_mode(1)
+    assert p.info()["mode"] == 1  # the persistent launch is what runs
     for k in (0, 6):
         p.set_launch(k, 256)
         for _ in range(3):
@@ -259,6 +260,27 @@ def test_persistent_mode_matches_launches(gpu, H, sphere4096, storage):
     t = p.time(2, 5)
     assert t["ms_stream"] > 0 and p.fault() == 0
     p.close()
+
+
+def test_persistent_mode_needs_fused_stage2(gpu, H, sphere4096):
+    """Mode 1 has no launch boundary for a t sum: a plan whose stage-2 tiles
+    read a materialised t refuses it (no silent fallback to the launches),
+    and mode 1 at creation selects t_mode 0 unless told otherwise."""
+    h = sphere4096
+    b = np.random.default_rng(17).uniform(-1, 1, h.nd)
+    p = H.Plan(h)  # default t_mode 1: a materialised t
+    assert not p.info()["t_fused"] and p.info()["mode"] == 0
+    with pytest.raises(H.HMVMError, match="t_mode 0"):
+        p.set_mode(1)
+    assert p.info()["mode"] == 0
+    y0 = p.apply_host(b)
+    p.close()
+    p = H.Plan(h, mode=1)
+    assert p.info()["t_fused"] and p.info()["mode"] == 1
+    assert rel(p.apply_host(b), y0) <= 1e-14 and p.fault() == 0
+    p.close()
+    with pytest.raises(H.HMVMError, match="t_mode 0"):
+        H.Plan(h, mode=1, t_mode=1)
 
 
 def test_modes_regroup_launch_a(gpu, H, sphere4096):
