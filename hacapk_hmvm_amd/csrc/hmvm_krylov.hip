@@ -26,8 +26,21 @@
 #include <vector>
 
 #include "hmvm_hip.h"
+#include "hmvm_status.hpp"
 
 namespace {
+
+int hip_fail(const char *what, hipError_t e) {
+  char m[512];
+  std::snprintf(m, sizeof m, "%s failed: %s", what, hipGetErrorString(e));
+  return hmvm::report_error(HMVM_ERR_HIP, m);
+}
+
+#define KRY_TRY(expr)                         \
+  do {                                        \
+    const hipError_t e_ = (expr);             \
+    if (e_ != hipSuccess) return hip_fail(#expr, e_); \
+  } while (0)
 
 constexpr int kBlock = 256;
 constexpr int kDotBlocks = 512;
@@ -207,7 +220,8 @@ struct Solver {
     if (rc) return rc;
     hipLaunchKernelGGL(bicg_init, grid(), dim3(kBlock), 0, st, n, b, r, sh, p, ap);
     dots(r, r, r, r, 1);
-    return hipGetLastError() == hipSuccess ? HMVM_OK : HMVM_ERR_HIP;
+    KRY_TRY(hipGetLastError());
+    return HMVM_OK;
   }
   // one BiCGSTAB iteration (m_HACApK_solve.f90:341-372)
   int iterate() {
@@ -221,12 +235,12 @@ struct Solver {
     dots(at, t, at, at, 3);
     hipLaunchKernelGGL(bicg_update, grid(), dim3(kBlock), 0, st, n, u, p, t, at, r, S);
     dots(sh, r, r, r, 4);
-    return hipGetLastError() == hipSuccess ? HMVM_OK : HMVM_ERR_HIP;
+    KRY_TRY(hipGetLastError());
+    return HMVM_OK;
   }
   int read_scalars(double *S_host) {
-    if (hipMemcpyAsync(S_host, S, sizeof(double) * kNumScalars, hipMemcpyDeviceToHost, st) != hipSuccess ||
-        hipStreamSynchronize(st) != hipSuccess)
-      return HMVM_ERR_HIP;
+    KRY_TRY(hipMemcpyAsync(S_host, S, sizeof(double) * kNumScalars, hipMemcpyDeviceToHost, st));
+    KRY_TRY(hipStreamSynchronize(st));
     return HMVM_OK;
   }
 };
@@ -243,16 +257,17 @@ int solve(Solver &s, int max_iter, int use_graph, int check_every, bool timed, h
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
   if (use_graph && S0[kStop] == 0.0 && max_iter > 0) {
-    if (hipStreamBeginCapture(s.st, hipStreamCaptureModeThreadLocal) != hipSuccess) return HMVM_ERR_HIP;
+    KRY_TRY(hipStreamBeginCapture(s.st, hipStreamCaptureModeThreadLocal));
     rc = s.iterate();
-    hipError_t e = hipStreamEndCapture(s.st, &graph);
+    const hipError_t e = hipStreamEndCapture(s.st, &graph);
     if (rc || e != hipSuccess) {
       if (graph) (void)hipGraphDestroy(graph);
-      return rc ? rc : HMVM_ERR_HIP;
+      return rc ? rc : hip_fail("hipStreamEndCapture", e);
     }
-    if (hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0) != hipSuccess) {
+    const hipError_t ei = hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0);
+    if (ei != hipSuccess) {
       (void)hipGraphDestroy(graph);
-      return HMVM_ERR_HIP;
+      return hip_fail("hipGraphInstantiate", ei);
     }
   }
   hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -265,9 +280,10 @@ int solve(Solver &s, int max_iter, int use_graph, int check_every, bool timed, h
   while (launched < max_iter && S0[kStop] == 0.0) {
     const int chunk = check_every > 0 ? std::min(check_every, max_iter - launched) : max_iter - launched;
     for (int c = 0; c < chunk && !rc; c++) {
-      if (gexec)
-        rc = hipGraphLaunch(gexec, s.st) == hipSuccess ? HMVM_OK : HMVM_ERR_HIP;
-      else
+      if (gexec) {
+        const hipError_t e = hipGraphLaunch(gexec, s.st);
+        rc = e == hipSuccess ? HMVM_OK : hip_fail("hipGraphLaunch", e);
+      } else
         rc = s.iterate();
     }
     if (rc) break;
@@ -291,14 +307,22 @@ int solve(Solver &s, int max_iter, int use_graph, int check_every, bool timed, h
   res->ms_loop = ms;
   res->ms_per_iter = launched > 0 ? ms / launched : 0.0;
   res->graph_used = gexec ? 1 : 0;
-  return S0[kStop] == 2.0 ? HMVM_ERR_BREAKDOWN : HMVM_OK;
+  if (S0[kStop] == 2.0) {
+    char m[256];
+    std::snprintf(m, sizeof m, "BiCGSTAB broke down after %d iterations (zero divisor or non-finite scalar)",
+                  res->iters);
+    return hmvm::report_error(HMVM_ERR_BREAKDOWN, m);
+  }
+  return HMVM_OK;
 }
 
 }  // namespace
 
 extern "C" int hmvm_bicgstab(hmvm_plan *plan, const double *b_host, double *u_host, int max_iter, double tol,
                              int use_graph, int check_every, hmvm_krylov_result *res) {
-  if (!plan || !b_host || !u_host || !res || max_iter < 0 || check_every < 0) return HMVM_ERR_ARG;
+  hmvm::report_clear();
+  if (!plan || !b_host || !u_host || !res || max_iter < 0 || check_every < 0)
+    return hmvm::report_error(HMVM_ERR_ARG, "hmvm_bicgstab: NULL argument or negative count");
   std::memset(res, 0, sizeof(*res));
   hmvm_plan_info info;
   int rc = hmvm_plan_get_info(plan, &info);
@@ -319,10 +343,10 @@ extern "C" int hmvm_bicgstab(hmvm_plan *plan, const double *b_host, double *u_ho
       if (st) (void)hipStreamDestroy(st);
     }
   } m;
-  if (hipStreamCreate(&m.st) != hipSuccess) return HMVM_ERR_HIP;
+  KRY_TRY(hipStreamCreate(&m.st));
   for (int i = 0; i < 10; i++) {
     const size_t bytes = i < 8 ? nb : i == 8 ? sizeof(double) * 2 * kDotBlocks : sizeof(double) * kNumScalars;
-    if (hipMalloc(&m.d[i], bytes) != hipSuccess) return HMVM_ERR_HIP;
+    KRY_TRY(hipMalloc(&m.d[i], bytes));
   }
   s.st = m.st;
   s.b = m.d[0];
@@ -335,11 +359,12 @@ extern "C" int hmvm_bicgstab(hmvm_plan *plan, const double *b_host, double *u_ho
   s.at = m.d[7];
   s.part = m.d[8];
   s.S = m.d[9];
-  if (hipMemcpyAsync(m.d[0], b_host, nb, hipMemcpyHostToDevice, s.st) != hipSuccess ||
-      hipMemcpyAsync(s.u, u_host, nb, hipMemcpyHostToDevice, s.st) != hipSuccess)
-    return HMVM_ERR_HIP;
+  KRY_TRY(hipMemcpyAsync(m.d[0], b_host, nb, hipMemcpyHostToDevice, s.st));
+  KRY_TRY(hipMemcpyAsync(s.u, u_host, nb, hipMemcpyHostToDevice, s.st));
   rc = solve(s, max_iter, use_graph, check_every, true, res);
-  if ((rc == HMVM_OK || rc == HMVM_ERR_BREAKDOWN) && hipMemcpy(u_host, s.u, nb, hipMemcpyDeviceToHost) != hipSuccess)
-    rc = HMVM_ERR_HIP;
+  if (rc == HMVM_OK || rc == HMVM_ERR_BREAKDOWN) {
+    const hipError_t e = hipMemcpy(u_host, s.u, nb, hipMemcpyDeviceToHost);
+    if (e != hipSuccess) rc = hip_fail("hipMemcpy (u)", e);
+  }
   return rc;
 }

@@ -18,6 +18,7 @@
 #include "hmvm_hip.h"
 #include "hmvm_kernels.hpp"
 #include "hmvm_plan.hpp"
+#include "hmvm_status.hpp"
 #include "io_records.hpp"
 
 using namespace hmvm;
@@ -41,6 +42,15 @@ void clear_error() {
   g_err = HMVM_OK;
   std::snprintf(g_msg, sizeof g_msg, "ok");
 }
+
+}  // namespace
+
+namespace hmvm {
+int report_error(int code, const char *msg) { return set_error(code, "%s", msg); }
+void report_clear() { clear_error(); }
+}  // namespace hmvm
+
+namespace {
 
 #define HIP_TRY(expr)                                                                      \
   do {                                                                                     \
