@@ -112,6 +112,15 @@ def seq_2_f32(h, b32, rowmat32):
     return v
 
 
+def convert_d2f(a):
+    """Element-wise FP64 -> FP32 (round to nearest), as convertD2F of
+    src_cpp/loadmatrix.cpp:252-342 converts every factor."""
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    out = np.empty(a.shape, dtype=np.float32)
+    lib().oracle_convert_d2f(_p(a), out.ctypes.data_as(c_float_p), a.size)
+    return out
+
+
 def omp_threads():
     return lib().oracle_omp_threads()
 

@@ -23,6 +23,47 @@ def rand_params(seed=1):
     return (rng.normal(size=n) * 0.2).astype(np.float32)
 
 
+# A 3 -> 2 -> 2 -> 1 network with hand-set weights and outputs worked out by
+# hand: standardisation (x - mean)/stdev, ReLU after both hidden layers, W
+# row-major as (out, in).  W2 is not symmetric and several units are cut by
+# the ReLU, so a transposed weight or a missing/wrong activation changes the
+# outputs (with W2 transposed the first output is 0.25, without ReLU 14.75).
+HAND = {
+    "nin": 3, "nh": 2,
+    "params": [1, 0, 2, 0, -1, 1,   # W1 (2 x 3)
+               0.5, -1,             # b1
+               1, 2, -3, 1,         # W2 (2 x 2)
+               0, 0.25,             # b2
+               2, -1,               # w3
+               0.5],                # b3
+    "mean": [1, 0, -1], "stdev": [2, 1, 0.5],
+    "X": [[3, 1, -0.5], [1, -2, -1], [-1, 3, -2], [1, 0, -1]],
+    # x_s = (1,1,1): h1 = relu(3.5, -1) = (3.5, 0), h2 = relu(3.5, -10.25), out = 2*3.5 + 0.5
+    # x_s = (0,-2,0): h1 = (0.5, 1), h2 = relu(2.5, -0.25), out = 5 + 0.5
+    # x_s = (-1,3,-2): h1 = relu(-4.5, -6) = 0, h2 = (0, 0.25), out = -0.25 + 0.5
+    # x_s = 0: h1 = (0.5, 0), h2 = relu(0.5, -1.25), out = 1 + 0.5
+    "out": [7.5, 5.5, 0.25, 1.5],
+}
+
+
+def test_oracle_forward_closed_form():
+    h = HAND
+    out, _ = cost_mlp.forward(np.array(h["params"], float), h["mean"], h["stdev"], np.array(h["X"], float),
+                              h["nin"], h["nh"])
+    assert np.array_equal(out, np.array(h["out"]))
+
+
+@pytest.mark.gpu
+def test_hip_forward_closed_form(gpu, H):
+    h = HAND
+    m = H.CostMLP(h["nin"], h["nh"], seed=1)
+    m.set(params=np.array(h["params"], np.float32), mean=np.array(h["mean"], np.float32),
+          stdev=np.array(h["stdev"], np.float32))
+    pred = m.predict(np.array(h["X"], np.float32))
+    assert np.allclose(pred, h["out"], rtol=0, atol=1e-6)
+    m.close()
+
+
 def test_oracle_gradient_matches_finite_differences():
     X, y = rand_problem(60, seed=3)
     X = X.astype(np.float64)

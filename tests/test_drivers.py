@@ -68,3 +68,50 @@ def test_cpp_driver_dumps_match_oracle(gpu, H, O, drivers, tmp_path):
         assert np.max(np.abs(got - ref) / np.abs(ref)) <= 1e-3  # %.3E dump
     gotf = np.array([float(v) for v in open(tmp_path / "result_hip_k-1_f.txt")])
     assert np.max(np.abs(gotf - ref) / np.abs(ref)) <= 2e-3
+
+
+def read_convert_dump(path):
+    recs = []
+    with open(path, "rb") as f:
+        count = int(np.frombuffer(f.read(4), dtype=np.int32)[0])
+        for _ in range(count):
+            n = int(np.frombuffer(f.read(8), dtype=np.int64)[0])
+            d = np.frombuffer(f.read(8 * n), dtype=np.float64)
+            s = np.frombuffer(f.read(4 * n), dtype=np.float32)
+            same = int(np.frombuffer(f.read(4), dtype=np.int32)[0])
+            recs.append((d, s, same))
+    return recs
+
+
+def test_convertD2F_rounds_every_element(H, tmp_path):
+    """The product convertD2F (C++ API) on a matrix whose values are the
+    IEEE tie, subnormal and overflow cases, and on the golden files (low-rank
+    leaves: a1, a2 and a2t): every converted element of rowmat, rowmat_t and
+    the per-leaf arrays is the double rounded to the nearest float32, and the
+    descriptors and offsets are unchanged (src_cpp/loadmatrix.cpp:252-342)."""
+    from f32_rounding import CASES, case_values, check, random_values
+    exe = str(tmp_path / "convert_dump")
+    subprocess.run(["g++", "-std=c++14", "-O2", "-I" + INC, os.path.join(DRV, "convert_dump.cpp"), "-o", exe,
+                    "-L" + LIBDIR, "-lhmvm_hip", "-Wl,-rpath," + LIBDIR], check=True)
+    d = H.dummymatrix(4, 4, 4)  # 16 dense 4 x 4 leaves, 256 values
+    vals = np.concatenate([case_values(), random_values(256 - len(CASES), 9)])
+    vals[-1] = np.nan
+    a = d.arrays()
+    a["rowmat"][:] = vals
+    special = str(tmp_path / "special_1.bin")
+    H.writeHmatrix1(special, d)
+    files = [special, os.path.join(GOLDEN, "sphere320_1.bin"), os.path.join(GOLDEN, "cube384_1.bin")]
+    for f in files:
+        out = str(tmp_path / "conv.bin")
+        r = subprocess.run([exe, f, out], stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True, timeout=120)
+        assert r.returncode == 0, r.stdout
+        recs = read_convert_dump(out)
+        assert len(recs) >= 2
+        if f == special:  # the loader handed over the crafted values
+            assert np.array_equal(recs[0][0].view(np.uint64)[:-1], vals.view(np.uint64)[:-1])
+            assert np.isnan(recs[0][0][-1])
+        for src, got, same in recs:
+            assert same == 1
+            check(src, got)
+    recs = read_convert_dump(str(tmp_path / "conv.bin"))  # the golden file has low-rank leaves: 3 records each
+    assert any(len(recs[i][0]) != len(recs[i + 1][0]) for i in range(2, len(recs) - 1))

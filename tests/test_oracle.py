@@ -170,3 +170,17 @@ def test_planted_mistakes_fail_golden(H, O):
     assert rel(O.seq_2(h, x), y) > 1e-8
     a["rowmat"][:] = saved
     assert np.array_equal(O.seq_2(h, x), y)
+
+
+def test_convert_d2f_rounds_to_nearest_even(O):
+    """oracle_convert_d2f against the IEEE 754 results fixed by the rules
+    (ties to even, subnormals, overflow to infinity, signed zero), and against
+    numpy's conversion on doubles spread over the whole range; NaN stays NaN.
+    A truncating or ties-away conversion fails the tie cases."""
+    from f32_rounding import case_bits, case_values, check, random_values
+    got = O.convert_d2f(case_values())
+    assert np.array_equal(got.view(np.uint32), case_bits())
+    r = random_values(200000, 5)
+    check(r, O.convert_d2f(r))
+    n = O.convert_d2f(np.array([np.nan, -np.nan, 1.0]))
+    assert np.isnan(n[0]) and np.isnan(n[1]) and n[2] == 1.0
