@@ -44,6 +44,11 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# relative l2 error bars against the FP64 oracle: FP64 the oracle bar, FP32
+# storage the FP32 target, BF16 factors (8 significant bits) a sanity bound;
+# a run that misses one still prints its line, then exits with status 1
+ERR_BAR = {"f64": 1e-13, "f32": 1e-6, "bf16": 1e-2}
+ERR_BAR_MULTI = {"f64": 1e-13, "f32": 2e-6, "bf16": 1e-2}  # several right-hand sides (FP32 MFMA accumulation)
 MFMA_PEAK_TFLOPS = {"f64": 78.6, "f32": 157.3, "bf16": 2500.0}
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_c2.json")
 
@@ -153,14 +158,17 @@ def run_c2(args, H, np):
                               "ysum": tm["ms_ysum"]}}
 
     forms = {st: form(st) for st in ("f64", "f32", "bf16")}
+    import oracle as O  # the checker, and the CPU baseline leg timed on the host cores
+    t0 = time.time()
+    y_or = O.seq_2(h, x)
+    seq_s = time.time() - t0
+    for st in forms:
+        e = rel(ys[st], y_or)
+        forms[st]["rel_err_vs_oracle"] = e
+        forms[st]["rel_err_over_aca_eps"] = round(e / args.eps, 6)
+        forms[st]["err_bar"] = ERR_BAR[st]
     cpu = None
     if not args.no_cpu_baseline:
-        import oracle as O  # CPU baseline leg: the oracle, timed on the host cores
-        t0 = time.time()
-        y_or = O.seq_2(h, x)
-        seq_s = time.time() - t0
-        for st in forms:
-            forms[st]["rel_err_vs_oracle"] = rel(ys[st], y_or)
         omp_ms = omp_time(O.omp_2, h, x)
         cpu = {"value": round(omp_ms, 3), "unit": "ms/HMVM", "cores": O.omp_threads(), "kind": "port",
                "sample": "full workload (N=%d, len=%d): oracle restatement of hmvm_omp_2, 10 runs, mean of the last "
@@ -184,6 +192,8 @@ def run_c2(args, H, np):
                      "unit": "GB/s", "frac": round(k_gbs / HBM_PEAK_GBS, 4), "traffic": traffic},
         "achievable_GBps": {"read_stream": round(probe_read, 1), "copy": round(probe_copy, 1)},
         "cpu_baseline": cpu,
+        "aca_eps": args.eps,
+        "errors_within_bars": all(forms[st]["rel_err_vs_oracle"] <= ERR_BAR[st] for st in forms),
     })
     return line
 
@@ -245,7 +255,10 @@ def run_c3(args, H, np):
         ms = tm["ms_stream"]
         tflops = 2.0 * info["len"] * nrhs / (ms * 1e-3) / 1e12
         b_alg = info["elem_bytes"] * info["len"] + 16 * h.nd * nrhs
-        forms[st] = {"ms": round(ms, 4), "rel_err_vs_fp64_oracle": rel(Y, Y_or), "TFLOPs": round(tflops, 2),
+        e = max(rel(Y[:, c], Y_or[:, c]) for c in range(nrhs))
+        forms[st] = {"ms": round(ms, 4), "rel_err_vs_fp64_oracle": rel(Y, Y_or), "max_column_rel_err": e,
+                     "rel_err_over_aca_eps": round(e / 1e-4, 6), "err_bar": ERR_BAR_MULTI[st],
+                     "TFLOPs": round(tflops, 2),
                      "mfma_frac_of_dense_peak": round(tflops / MFMA_PEAK_TFLOPS[st], 4),
                      "hbm_frac": round(b_alg / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
                      "launch_ms": {"tiles_a": tm["ms_a"], "tsum": tm["ms_tsum"], "tiles_b": tm["ms_b"],
@@ -256,6 +269,7 @@ def run_c3(args, H, np):
                      forms["f64"]["ms"], "ms/HMVM(32 RHS)", args, "f64",
                      {"workload": "HMVM Y=A*X, X N x 32, N=101250 cube, ACA eps=1e-4", "len": h.mat2.len})
     line["forms"] = forms
+    line["errors_within_bars"] = all(forms[st]["max_column_rel_err"] <= ERR_BAR_MULTI[st] for st in forms)
     bound = max(8.0 * h.mat2.len / (HBM_PEAK_GBS * 1e9), 2.0 * h.mat2.len * nrhs / (MFMA_PEAK_TFLOPS["f64"] * 1e12))
     line["roofline"] = {"bound": "hbm", "achieved": round(bound * 1e3 / forms["f64"]["ms"] * HBM_PEAK_GBS, 1),
                         "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(bound * 1e3 / forms["f64"]["ms"], 4),
@@ -362,6 +376,8 @@ def run_c4(args, H, np):
     line.update({"ms_launches": round(tm["ms_stream"], 5), "ms_graph": round(tg["ms_stream"], 5),
                  "hbm_frac": round(info["algorithmic_bytes"] / (tg["ms_stream"] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
                  "rel_err_vs_fp64_oracle_max": max(errs), "rel_err_vs_fp64_oracle_median": float(np.median(errs)),
+                 "rel_err_over_aca_eps_max": max(e / ep for e, ep in zip(errs, eps)),
+                 "errors_within_bars": max(errs) <= ERR_BAR["f32"],
                  "tuning": tuning})
     return line
 
@@ -389,6 +405,7 @@ def run_c5(args, H, np):
         tm = p.time(3, 10)
         u, res = p.bicgstab(b, max_iter=50, tol=0.0, use_graph=True, check_every=0)
         forms[st] = {"ms_hmvm": round(tm["ms_stream"], 4), "rel_err_vs_fp64_oracle": rel(y, y_or),
+                     "rel_err_over_aca_eps": round(rel(y, y_or) / 1e-6, 6), "err_bar": ERR_BAR[st],
                      "hbm_frac": round(info["algorithmic_bytes"] / (tm["ms_stream"] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
                      "bicgstab": {"iters": res["iters"], "hmvms": res["hmvms"], "ms_loop": round(res["ms_loop"], 3),
                                   "ms_per_hmvm_in_loop": round(res["ms_loop"] / (2 * res["iters"]), 4),
@@ -412,6 +429,7 @@ def run_c5(args, H, np):
                                   "ACA eps=1e-6, int64 offsets" % h.nd, "len": int(h.mat2.len)})
     line["forms"] = forms
     line["tuning_band"] = tuning
+    line["errors_within_bars"] = all(forms[st]["rel_err_vs_fp64_oracle"] <= ERR_BAR[st] for st in forms)
     return line
 
 
@@ -445,7 +463,11 @@ def main():
     if H.device_count() < 1:
         raise SystemExit("bench.py: no HIP device visible")
     fn = {"c1": run_c1, "c2": run_c2, "c3": run_c3, "c4": run_c4, "c5": run_c5, "tune": run_tune}[args.config]
-    print(json.dumps(fn(args, H, np)), flush=True)
+    line = fn(args, H, np)
+    print(json.dumps(line), flush=True)
+    if line.get("errors_within_bars") is False:
+        log("bench.py: a relative error exceeds its bar (%s)" % ERR_BAR)
+        sys.exit(1)
 
 
 if __name__ == "__main__":
