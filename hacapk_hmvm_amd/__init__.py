@@ -259,6 +259,9 @@ def _load():
                                                 P(hmvm_timing)]),
         "hmvm_bicgstab": (ctypes.c_int, [ctypes.c_void_p, c_double_p, c_double_p, ctypes.c_int, ctypes.c_double,
                                          ctypes.c_int, ctypes.c_int, P(hmvm_krylov_result)]),
+        "hmvm_bicgstab_static": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
+                                                ctypes.c_double, ctypes.c_int, ctypes.c_void_p,
+                                                P(hmvm_krylov_result)]),
         "hmvm_plan_check": (ctypes.c_int, [P(matrix2), P(hmvm_options), P(hmvm_plan_info)]),
         "hmvm_plan_set_launch": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]),
         "hmvm_plan_set_mode": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),

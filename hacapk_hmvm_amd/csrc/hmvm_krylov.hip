@@ -23,7 +23,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
-#include <vector>
+#include <mutex>
 
 #include "hmvm_hip.h"
 #include "hmvm_status.hpp"
@@ -367,4 +367,76 @@ extern "C" int hmvm_bicgstab(hmvm_plan *plan, const double *b_host, double *u_ho
     if (e != hipSuccess) rc = hip_fail("hipMemcpy (u)", e);
   }
   return rc;
+}
+
+// ---- the allocation-free variant ----------------------------------------------
+// Work space of hmvm_bicgstab_static: the six work vectors of a solve of at
+// most kStaticMaxN unknowns, the dot partials and the scalars, in one array
+// of the library's code object (48 MiB + 8 KiB; each device the library runs
+// on holds its own copy until the process ends).  Every call on a device
+// uses that device's copy: calls that overlap on one device race.
+constexpr int kStaticMaxN = HMVM_BICGSTAB_STATIC_MAX_N;
+constexpr size_t kScratchDoubles = 6 * (size_t)kStaticMaxN + 2 * kDotBlocks + kNumScalars;
+static __device__ __align__(256) double scratch[kScratchDoubles];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double *g_scratch_ptr[kMaxDevices] = {};
+
+// The device address of `scratch` on the current device, resolved once per
+// device (host code never passes the symbol's own name to a kernel: on the
+// host it is only a stand-in).
+int scratch_on_current_device(double **out) {
+  int dev = -1;
+  KRY_TRY(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices)
+    return hmvm::report_error(HMVM_ERR_UNSUPPORTED, "hmvm_bicgstab_static: device index beyond 63");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void *p = nullptr;
+    KRY_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    g_scratch_ptr[dev] = (double *)p;
+  }
+  *out = g_scratch_ptr[dev];
+  return HMVM_OK;
+}
+
+}  // namespace
+
+extern "C" int hmvm_bicgstab_static(hmvm_plan *plan, const double *d_b, double *d_u, int max_iter, double tol,
+                                    int check_every, void *stream, hmvm_krylov_result *res) {
+  hmvm::report_clear();
+  if (!plan || !d_b || !d_u || !res || max_iter < 0 || check_every < 0)
+    return hmvm::report_error(HMVM_ERR_ARG, "hmvm_bicgstab_static: NULL argument or negative count");
+  std::memset(res, 0, sizeof(*res));
+  hmvm_plan_info info;
+  int rc = hmvm_plan_get_info(plan, &info);
+  if (rc) return rc;
+  if (info.nd > kStaticMaxN) {
+    char m[256];
+    std::snprintf(m, sizeof m, "hmvm_bicgstab_static: nd = %d exceeds the fixed work space (at most %d unknowns); "
+                  "use hmvm_bicgstab", info.nd, kStaticMaxN);
+    return hmvm::report_error(HMVM_ERR_ARG, m);
+  }
+  double *w = nullptr;
+  if ((rc = scratch_on_current_device(&w))) return rc;
+  Solver s;
+  s.n = info.nd;
+  s.tol = tol;
+  s.plan = plan;
+  s.st = (hipStream_t)stream;
+  s.b = d_b;
+  s.u = d_u;
+  const size_t n = (size_t)s.n;
+  s.r = w;
+  s.sh = w + n;
+  s.p = w + 2 * n;
+  s.ap = w + 3 * n;
+  s.t = w + 4 * n;
+  s.at = w + 5 * n;
+  s.part = w + 6 * n;
+  s.S = s.part + 2 * kDotBlocks;
+  return solve(s, max_iter, 0, check_every, false, res);
 }

@@ -340,6 +340,21 @@ typedef struct hmvm_krylov_result {
 int hmvm_bicgstab(hmvm_plan *plan, const double *b, double *u, int max_iter, double tol, int use_graph,
                   int check_every, hmvm_krylov_result *result);
 
+/* hmvm_bicgstab without any device allocation.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (tight solver
+ * loops, harnesses that forbid hipMalloc).  b and u are DEVICE vectors of nd
+ * doubles (u: initial guess on entry, iterate on return); the work vectors
+ * live in a fixed array of the library, one per device, so nd is at most
+ * HMVM_BICGSTAB_STATIC_MAX_N (larger plans get HMVM_ERR_ARG).  Runs on
+ * `stream` (NULL: the null stream) with the plan's device current, eagerly
+ * (no graph), and returns when the solve has stopped; ms_loop is not timed
+ * (0).  Iterates, stopping rule and results are those of hmvm_bicgstab with
+ * use_graph = 0.  Every call on a device shares that device's work array:
+ * calls that overlap on one device race. */
+#define HMVM_BICGSTAB_STATIC_MAX_N 1048576
+int hmvm_bicgstab_static(hmvm_plan *plan, const double *d_b, double *d_u, int max_iter, double tol,
+                         int check_every, void *stream, hmvm_krylov_result *result);
+
 /* ---- schedule introspection (host only) -------------------------------------
  * The schedule a plan runs, as plain arrays: the autotuner's features, and
  * the CPU tests that replay it.  Element (r, j) of a tile's panel p is
