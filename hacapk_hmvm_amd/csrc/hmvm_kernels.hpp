@@ -268,8 +268,11 @@ __device__ __forceinline__ void wide_tile(const TileDesc &td, int shift, int lpc
   for (int q = 0; q < R; ++q)
 #pragma unroll
     for (int v = 0; v < V; ++v) acc[q][v] = 0;
+  PanelDesc pd_next = panels[td.panel0];
   for (int p = 0; p < td.npanel; ++p) {
-    const PanelDesc pd = panels[td.panel0 + p];
+    const PanelDesc pd = pd_next;
+    // the next panel's descriptor is requested before this panel streams
+    if (p + 1 < td.npanel) pd_next = panels[td.panel0 + p + 1];
     A accp[R][V];
 #pragma unroll
     for (int q = 0; q < R; ++q)
@@ -370,35 +373,41 @@ __device__ __forceinline__ void grouped_tile(const TileDesc &td, int shift, int 
   for (int v = 0; v < V; ++v) acc[v] = 0;
   // columns [0, nc) at base (column stride ld); entry(k) gives the scaled
   // vector entry of column k
+  auto load_batch = [&](T (&a)[kLoads], const S *__restrict__ base, int ld, int c0, int cn, int sb) {
+#pragma unroll
+    for (int u = 0; u < kLoads; ++u) {
+      const int j = sb * step + u * g + jg;
+      a[u] = live && j < cn ? load_wide<S, kNT>(base + (int64_t)(c0 + j) * ld) : T{};
+    }
+  };
   auto run_columns = [&](const S *__restrict__ base, int ld, int nc, auto entry) {
     for (int c0 = 0; c0 < nc; c0 += chunk) {
       const int cn = nc - c0 < chunk ? nc - c0 : chunk;
       wave_lds_sync();  // the previous chunk's LDS reads are done
-      for (int q0 = 0; q0 < kStage / 64 && q0 * 64 < cn; q0 += 4) {
-        double e[4];
+      // The chunk's vector entries and its first batch of matrix words are
+      // requested together, entries first: the matrix stream does not wait
+      // for the entries' round trip (one memory latency less per chunk), and
+      // the entries can be stored while the matrix words are in flight.
+      double e[kStage / 64];
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          const int k = (q0 + q) * 64 + lane;
-          e[q] = k < cn ? entry(c0 + k) : 0.0;
-        }
-#pragma unroll
-        for (int q = 0; q < 4; ++q) lds[(q0 + q) * 64 + lane] = e[q];
+      for (int q = 0; q < kStage / 64; ++q) {
+        const int k = q * 64 + lane;
+        e[q] = q * 64 < cn && k < cn ? entry(c0 + k) : 0.0;
       }
+      T a[kLoads];
+      load_batch(a, base, ld, c0, cn, 0);
+#pragma unroll
+      for (int q = 0; q < kStage / 64; ++q)
+        if (q * 64 < cn) lds[q * 64 + lane] = e[q];
       wave_lds_sync();
       const int nb = (cn + step - 1) / step;
       for (int sb = 0; sb < nb; ++sb) {
-        T a[kLoads];
+        if (sb > 0) load_batch(a, base, ld, c0, cn, sb);
         A b[kLoads];
 #pragma unroll
         for (int u = 0; u < kLoads; ++u) {
           const int j = sb * step + u * g + jg;
-          if (live && j < cn) {
-            a[u] = load_wide<S, kNT>(base + (int64_t)(c0 + j) * ld);
-            b[u] = (A)lds[j];
-          } else {
-            a[u] = T{};
-            b[u] = 0;
-          }
+          b[u] = live && j < cn ? (A)lds[j] : (A)0;
         }
         // (no software pipelining here: a second batch of registers costs
         // more occupancy than it gains, measured; the wide path pipelines)

@@ -4,10 +4,12 @@
 // It predicts the duration of a tile (one wavefront's share of a launch) from
 // the tile's features; the schedule search uses the predictions to choose
 // tile widths, launch variants and the dispatch order without timing every
-// candidate.  FP32 throughout.  The network is tiny (nh <= 64), so the
-// kernels are plain per-sample loops with the parameters staged in LDS;
-// weight gradients are reduced over samples in fixed-size chunks and the
-// chunks are summed in a fixed order, so a gradient is bit-reproducible.
+// candidate.  FP32 throughout.  The network is tiny (nh <= 64): one
+// workgroup takes 64 samples through the forward and backward passes with
+// the parameters and activations in LDS (register-tiled matrix products),
+// and one more launch adds the chunks' gradients in chunk order and applies
+// the Adam step, so a training step is two launches and a gradient is
+// bit-reproducible.
 //
 // Parameter vector: W1 (nh x nin), b1 (nh), W2 (nh x nh), b2 (nh), w3 (nh), b3.
 #include <hip/hip_runtime.h>
@@ -26,7 +28,7 @@ namespace {
 
 constexpr int kMaxIn = 32;
 constexpr int kMaxH = 64;
-constexpr int kChunk = 256;  // samples per weight-gradient partial
+constexpr int kChunk = 64;  // samples per workgroup (one gradient partial)
 
 struct Layout {
   int nin, nh;
@@ -42,151 +44,289 @@ struct Layout {
   }
 };
 
-// Forward pass of rows idx[s] (or s when idx is NULL); stores the
-// standardised input, both hidden activations and the output.
-__global__ __launch_bounds__(256) void mlp_forward_kernel(int n, int nin, int nh, const float *__restrict__ X,
-                                                          const int *__restrict__ idx,
-                                                          const float *__restrict__ params,
-                                                          const float *__restrict__ mean,
-                                                          const float *__restrict__ inv_std,
-                                                          float *__restrict__ XS, float *__restrict__ H1,
-                                                          float *__restrict__ H2, float *__restrict__ out) {
-  extern __shared__ float sp[];
-  const Layout L(nin, nh);
-  for (int i = threadIdx.x; i < L.total; i += blockDim.x) sp[i] = params[i];
-  __syncthreads();
-  const int s = (int)(blockIdx.x * blockDim.x + threadIdx.x);
-  if (s >= n) return;
-  const int row = idx ? idx[s] : s;
-  float xs[kMaxIn];
-  for (int i = 0; i < nin; ++i) {
-    xs[i] = (X[(int64_t)row * nin + i] - mean[i]) * inv_std[i];
-    if (XS) XS[(int64_t)s * nin + i] = xs[i];
+// ---- one chunk of samples per workgroup -------------------------------------
+// A workgroup of 256 threads takes kChunk = 64 samples (rows idx[s], or s
+// when idx is NULL) through the whole network: the parameters and every
+// activation of the chunk live in LDS, and each of the five matrix products
+//   Z1 = Xs W1^T (64 x nh, K = nin)     Z2 = H1 W2^T (64 x nh, K = nh)
+//   D1 = D2 W2   (64 x nh, K = nh)      gW2 = D2^T H1 (nh x nh, K = 64)
+//   gW1 = D1^T Xs (nh x nin, K = 64)
+// is computed with 4 x 4 register tiles (thread t: rows 4(t/16) .. +3,
+// columns t%16 + 16c, c < 4), so one LDS word feeds four FMAs.  Sums over
+// the chunk's samples run in sample order inside a thread; the chunk's
+// gradient goes to its own row of `partial`, and the chunks are added in
+// chunk order afterwards, so a gradient is bit-reproducible.
+// Samples past n in the last chunk carry zero error and add nothing.
+constexpr int kThreads = 256;
+
+// LDS row strides (floats): odd, so the rows a column of threads reads fall
+// in different banks.
+__host__ __device__ constexpr int pad_odd(int k) { return k | 1; }
+
+struct ChunkLds {  // float offsets into the dynamic LDS array
+  int w1, b1, w2, b2, w3, b3, xs, h1, h2, e, o, yt, total;
+  __host__ __device__ ChunkLds(int nin, int nh) {
+    const int si = pad_odd(nin), sh = pad_odd(nh);
+    w1 = 0;                 // nh rows of nin (stride si)
+    b1 = w1 + nh * si;
+    w2 = b1 + nh;           // nh rows of nh (stride sh)
+    b2 = w2 + nh * sh;
+    w3 = b2 + nh;
+    b3 = w3 + nh;
+    xs = b3 + 1;            // kChunk rows of nin (stride si)
+    h1 = xs + kChunk * si;  // kChunk rows of nh: H1, later D1
+    h2 = h1 + kChunk * sh;  // kChunk rows of nh: H2, later D2
+    e = h2 + kChunk * sh;   // kChunk output errors (out - y)/n
+    o = e + kChunk;         // kChunk outputs
+    yt = o + kChunk;        // kChunk targets
+    total = yt + kChunk;
   }
-  float h1[kMaxH];
-  for (int k = 0; k < nh; ++k) {
-    float a = sp[L.b1 + k];
-    for (int i = 0; i < nin; ++i) a += sp[L.w1 + k * nin + i] * xs[i];
-    h1[k] = a > 0.f ? a : 0.f;
-    if (H1) H1[(int64_t)s * nh + k] = h1[k];
+};
+
+// C[r][c] (+)= sum_k A[r][k] * B[c][k] for r < 64, c < ncols (row strides
+// sa, sb): thread t holds rows 4(t/16)+i and columns t%16 + 16j.
+__device__ __forceinline__ void tile_rows_cols(const float *A, int sa, const float *B, int sb, int K, int ncols,
+                                               float (&acc)[4][4]) {
+  const int r0 = 4 * ((int)threadIdx.x >> 4), c0 = (int)threadIdx.x & 15;
+  for (int k = 0; k < K; ++k) {
+    float a[4], b[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) a[i] = A[(r0 + i) * sa + k];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) b[j] = c0 + 16 * j < ncols ? B[(c0 + 16 * j) * sb + k] : 0.f;
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) acc[i][j] += a[i] * b[j];
   }
-  float o = sp[L.b3];
-  for (int j = 0; j < nh; ++j) {
-    float a = sp[L.b2 + j];
-    for (int k = 0; k < nh; ++k) a += sp[L.w2 + j * nh + k] * h1[k];
-    a = a > 0.f ? a : 0.f;
-    if (H2) H2[(int64_t)s * nh + j] = a;
-    o += sp[L.w3 + j] * a;
-  }
-  out[s] = o;
 }
 
-// Output error and hidden deltas of every sample:
-//   e = (out - y)/n, D2[j] = e w3[j] [H2>0], D1[k] = sum_j W2[j][k] D2[j] [H1>0].
-__global__ __launch_bounds__(256) void mlp_delta_kernel(int n, int nin, int nh, float inv_n,
-                                                        const float *__restrict__ y, const int *__restrict__ idx,
-                                                        const float *__restrict__ params,
-                                                        const float *__restrict__ H1, const float *__restrict__ H2,
-                                                        const float *__restrict__ out, float *__restrict__ E,
-                                                        float *__restrict__ D1, float *__restrict__ D2) {
-  extern __shared__ float sp[];
-  const Layout L(nin, nh);
-  for (int i = threadIdx.x; i < L.total; i += blockDim.x) sp[i] = params[i];
-  __syncthreads();
-  const int s = (int)(blockIdx.x * blockDim.x + threadIdx.x);
-  if (s >= n) return;
-  const int row = idx ? idx[s] : s;
-  const float e = (out[s] - y[row]) * inv_n;
-  E[s] = e;
-  float d2[kMaxH];
-  for (int j = 0; j < nh; ++j) {
-    d2[j] = H2[(int64_t)s * nh + j] > 0.f ? e * sp[L.w3 + j] : 0.f;
-    D2[(int64_t)s * nh + j] = d2[j];
+// C[r][c] = sum_s A[s][r] * B[s][c] over the chunk's samples s (the
+// transposed products of the weight gradients): rows r < nr, columns c < nc.
+__device__ __forceinline__ void tile_grad(const float *A, int sa, int nr, const float *B, int sb, int nc,
+                                          float (&acc)[4][4], int r0, int c0) {
+  for (int s = 0; s < kChunk; ++s) {
+    float a[4], b[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) a[i] = r0 + i < nr ? A[s * sa + r0 + i] : 0.f;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) b[j] = c0 + 16 * j < nc ? B[s * sb + c0 + 16 * j] : 0.f;
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) acc[i][j] += a[i] * b[j];
   }
-  for (int k = 0; k < nh; ++k) {
+}
+
+// kMode 0: predict (out[s] for the chunk's samples); 1: loss and gradient
+// partials of the chunk (partial[chunk * total + p], loss_part[chunk]).
+template <int kMode>
+__global__ __launch_bounds__(kThreads) void mlp_chunk_kernel(
+    int n, int nin, int nh, float inv_n, const float *__restrict__ X, const float *__restrict__ y,
+    const int *__restrict__ idx, const float *__restrict__ params, const float *__restrict__ mean,
+    const float *__restrict__ inv_std, float *__restrict__ out, float *__restrict__ partial,
+    float *__restrict__ loss_part) {
+  extern __shared__ float lds[];
+  const Layout L(nin, nh);
+  const ChunkLds C(nin, nh);
+  const int si = pad_odd(nin), sh = pad_odd(nh);
+  const int tid = (int)threadIdx.x;
+  const int s0 = (int)blockIdx.x * kChunk;
+  const int ns = min(kChunk, n - s0);
+  // parameters (global layout) -> LDS (padded rows)
+  for (int p = tid; p < L.total; p += kThreads) {
+    const float v = params[p];
+    if (p < L.b1) lds[C.w1 + (p / nin) * si + p % nin] = v;
+    else if (p < L.w2) lds[C.b1 + p - L.b1] = v;
+    else if (p < L.b2) lds[C.w2 + ((p - L.w2) / nh) * sh + (p - L.w2) % nh] = v;
+    else if (p < L.w3) lds[C.b2 + p - L.b2] = v;
+    else if (p < L.b3) lds[C.w3 + p - L.w3] = v;
+    else lds[C.b3] = v;
+  }
+  // standardised inputs (zero rows past n)
+  for (int q = tid; q < kChunk * nin; q += kThreads) {
+    const int s = q / nin, i = q - s * nin;
+    float v = 0.f;
+    if (s < ns) {
+      const int row = idx ? idx[s0 + s] : s0 + s;
+      v = (X[(int64_t)row * nin + i] - mean[i]) * inv_std[i];
+    }
+    lds[C.xs + s * si + i] = v;
+  }
+  __syncthreads();
+  const int r0 = 4 * (tid >> 4), c0 = tid & 15;
+  float acc[4][4];
+  // H1 = relu(Xs W1^T + b1)
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[i][j] = 0.f;
+  tile_rows_cols(lds + C.xs, si, lds + C.w1, si, nin, nh, acc);
+#pragma unroll
+  for (int j = 0; j < 4; ++j)
+    if (c0 + 16 * j < nh)
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+        lds[C.h1 + (r0 + i) * sh + c0 + 16 * j] = fmaxf(acc[i][j] + lds[C.b1 + c0 + 16 * j], 0.f);
+  __syncthreads();
+  // H2 = relu(H1 W2^T + b2)
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[i][j] = 0.f;
+  tile_rows_cols(lds + C.h1, sh, lds + C.w2, sh, nh, nh, acc);
+#pragma unroll
+  for (int j = 0; j < 4; ++j)
+    if (c0 + 16 * j < nh)
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+        lds[C.h2 + (r0 + i) * sh + c0 + 16 * j] = fmaxf(acc[i][j] + lds[C.b2 + c0 + 16 * j], 0.f);
+  __syncthreads();
+  // out = H2 w3 + b3; training: e = (out - y)/n, zero past n
+  if (tid < kChunk) {
+    float o = lds[C.b3];
+    for (int j = 0; j < nh; ++j) o += lds[C.w3 + j] * lds[C.h2 + tid * sh + j];
+    if (kMode == 0) {
+      if (tid < ns) out[s0 + tid] = o;
+    } else {
+      float e = 0.f, yv = 0.f;
+      if (tid < ns) {
+        yv = y[idx ? idx[s0 + tid] : s0 + tid];
+        e = (o - yv) * inv_n;
+      }
+      lds[C.e + tid] = e;
+      lds[C.o + tid] = o;
+      lds[C.yt + tid] = yv;
+    }
+  }
+  if (kMode == 0) return;
+  __syncthreads();
+  float *part = partial + (int64_t)blockIdx.x * L.total;
+  if (tid == 0) {  // loss: sum of squared errors of the chunk, in sample order
+    double a = 0;
+    for (int s = 0; s < ns; ++s) {
+      const double d = (double)lds[C.o + s] - (double)lds[C.yt + s];
+      a += d * d;
+    }
+    loss_part[blockIdx.x] = (float)a;
+  }
+  // gw3[j] = sum_s e[s] H2[s][j], gb3 = sum_s e[s]  (before H2 becomes D2)
+  if (tid < nh) {
     float a = 0.f;
-    if (H1[(int64_t)s * nh + k] > 0.f)
-      for (int j = 0; j < nh; ++j) a += sp[L.w2 + j * nh + k] * d2[j];
-    D1[(int64_t)s * nh + k] = a;
+    for (int s = 0; s < kChunk; ++s) a += lds[C.e + s] * lds[C.h2 + s * sh + tid];
+    part[L.w3 + tid] = a;
+  } else if (tid == 64) {
+    float a = 0.f;
+    for (int s = 0; s < kChunk; ++s) a += lds[C.e + s];
+    part[L.b3] = a;
   }
-}
-
-// Partial weight gradients over one chunk of kChunk samples: thread p of the
-// grid computes parameter p of chunk blockIdx.y.
-//   W1[k][i] += D1[s][k] XS[s][i]   b1[k] += D1[s][k]
-//   W2[j][k] += D2[s][j] H1[s][k]   b2[j] += D2[s][j]
-//   w3[j]    += E[s] H2[s][j]       b3    += E[s]
-__global__ __launch_bounds__(256) void mlp_wgrad_kernel(int n, int nin, int nh, const float *__restrict__ XS,
-                                                        const float *__restrict__ H1, const float *__restrict__ H2,
-                                                        const float *__restrict__ E, const float *__restrict__ D1,
-                                                        const float *__restrict__ D2, float *__restrict__ partial) {
-  const Layout L(nin, nh);
-  const int p = (int)(blockIdx.x * blockDim.x + threadIdx.x);
-  if (p >= L.total) return;
-  const int s0 = (int)blockIdx.y * kChunk;
-  const int s1 = min(n, s0 + kChunk);
-  float a = 0.f;
-  if (p < L.b1) {
-    const int k = p / nin, i = p - k * nin;
-    for (int s = s0; s < s1; ++s) a += D1[(int64_t)s * nh + k] * XS[(int64_t)s * nin + i];
-  } else if (p < L.w2) {
-    const int k = p - L.b1;
-    for (int s = s0; s < s1; ++s) a += D1[(int64_t)s * nh + k];
-  } else if (p < L.b2) {
-    const int j = (p - L.w2) / nh, k = (p - L.w2) - j * nh;
-    for (int s = s0; s < s1; ++s) a += D2[(int64_t)s * nh + j] * H1[(int64_t)s * nh + k];
-  } else if (p < L.w3) {
-    const int j = p - L.b2;
-    for (int s = s0; s < s1; ++s) a += D2[(int64_t)s * nh + j];
-  } else if (p < L.b3) {
-    const int j = p - L.w3;
-    for (int s = s0; s < s1; ++s) a += E[s] * H2[(int64_t)s * nh + j];
-  } else {
-    for (int s = s0; s < s1; ++s) a += E[s];
-  }
-  partial[(int64_t)blockIdx.y * L.total + p] = a;
-}
-
-// grad[p] = sum over chunks in order; loss = 0.5/n sum (out - y)^2 by one
-// block in a fixed order.
-__global__ __launch_bounds__(256) void mlp_reduce_kernel(int total, int nchunks, const float *__restrict__ partial,
-                                                         float *__restrict__ grad) {
-  const int p = (int)(blockIdx.x * blockDim.x + threadIdx.x);
-  if (p >= total) return;
-  float a = 0.f;
-  for (int c = 0; c < nchunks; ++c) a += partial[(int64_t)c * total + p];
-  grad[p] = a;
-}
-
-__global__ __launch_bounds__(256) void mlp_loss_kernel(int n, const float *__restrict__ out,
-                                                       const float *__restrict__ y, const int *__restrict__ idx,
-                                                       float *__restrict__ loss) {
-  __shared__ double red[256];
-  double a = 0;
-  for (int s = threadIdx.x; s < n; s += blockDim.x) {
-    const double d = (double)out[s] - (double)y[idx ? idx[s] : s];
-    a += d * d;
-  }
-  red[threadIdx.x] = a;
   __syncthreads();
-  for (int w = 128; w > 0; w >>= 1) {
-    if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
-    __syncthreads();
+  // D2 = e w3^T [H2 > 0], in place of H2
+  for (int q = tid; q < kChunk * nh; q += kThreads) {
+    const int s = q / nh, j = q - s * nh;
+    float *h = lds + C.h2 + s * sh + j;
+    *h = *h > 0.f ? lds[C.e + s] * lds[C.w3 + j] : 0.f;
   }
-  if (threadIdx.x == 0) *loss = (float)(0.5 * red[0] / n);
+  __syncthreads();
+  // gW2[j][k] = sum_s D2[s][j] H1[s][k]; gb2[j] = sum_s D2[s][j]
+  {
+    // thread: rows j = 4(t/16)..+3 of gW2, columns k = t%16 + 16c
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) acc[i][j] = 0.f;
+    tile_grad(lds + C.h2, sh, nh, lds + C.h1, sh, nh, acc, r0, c0);
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+      if (r0 + i < nh)
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+          if (c0 + 16 * j < nh) part[L.w2 + (r0 + i) * nh + c0 + 16 * j] = acc[i][j];
+    if (tid < nh) {
+      float a = 0.f;
+      for (int s = 0; s < kChunk; ++s) a += lds[C.h2 + s * sh + tid];
+      part[L.b2 + tid] = a;
+    }
+  }
+  // D1 = (D2 W2) [H1 > 0]: D1[s][k] = sum_j D2[s][j] W2[j][k]
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[i][j] = 0.f;
+  for (int jj = 0; jj < nh; ++jj) {
+    float a[4], b[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) a[i] = lds[C.h2 + (r0 + i) * sh + jj];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) b[j] = c0 + 16 * j < nh ? lds[C.w2 + jj * sh + c0 + 16 * j] : 0.f;
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) acc[i][j] += a[i] * b[j];
+  }
+  __syncthreads();  // every read of H1 (gW2 above) is done before D1 replaces it
+#pragma unroll
+  for (int j = 0; j < 4; ++j)
+    if (c0 + 16 * j < nh)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        float *h = lds + C.h1 + (r0 + i) * sh + c0 + 16 * j;
+        *h = *h > 0.f ? acc[i][j] : 0.f;
+      }
+  __syncthreads();
+  // gW1[k][i] = sum_s D1[s][k] Xs[s][i]; gb1[k] = sum_s D1[s][k]
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[i][j] = 0.f;
+  // rows k (nh) = 4(t/16)..+3, columns i (nin <= 32) = t%16 + 16c, c < 2 used
+  tile_grad(lds + C.h1, sh, nh, lds + C.xs, si, nin, acc, r0, c0);
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+    if (r0 + i < nh)
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+        if (c0 + 16 * j < nin) part[L.w1 + (r0 + i) * nin + c0 + 16 * j] = acc[i][j];
+  if (tid < nh) {
+    float a = 0.f;
+    for (int s = 0; s < kChunk; ++s) a += lds[C.h1 + s * sh + tid];
+    part[L.b1 + tid] = a;
+  }
 }
 
-__global__ __launch_bounds__(256) void mlp_adam_kernel(int total, float *__restrict__ params,
-                                                       const float *__restrict__ grad, float *__restrict__ m1,
-                                                       float *__restrict__ m2, float lr, float c1, float c2) {
+// grad[p] = sum of the chunks' partials in chunk order; with kAdam the Adam
+// step (beta1 0.9, beta2 0.999, bias corrections c1, c2) is applied to the
+// parameters instead of storing the gradient.  Block 0 also adds the chunks'
+// squared errors: loss = 0.5/n * sum.
+template <bool kAdam>
+__global__ __launch_bounds__(256) void mlp_reduce_kernel(int total, int nchunks, const float *__restrict__ partial,
+                                                         const float *__restrict__ loss_part, float half_inv_n,
+                                                         float *__restrict__ grad, float *__restrict__ params,
+                                                         float *__restrict__ m1, float *__restrict__ m2, float lr,
+                                                         float c1, float c2, float *__restrict__ loss) {
   const int p = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (loss && p == 0) {
+    double a = 0;
+    for (int c = 0; c < nchunks; ++c) a += (double)loss_part[c];
+    *loss = (float)(a * half_inv_n);
+  }
   if (p >= total) return;
-  const float g = grad[p];
-  const float a = 0.9f * m1[p] + 0.1f * g;
-  const float b = 0.999f * m2[p] + 0.001f * g * g;
-  m1[p] = a;
-  m2[p] = b;
-  params[p] -= lr * (a * c1) / (sqrtf(b * c2) + 1e-8f);
+  float g = 0.f;
+  for (int c = 0; c < nchunks; ++c) g += partial[(int64_t)c * total + p];
+  if (kAdam) {
+    const float a = 0.9f * m1[p] + 0.1f * g;
+    const float b = 0.999f * m2[p] + 0.001f * g * g;
+    m1[p] = a;
+    m2[p] = b;
+    params[p] -= lr * (a * c1) / (sqrtf(b * c2) + 1e-8f);
+  } else {
+    grad[p] = g;
+  }
+}
+
+__global__ void mlp_zero_kernel(int n, float *__restrict__ a) {
+  const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (i < n) a[i] = 0.f;
 }
 
 struct DevBuf {
@@ -230,21 +370,31 @@ struct hmvm_mlp {
 
 namespace {
 
-// Device workspace for one pass over n samples.
+// Device workspace for passes over up to `batch` samples of `rows` rows.
 struct Work {
-  DevBuf X, y, idx, params, mean, istd, XS, H1, H2, out, E, D1, D2, partial, grad, loss;
+  DevBuf X, y, idx, params, mean, istd, out, partial, loss_part, grad, loss;
   int n = 0, nrows = 0;
   int setup(const hmvm_mlp *m, const float *hX, const float *hy, int rows, int batch) {
     const Layout &L = m->L;
     nrows = rows;
     n = batch;
     MLP_TRY(X.alloc(sizeof(float) * (size_t)rows * L.nin));
+    {
+      int dev = 0, max_lds = 0;
+      MLP_TRY(hipGetDevice(&dev));
+      MLP_TRY(hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+      if (lds_bytes(m) > (size_t)max_lds) {
+        std::fprintf(stderr, "hmvm_mlp: %zu bytes of LDS per workgroup exceed the device's %d\n", lds_bytes(m),
+                     max_lds);
+        return HMVM_ERR_UNSUPPORTED;
+      }
+    }
     MLP_TRY(hipMemcpy(X.p, hX, sizeof(float) * (size_t)rows * L.nin, hipMemcpyHostToDevice));
     if (hy) {
       MLP_TRY(y.alloc(sizeof(float) * (size_t)rows));
       MLP_TRY(hipMemcpy(y.p, hy, sizeof(float) * (size_t)rows, hipMemcpyHostToDevice));
     }
-    MLP_TRY(idx.alloc(sizeof(int) * (size_t)batch));
+    MLP_TRY(idx.alloc(sizeof(int) * (size_t)rows));
     MLP_TRY(params.alloc(sizeof(float) * L.total));
     MLP_TRY(hipMemcpy(params.p, m->params.data(), sizeof(float) * L.total, hipMemcpyHostToDevice));
     std::vector<float> istd_h(L.nin);
@@ -253,39 +403,30 @@ struct Work {
     MLP_TRY(istd.alloc(sizeof(float) * L.nin));
     MLP_TRY(hipMemcpy(mean.p, m->mean.data(), sizeof(float) * L.nin, hipMemcpyHostToDevice));
     MLP_TRY(hipMemcpy(istd.p, istd_h.data(), sizeof(float) * L.nin, hipMemcpyHostToDevice));
-    MLP_TRY(XS.alloc(sizeof(float) * (size_t)batch * L.nin));
-    MLP_TRY(H1.alloc(sizeof(float) * (size_t)batch * L.nh));
-    MLP_TRY(H2.alloc(sizeof(float) * (size_t)batch * L.nh));
-    MLP_TRY(out.alloc(sizeof(float) * (size_t)batch));
-    MLP_TRY(E.alloc(sizeof(float) * (size_t)batch));
-    MLP_TRY(D1.alloc(sizeof(float) * (size_t)batch * L.nh));
-    MLP_TRY(D2.alloc(sizeof(float) * (size_t)batch * L.nh));
+    MLP_TRY(out.alloc(sizeof(float) * (size_t)rows));
     const int nch = (batch + kChunk - 1) / kChunk;
     MLP_TRY(partial.alloc(sizeof(float) * (size_t)nch * L.total));
+    MLP_TRY(loss_part.alloc(sizeof(float) * (size_t)nch));
     MLP_TRY(grad.alloc(sizeof(float) * L.total));
     MLP_TRY(loss.alloc(sizeof(float)));
     return HMVM_OK;
   }
-  // forward (+ loss) on nb samples selected by d_idx (NULL: rows 0..nb-1)
-  void forward(const hmvm_mlp *m, int nb, const int *d_idx, hipStream_t st) {
+  size_t lds_bytes(const hmvm_mlp *m) const { return sizeof(float) * (size_t)ChunkLds(m->L.nin, m->L.nh).total; }
+  // out[s] = f(X[s]) for the first nb rows
+  void predict(const hmvm_mlp *m, int nb, hipStream_t st) {
     const Layout &L = m->L;
-    const size_t lds = sizeof(float) * L.total;
-    hipLaunchKernelGGL(mlp_forward_kernel, dim3((nb + 255) / 256), dim3(256), lds, st, nb, L.nin, L.nh,
-                       X.as<float>(), d_idx, params.as<float>(), mean.as<float>(), istd.as<float>(), XS.as<float>(),
-                       H1.as<float>(), H2.as<float>(), out.as<float>());
+    hipLaunchKernelGGL(mlp_chunk_kernel<0>, dim3((nb + kChunk - 1) / kChunk), dim3(kThreads), lds_bytes(m), st, nb,
+                       L.nin, L.nh, 0.f, X.as<float>(), (const float *)nullptr, (const int *)nullptr,
+                       params.as<float>(), mean.as<float>(), istd.as<float>(), out.as<float>(), (float *)nullptr,
+                       (float *)nullptr);
   }
-  void backward(const hmvm_mlp *m, int nb, const int *d_idx, hipStream_t st) {
+  // loss and gradient partials of nb samples selected by d_idx (NULL: rows 0..nb-1)
+  void chunks(const hmvm_mlp *m, int nb, const int *d_idx, hipStream_t st) {
     const Layout &L = m->L;
-    const size_t lds = sizeof(float) * L.total;
-    hipLaunchKernelGGL(mlp_delta_kernel, dim3((nb + 255) / 256), dim3(256), lds, st, nb, L.nin, L.nh,
-                       1.0f / (float)nb, y.as<float>(), d_idx, params.as<float>(), H1.as<float>(), H2.as<float>(),
-                       out.as<float>(), E.as<float>(), D1.as<float>(), D2.as<float>());
-    const int nch = (nb + kChunk - 1) / kChunk;
-    hipLaunchKernelGGL(mlp_wgrad_kernel, dim3((L.total + 255) / 256, nch), dim3(256), 0, st, nb, L.nin, L.nh,
-                       XS.as<float>(), H1.as<float>(), H2.as<float>(), E.as<float>(), D1.as<float>(), D2.as<float>(),
-                       partial.as<float>());
-    hipLaunchKernelGGL(mlp_reduce_kernel, dim3((L.total + 255) / 256), dim3(256), 0, st, L.total, nch,
-                       partial.as<float>(), grad.as<float>());
+    hipLaunchKernelGGL(mlp_chunk_kernel<1>, dim3((nb + kChunk - 1) / kChunk), dim3(kThreads), lds_bytes(m), st, nb,
+                       L.nin, L.nh, 1.0f / (float)nb, X.as<float>(), y.as<float>(), d_idx, params.as<float>(),
+                       mean.as<float>(), istd.as<float>(), (float *)nullptr, partial.as<float>(),
+                       loss_part.as<float>());
   }
 };
 
@@ -364,7 +505,7 @@ int hmvm_mlp_predict(hmvm_mlp *m, const float *X, int n, float *y) {
   Work w;
   int rc = w.setup(m, X, nullptr, n, n);
   if (rc) return rc;
-  w.forward(m, n, nullptr, m->st);
+  w.predict(m, n, m->st);
   MLP_TRY(hipGetLastError());
   MLP_TRY(hipMemcpyAsync(y, w.out.p, sizeof(float) * (size_t)n, hipMemcpyDeviceToHost, m->st));
   MLP_TRY(hipStreamSynchronize(m->st));
@@ -376,17 +517,22 @@ int hmvm_mlp_loss_grad(hmvm_mlp *m, const float *X, const float *y, int n, float
   Work w;
   int rc = w.setup(m, X, y, n, n);
   if (rc) return rc;
-  w.forward(m, n, nullptr, m->st);
-  hipLaunchKernelGGL(mlp_loss_kernel, dim3(1), dim3(256), 0, m->st, n, w.out.as<float>(), w.y.as<float>(),
-                     (const int *)nullptr, w.loss.as<float>());
-  w.backward(m, n, nullptr, m->st);
+  const Layout &L = m->L;
+  w.chunks(m, n, nullptr, m->st);
+  const int nch = (n + kChunk - 1) / kChunk;
+  hipLaunchKernelGGL(mlp_reduce_kernel<false>, dim3((L.total + 255) / 256), dim3(256), 0, m->st, L.total, nch,
+                     w.partial.as<float>(), w.loss_part.as<float>(), 0.5f / (float)n, w.grad.as<float>(),
+                     (float *)nullptr, (float *)nullptr, (float *)nullptr, 0.f, 0.f, 0.f, w.loss.as<float>());
   MLP_TRY(hipGetLastError());
   if (loss) MLP_TRY(hipMemcpyAsync(loss, w.loss.p, sizeof(float), hipMemcpyDeviceToHost, m->st));
-  if (grad) MLP_TRY(hipMemcpyAsync(grad, w.grad.p, sizeof(float) * m->L.total, hipMemcpyDeviceToHost, m->st));
+  if (grad) MLP_TRY(hipMemcpyAsync(grad, w.grad.p, sizeof(float) * L.total, hipMemcpyDeviceToHost, m->st));
   MLP_TRY(hipStreamSynchronize(m->st));
   return HMVM_OK;
 }
 
+// Minibatch Adam: one launch of the chunk kernel (forward and backward of
+// the minibatch, one workgroup per 64 samples) and one reduce-and-update
+// launch per step; the host only reshuffles the sample order once an epoch.
 int hmvm_mlp_train(hmvm_mlp *m, const float *X, const float *y, int n, int epochs, int batch, float lr,
                    unsigned long long seed, float *final_loss) {
   if (!m || !X || !y || n < 1 || epochs < 0 || batch < 1 || !(lr > 0)) return HMVM_ERR_ARG;
@@ -398,8 +544,8 @@ int hmvm_mlp_train(hmvm_mlp *m, const float *X, const float *y, int n, int epoch
   DevBuf m1, m2;
   MLP_TRY(m1.alloc(sizeof(float) * L.total));
   MLP_TRY(m2.alloc(sizeof(float) * L.total));
-  MLP_TRY(hipMemsetAsync(m1.p, 0, sizeof(float) * L.total, m->st));
-  MLP_TRY(hipMemsetAsync(m2.p, 0, sizeof(float) * L.total, m->st));
+  hipLaunchKernelGGL(mlp_zero_kernel, dim3((L.total + 255) / 256), dim3(256), 0, m->st, L.total, m1.as<float>());
+  hipLaunchKernelGGL(mlp_zero_kernel, dim3((L.total + 255) / 256), dim3(256), 0, m->st, L.total, m2.as<float>());
   std::vector<int> perm(n);
   std::iota(perm.begin(), perm.end(), 0);
   uint64_t s = seed * 0x2545F4914F6CDD1Dull + 1;
@@ -414,23 +560,25 @@ int hmvm_mlp_train(hmvm_mlp *m, const float *X, const float *y, int n, int epoch
     MLP_TRY(hipMemcpyAsync(w.idx.p, perm.data(), sizeof(int) * (size_t)n, hipMemcpyHostToDevice, m->st));
     for (int b = 0; b < nbatches; b++) {
       const int b0 = b * batch, nb = std::min(batch, n - b0);
-      const int *d_idx = w.idx.as<int>() + b0;
-      w.forward(m, nb, d_idx, m->st);
-      w.backward(m, nb, d_idx, m->st);
+      w.chunks(m, nb, w.idx.as<int>() + b0, m->st);
       step++;
       const float c1 = 1.0f / (1.0f - std::pow(0.9f, (float)step));
       const float c2 = 1.0f / (1.0f - std::pow(0.999f, (float)step));
-      hipLaunchKernelGGL(mlp_adam_kernel, dim3((L.total + 255) / 256), dim3(256), 0, m->st, L.total,
-                         w.params.as<float>(), w.grad.as<float>(), m1.as<float>(), m2.as<float>(), lr, c1, c2);
+      hipLaunchKernelGGL(mlp_reduce_kernel<true>, dim3((L.total + 255) / 256), dim3(256), 0, m->st, L.total,
+                         (nb + kChunk - 1) / kChunk, w.partial.as<float>(), w.loss_part.as<float>(), 0.f,
+                         (float *)nullptr, w.params.as<float>(), m1.as<float>(), m2.as<float>(), lr, c1, c2,
+                         (float *)nullptr);
     }
     MLP_TRY(hipGetLastError());
     // the permutation buffer is rewritten next epoch: finish this one first
     MLP_TRY(hipStreamSynchronize(m->st));
   }
   MLP_TRY(hipMemcpyAsync(m->params.data(), w.params.p, sizeof(float) * L.total, hipMemcpyDeviceToHost, m->st));
-  w.forward(m, n, nullptr, m->st);
-  hipLaunchKernelGGL(mlp_loss_kernel, dim3(1), dim3(256), 0, m->st, n, w.out.as<float>(), w.y.as<float>(),
-                     (const int *)nullptr, w.loss.as<float>());
+  w.chunks(m, n, nullptr, m->st);
+  const int nch = (n + kChunk - 1) / kChunk;
+  hipLaunchKernelGGL(mlp_reduce_kernel<false>, dim3((L.total + 255) / 256), dim3(256), 0, m->st, L.total, nch,
+                     w.partial.as<float>(), w.loss_part.as<float>(), 0.5f / (float)n, w.grad.as<float>(),
+                     (float *)nullptr, (float *)nullptr, (float *)nullptr, 0.f, 0.f, 0.f, w.loss.as<float>());
   float l = 0;
   MLP_TRY(hipMemcpyAsync(&l, w.loss.p, sizeof(float), hipMemcpyDeviceToHost, m->st));
   MLP_TRY(hipStreamSynchronize(m->st));

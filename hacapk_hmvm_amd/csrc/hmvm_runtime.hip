@@ -775,44 +775,44 @@ int create_plan(Source &src, const hmvm_options *opt, hmvm_plan **out) {
   return HMVM_OK;
 }
 
+// kind: unused by the one-wave-per-tile launch (kept so every launch names
+// its part of the HMVM: 0 A, 1 B, 2 stage-1 part of A, 3 dense part)
 template <typename S, bool kScaled, int kVar>
-void launch_tiles_v(hmvm_plan *p, dim3 grid, dim3 block, hipStream_t st, const TileDesc *d_tiles, int ntiles,
+void launch_tiles_v(hmvm_plan *p, int kind, int bt, hipStream_t st, const TileDesc *d_tiles, int ntiles,
                     const double *x, const double *t, const double *tp, double *out) {
+  (void)kind;
   constexpr int L = 8;
   constexpr bool kBV = (kVar & 1) != 0;
   constexpr bool kPipe = (kVar & 2) != 0, kNT = (kVar & 4) != 0;
-  const size_t lds = sizeof(double) * (block.x / 64) * lds_wave_doubles<S>();
-  hipLaunchKernelGGL((tile_kernel<S, kScaled, L, kPipe, kNT, kBV>), grid, block, lds, st, d_tiles, ntiles,
-                     p->d_panels, (const S *)p->d_vals, x, t, tp, (const float *)p->d_scales, out);
+  const int wpb = bt / 64;
+  const size_t lds = sizeof(double) * wpb * lds_wave_doubles<S>();
+  hipLaunchKernelGGL((tile_kernel<S, kScaled, L, kPipe, kNT, kBV>), dim3((ntiles + wpb - 1) / wpb), dim3(bt), lds, st,
+                     d_tiles, ntiles, p->d_panels, (const S *)p->d_vals, x, t, tp, (const float *)p->d_scales, out);
 }
 
 template <typename S, bool kScaled>
-void launch_tiles_s(hmvm_plan *p, int kern, dim3 grid, dim3 block, hipStream_t st, const TileDesc *d_tiles,
-                    int ntiles, const double *x, const double *t, const double *tp, double *out) {
+void launch_tiles_s(hmvm_plan *p, int kern, int kind, int bt, hipStream_t st, const TileDesc *d_tiles, int ntiles,
+                    const double *x, const double *t, const double *tp, double *out) {
+#define HMVM_TILE_CASE(v) \
+  case v: launch_tiles_v<S, kScaled, v>(p, kind, bt, st, d_tiles, ntiles, x, t, tp, out); break;
   switch (kern) {
-    case 0: launch_tiles_v<S, kScaled, 0>(p, grid, block, st, d_tiles, ntiles, x, t, tp, out); break;
-    case 1: launch_tiles_v<S, kScaled, 1>(p, grid, block, st, d_tiles, ntiles, x, t, tp, out); break;
-    case 2: launch_tiles_v<S, kScaled, 2>(p, grid, block, st, d_tiles, ntiles, x, t, tp, out); break;
-    case 3: launch_tiles_v<S, kScaled, 3>(p, grid, block, st, d_tiles, ntiles, x, t, tp, out); break;
-    case 4: launch_tiles_v<S, kScaled, 4>(p, grid, block, st, d_tiles, ntiles, x, t, tp, out); break;
-    case 5: launch_tiles_v<S, kScaled, 5>(p, grid, block, st, d_tiles, ntiles, x, t, tp, out); break;
-    case 6: launch_tiles_v<S, kScaled, 6>(p, grid, block, st, d_tiles, ntiles, x, t, tp, out); break;
-    default: launch_tiles_v<S, kScaled, 7>(p, grid, block, st, d_tiles, ntiles, x, t, tp, out); break;
+    HMVM_TILE_CASE(0) HMVM_TILE_CASE(1) HMVM_TILE_CASE(2) HMVM_TILE_CASE(3)
+    HMVM_TILE_CASE(4) HMVM_TILE_CASE(5) HMVM_TILE_CASE(6)
+    default: launch_tiles_v<S, kScaled, 7>(p, kind, bt, st, d_tiles, ntiles, x, t, tp, out); break;
   }
+#undef HMVM_TILE_CASE
 }
 
 template <typename S>
-void launch_tiles(hmvm_plan *p, int which, const TileDesc *d_tiles, int ntiles, const double *x, const double *t,
-                  const double *tp, double *out, bool scaled, hipStream_t st) {
+void launch_tiles(hmvm_plan *p, int which, int kind, const TileDesc *d_tiles, int ntiles, const double *x,
+                  const double *t, const double *tp, double *out, bool scaled, hipStream_t st) {
   if (ntiles == 0) return;
   const int bt = which ? p->block_threads_b : p->block_threads;
   const int kern = which ? p->kernel_b : p->kernel;
-  const int wpb = bt / 64;
-  const dim3 grid((ntiles + wpb - 1) / wpb), block(bt);
   if (scaled)
-    launch_tiles_s<S, true>(p, kern, grid, block, st, d_tiles, ntiles, x, t, tp, out);
+    launch_tiles_s<S, true>(p, kern, kind, bt, st, d_tiles, ntiles, x, t, tp, out);
   else
-    launch_tiles_s<S, false>(p, kern, grid, block, st, d_tiles, ntiles, x, t, tp, out);
+    launch_tiles_s<S, false>(p, kern, kind, bt, st, d_tiles, ntiles, x, t, tp, out);
 }
 
 // which: 0 launch A (all), 1 launch B, 2 stage-1 tiles of A only, 3 dense tiles of A only
@@ -822,13 +822,13 @@ void launch_tiles_both(hmvm_plan *p, int which, const double *x, hipStream_t st)
   const bool bf = p->storage == kBF16;
   const int na = (int)P.tiles_a.size(), n1 = P.n_stage1_tiles;
   if (which == 0)
-    launch_tiles<S>(p, 0, p->d_tiles_a, na, x, nullptr, nullptr, p->d_slots, bf, st);
+    launch_tiles<S>(p, 0, 0, p->d_tiles_a, na, x, nullptr, nullptr, p->d_slots, bf, st);
   else if (which == 2)
-    launch_tiles<S>(p, 0, p->d_tiles_a, n1, x, nullptr, nullptr, p->d_slots, bf, st);
+    launch_tiles<S>(p, 0, 2, p->d_tiles_a, n1, x, nullptr, nullptr, p->d_slots, bf, st);
   else if (which == 3)
-    launch_tiles<S>(p, 0, p->d_tiles_a + n1, na - n1, x, nullptr, nullptr, p->d_slots, bf, st);
+    launch_tiles<S>(p, 0, 3, p->d_tiles_a + n1, na - n1, x, nullptr, nullptr, p->d_slots, bf, st);
   else
-    launch_tiles<S>(p, 1, p->d_tiles_b, (int)P.tiles_b.size(), x, p->d_t, p->d_slots, p->d_slots, bf, st);
+    launch_tiles<S>(p, 1, 1, p->d_tiles_b, (int)P.tiles_b.size(), x, p->d_t, p->d_slots, p->d_slots, bf, st);
 }
 
 void launch_tiles_part(hmvm_plan *p, int which, const double *x, hipStream_t st) {

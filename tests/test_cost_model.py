@@ -121,6 +121,26 @@ def test_hip_training_fits(gpu, H):
     m.close()
 
 
+@pytest.mark.gpu
+def test_hip_training_throughput(gpu, H):
+    """A training pass costs two launches per minibatch with the work spread
+    over one workgroup per 64 samples: 20 epochs over 200,000 samples
+    (batch 4096) in well under the time of a benchmark run, and the loss
+    drops."""
+    import time
+    X, y = rand_problem(200000, seed=10)
+    m = H.CostMLP(NIN, NH, seed=4)
+    m.fit_scaler(X)
+    l0, _ = m.loss_grad(X, y)
+    t0 = time.time()
+    lf = m.train(X, y, epochs=20, batch=4096, lr=3e-3, seed=5)
+    dt = time.time() - t0
+    print("20 epochs x 200k samples: %.2f s, loss %.4f -> %.4f" % (dt, l0, lf))
+    assert lf < 0.2 * l0
+    assert dt < 10.0
+    m.close()
+
+
 def test_features_clamped_to_training_range(H):
     """Launch-level features beyond every training launch (a large batched
     plan) are clamped to the training range before prediction."""
