@@ -70,7 +70,8 @@ class hmvm_options(ctypes.Structure):
 
 
 class hmvm_tile(ctypes.Structure):
-    _fields_ = [("nrows", ctypes.c_int), ("panel0", ctypes.c_int), ("npanel", ctypes.c_int), ("out_off", ctypes.c_int)]
+    _fields_ = [("nrows", ctypes.c_int), ("panel0", ctypes.c_int), ("npanel", ctypes.c_int), ("out_off", ctypes.c_int),
+                ("map_off", ctypes.c_int), ("reserved", ctypes.c_int * 3)]
 
 
 class hmvm_panel(ctypes.Structure):
@@ -97,7 +98,8 @@ class hmvm_schedule(ctypes.Structure):
                 ("n_zslots", ctypes.c_longlong),
                 ("tiles_a", ctypes.POINTER(hmvm_tile)), ("tiles_b", ctypes.POINTER(hmvm_tile)),
                 ("panels", ctypes.POINTER(hmvm_panel)), ("repack", ctypes.POINTER(hmvm_repack)),
-                ("tsum", hmvm_segsum), ("ysum", hmvm_segsum)]
+                ("tsum", hmvm_segsum), ("ysum", hmvm_segsum),
+                ("t_direct", ctypes.c_int), ("ntmap", ctypes.c_longlong), ("tmap", ctypes.POINTER(ctypes.c_int))]
 
 
 class hmvm_plan_info(ctypes.Structure):
@@ -112,7 +114,7 @@ class hmvm_plan_info(ctypes.Structure):
                 ("elems_stage2", ctypes.c_longlong),
                 ("vec_reads_a", ctypes.c_longlong), ("vec_reads_b", ctypes.c_longlong),
                 ("algorithmic_bytes", ctypes.c_longlong), ("device_bytes", ctypes.c_longlong),
-                ("mode", ctypes.c_int), ("pad_", ctypes.c_int)]
+                ("mode", ctypes.c_int), ("t_direct", ctypes.c_int)]
 
     def as_dict(self):
         return {name: getattr(self, name) for name, _ in self._fields_}
@@ -666,7 +668,8 @@ def _struct_array(ptr, n, dtype):
     return raw.view(dtype).copy()
 
 
-TILE_DT = np.dtype([("nrows", "<i4"), ("panel0", "<i4"), ("npanel", "<i4"), ("out_off", "<i4")])
+TILE_DT = np.dtype([("nrows", "<i4"), ("panel0", "<i4"), ("npanel", "<i4"), ("out_off", "<i4"), ("map_off", "<i4"),
+                    ("reserved", "<i4", (3,))])
 PANEL_DT = np.dtype([("val_off", "<i8"), ("ld", "<i4"), ("ncols", "<i4"), ("vec_off", "<i4"), ("vec_sel", "<i4"),
                      ("factor", "<i4"), ("vec_chunks", "<i4")])
 REPACK_DT = np.dtype([("src_off", "<i8"), ("dst_off", "<i8"), ("rows", "<i4"), ("cols", "<i4"), ("src_rs", "<i4"),
@@ -701,7 +704,8 @@ def schedule(h, source="rowmat", **kw):
                 "tiles_b": _struct_array(s.tiles_b, s.ntiles_b, TILE_DT),
                 "panels": _struct_array(s.panels, s.npanels, PANEL_DT),
                 "repack": _struct_array(s.repack, s.nrepack, REPACK_DT),
-                "tsum": seg(s.tsum), "ysum": seg(s.ysum)}
+                "tsum": seg(s.tsum), "ysum": seg(s.ysum), "t_direct": bool(s.t_direct),
+                "tmap": np.ctypeslib.as_array(s.tmap, (max(s.ntmap, 1),))[:s.ntmap].copy()}
     finally:
         lib.hmvm_schedule_release(ctypes.byref(s))
 

@@ -80,10 +80,15 @@ constexpr int kNumKernelVariants = 8;
 // written earlier in the same launch (persistent mode), so they are read with
 // device-coherent vector loads instead of possibly-scalar cached ones.
 template <bool kCoherent = false>
-__device__ __forceinline__ double vec_entry(const PanelDesc &pd, const double *__restrict__ x,
-                                            const double *__restrict__ t, const double *tp, int j) {
+__device__ __forceinline__ double vec_entry(const PanelDesc &pd, const double *__restrict__ x, const double *t,
+                                            const double *tp, int j) {
   if (pd.vec_sel == kVecX) return x[pd.vec_off + j];
-  if (pd.vec_sel == kVecT) return t[pd.vec_off + j];
+  if (pd.vec_sel == kVecT) {
+    if constexpr (kCoherent)  // t was written earlier in the same launch
+      return __hip_atomic_load(t + pd.vec_off + j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    else
+      return t[pd.vec_off + j];
+  }
   const int nch = pd.vec_chunks >> 16, stride = pd.vec_chunks & 0xFFFF;
   double s = 0;
   for (int c = 0; c < nch; ++c) {
@@ -104,6 +109,29 @@ __device__ __forceinline__ float panel_scale(const PanelDesc &pd, const float *_
   if (pd.factor < 0 || pd.vec_sel == kVecT) return 1.0f;
   const float s = scales[pd.factor];
   return pd.vec_sel == kVecTP ? s * scales[pd.factor - 1] : s;
+}
+
+// Where a tile's rows go: its output slot (out + out_off + row), or, for a
+// stage-1 tile with a row map (t_mode 3), the entry of t the map names, with
+// both factor scales of that entry's leaf applied under BF16 storage
+// (tfac[e] = the leaf's a1 factor id, its a2 factor is the next id).
+struct TileOut {
+  double *out;
+  const int32_t *tmap;
+  double *tout;
+  const int32_t *tfac;
+};
+
+template <bool kScaled>
+__device__ __forceinline__ void store_row(const TileDesc &td, int row, double v, const TileOut &o,
+                                          const float *__restrict__ scales) {
+  if (td.map_off < 0) {
+    o.out[td.out_off + row] = v;
+    return;
+  }
+  const int e = o.tmap[td.map_off + row];
+  if constexpr (kScaled) v *= (double)scales[o.tfac[e]] * (double)scales[o.tfac[e] + 1];
+  o.tout[e] = v;
 }
 
 // ---- wide lane loads ----------------------------------------------------------
@@ -258,9 +286,9 @@ __device__ __forceinline__ void wide_panel(const S *__restrict__ vals, const Pan
 
 template <typename S, int R, int kLoads, bool kPipe, bool kNT, bool kScaled, bool kCoh, bool kBV>
 __device__ __forceinline__ void wide_tile(const TileDesc &td, int shift, int lpc, const PanelDesc *__restrict__ panels,
-                                          const S *__restrict__ vals, const double *__restrict__ x,
-                                          const double *__restrict__ t, const double *tp,
-                                          const float *__restrict__ scales, double *__restrict__ out, int lane) {
+                                          const S *__restrict__ vals, const double *__restrict__ x, const double *t,
+                                          const double *tp, const float *__restrict__ scales, const TileOut &o,
+                                          int lane) {
   using A = typename AccOf<S>::T;
   constexpr int V = Wide<S>::V;
   A acc[R][V];
@@ -290,7 +318,7 @@ __device__ __forceinline__ void wide_tile(const TileDesc &td, int shift, int lpc
 #pragma unroll
     for (int v = 0; v < V; ++v) {
       const int row = V * (lane + 64 * q) + v - shift;
-      if (row >= 0 && row < td.nrows) out[td.out_off + row] = (double)acc[q][v];
+      if (row >= 0 && row < td.nrows) store_row<kScaled>(td, row, (double)acc[q][v], o, scales);
     }
 }
 
@@ -318,9 +346,14 @@ __host__ __device__ constexpr int lds_wave_doubles() {
 // Vector entry j of a panel given its fields (vec_entry's rule).
 template <bool kCoh>
 __device__ __forceinline__ double entry_of(int sel, int off, int chunks, const double *__restrict__ x,
-                                           const double *__restrict__ t, const double *tp, int j) {
+                                           const double *t, const double *tp, int j) {
   if (sel == kVecX) return x[off + j];
-  if (sel == kVecT) return t[off + j];
+  if (sel == kVecT) {
+    if constexpr (kCoh)  // t was written earlier in the same launch
+      return __hip_atomic_load(t + off + j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    else
+      return t[off + j];
+  }
   const int nch = chunks >> 16, stride = chunks & 0xFFFF;
   double s = 0;
   for (int c = 0; c < nch; ++c) {
@@ -350,9 +383,9 @@ __device__ __forceinline__ double entry_of(int sel, int off, int chunks, const d
 template <typename S, int kLoads, bool kNT, bool kScaled, bool kCoh>
 __device__ __forceinline__ void grouped_tile(const TileDesc &td, int shift, int lpc,
                                              const PanelDesc *__restrict__ panels, const S *__restrict__ vals,
-                                             const double *__restrict__ x, const double *__restrict__ t,
-                                             const double *tp, const float *__restrict__ scales,
-                                             double *__restrict__ out, int lane, double *lds) {
+                                             const double *__restrict__ x, const double *t, const double *tp,
+                                             const float *__restrict__ scales, const TileOut &o, int lane,
+                                             double *lds) {
   using A = typename AccOf<S>::T;
   using W = Wide<S>;
   using T = typename W::T;
@@ -465,39 +498,39 @@ __device__ __forceinline__ void grouped_tile(const TileDesc &td, int shift, int 
     }
   }
   // The g partial sums of every row meet in the wave's LDS region: lane
-  // (jg, c) stores its V sums at jg*lpc*V + c*V + v, then lane o sums output
-  // row o = c*V + v over the groups in group order, so the stores to the slot
-  // are contiguous across lanes.
+  // (jg, c) stores its V sums at jg*lpc*V + c*V + v, then lane ro sums output
+  // row ro = c*V + v over the groups in group order, so the stores to the
+  // slot are contiguous across lanes.
   double *r = lds;
   wave_lds_sync();  // staged entries read before they are overwritten
 #pragma unroll
   for (int v = 0; v < V; ++v) r[lane * V + v] = (double)acc[v];
   wave_lds_sync();
-  for (int o = lane; o < lpc * V; o += 64) {
+  for (int ro = lane; ro < lpc * V; ro += 64) {
     double s = 0;
-    for (int q = 0; q < g; ++q) s += r[q * lpc * V + o];
-    const int row = o - shift;
-    if (row >= 0 && row < td.nrows) out[td.out_off + row] = s;
+    for (int q = 0; q < g; ++q) s += r[q * lpc * V + ro];
+    const int row = ro - shift;
+    if (row >= 0 && row < td.nrows) store_row<kScaled>(td, row, s, o, scales);
   }
 }
 
 template <typename S, bool kScaled, int kLoads, bool kPipe, bool kNT, bool kCoh, bool kBV>
 __device__ __forceinline__ void run_tile(const TileDesc &td, const PanelDesc *__restrict__ panels,
                                          const S *__restrict__ vals, const double *__restrict__ x,
-                                         const double *__restrict__ t, const double *tp,
-                                         const float *__restrict__ scales, double *out, int lane,
+                                         const double *t, const double *tp,
+                                         const float *__restrict__ scales, const TileOut &o, int lane,
                                          double *lds) {
   constexpr int V = Wide<S>::V;
   const int shift = (int)(panels[td.panel0].val_off % V);
   const int lpc = (td.nrows + shift + V - 1) / V;  // lanes per column
   if (lpc <= 32) {
-    grouped_tile<S, kLoads, kNT, kScaled, kCoh>(td, shift, lpc, panels, vals, x, t, tp, scales, out, lane, lds);
+    grouped_tile<S, kLoads, kNT, kScaled, kCoh>(td, shift, lpc, panels, vals, x, t, tp, scales, o, lane, lds);
   } else if (lpc <= 64) {
-    wide_tile<S, 1, kLoads, kPipe, kNT, kScaled, kCoh, kBV>(td, shift, lpc, panels, vals, x, t, tp, scales, out, lane);
+    wide_tile<S, 1, kLoads, kPipe, kNT, kScaled, kCoh, kBV>(td, shift, lpc, panels, vals, x, t, tp, scales, o, lane);
   } else if (lpc <= 128) {
-    wide_tile<S, 2, kLoads, kPipe, kNT, kScaled, kCoh, kBV>(td, shift, lpc, panels, vals, x, t, tp, scales, out, lane);
+    wide_tile<S, 2, kLoads, kPipe, kNT, kScaled, kCoh, kBV>(td, shift, lpc, panels, vals, x, t, tp, scales, o, lane);
   } else {  // lpc <= 192: the schedule check bounds the rows of every tile
-    wide_tile<S, 3, kLoads, kPipe, kNT, kScaled, kCoh, kBV>(td, shift, lpc, panels, vals, x, t, tp, scales, out, lane);
+    wide_tile<S, 3, kLoads, kPipe, kNT, kScaled, kCoh, kBV>(td, shift, lpc, panels, vals, x, t, tp, scales, o, lane);
   }
 }
 
@@ -509,15 +542,14 @@ __global__ __launch_bounds__(256) void tile_kernel(const TileDesc *__restrict__ 
                                                    const double *__restrict__ x,
                                                    const double *__restrict__ t,
                                                    const double *__restrict__ tp,
-                                                   const float *__restrict__ scales,
-                                                   double *__restrict__ out) {
+                                                   const float *__restrict__ scales, TileOut o) {
   const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
   const int lane = (int)(threadIdx.x & 63);
   const int tid = (int)blockIdx.x * (int)(blockDim.x >> 6) + wave;
   if (tid >= ntiles) return;
   extern __shared__ double tile_lds[];
   const TileDesc td = tiles[tid];
-  run_tile<S, kScaled, kLoads, kPipe, kNT, false, kBV>(td, panels, vals, x, t, tp, scales, out, lane,
+  run_tile<S, kScaled, kLoads, kPipe, kNT, false, kBV>(td, panels, vals, x, t, tp, scales, o, lane,
                                                        tile_lds + wave * lds_wave_doubles<S>());
 }
 
@@ -532,8 +564,7 @@ __global__ __launch_bounds__(256) void tile_kernel_timed(const TileDesc *__restr
                                                          const double *__restrict__ x,
                                                          const double *__restrict__ t,
                                                          const double *__restrict__ tp,
-                                                         const float *__restrict__ scales,
-                                                         double *__restrict__ out,
+                                                         const float *__restrict__ scales, TileOut o,
                                                          long long *__restrict__ stamps) {
   const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
   const int lane = (int)(threadIdx.x & 63);
@@ -542,7 +573,7 @@ __global__ __launch_bounds__(256) void tile_kernel_timed(const TileDesc *__restr
   const long long t0 = (long long)wall_clock64();
   extern __shared__ double tile_lds[];
   const TileDesc td = tiles[tid];
-  run_tile<S, kScaled, kLoads, kPipe, kNT, false, kBV>(td, panels, vals, x, t, tp, scales, out, lane,
+  run_tile<S, kScaled, kLoads, kPipe, kNT, false, kBV>(td, panels, vals, x, t, tp, scales, o, lane,
                                                        tile_lds + wave * lds_wave_doubles<S>());
   const long long t1 = (long long)wall_clock64();
   if (lane == 0) {
@@ -603,19 +634,21 @@ template <typename S, bool kScaled, int kLoads, bool kPipe, bool kNT, bool kBV>
 __global__ __launch_bounds__(256) void persistent_kernel(
     const TileDesc *__restrict__ tiles_a, int n1, int na, const TileDesc *__restrict__ tiles_b, int nt,
     int nitems, int nwaves, const PanelDesc *__restrict__ panels, const S *__restrict__ vals,
-    const double *__restrict__ x, const double *__restrict__ t, double *slots, const float *__restrict__ scales,
+    const double *__restrict__ x, double *t, double *slots, const float *__restrict__ scales,
+    const int32_t *__restrict__ tmap, const int32_t *__restrict__ tfac,
     const int32_t *__restrict__ yseg, const int32_t *__restrict__ yptr, const int64_t *__restrict__ ylist,
     int nd, double *__restrict__ y, int accumulate, PersistCounters *ctr) {
   const int lane = (int)(threadIdx.x & 63);
   extern __shared__ double tile_lds[];
   double *lds = tile_lds + (threadIdx.x >> 6) * lds_wave_doubles<S>();
+  const TileOut o{slots, tmap, t, tfac};  // stage-1 rows with a map go to t
   bool s1_seen = false, all_seen = false;
   for (;;) {
     const int item = wave_take(&ctr->head, lane);
     if (item >= nitems) break;
     if (item < na) {
       const TileDesc td = tiles_a[item];
-      run_tile<S, kScaled, kLoads, kPipe, kNT, true, kBV>(td, panels, vals, x, t, slots, scales, slots, lane, lds);
+      run_tile<S, kScaled, kLoads, kPipe, kNT, true, kBV>(td, panels, vals, x, t, slots, scales, o, lane, lds);
       if (item < n1) wave_signal(&ctr->s1_done, lane);
       wave_signal(&ctr->tiles_done, lane);
     } else if (item < nt) {
@@ -624,7 +657,7 @@ __global__ __launch_bounds__(256) void persistent_kernel(
         s1_seen = true;
       }
       const TileDesc td = tiles_b[item - na];
-      run_tile<S, kScaled, kLoads, kPipe, kNT, true, kBV>(td, panels, vals, x, t, slots, scales, slots, lane, lds);
+      run_tile<S, kScaled, kLoads, kPipe, kNT, true, kBV>(td, panels, vals, x, t, slots, scales, o, lane, lds);
       wave_signal(&ctr->tiles_done, lane);
     } else {
       if (!all_seen) {
@@ -719,6 +752,20 @@ __device__ __forceinline__ mf_f4 mfma_step(mf_s4 a, mf_s4 b, mf_f4 c) {
 __device__ __forceinline__ short bf16_bits(float f) {
   const uint32_t b = __float_as_uint(f);
   return (short)(uint16_t)((b + 0x7FFFu + ((b >> 16) & 1u)) >> 16);
+}
+
+// Row `row`, right-hand side `col` of a multi-RHS tile: store_row with rows
+// of nrhs entries (slots and t in the row-major multi-RHS layout).
+template <bool kScaled>
+__device__ __forceinline__ void store_row_multi(const TileDesc &td, int row, int col, int nrhs, double v,
+                                                const TileOut &o, const float *__restrict__ scales) {
+  if (td.map_off < 0) {
+    o.out[(int64_t)(td.out_off + row) * nrhs + col] = v;
+    return;
+  }
+  const int e = o.tmap[td.map_off + row];
+  if constexpr (kScaled) v *= (double)scales[o.tfac[e]] * (double)scales[o.tfac[e] + 1];
+  o.tout[(int64_t)e * nrhs + col] = v;
 }
 
 // Vector entry (col, rhs) of a panel in the multi-RHS layout, zero past the
@@ -901,7 +948,7 @@ template <typename S, bool kScaled, int NB, bool kHasTP>
 __device__ __forceinline__ void multi_tile_narrow(const TileDesc &td, const PanelDesc *__restrict__ panels,
                                                   const S *__restrict__ vals, const double *__restrict__ X,
                                                   const double *__restrict__ T, const double *__restrict__ TP,
-                                                  const float *__restrict__ scales, double *__restrict__ out,
+                                                  const float *__restrict__ scales, const TileOut &o,
                                                   int lane) {
   static_assert(sizeof(S) == 4, "the narrow path is FP32 only");
   using M = Mfma<S>;
@@ -935,8 +982,7 @@ __device__ __forceinline__ void multi_tile_narrow(const TileDesc &td, const Pane
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
           const int row = 16 * ch + M::row(lane, v);
-          if (row < td.nrows)
-            out[(int64_t)(td.out_off + row) * nrhs + 16 * cb + (lane & 15)] = (double)acc[ch][cb][v];
+          if (row < td.nrows) store_row_multi<kScaled>(td, row, 16 * cb + (lane & 15), nrhs, (double)acc[ch][cb][v], o, scales);
         }
 }
 
@@ -948,8 +994,7 @@ __global__ __launch_bounds__(256) void tile_multi_kernel(const TileDesc *__restr
                                                          const S *__restrict__ vals, const double *__restrict__ X,
                                                          const double *__restrict__ T,
                                                          const double *__restrict__ TP,
-                                                         const float *__restrict__ scales,
-                                                         double *__restrict__ out) {
+                                                         const float *__restrict__ scales, TileOut o) {
   using M = Mfma<S>;
   using Acc = typename M::Acc;
   using Sh = MultiShape<S, NB>;
@@ -962,7 +1007,7 @@ __global__ __launch_bounds__(256) void tile_multi_kernel(const TileDesc *__restr
   const TileDesc td = tiles[tid];
   if constexpr (sizeof(S) == 4) {
     if (td.nrows <= 8 * V) {
-      multi_tile_narrow<S, kScaled, NB, kHasTP>(td, panels, vals, X, T, TP, scales, out, lane);
+      multi_tile_narrow<S, kScaled, NB, kHasTP>(td, panels, vals, X, T, TP, scales, o, lane);
       return;
     }
   }
@@ -1006,7 +1051,7 @@ __global__ __launch_bounds__(256) void tile_multi_kernel(const TileDesc *__restr
             // result row i of MFMA m: row V*i+m of the chunk, minus the shift
             const int row = V * (16 * ch + g0 + M::row(lane, v)) + m - shift;
             if (row >= 0 && row < td.nrows)
-              out[(int64_t)(td.out_off + row) * nrhs + 16 * cb + (lane & 15)] = (double)acc[ch][m][cb][v];
+              store_row_multi<kScaled>(td, row, 16 * cb + (lane & 15), nrhs, (double)acc[ch][m][cb][v], o, scales);
           }
   }
 }

@@ -65,7 +65,7 @@ std::string build_plan(const LeafArrays &la, const PlanOptions &opt, Plan *plan)
   P.nd = la.nd;
   P.nlf = la.nlf;
   if (la.nd < 1 || la.nlf < 1) return "empty matrix";
-  if (opt.w1 < 64 || opt.w2 < 64 || opt.align < 1 || opt.t_mode < 0 || opt.t_mode > 2) return "bad plan options";
+  if (opt.w1 < 64 || opt.w2 < 64 || opt.align < 1 || opt.t_mode < 0 || opt.t_mode > 3) return "bad plan options";
   const int MR = opt.max_rows;
   const int MRb = opt.max_rows_b > 0 ? opt.max_rows_b : MR;  // stage-2 tiles
   const int lim = max_tile_rows_limit(lane_elems(opt.storage));
@@ -121,6 +121,102 @@ std::string build_plan(const LeafArrays &la, const PlanOptions &opt, Plan *plan)
   int64_t off = 0;
   std::vector<TileDesc> t1, td, t2;        // stage-1, dense, stage-2 tiles
 
+  std::vector<char> mat(nlf, 0);  // stage 2 of this leaf reads the materialised t
+  if (opt.t_mode == 3) {
+    // ---- stage 1 in bands, t written directly (t_mode 3) -----------------------------
+    // t in row-cluster order, ranks contiguous (so that consecutive a2 factors
+    // of a row cluster multiply one contiguous slice of t and merge)
+    int64_t tp = 0;
+    for (auto &g : rowg)
+      for (int i : g.second)
+        if (la.ltmtx[i] == 1 && la.kt[i] > 0) {
+          mat[i] = 1;
+          t_idx[i] = tp;
+          tp += la.kt[i];
+        }
+    P.n_t = tp;
+    if (P.n_t > INT32_MAX) return "t exceeds int32";
+    P.factor_of_trow.assign((size_t)P.n_t, 0);
+    for (int i = 0; i < nlf; i++)
+      for (int k = 0; mat[i] && k < la.kt[i]; k++) P.factor_of_trow[t_idx[i] + k] = 2 * i;
+    // A column cluster's stacked a1 factors (K rank rows x ndt columns) are
+    // cut into bands of near-equal height r (a multiple of V, about w1/ndt
+    // rows, at most MR): band b is stored on its own, column-major with
+    // stride rp (r rounded up to V), so a tile reads one contiguous block and
+    // computes its rank rows over all of the cluster's columns.
+    for (auto &g : colg) {
+      const int nstrtt = g.first.first, ndt = g.first.second;
+      const std::vector<int> &ls = g.second;
+      int64_t K64 = 0;
+      for (int i : ls) K64 += la.kt[i];
+      if (K64 > INT_MAX / 2) return "rank stack too large";
+      const int K = (int)K64;
+      if (K == 0) continue;
+      const int rt = std::max<int64_t>(V, std::min<int64_t>(MR, (int64_t)opt.w1 / ndt / V * V));
+      const int nb = (K + rt - 1) / rt;
+      const int rb = std::min<int64_t>(MR, align_up((K + nb - 1) / nb, V));
+      // stack row -> (leaf, rank) in stacking order
+      std::vector<int> row_leaf((size_t)K), row_k((size_t)K), leaf_kofs(ls.size());
+      {
+        int kofs = 0;
+        for (size_t q = 0; q < ls.size(); q++) {
+          leaf_kofs[q] = kofs;
+          for (int k = 0; k < la.kt[ls[q]]; k++) {
+            row_leaf[(size_t)kofs + k] = ls[q];
+            row_k[(size_t)kofs + k] = k;
+          }
+          kofs += la.kt[ls[q]];
+        }
+      }
+      for (int kb0 = 0; kb0 < K; kb0 += rb) {
+        const int nr = std::min(rb, K - kb0);
+        const int rp = (int)align_up(nr, V);
+        const int64_t base = align_up(off, opt.align);
+        off = base + (int64_t)ndt * rp;
+        // the pieces of the band's leaves
+        for (size_t q = 0; q < ls.size(); q++) {
+          const int i = ls[q], kofs = leaf_kofs[q], kt = la.kt[i];
+          const int a = std::max(kofs, kb0), b = std::min(kofs + kt, kb0 + nr);
+          if (a >= b) continue;
+          RepackDesc r;
+          r.src_off = la.src_a1[i] + (int64_t)(a - kofs) * ndt;
+          r.dst_off = base + (a - kb0);
+          r.rows = b - a;  // k
+          r.cols = ndt;    // it
+          r.src_rs = ndt;
+          r.src_cs = 1;
+          r.dst_cs = rp;
+          r.factor = 2 * i;
+          P.repack.push_back(r);
+        }
+        PanelDesc pd;
+        pd.val_off = base;
+        pd.ld = rp;
+        pd.ncols = ndt;
+        pd.vec_off = nstrtt - 1;
+        pd.vec_sel = kVecX;
+        pd.factor = -1;  // the t entries get the leaves' scales when they are stored
+        pd.vec_chunks = 0;
+        TileDesc tdsc;
+        tdsc.nrows = nr;
+        tdsc.panel0 = (int)P.panels.size();
+        tdsc.npanel = 1;
+        tdsc.out_off = 0;
+        tdsc.map_off = (int)P.tmap.size();
+        for (int r = 0; r < nr; r++)
+          P.tmap.push_back((int32_t)(t_idx[row_leaf[(size_t)kb0 + r]] + row_k[(size_t)kb0 + r]));
+        P.panels.push_back(pd);
+        t1.push_back(tdsc);
+        P.elems_stage1 += (int64_t)nr * ndt;
+        P.vec_a += ndt;
+      }
+    }
+    if (P.tmap.size() > (size_t)INT32_MAX) return "t map exceeds int32";
+    P.tsum.n = 0;
+    P.tsum.ptr.assign(1, 0);
+    P.t_fused = false;
+    P.t_direct = true;
+  } else {
   // ---- stage 1 layout + tiles -----------------------------------------------------
   int64_t tpos = 0;
   for (auto &g : colg) {
@@ -221,7 +317,6 @@ std::string build_plan(const LeafArrays &la, const PlanOptions &opt, Plan *plan)
   //             repeated by several row tiles or span more than two chunks
   bool any_forced = false;
   for (int i = 0; i < nlf; i++) any_forced = any_forced || forced[i];
-  std::vector<char> mat(nlf, 0);
   for (int i = 0; i < nlf; i++) {
     if (la.ltmtx[i] != 1 || la.kt[i] == 0) continue;
     if (opt.t_mode == 1 || (opt.t_mode == 0 && any_forced) || forced[i])
@@ -263,6 +358,7 @@ std::string build_plan(const LeafArrays &la, const PlanOptions &opt, Plan *plan)
     P.tsum.ptr.push_back((int)P.tsum.list.size());
   }
   P.t_fused = P.n_t == 0;
+  }  // t_mode 0..2
 
   // ---- dense / stage 2 layout + tiles ------------------------------------------------
   struct Slot { int64_t row0; int nr; int64_t z; };
@@ -469,6 +565,7 @@ std::string verify_plan(const Plan &P, const LeafArrays &la) {
   const int64_t npan = (int64_t)P.panels.size();
   std::vector<int64_t> seen2(P.nlf, 0);
   int64_t seen1 = 0;
+  std::vector<char> t_written(P.t_direct ? (size_t)P.n_t : 0, 0);
   for (int list = 0; list < 2; list++) {
     const std::vector<TileDesc> &tl = list == 0 ? P.tiles_a : P.tiles_b;
     for (const TileDesc &t : tl) {
@@ -514,9 +611,20 @@ std::string verify_plan(const Plan &P, const LeafArrays &la) {
         if (d.vec_sel == kVecTP && d.factor < 1) return "stage-2 panel without an a1 factor";
         if (d.vec_sel == kVecX && d.factor < 0) {  // stage-1 panel
           if (list != 0 || t.npanel != 1) return "stage-1 tile malformed";
-          if ((int64_t)t.out_off + t.nrows > P.n_tp) return "stage-1 slot out of range";
+          if (P.t_direct) {  // rows go to t through the map, each t entry once
+            if (t.map_off < 0 || (int64_t)t.map_off + t.nrows > (int64_t)P.tmap.size()) return "t map range";
+            for (int r = 0; r < t.nrows; r++) {
+              const int32_t e = P.tmap[(size_t)t.map_off + r];
+              if (e < 0 || e >= P.n_t) return "t map entry out of range";
+              if (t_written[(size_t)e]++) return "t entry written twice";
+            }
+          } else {
+            if (t.map_off >= 0) return "row map without direct t";
+            if ((int64_t)t.out_off + t.nrows > P.n_tp) return "stage-1 slot out of range";
+          }
           seen1 += (int64_t)t.nrows * d.ncols;
         } else {
+          if (t.map_off >= 0) return "row map on a dense or stage-2 tile";
           if (d.factor < 0 || d.factor >= P.nfactors || d.factor % 2 != 1) return "factor id out of range";
           if (t.out_off < P.n_tp || (int64_t)t.out_off + t.nrows > P.n_tp + P.n_z) return "output slot out of range";
           if ((list == 1) != (d.vec_sel != kVecX)) return "panel in the wrong launch";
@@ -558,10 +666,12 @@ std::string verify_plan(const Plan &P, const LeafArrays &la) {
     if (la.ltmtx[i] == 1) want1 += (int64_t)la.kt[i] * la.ndt[i];
   }
   if (seen1 != want1) return "stage-1 coverage wrong";
+  for (char w : t_written)
+    if (!w) return "t entry never written";
 
   // 3. segment sums index inside their buffers
   if ((int64_t)P.tsum.seg.size() != P.tsum.n || (int64_t)P.ysum.seg.size() != P.ysum.n || P.ysum.n != P.nd ||
-      P.tsum.n != P.n_t)
+      P.tsum.n != (P.t_direct ? 0 : P.n_t))
     return "segment-sum sizes";
   if ((int64_t)P.factor_of_trow.size() != P.n_t) return "t factor table size";
   for (int32_t f : P.factor_of_trow)

@@ -25,10 +25,17 @@
 // streaming loop.
 //
 // Launch A runs the stage-1 tiles and the dense tiles (independent of each
-// other); a segment sum adds the stage-1 partial slots into t (t_mode 1, or
-// for the leaves t_mode 2 picks); launch B runs the stage-2 tiles, which read
-// t or sum the partial slots of their leaf on the fly (t_mode 0); a final
-// segment sum adds the private output slots of all tiles into y in a fixed
+// other).  With t_mode 3 every stage-1 tile is a band of rank rows of its
+// column cluster's stack over ALL of the cluster's columns, so each entry of
+// t is computed by exactly one tile, which writes it straight into t through
+// a per-row index map (t is kept in row-cluster order); no t sum launch, but
+// a large cluster then gets thin bands that each re-read its x slice, which
+// measured slower than the partial slots at N = 101,250.  Otherwise (the
+// default) stage-1 tiles cover column chunks and write partial slots; a
+// segment sum adds them into t (t_mode 1, or for the leaves t_mode 2 picks),
+// or the stage-2 tiles sum the partial slots of their leaf on the fly
+// (t_mode 0).  Launch B runs the stage-2 tiles; a final segment sum adds the
+// private output slots of all dense and stage-2 tiles into y in a fixed
 // order: y is accumulated without atomics and the result is bit-reproducible
 // run to run.
 #pragma once
@@ -66,11 +73,13 @@ struct PanelDesc {        // 32 bytes, read with scalar loads
   int32_t vec_chunks;     // kVecTP: (nch << 16) | stride
 };
 
-struct TileDesc {         // 16 bytes
+struct TileDesc {         // 32 bytes (one scalar load)
   int32_t nrows;          // 1..Plan::max_rows
   int32_t panel0;
   int32_t npanel;
-  int32_t out_off;        // first element of the tile's output slot
+  int32_t out_off;        // first element of the tile's output slot (map_off < 0)
+  int32_t map_off = -1;   // >= 0: row r of the tile goes to t[tmap[map_off + r]]
+  int32_t reserved[3] = {0, 0, 0};
 };
 
 // One source->device copy: element (r, c) of a rows x cols factor.
@@ -96,7 +105,8 @@ struct PlanOptions {
   int w2 = 16384;         // target elements per dense / stage-2 tile
   int align = 16;         // factor start alignment in elements
   int order = 0;          // 0: memory order, 1: largest tiles first
-  int t_mode = 0;         // stage-2 vector: 0 partial sums on the fly, 1 materialised t, 2 per leaf
+  int t_mode = 0;         // stage-2 vector: 0 partial sums on the fly, 1 materialised t, 2 per leaf,
+                          // 3 t written directly by banded stage-1 tiles
   int max_rows = kMaxTileRows;  // rows per tile (rank rows per stage-1 group, leaf rows per chunk)
   int max_rows_b = 0;           // rows per stage-2 tile (0: max_rows)
 };
@@ -122,6 +132,8 @@ struct Plan {
   int64_t len = 0;                   // reference element count
   int nfactors = 0;
   bool t_fused = true;               // no stage-2 panel reads the materialised t (no t sum launch)
+  bool t_direct = false;             // t_mode 3: stage-1 tiles write t themselves (no t sum launch)
+  std::vector<int32_t> tmap;         // t_mode 3: t index of every row of every stage-1 tile
   std::vector<RepackDesc> repack;
   std::vector<int32_t> factor_of_trow;  // t index -> a1 factor id (BF16 t scaling)
   std::vector<TileDesc> tiles_a, tiles_b;  // tiles_a: stage-1 and dense tiles

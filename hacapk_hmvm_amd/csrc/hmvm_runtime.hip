@@ -141,6 +141,7 @@ struct hmvm_plan {
   PanelDesc *d_panels = nullptr;
   double *d_t = nullptr, *d_slots = nullptr;  // stage-1 partial slots, then output slots
   int32_t *d_tfac = nullptr;
+  int32_t *d_tmap = nullptr;  // t_mode 3: t index of every stage-1 tile row
   int64_t *d_tlist = nullptr;
   int32_t *d_yseg = nullptr, *d_yptr = nullptr;
   int64_t *d_ylist = nullptr;
@@ -151,7 +152,7 @@ struct hmvm_plan {
 
   ~hmvm_plan() {
     void *ptrs[] = {d_vals,  d_tiles_a, d_tiles_b, d_panels, d_t,      d_slots,
-                    d_tfac,  d_tlist,   d_yseg,    d_yptr,   d_ylist,  d_scales, d_x,    d_y,
+                    d_tfac,  d_tmap,    d_tlist,   d_yseg,   d_yptr,   d_ylist,  d_scales, d_x,    d_y,
                     d_trng,  d_yrng,
                     d_ctr,   d_mslots,  d_mt,      d_mX,     d_mY};
     for (void *p : ptrs)
@@ -426,8 +427,10 @@ PlanOptions plan_options(const hmvm_options *o, int64_t len) {
   // t_mode < 0: materialise t for every leaf.  With t in row-cluster order
   // the stage-2 factors of a cluster merge into one panel, which beats summing
   // the stage-1 partials on the fly in all three storages (measured at
-  // N = 101,250) despite the extra launch.  The persistent mode has no launch
-  // boundary for the t sum, so there the stage-2 tiles sum the partials.
+  // N = 101,250) despite the extra launch, and beats t_mode 3 (stage-1 bands
+  // writing t directly: thin bands over large clusters re-read x).  The
+  // persistent mode has no launch boundary for the t sum, so there the
+  // stage-2 tiles sum the partials.
   po.t_mode = o->t_mode >= 0 ? o->t_mode : o->mode == 1 ? 0 : 1;
   return po;
 }
@@ -716,9 +719,9 @@ int create_plan(Source &src, const hmvm_options *opt, hmvm_plan **out) {
   err = verify_plan(p->plan, src.la);
   if (!err.empty()) return set_error(HMVM_ERR_MATRIX, "schedule check failed: %s", err.c_str());
   Plan &P = p->plan;
-  if (p->mode == 1 && !P.t_fused)
-    return set_error(HMVM_ERR_UNSUPPORTED, "mode 1 (persistent launch) needs t_mode 0: stage-2 tiles of this plan "
-                                           "read a materialised t");
+  if (p->mode == 1 && !P.t_fused && !P.t_direct)
+    return set_error(HMVM_ERR_UNSUPPORTED, "mode 1 (persistent launch) needs t_mode 0 or 3: this plan sums t in a "
+                                           "launch of its own");
   for (const PanelDesc &pd : P.panels) p->has_tp = p->has_tp || pd.vec_sel == kVecTP;
   p->a_mixed = order_mixed(P);
   HIP_TRY(hipGetDevice(&p->device));
@@ -744,6 +747,7 @@ int create_plan(Source &src, const hmvm_options *opt, hmvm_plan **out) {
       (rc = upload(&p->d_yrng, seg_ranges(P.ysum))))
     return rc;
   if (p->mode != 0 && (rc = partition_a(p.get()))) return rc;
+  if (P.t_direct && (rc = upload(&p->d_tmap, P.tmap))) return rc;
   if (p->storage == kBF16) {
     if ((rc = upload(&p->d_tfac, P.factor_of_trow))) return rc;
     if ((rc = dalloc(&p->d_scales, (size_t)P.nfactors))) return rc;
@@ -775,6 +779,12 @@ int create_plan(Source &src, const hmvm_options *opt, hmvm_plan **out) {
   return HMVM_OK;
 }
 
+// Where the tiles of one launch store their rows: the slots, and for
+// stage-1 tiles with a row map the plan's t (or the multi-RHS T).
+TileOut tile_out(const hmvm_plan *p, double *out, double *tout) {
+  return TileOut{out, p->d_tmap, tout, p->d_tfac};
+}
+
 // kind: unused by the one-wave-per-tile launch (kept so every launch names
 // its part of the HMVM: 0 A, 1 B, 2 stage-1 part of A, 3 dense part)
 template <typename S, bool kScaled, int kVar>
@@ -787,7 +797,8 @@ void launch_tiles_v(hmvm_plan *p, int kind, int bt, hipStream_t st, const TileDe
   const int wpb = bt / 64;
   const size_t lds = sizeof(double) * wpb * lds_wave_doubles<S>();
   hipLaunchKernelGGL((tile_kernel<S, kScaled, L, kPipe, kNT, kBV>), dim3((ntiles + wpb - 1) / wpb), dim3(bt), lds, st,
-                     d_tiles, ntiles, p->d_panels, (const S *)p->d_vals, x, t, tp, (const float *)p->d_scales, out);
+                     d_tiles, ntiles, p->d_panels, (const S *)p->d_vals, x, t, tp, (const float *)p->d_scales,
+                     tile_out(p, out, p->d_t));
 }
 
 template <typename S, bool kScaled>
@@ -851,7 +862,7 @@ void launch_timed_v(hmvm_plan *p, int bt, const TileDesc *d_tiles, int ntiles, c
   const size_t lds = sizeof(double) * wpb * lds_wave_doubles<S>();
   hipLaunchKernelGGL((tile_kernel_timed<S, kScaled, L, kPipe, kNT, kBV>), dim3((ntiles + wpb - 1) / wpb),
                      dim3(bt), lds, st, d_tiles, ntiles, p->d_panels, (const S *)p->d_vals, x, t, tp,
-                     (const float *)p->d_scales, out, stamps);
+                     (const float *)p->d_scales, tile_out(p, out, p->d_t), stamps);
 }
 
 template <typename S, bool kScaled>
@@ -932,28 +943,33 @@ int reserve_multi(hmvm_plan *p, int nrhs) {
   return HMVM_OK;
 }
 
+// T: the t read by stage-2 panels (launch B; NULL in launch A, whose
+// stage-1 tiles with a row map write it instead)
 template <typename S, bool kScaled, int NB>
-void launch_multi_tiles(hmvm_plan *p, const TileDesc *d_tiles, int ntiles, const double *X, hipStream_t st) {
+void launch_multi_tiles(hmvm_plan *p, const TileDesc *d_tiles, int ntiles, const double *X, const double *T,
+                        hipStream_t st) {
   if (ntiles == 0) return;
   const int wpb = p->block_threads / 64;
+  const TileOut o = tile_out(p, p->d_mslots, p->d_mt);
   if (p->has_tp)
     hipLaunchKernelGGL((tile_multi_kernel<S, kScaled, NB, true>), dim3((ntiles + wpb - 1) / wpb),
-                       dim3(p->block_threads), 0, st, d_tiles, ntiles, p->d_panels, (const S *)p->d_vals, X, p->d_mt,
-                       p->d_mslots, (const float *)p->d_scales, p->d_mslots);
+                       dim3(p->block_threads), 0, st, d_tiles, ntiles, p->d_panels, (const S *)p->d_vals, X, T,
+                       p->d_mslots, (const float *)p->d_scales, o);
   else
     hipLaunchKernelGGL((tile_multi_kernel<S, kScaled, NB, false>), dim3((ntiles + wpb - 1) / wpb),
-                       dim3(p->block_threads), 0, st, d_tiles, ntiles, p->d_panels, (const S *)p->d_vals, X, p->d_mt,
-                       p->d_mslots, (const float *)p->d_scales, p->d_mslots);
+                       dim3(p->block_threads), 0, st, d_tiles, ntiles, p->d_panels, (const S *)p->d_vals, X, T,
+                       p->d_mslots, (const float *)p->d_scales, o);
 }
 
 template <typename S, bool kScaled>
-void launch_multi_nb(hmvm_plan *p, const TileDesc *d_tiles, int ntiles, const double *X, int nrhs, hipStream_t st) {
+void launch_multi_nb(hmvm_plan *p, const TileDesc *d_tiles, int ntiles, const double *X, const double *T, int nrhs,
+                     hipStream_t st) {
   if (nrhs == 16)
-    launch_multi_tiles<S, kScaled, 1>(p, d_tiles, ntiles, X, st);
+    launch_multi_tiles<S, kScaled, 1>(p, d_tiles, ntiles, X, T, st);
   else if (nrhs == 32)
-    launch_multi_tiles<S, kScaled, 2>(p, d_tiles, ntiles, X, st);
+    launch_multi_tiles<S, kScaled, 2>(p, d_tiles, ntiles, X, T, st);
   else
-    launch_multi_tiles<S, kScaled, 4>(p, d_tiles, ntiles, X, st);
+    launch_multi_tiles<S, kScaled, 4>(p, d_tiles, ntiles, X, T, st);
 }
 
 void launch_multi_phase(hmvm_plan *p, int phase, const double *X, double *Y, int nrhs, int accumulate,
@@ -963,12 +979,13 @@ void launch_multi_phase(hmvm_plan *p, int phase, const double *X, double *Y, int
   if (phase == 0 || phase == 2) {
     const TileDesc *tl = phase == 0 ? p->d_tiles_a : p->d_tiles_b;
     const int nt = (int)(phase == 0 ? P.tiles_a.size() : P.tiles_b.size());
+    const double *T = phase == 0 ? nullptr : p->d_mt;
     if (p->storage == kF64)
-      launch_multi_nb<double, false>(p, tl, nt, X, nrhs, st);
+      launch_multi_nb<double, false>(p, tl, nt, X, T, nrhs, st);
     else if (p->storage == kF32)
-      launch_multi_nb<float, false>(p, tl, nt, X, nrhs, st);
+      launch_multi_nb<float, false>(p, tl, nt, X, T, nrhs, st);
     else
-      launch_multi_nb<uint16_t, true>(p, tl, nt, X, nrhs, st);
+      launch_multi_nb<uint16_t, true>(p, tl, nt, X, T, nrhs, st);
   } else if (phase == 1) {
     // two columns per thread (measured 15-20% faster than one at N=101,250);
     // slots and t are the plan's own allocations, nrhs is even
@@ -1029,7 +1046,8 @@ int launch_persistent_v(hmvm_plan *p, const double *x, double *y, int accumulate
   const int nitems = nt + (P.nd + 63) / 64;
   hipLaunchKernelGGL(kfn, dim3(blocks), dim3(p->block_threads), lds, st, p->d_tiles_a, ns1, na, p->d_tiles_b, nt,
                      nitems, nwaves, p->d_panels, (const S *)p->d_vals, x, p->d_t, p->d_slots,
-                     (const float *)p->d_scales, p->d_yseg, p->d_yptr, p->d_ylist, P.nd, y, accumulate, p->d_ctr);
+                     (const float *)p->d_scales, p->d_tmap, p->d_tfac, p->d_yseg, p->d_yptr, p->d_ylist, P.nd, y,
+                     accumulate, p->d_ctr);
   return HMVM_OK;
 }
 
@@ -1055,7 +1073,9 @@ int launch_persistent(hmvm_plan *p, const double *x, double *y, int accumulate, 
 
 // The persistent mode needs every stage-2 tile to read its leaf's stage-1
 // partial slots directly (no materialised t).
-bool persistent_ok(const hmvm_plan *p) { return p->mode == 1 && p->plan.t_fused && !p->a_mixed; }
+bool persistent_ok(const hmvm_plan *p) {
+  return p->mode == 1 && (p->plan.t_fused || p->plan.t_direct) && !p->a_mixed;
+}
 
 // Dense tiles depend only on x and write their own output slots, so in mode 2
 // they run on a second stream, concurrently with the stage-1 tiles, the t sum
@@ -1117,6 +1137,7 @@ void fill_info(const hmvm_plan *p, const Plan &P, int storage, hmvm_plan_info *i
   info->algorithmic_bytes = (long long)info->elem_bytes * P.len + 16LL * P.nd;
   info->device_bytes = p ? (long long)p->device_bytes : 0;
   info->mode = !p ? 0 : persistent_ok(p) ? 1 : overlap_ok(p) ? 2 : 0;
+  info->t_direct = P.t_direct ? 1 : 0;
 }
 
 // Reads and clears the fault word of the persistent launches (after the
@@ -1464,8 +1485,8 @@ int hmvm_plan_time_multi(hmvm_plan *plan, int nrhs, int warmup, int iters, hmvm_
 int hmvm_plan_set_mode(hmvm_plan *plan, int mode) {
   if (!plan) return set_error(HMVM_ERR_ARG, "NULL plan");
   if (mode < 0 || mode > 2) return set_error(HMVM_ERR_ARG, "mode must be 0, 1 or 2");
-  if (mode == 1 && !plan->plan.t_fused)
-    return set_error(HMVM_ERR_UNSUPPORTED, "mode 1 (persistent launch) needs a plan built with t_mode 0");
+  if (mode == 1 && !plan->plan.t_fused && !plan->plan.t_direct)
+    return set_error(HMVM_ERR_UNSUPPORTED, "mode 1 (persistent launch) needs a plan built with t_mode 0 or 3");
   if (mode != 0) {
     const int rc = partition_a(plan);
     if (rc) return rc;
@@ -1572,7 +1593,11 @@ int export_schedule(const Source &s, const hmvm_options *opt, hmvm_schedule *out
   out->repack = (hmvm_repack *)copy_out(P.repack);
   export_segsum(P.tsum, &out->tsum);
   export_segsum(P.ysum, &out->ysum);
-  if (!out->tiles_a || !out->tiles_b || !out->panels || !out->repack || !out->tsum.seg || !out->ysum.seg) {
+  out->t_direct = P.t_direct ? 1 : 0;
+  out->ntmap = (long long)P.tmap.size();
+  out->tmap = copy_out(P.tmap);
+  if (!out->tiles_a || !out->tiles_b || !out->panels || !out->repack || !out->tsum.seg || !out->ysum.seg ||
+      !out->tmap) {
     hmvm_schedule_release(out);
     return set_error(HMVM_ERR_NOMEM, "schedule export");
   }
@@ -1621,7 +1646,7 @@ int hmvm_schedule_export_batch(const matrix2 *const *mats, int count, int a2_sou
 void hmvm_schedule_release(hmvm_schedule *s) {
   if (!s) return;
   void *ptrs[] = {s->tiles_a, s->tiles_b, s->panels, s->repack, s->tsum.seg, s->tsum.ptr, s->tsum.list,
-                  s->ysum.seg, s->ysum.ptr, s->ysum.list};
+                  s->ysum.seg, s->ysum.ptr, s->ysum.list, s->tmap};
   for (void *p : ptrs) std::free(p);
   std::memset(s, 0, sizeof(*s));
 }

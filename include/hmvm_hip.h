@@ -107,16 +107,17 @@ typedef struct hmvm_options {
   int order;             /* tile order in a launch: 0 memory, 1 largest first   */
                          /* (launch A: stage-1 and dense tiles in one order)    */
   int mode;              /* 0: launches on one stream; 1: one persistent launch */
-                         /* with a device work queue (needs every stage-2 tile  */
-                         /* to sum its stage-1 partials itself: t_mode 0, which */
-                         /* t_mode -1 then selects; a plan with a materialised  */
-                         /* t is refused with HMVM_ERR_UNSUPPORTED); 2:         */
-                         /* launches, dense tiles on a second stream concurrent */
-                         /* with stage 1/2                                      */
+                         /* with a device work queue (needs t_mode 0 or 3: no   */
+                         /* t sum launch; a plan whose t is summed by a launch  */
+                         /* of its own is refused with HMVM_ERR_UNSUPPORTED);   */
+                         /* 2: launches, dense tiles on a second stream         */
+                         /* concurrent with stage 1/2                           */
   int t_mode;            /* stage-2 vector entries: 0 sum the stage-1 partial   */
                          /* slots on the fly, 1 read a materialised t (one more */
-                         /* launch), 2 choose per leaf, -1 default (1; 0 in     */
-                         /* mode 1)                                             */
+                         /* launch), 2 choose per leaf, 3 t written directly by */
+                         /* the stage-1 tiles (bands of rank rows over all of a */
+                         /* column cluster's columns; no extra launch), -1      */
+                         /* default (1; 0 in mode 1)                            */
   int max_rows;          /* rows per tile (0: by storage); at most 192*V-V+1,  */
                          /* V = 16 bytes / element size; when set, also for the */
                          /* stage-2 tiles unless max_rows_b is set              */
@@ -140,7 +141,7 @@ typedef struct hmvm_plan_info {
   long long algorithmic_bytes; /* elem_bytes*len + 16*nd (x read, y written)    */
   long long device_bytes;      /* HBM held by the plan                          */
   int mode;                    /* execution mode an apply runs (hmvm_options)   */
-  int pad_;
+  int t_direct;                /* 1: stage-1 tiles write t (t_mode 3)           */
 } hmvm_plan_info;
 
 typedef struct hmvm_timing {
@@ -358,12 +359,13 @@ int hmvm_bicgstab_static(hmvm_plan *plan, const double *d_b, double *d_u, int ma
 /* ---- schedule introspection (host only) -------------------------------------
  * The schedule a plan runs, as plain arrays: the autotuner's features, and
  * the CPU tests that replay it.  Element (r, j) of a tile's panel p is
- * values[val_off + r + j*ld]; the panel's vector entry j is x[vec_off + j]
- * (vec_sel 0), t[vec_off + j] (1) or sum_{c<nch} slots[vec_off + c*stride + j]
- * (2, vec_chunks = nch<<16 | stride).  Repack record: values[dst_off + r +
+ * values[val_off + r + j*ld]; row r of a tile goes to slot out_off + r, or to
+ * t[tmap[map_off + r]] when map_off >= 0; the panel's vector entry j is
+ * x[vec_off + j] (vec_sel 0), t[vec_off + j] (1) or
+ * sum_{c<nch} slots[vec_off + c*stride + j] (2, vec_chunks = nch<<16 | stride).  Repack record: values[dst_off + r +
  * c*dst_cs] = source[src_off + r*src_rs + c*src_cs].  Segment sums:
  * out[i] (+)= sum_{e in ptr[seg[i]]..ptr[seg[i]+1]} slots[list[e] + i]. */
-typedef struct hmvm_tile { int nrows, panel0, npanel, out_off; } hmvm_tile;
+typedef struct hmvm_tile { int nrows, panel0, npanel, out_off, map_off, reserved[3]; } hmvm_tile;
 typedef struct hmvm_panel {
   long long val_off;
   int ld, ncols, vec_off, vec_sel, factor, vec_chunks;
@@ -384,6 +386,9 @@ typedef struct hmvm_schedule {
   hmvm_panel *panels;
   hmvm_repack *repack;
   hmvm_segsum tsum, ysum;
+  int t_direct;               /* stage-1 tiles with map_off >= 0 write t[tmap[map_off + r]] */
+  long long ntmap;
+  int *tmap;
 } hmvm_schedule;
 
 int hmvm_schedule_export(const matrix2 *mat2, int a2_source, const hmvm_options *opt, hmvm_schedule *out);

@@ -102,7 +102,7 @@ def test_tile_order_and_phase_launch_keep_bits(gpu, H, mats):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("t_mode", [0, 1, 2])
+@pytest.mark.parametrize("t_mode", [0, 1, 2, 3])
 def test_t_modes(gpu, H, O, mats, t_mode):
     m = mats[2]
     x = np.random.default_rng(16).uniform(-1, 1, m.nd)
@@ -112,6 +112,8 @@ def test_t_modes(gpu, H, O, mats, t_mode):
     assert rel(y, ref) <= 1e-13
     if t_mode == 1:
         assert not p.info()["t_fused"]
+    if t_mode == 3:  # t written by the stage-1 tiles: no partial slots
+        assert p.info()["t_direct"] and p.info()["n_tslots"] == 0
     # the same t mode is bit-reproducible; modes differ in how panels merge
     assert np.array_equal(p.apply_host(x), y)
     p.close()

@@ -238,14 +238,17 @@ def test_rank_rows_straddle_row_groups(gpu, H, O, tmp_path):
         p.close()
 
 
+@pytest.mark.parametrize("t_mode", [0, 3])
 @pytest.mark.parametrize("storage", ["f64", "f32", "bf16"])
-def test_persistent_mode_matches_launches(gpu, H, sphere4096, storage):
+def test_persistent_mode_matches_launches(gpu, H, sphere4096, storage, t_mode):
     """The single persistent launch (device-wide work queue) computes the same
-    bits as the three-launch mode, repeatedly (its counters reset themselves)."""
+    bits as the launches, repeatedly (its counters reset themselves), with
+    stage 2 summing the partial slots (t_mode 0) or reading the t that the
+    stage-1 tiles of the same launch wrote (t_mode 3)."""
     h = sphere4096
     b = np.random.default_rng(14).uniform(-1, 1, h.nd)
-    p = H.Plan(h, storage=storage, t_mode=0)  # the persistent queue needs stage 2 to read the partial slots
-    assert p.info()["t_fused"]
+    p = H.Plan(h, storage=storage, t_mode=t_mode)  # no t sum launch in either mode
+    assert p.info()["t_fused"] or p.info()["t_direct"]
     y0 = p.apply_host(b)
     p.set_mode(1)
     assert p.info()["mode"] == 1  # the persistent launch is what runs

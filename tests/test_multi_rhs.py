@@ -68,7 +68,7 @@ def test_multi_rhs_rank_rows_straddle(gpu, H, O, tmp_path):
 
 
 @pytest.mark.parametrize("storage,tol", [("f64", 1e-13), ("f32", 2e-6), ("bf16", 2e-2)])
-@pytest.mark.parametrize("t_mode", [0, 2])
+@pytest.mark.parametrize("t_mode", [0, 2, 3])
 def test_multi_rhs_partial_slot_panels(gpu, H, hm, XY, storage, tol, t_mode):
     """Stage-2 panels that sum stage-1 partial slots (t_mode 0 and 2) take the
     kernel instantiation with the summing loop."""

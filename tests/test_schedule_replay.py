@@ -38,7 +38,10 @@ def replay(H, h, x, source="rowmat", **kw):
                 blk = vals[p["val_off"] + rr + jj * p["ld"]]
                 assert not np.isnan(blk).any()
                 acc += blk @ vector(p, p["ncols"])
-            slots[td["out_off"]:td["out_off"] + td["nrows"]] = acc
+            if td["map_off"] >= 0:  # stage-1 rows straight into t
+                t[s["tmap"][td["map_off"]:td["map_off"] + td["nrows"]]] = acc
+            else:
+                slots[td["out_off"]:td["out_off"] + td["nrows"]] = acc
 
     def segsum(ss):
         out = np.zeros(ss["n"])
@@ -48,7 +51,7 @@ def replay(H, h, x, source="rowmat", **kw):
         return out
 
     run(s["tiles_a"])
-    if not s["t_fused"]:
+    if not s["t_fused"] and not s["t_direct"]:
         t[:s["n_t"]] = segsum(s["tsum"])
     run(s["tiles_b"])
     return segsum(s["ysum"]), s
@@ -61,15 +64,18 @@ def rel(a, b):
 @pytest.mark.parametrize("kw", [{}, {"w1": 64, "w2": 64}, {"w1": 500, "w2": 3000, "order": 1},
                                 {"w1": 1 << 20, "w2": 1 << 20}])
 @pytest.mark.parametrize("source", ["rowmat", "rowmat_t"])
-@pytest.mark.parametrize("t_mode", [0, 1, 2])
+@pytest.mark.parametrize("t_mode", [0, 1, 2, 3])
 def test_replay_matches_oracle(H, O, kw, source, t_mode):
     h = H.generate(1500, geometry="cube", eps=1e-6)
     x = np.random.default_rng(3).uniform(-1, 1, h.nd)
     y, s = replay(H, h, x, source, t_mode=t_mode, **kw)
     if t_mode == 0:
         assert s["t_fused"]   # every stage-2 panel sums its partial slots
-    if t_mode == 1:
+    if t_mode in (1, 3):
         assert not s["t_fused"] and s["n_t"] == int(h.arrays()["kt"][h.arrays()["ltmtx"] == 1].sum())
+    if t_mode == 3:  # no partial slots, no t sum: every t entry written once by a stage-1 tile
+        assert s["t_direct"] and s["n_tslots"] == 0 and s["tsum"]["n"] == 0
+        assert np.array_equal(np.sort(s["tmap"]), np.arange(s["n_t"]))
     assert rel(y, O.seq_2(h, x)) <= 1e-13
 
 
