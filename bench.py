@@ -185,7 +185,22 @@ def run_c2(args, H, np):
                       "N": nd, "nlf": info64["nlf"], "len": length, "ktmax": h.mat2.ktmax,
                       "bytes_fp64": info64["algorithmic_bytes"], "parallelism": "single GPU"})
     line["data"] = "synthetic: generated H-matrix (1/(4 pi r), cube surface), x ~ U[-1,1] seed 20261202"
+    tuning = None
+    if not args.no_tune:
+        # schedule from the learned cost model against the exhaustive search on
+        # this matrix (the network is trained on other generated matrices)
+        t0 = time.time()
+        model, X, yy = train_cost_model(H, np, ("f64", "f32"))
+        tuning = {st: tune_report(H, np, h, st, model, args, X, yy) for st in ("f64", "f32")}
+        for st in tuning:
+            log("tune %s: %s" % (st, tuning[st]))
+        log("tuning done in %.1fs" % (time.time() - t0))
     line.update({
+        "learned_over_exhaustive": None if tuning is None else {
+            st: tuning[st]["learned_over_exhaustive"] for st in tuning},
+        "heldout_abs_log2_err": None if tuning is None else {
+            st: tuning[st].get("heldout_abs_log2_err") for st in tuning},
+        "tuning": tuning,
         "hbm_frac_whole_hmvm": round(whole_gbs / HBM_PEAK_GBS, 4),
         "fp64": forms["f64"], "fp32": forms["f32"], "bf16": forms["bf16"],
         "roofline": {"bound": "hbm", "kernel": kname, "achieved": round(k_gbs, 1), "peak": HBM_PEAK_GBS,
@@ -285,8 +300,11 @@ def train_cost_model(H, np, storages, seed=0):
     specs = [(20000, "sphere", 1e-4, 32), (36000, "cube", 1e-6, 32), (28000, "cube", 1e-4, 32),
              (44000, "sphere", 1e-6, 32), (70000, "sphere", 1e-4, 32), (130000, "cube", 1e-4, 32),
              (40000, "sphere", 1e-6, 64), (60000, "cube", 1e-4, 64)]
+    t_start = time.time()
     for i, (n, geo, eps, leaf) in enumerate(specs):
+        t0 = time.time()
         m = H.generate(n, geometry=geo, eps=eps, leaf=leaf, seed=1000 + i)
+        log("training matrix %d (N=%d): generated in %.1fs" % (i, n, time.time() - t0))
         for st in storages:
             X, y, r = AT.collect(m, st, w_scales=((1.0, 1.0), (0.5, 2.0), (2.0, 0.5)), iters=2, max_tiles=6000,
                                  seed=seed + i)
@@ -294,6 +312,7 @@ def train_cost_model(H, np, storages, seed=0):
             ys.append(y)
             recs += r
         m.free()
+        log("training matrix %d: tiles timed, %.1fs so far" % (i, time.time() - t_start))
     X, y = np.concatenate(Xs), np.concatenate(ys)
     model = AT.CostModel(seed=seed + 5)
     t0 = time.time()
@@ -447,7 +466,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--w1", type=int, default=0)
     ap.add_argument("--w2", type=int, default=0)
-    ap.add_argument("--no-tune", action="store_true", help="c5: skip the schedule check on a row band")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="c2: skip the learned-vs-exhaustive schedule search; c5: skip it on a row band")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:

@@ -175,9 +175,14 @@ class CostModel:
             return X
         return np.clip(X, self.lo, self.hi).astype(np.float32)
 
-    def fit(self, X, y, recs, epochs=80, batch=512, lr=2e-3, seed=3):
+    def fit(self, X, y, recs, epochs=80, batch=None, lr=2e-3, seed=3):
+        """Trains the network on (features, log2 us) and calibrates the launch
+        model.  batch=None: about 512 steps per epoch (512 to 8192 samples a
+        step), so that millions of training tiles cost seconds, not minutes."""
         self.lo, self.hi = X.min(axis=0), X.max(axis=0)
         self.mlp.fit_scaler(X)
+        if batch is None:
+            batch = int(min(8192, max(512, len(y) // 512)))
         loss = self.mlp.train(X, y, epochs=epochs, batch=batch, lr=lr, seed=seed)
         by = {}
         for r in recs:

@@ -22,6 +22,13 @@ def test_mat(H):
     return H.generate(14000, geometry="cube", eps=1e-4, seed=99)
 
 
+@pytest.fixture(scope="module")
+def big_mat(H):
+    """Large enough (FP32 0.7 GB, more than twice the 256 MB Infinity Cache)
+    that a schedule's time is stable to about a percent."""
+    return H.generate(70000, geometry="cube", eps=1e-4, seed=98)
+
+
 @pytest.mark.gpu
 def test_schedule_features_match_plan(gpu, H, test_mat):
     for storage in ("f64", "bf16"):
@@ -39,7 +46,7 @@ def test_schedule_features_match_plan(gpu, H, test_mat):
 
 
 @pytest.mark.gpu
-def test_learned_model_beats_baseline_and_tracks_exhaustive(gpu, H, train_mats, test_mat):
+def test_learned_model_beats_baseline_and_tracks_exhaustive(gpu, H, train_mats, test_mat, big_mat):
     storage = "f32"
     Xs, ys, recs = [], [], []
     for i, m in enumerate(train_mats):
@@ -55,10 +62,18 @@ def test_learned_model_beats_baseline_and_tracks_exhaustive(gpu, H, train_mats, 
     err_lin = np.mean(np.abs(AT.linear_baseline(X, y)(Xt) - yt))
     print("held-out mean |log2 error|: mlp %.3f  linear %.3f" % (err_mlp, err_lin))
     assert err_mlp < err_lin
-    best, _, _ = AT.exhaustive(test_mat, storage, warmup=2, iters=6)
-    choice, p, _ = AT.learned(test_mat, storage, model)
-    ms_learned = p.time(2, 6)["ms_stream"]
-    p.close()
-    print("exhaustive %.4f ms, learned %.4f ms (%s)" % (best["ms"], ms_learned, choice))
-    # small matrices time noisily; the 5% target is measured at full size by bench.py
-    assert ms_learned <= 1.25 * best["ms"]
+    # schedule quality on the large matrix: the learned schedule within 5% of
+    # the exhaustive one, both timed afresh, interleaved, median of five
+    best, _, _ = AT.exhaustive(big_mat, storage, warmup=2, iters=6)
+    p_ex = H.Plan(big_mat, storage=storage, w1=best["w1"], w2=best["w2"])
+    AT._apply_choice(p_ex, best)
+    choice, p_l, _ = AT.learned(big_mat, storage, model)
+    t_ex, t_l = [], []
+    for _ in range(5):
+        t_ex.append(p_ex.time(3, 20)["ms_stream"])
+        t_l.append(p_l.time(3, 20)["ms_stream"])
+    p_ex.close()
+    p_l.close()
+    ms_ex, ms_learned = float(np.median(t_ex)), float(np.median(t_l))
+    print("exhaustive %.4f ms, learned %.4f ms (%s)" % (ms_ex, ms_learned, choice))
+    assert ms_learned <= 1.05 * ms_ex
