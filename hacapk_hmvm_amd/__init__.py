@@ -71,7 +71,7 @@ class hmvm_options(ctypes.Structure):
 
 class hmvm_tile(ctypes.Structure):
     _fields_ = [("nrows", ctypes.c_int), ("panel0", ctypes.c_int), ("npanel", ctypes.c_int), ("out_off", ctypes.c_int),
-                ("map_off", ctypes.c_int), ("reserved", ctypes.c_int * 3)]
+                ("map_off", ctypes.c_int), ("flags", ctypes.c_int), ("reserved", ctypes.c_int * 2)]
 
 
 class hmvm_panel(ctypes.Structure):
@@ -142,7 +142,8 @@ HMVM_STORAGE_F64, HMVM_STORAGE_F32, HMVM_STORAGE_BF16 = 0, 1, 2
 HMVM_A2_ROWMAJOR, HMVM_A2_TRANSPOSED = 0, 1
 STORAGE = {"f64": 0, "f32": 1, "bf16": 2}
 HACAPK_WITH_ROWMAT_T = 1
-HMVM_NFEAT = 17
+HMVM_NFEAT = 18
+HMVM_TILE_ROW_PER_LANE = 1
 
 
 HMVM_ERR_ARG, HMVM_ERR_MATRIX, HMVM_ERR_HIP, HMVM_ERR_NOMEM = -101, -102, -103, -104
@@ -211,6 +212,8 @@ def _load():
         "hmvm_plan_profile_tiles": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, c_double_p,
                                                    c_double_p, c_double_p]),
         "hmvm_plan_set_tile_order": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, c_int_p]),
+        "hmvm_plan_set_tile_flags": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, c_int_p]),
+        "hmvm_plan_get_tile_flags": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, c_int_p]),
         "hmvm_tile_occupancy": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, c_int_p]),
         "hmvm_schedule_export_batch": (ctypes.c_int, [P(P(matrix2)), ctypes.c_int, ctypes.c_int, P(hmvm_options),
                                                       P(hmvm_schedule)]),
@@ -669,7 +672,7 @@ def _struct_array(ptr, n, dtype):
 
 
 TILE_DT = np.dtype([("nrows", "<i4"), ("panel0", "<i4"), ("npanel", "<i4"), ("out_off", "<i4"), ("map_off", "<i4"),
-                    ("reserved", "<i4", (3,))])
+                    ("flags", "<i4"), ("reserved", "<i4", (2,))])
 PANEL_DT = np.dtype([("val_off", "<i8"), ("ld", "<i4"), ("ncols", "<i4"), ("vec_off", "<i4"), ("vec_sel", "<i4"),
                      ("factor", "<i4"), ("vec_chunks", "<i4")])
 REPACK_DT = np.dtype([("src_off", "<i8"), ("dst_off", "<i8"), ("rows", "<i4"), ("cols", "<i4"), ("src_rs", "<i4"),
@@ -817,6 +820,25 @@ class Plan:
         if rc:
             raise HMVMError("hmvm_plan_profile_tiles: %s" % lib.hmvm_hip_last_error_message().decode())
         return d[:n], s0[:n], lm.value
+
+    def set_tile_flags(self, which, flags):
+        """Per-tile kernel path of launch A (0) or B (1), in dispatch order:
+        HMVM_TILE_ROW_PER_LANE = lane-per-row path, 0 = column-split (default)."""
+        flags = np.ascontiguousarray(flags, dtype=np.int32)
+        n = self.info()["ntiles_a" if which == 0 else "ntiles_b"]
+        if flags.shape != (n,):
+            raise HMVMError("one flag per tile (%d)" % n)
+        rc = lib.hmvm_plan_set_tile_flags(self._p, which, flags.ctypes.data_as(c_int_p))
+        if rc:
+            raise HMVMError("hmvm_plan_set_tile_flags: %s" % lib.hmvm_hip_last_error_message().decode())
+
+    def tile_flags(self, which):
+        n = self.info()["ntiles_a" if which == 0 else "ntiles_b"]
+        f = np.zeros(max(n, 1), dtype=np.int32)
+        rc = lib.hmvm_plan_get_tile_flags(self._p, which, f.ctypes.data_as(c_int_p))
+        if rc:
+            raise HMVMError("hmvm_plan_get_tile_flags: %s" % lib.hmvm_hip_last_error_message().decode())
+        return f[:n]
 
     def set_tile_order(self, which, perm):
         perm = np.ascontiguousarray(perm, dtype=np.int32)

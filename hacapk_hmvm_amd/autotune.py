@@ -7,19 +7,25 @@ reads the CSV.  Here the schedule of a plan has these knobs:
   w1, w2          tile widths (target elements per stage-1 / dense+stage-2 tile)
   kernel_a/b      tile-kernel variant of launch A / launch B (0..7)
   bt_a/b          workgroup size of launch A / launch B (64, 128, 256)
+  path, per tile  column-split (lane groups take different columns) or
+                  lane-per-row (lanes own row groups, read whole columns),
+                  for every tile of at most 32 lanes per column
   order           dispatch order of the tiles of each launch
 
 Two ways to choose them:
 
 exhaustive(...)  builds a plan for every (w1, w2), times every (kernel, bt)
-                 for each launch on the GPU and keeps the fastest of each;
-                 this is the ground truth and the fallback.
+                 and both uniform paths (every eligible tile column-split, or
+                 every one lane-per-row) for each launch on the GPU and keeps
+                 the fastest of each; this is the ground truth and the
+                 fallback.  It cannot try per-tile mixes (2^tiles of them).
 learned(...)     builds the candidate schedules on the host only, predicts
                  every tile's duration with the MLP (HIP inference kernel),
                  replays the hardware's greedy dispatch of the tiles (longest
                  predicted first) onto the kernel's resident wavefront slots
                  and picks the candidate with the smallest predicted launch
-                 times.  Only the chosen plan is uploaded.
+                 times; then it gives every eligible tile the path predicted
+                 faster for it.  Only the chosen plans are uploaded.
 
 The MLP is trained on tile durations measured in situ (a clock-stamping copy
 of the tile kernel) on generated H-matrices, by HIP forward/backward/Adam
@@ -58,9 +64,11 @@ def _length(h):
 
 
 # ---- features from an exported schedule (host only) ------------------------------
-def schedule_features(sched, which, storage, kernel, block_threads, nrhs=1):
+def schedule_features(sched, which, storage, kernel, block_threads, nrhs=1, row_per_lane=None):
     """Tile features of launch A (0) or B (1) of an exported schedule, as
-    hmvm_plan_tile_features computes them for an uploaded plan."""
+    hmvm_plan_tile_features computes them for an uploaded plan.
+    row_per_lane: per-tile path override (bool array; None: the schedule's
+    tile flags)."""
     tiles = sched["tiles_a"] if which == 0 else sched["tiles_b"]
     pan = sched["panels"]
     V = 16 // EB[storage]
@@ -97,7 +105,25 @@ def schedule_features(sched, which, storage, kernel, block_threads, nrhs=1):
     slots = max(occupancy(storage, kernel, block_threads), 1)
     f[:, 15] = np.log2(min(n, slots))  # saturation of the launch
     f[:, 16] = np.log2(slots)
+    flag = (tiles["flags"] & 1) != 0 if row_per_lane is None else np.asarray(row_per_lane, dtype=bool)
+    f[:, 17] = ((lpc > 32) | flag).astype(np.float32)
     return f
+
+
+def eligible(X):
+    """Tiles with the choice of path (at most 32 lanes per column), from features."""
+    return X[:, 6] <= 5.0 + 1e-6
+
+
+def path_flags(model, X):
+    """Per-tile flags: HMVM_TILE_ROW_PER_LANE where the model predicts the
+    lane-per-row path faster than the column-split one."""
+    e = eligible(X)
+    X0, X1 = X.copy(), X.copy()
+    X0[:, 17] = np.where(e, 0.0, 1.0)
+    X1[:, 17] = 1.0
+    d0, d1 = model.tile_us(X0), model.tile_us(X1)
+    return (e & (d1 < d0)).astype(np.int32)
 
 
 # ---- training data -------------------------------------------------------------------
@@ -117,15 +143,21 @@ def collect(h, storage, variants=None, w_scales=((1.0, 1.0),), iters=2, max_tile
         for k, bt in variants:
             p.set_launch(k, bt)
             for which in (0, 1):
-                d, _, lms = p.profile_tiles(which, iters=iters)
-                if len(d) == 0:
-                    continue
-                f = p.tile_features(which)
-                recs.append({"storage": storage, "which": which, "kernel": k, "bt": bt, "w1": w1, "w2": w2,
-                             "sum_us": float(d.sum()), "launch_ms": lms, "ntiles": len(d), "X": f})
-                sel = rng.choice(len(d), min(len(d), max_tiles), replace=False)
-                Xs.append(f[sel])
-                ys.append(np.log2(np.maximum(d[sel], 1e-2)).astype(np.float32))
+                # both paths: the default, then every eligible tile lane-per-row
+                for rpl in (False, True):
+                    f = p.tile_features(which)
+                    if len(f) == 0 or (rpl and not eligible(f).any()):
+                        continue
+                    p.set_tile_flags(which, eligible(f).astype(np.int32) if rpl else np.zeros(len(f), np.int32))
+                    d, _, lms = p.profile_tiles(which, iters=iters)
+                    f = p.tile_features(which)
+                    recs.append({"storage": storage, "which": which, "kernel": k, "bt": bt, "w1": w1, "w2": w2,
+                                 "row_per_lane": rpl, "sum_us": float(d.sum()), "launch_ms": lms, "ntiles": len(d),
+                                 "X": f})
+                    sel = rng.choice(len(d), min(len(d), max_tiles), replace=False)
+                    Xs.append(f[sel])
+                    ys.append(np.log2(np.maximum(d[sel], 1e-2)).astype(np.float32))
+                p.set_tile_flags(which, np.zeros(len(f), np.int32))
         p.close()
     return np.concatenate(Xs), np.concatenate(ys), recs
 
@@ -226,10 +258,15 @@ def linear_baseline(X, y):
 def _apply_choice(p, choice):
     p.set_phase_launch(0, choice["kernel_a"], choice["bt_a"])
     p.set_phase_launch(1, choice["kernel_b"], choice["bt_b"])
+    for which, key in ((0, "rpl_a"), (1, "rpl_b")):
+        if key in choice:  # uniform path of the launch (exhaustive search)
+            f = p.tile_features(which)
+            p.set_tile_flags(which, (eligible(f) & bool(choice[key])).astype(np.int32))
 
 
-def exhaustive(h, storage, warmup=2, iters=8, w_choices=None, kernels=KERNELS, bts=BLOCK_THREADS):
-    """Times every candidate on the GPU; returns (best choice, all results, seconds spent)."""
+def exhaustive(h, storage, warmup=2, iters=8, w_choices=None, kernels=KERNELS, bts=BLOCK_THREADS, paths=True):
+    """Times every candidate on the GPU; returns (best choice, all results, seconds spent).
+    paths: also time every launch with all eligible tiles lane-per-row."""
     t0 = time.time()
     w_choices = w_choices or list(itertools.product(W1_CHOICES, W2_CHOICES))
     results = []
@@ -238,16 +275,22 @@ def exhaustive(h, storage, warmup=2, iters=8, w_choices=None, kernels=KERNELS, b
         w1, w2 = widths(storage, a, b, _length(h))
         p = Plan(h, storage=storage, w1=w1, w2=w2)
         per = {}
-        for k in kernels:
-            for bt in bts:
-                p.set_launch(k, bt)
-                t = p.time(warmup, iters)
-                per[(k, bt)] = t
-                results.append({"w1": w1, "w2": w2, "kernel": k, "bt": bt, "ms_a": t["ms_a"], "ms_b": t["ms_b"],
-                                "ms_other": t["ms_tsum"] + t["ms_ysum"], "ms_stream": t["ms_stream"]})
+        elig = [eligible(p.tile_features(which)) for which in (0, 1)]
+        for rpl in ((False, True) if paths else (False,)):
+            for which in (0, 1):
+                p.set_tile_flags(which, (elig[which] & rpl).astype(np.int32))
+            for k in kernels:
+                for bt in bts:
+                    p.set_launch(k, bt)
+                    t = p.time(warmup, iters)
+                    per[(k, bt, rpl)] = t
+                    results.append({"w1": w1, "w2": w2, "kernel": k, "bt": bt, "row_per_lane": rpl,
+                                    "ms_a": t["ms_a"], "ms_b": t["ms_b"], "ms_other": t["ms_tsum"] + t["ms_ysum"],
+                                    "ms_stream": t["ms_stream"]})
         ka = min(per, key=lambda v: per[v]["ms_a"])
         kb = min(per, key=lambda v: per[v]["ms_b"])
-        choice = {"w1": w1, "w2": w2, "kernel_a": ka[0], "bt_a": ka[1], "kernel_b": kb[0], "bt_b": kb[1]}
+        choice = {"w1": w1, "w2": w2, "kernel_a": ka[0], "bt_a": ka[1], "kernel_b": kb[0], "bt_b": kb[1],
+                  "rpl_a": ka[2], "rpl_b": kb[2]}
         _apply_choice(p, choice)
         choice["ms"] = p.time(warmup, iters)["ms_stream"]
         if best is None or choice["ms"] < best["ms"]:
@@ -285,34 +328,41 @@ def learned(h, storage, model, w_choices=None, kernels=KERNELS, bts=BLOCK_THREAD
                       "bt_b": pred[1][2], "pred_ms": pred[0][0] + pred[1][0]})
     cands.sort(key=lambda c: c["pred_ms"])
 
-    def build(c, ordered):
+    def build(c, ordered, paths):
         p = Plan(h, storage=storage) if c is None else Plan(h, storage=storage, w1=c["w1"], w2=c["w2"])
         if c is not None:
             _apply_choice(p, c)
-            if ordered:
-                for which in (0, 1):
+            for which in (0, 1):
+                if paths:  # each eligible tile on the path predicted faster for it
+                    X = p.tile_features(which)
+                    if len(X):
+                        p.set_tile_flags(which, path_flags(model, X))
+                if ordered:
                     d = model.tile_us(p.tile_features(which))
                     if len(d):
                         p.set_tile_order(which, np.argsort(-d, kind="stable"))
         return p
 
     if verify <= 0:
-        best = dict(cands[0], reordered=bool(reorder))
-        return best, build(cands[0], reorder), time.time() - t0
-    trials = [(None, False)]
+        best = dict(cands[0], reordered=bool(reorder), tile_paths=True)
+        return best, build(cands[0], reorder, True), time.time() - t0
+    trials = [(None, False, False)]
     for c in cands[:verify]:
-        trials += [(c, False)] + ([(c, True)] if reorder else [])
+        trials += [(c, False, False)] + ([(c, True, False), (c, True, True)] if reorder else [(c, False, True)])
     best, best_ms = None, None
-    for c, ordered in trials:
-        p = build(c, ordered)
+    for c, ordered, paths in trials:
+        p = build(c, ordered, paths)
         ms = p.time(2, 5)["ms_stream"]
         p.close()
         if best_ms is None or ms < best_ms:
-            best, best_ms = (c, ordered), ms
-    c, ordered = best
+            best, best_ms = (c, ordered, paths), ms
+    c, ordered, paths = best
     choice = {"defaults": True} if c is None else dict(c)
-    choice.update({"reordered": ordered, "timed_trials": len(trials)})
-    return choice, build(c, ordered), time.time() - t0
+    choice.update({"reordered": ordered, "tile_paths": paths, "timed_trials": len(trials)})
+    p = build(c, ordered, paths)
+    if paths:
+        choice["row_per_lane_tiles"] = [int(p.tile_flags(w).sum()) for w in (0, 1)]
+    return choice, p, time.time() - t0
 
 
 def default_plan(h, storage):

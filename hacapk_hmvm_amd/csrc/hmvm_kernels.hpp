@@ -523,9 +523,9 @@ __device__ __forceinline__ void run_tile(const TileDesc &td, const PanelDesc *__
   constexpr int V = Wide<S>::V;
   const int shift = (int)(panels[td.panel0].val_off % V);
   const int lpc = (td.nrows + shift + V - 1) / V;  // lanes per column
-  if (lpc <= 32) {
+  if (lpc <= 32 && !(td.flags & kTileRowPerLane)) {
     grouped_tile<S, kLoads, kNT, kScaled, kCoh>(td, shift, lpc, panels, vals, x, t, tp, scales, o, lane, lds);
-  } else if (lpc <= 64) {
+  } else if (lpc <= 64) {  // (and flagged tiles of at most 32 lanes per column)
     wide_tile<S, 1, kLoads, kPipe, kNT, kScaled, kCoh, kBV>(td, shift, lpc, panels, vals, x, t, tp, scales, o, lane);
   } else if (lpc <= 128) {
     wide_tile<S, 2, kLoads, kPipe, kNT, kScaled, kCoh, kBV>(td, shift, lpc, panels, vals, x, t, tp, scales, o, lane);

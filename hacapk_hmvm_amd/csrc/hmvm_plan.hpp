@@ -79,8 +79,17 @@ struct TileDesc {         // 32 bytes (one scalar load)
   int32_t npanel;
   int32_t out_off;        // first element of the tile's output slot (map_off < 0)
   int32_t map_off = -1;   // >= 0: row r of the tile goes to t[tmap[map_off + r]]
-  int32_t reserved[3] = {0, 0, 0};
+  int32_t flags = 0;      // kTileRowPerLane: lane-per-row path even when lanes per column <= 32
+  int32_t reserved[2] = {0, 0};
 };
+
+// Per-tile kernel path.  A tile needing at most 32 lanes per column runs the
+// column-split path by default (the wave's lane groups take different
+// columns, an LDS reduction adds them); with this flag it runs the
+// lane-per-row path instead (each lane owns row groups and reads whole
+// columns; lanes past the tile's rows idle).  Tiles of more than 32 lanes
+// per column always run the lane-per-row path.
+constexpr int32_t kTileRowPerLane = 1;
 
 // One source->device copy: element (r, c) of a rows x cols factor.
 struct RepackDesc {

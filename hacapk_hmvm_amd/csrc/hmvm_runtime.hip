@@ -436,6 +436,7 @@ PlanOptions plan_options(const hmvm_options *o, int64_t len) {
 }
 
 static_assert(sizeof(hmvm_tile) == sizeof(TileDesc), "tile record layout");
+static_assert(HMVM_TILE_ROW_PER_LANE == kTileRowPerLane, "tile flag values");
 static_assert(sizeof(hmvm_panel) == sizeof(PanelDesc), "panel record layout");
 static_assert(sizeof(hmvm_repack) == sizeof(RepackDesc), "repack record layout");
 
@@ -2066,7 +2067,31 @@ int hmvm_plan_tile_features(const hmvm_plan *plan, int which, int nrhs, float *f
     f[14] = std::log2((float)std::max(nrhs, 1));
     f[15] = f_sat;
     f[16] = f_slots;
+    f[17] = (lpc > 32 || (td.flags & kTileRowPerLane)) ? 1.0f : 0.0f;
   }
+  return HMVM_OK;
+}
+
+int hmvm_plan_set_tile_flags(hmvm_plan *plan, int which, const int *flags) {
+  if (!plan || (which != 0 && which != 1) || !flags) return set_error(HMVM_ERR_ARG, "bad argument");
+  Plan &P = plan->plan;
+  std::vector<TileDesc> &tl = which == 0 ? P.tiles_a : P.tiles_b;
+  std::vector<TileDesc> nt = tl;
+  for (size_t i = 0; i < nt.size(); i++) {
+    if (flags[i] & ~HMVM_TILE_ROW_PER_LANE) return set_error(HMVM_ERR_ARG, "unknown tile flag %d", flags[i]);
+    nt[i].flags = flags[i];
+  }
+  TileDesc *d = which == 0 ? plan->d_tiles_a : plan->d_tiles_b;
+  HIP_TRY(hipStreamSynchronize(plan->stream));
+  HIP_TRY(hipMemcpy(d, nt.data(), sizeof(TileDesc) * nt.size(), hipMemcpyHostToDevice));
+  tl.swap(nt);
+  return HMVM_OK;
+}
+
+int hmvm_plan_get_tile_flags(const hmvm_plan *plan, int which, int *flags) {
+  if (!plan || (which != 0 && which != 1) || !flags) return set_error(HMVM_ERR_ARG, "bad argument");
+  const std::vector<TileDesc> &tl = which == 0 ? plan->plan.tiles_a : plan->plan.tiles_b;
+  for (size_t i = 0; i < tl.size(); i++) flags[i] = tl[i].flags;
   return HMVM_OK;
 }
 

@@ -277,6 +277,8 @@ int hmvm_plan_time_graph(hmvm_plan *plan, int count, int warmup, int iters, hmvm
  *   number of wavefronts of the launch's variant resident at once on the
  *   device (hmvm_tile_occupancy): how saturated the launch is, bounded so
  *   that no launch size lies outside what a model has seen
+ *   [17] 1 when the tile runs the lane-per-row path (more than 32 lanes per
+ *   column, or HMVM_TILE_ROW_PER_LANE set), 0 for the column-split path
  * hmvm_plan_profile_tiles runs the launch with a clock-stamping copy of the
  * tile kernel (same results) and returns each tile's mean duration and start
  * offset in microseconds, plus the mean launch time.
@@ -284,8 +286,19 @@ int hmvm_plan_time_graph(hmvm_plan *plan, int count, int warmup, int iters, hmvm
  * are unchanged: tiles write disjoint slots).  Modes 1 and 2 need launch A as
  * "stage-1 tiles, then dense": selecting them regroups launch A so, keeping
  * each part's order. */
-enum { HMVM_NFEAT = 17 };
+enum { HMVM_NFEAT = 18 };
 int hmvm_plan_tile_features(const hmvm_plan *plan, int which, int nrhs, float *feat);
+/* Per-tile kernel path of launch A (which = 0) or B (1), in the launch's
+ * current dispatch order: flags[i] = HMVM_TILE_ROW_PER_LANE makes tile i run
+ * the lane-per-row path (each lane owns row groups and reads whole columns)
+ * instead of the column-split path (lane groups take different columns, an
+ * LDS reduction adds them); 0 restores the default.  Only tiles of at most
+ * 32 lanes per column have the choice; the flag is ignored on the others.
+ * Results differ from the default only in the summation order inside a
+ * tile.  hmvm_plan_get_tile_flags reads them back. */
+enum { HMVM_TILE_ROW_PER_LANE = 1 };
+int hmvm_plan_set_tile_flags(hmvm_plan *plan, int which, const int *flags);
+int hmvm_plan_get_tile_flags(const hmvm_plan *plan, int which, int *flags);
 int hmvm_plan_profile_tiles(hmvm_plan *plan, int which, int iters, double *tile_us, double *start_us,
                             double *launch_ms);
 int hmvm_plan_set_tile_order(hmvm_plan *plan, int which, const int *perm);
@@ -365,7 +378,7 @@ int hmvm_bicgstab_static(hmvm_plan *plan, const double *d_b, double *d_u, int ma
  * sum_{c<nch} slots[vec_off + c*stride + j] (2, vec_chunks = nch<<16 | stride).  Repack record: values[dst_off + r +
  * c*dst_cs] = source[src_off + r*src_rs + c*src_cs].  Segment sums:
  * out[i] (+)= sum_{e in ptr[seg[i]]..ptr[seg[i]+1]} slots[list[e] + i]. */
-typedef struct hmvm_tile { int nrows, panel0, npanel, out_off, map_off, reserved[3]; } hmvm_tile;
+typedef struct hmvm_tile { int nrows, panel0, npanel, out_off, map_off, flags, reserved[2]; } hmvm_tile;
 typedef struct hmvm_panel {
   long long val_off;
   int ld, ncols, vec_off, vec_sel, factor, vec_chunks;

@@ -265,6 +265,43 @@ def test_persistent_mode_matches_launches(gpu, H, sphere4096, storage, t_mode):
     p.close()
 
 
+@pytest.mark.parametrize("storage,tol", [("f64", 1e-13), ("f32", 1e-6), ("bf16", 2e-2)])
+def test_per_tile_paths(gpu, H, O, sphere4096, storage, tol):
+    """Every tile of at most 32 lanes per column may run the lane-per-row path
+    instead of the column-split one, chosen per tile: any mix gives the
+    oracle's result to the storage's bar (the summation order inside a tile
+    differs), the same bits on every repeat, and the flags and the path
+    feature read back as set; tiles without the choice ignore the flag."""
+    h = sphere4096
+    b = np.random.default_rng(19).uniform(-1, 1, h.nd)
+    ref = O.seq_2(h, b)
+    p = H.Plan(h, storage=storage)
+    y0 = p.apply_host(b)
+    rng = np.random.default_rng(20)
+    for which in (0, 1):
+        f = p.tile_features(which)
+        elig = f[:, 6] <= 5.0
+        assert np.all(f[~elig, 17] == 1) and np.all(f[elig, 17] == 0)
+    for mix in ("all", "random"):
+        for which in (0, 1):
+            n = p.info()["ntiles_a" if which == 0 else "ntiles_b"]
+            fl = np.ones(n, np.int32) if mix == "all" else rng.integers(0, 2, n).astype(np.int32)
+            p.set_tile_flags(which, fl)
+            assert np.array_equal(p.tile_flags(which), fl)
+            f = p.tile_features(which)
+            elig = f[:, 6] <= 5.0
+            assert np.array_equal(f[:, 17] == 1, ~elig | (fl == 1))
+        y = p.apply_host(b)
+        assert rel(y, ref) <= tol
+        assert np.array_equal(p.apply_host(b), y)
+    for which in (0, 1):
+        p.set_tile_flags(which, np.zeros(p.info()["ntiles_a" if which == 0 else "ntiles_b"], np.int32))
+    assert np.array_equal(p.apply_host(b), y0)
+    with pytest.raises(H.HMVMError):
+        p.set_tile_flags(0, np.full(p.info()["ntiles_a"], 2, np.int32))
+    p.close()
+
+
 def test_persistent_mode_needs_fused_stage2(gpu, H, sphere4096):
     """Mode 1 has no launch boundary for a t sum: a plan whose stage-2 tiles
     read a materialised t refuses it (no silent fallback to the launches),
