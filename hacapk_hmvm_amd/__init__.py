@@ -69,15 +69,16 @@ class hmvm_options(ctypes.Structure):
                 ("max_rows_b", ctypes.c_int)]
 
 
-class hmvm_tile(ctypes.Structure):
-    _fields_ = [("nrows", ctypes.c_int), ("panel0", ctypes.c_int), ("npanel", ctypes.c_int), ("out_off", ctypes.c_int),
-                ("map_off", ctypes.c_int), ("flags", ctypes.c_int), ("reserved", ctypes.c_int * 2)]
-
-
 class hmvm_panel(ctypes.Structure):
     _fields_ = [("val_off", ctypes.c_longlong), ("ld", ctypes.c_int), ("ncols", ctypes.c_int),
                 ("vec_off", ctypes.c_int), ("vec_sel", ctypes.c_int), ("factor", ctypes.c_int),
                 ("vec_chunks", ctypes.c_int)]
+
+
+class hmvm_tile(ctypes.Structure):
+    _fields_ = [("nrows", ctypes.c_int), ("panel0", ctypes.c_int), ("npanel", ctypes.c_int), ("out_off", ctypes.c_int),
+                ("map_off", ctypes.c_int), ("flags", ctypes.c_int), ("ncols", ctypes.c_int), ("pad", ctypes.c_int),
+                ("first", hmvm_panel)]
 
 
 class hmvm_repack(ctypes.Structure):
@@ -671,10 +672,10 @@ def _struct_array(ptr, n, dtype):
     return raw.view(dtype).copy()
 
 
-TILE_DT = np.dtype([("nrows", "<i4"), ("panel0", "<i4"), ("npanel", "<i4"), ("out_off", "<i4"), ("map_off", "<i4"),
-                    ("flags", "<i4"), ("reserved", "<i4", (2,))])
 PANEL_DT = np.dtype([("val_off", "<i8"), ("ld", "<i4"), ("ncols", "<i4"), ("vec_off", "<i4"), ("vec_sel", "<i4"),
                      ("factor", "<i4"), ("vec_chunks", "<i4")])
+TILE_DT = np.dtype([("nrows", "<i4"), ("panel0", "<i4"), ("npanel", "<i4"), ("out_off", "<i4"), ("map_off", "<i4"),
+                    ("flags", "<i4"), ("ncols", "<i4"), ("pad", "<i4"), ("first", PANEL_DT)])
 REPACK_DT = np.dtype([("src_off", "<i8"), ("dst_off", "<i8"), ("rows", "<i4"), ("cols", "<i4"), ("src_rs", "<i4"),
                       ("src_cs", "<i4"), ("dst_cs", "<i4"), ("factor", "<i4")])
 

@@ -296,7 +296,7 @@ __device__ __forceinline__ void wide_tile(const TileDesc &td, int shift, int lpc
   for (int q = 0; q < R; ++q)
 #pragma unroll
     for (int v = 0; v < V; ++v) acc[q][v] = 0;
-  PanelDesc pd_next = panels[td.panel0];
+  PanelDesc pd_next = td.first;
   for (int p = 0; p < td.npanel; ++p) {
     const PanelDesc pd = pd_next;
     // the next panel's descriptor is requested before this panel streams
@@ -406,14 +406,16 @@ __device__ __forceinline__ void grouped_tile(const TileDesc &td, int shift, int 
   for (int v = 0; v < V; ++v) acc[v] = 0;
   // columns [0, nc) at base (column stride ld); entry(k) gives the scaled
   // vector entry of column k
-  auto load_batch = [&](T (&a)[kLoads], const S *__restrict__ base, int ld, int c0, int cn, int sb) {
+  T a[kLoads];  // the batch of matrix words in flight
+  auto load_batch = [&](const S *__restrict__ base, int ld, int c0, int cn, int sb) {
 #pragma unroll
     for (int u = 0; u < kLoads; ++u) {
       const int j = sb * step + u * g + jg;
       a[u] = live && j < cn ? load_wide<S, kNT>(base + (int64_t)(c0 + j) * ld) : T{};
     }
   };
-  auto run_columns = [&](const S *__restrict__ base, int ld, int nc, auto entry) {
+  // preloaded: the first batch of the first chunk is already in flight
+  auto run_columns = [&](const S *__restrict__ base, int ld, int nc, auto entry, bool preloaded) {
     for (int c0 = 0; c0 < nc; c0 += chunk) {
       const int cn = nc - c0 < chunk ? nc - c0 : chunk;
       wave_lds_sync();  // the previous chunk's LDS reads are done
@@ -427,15 +429,14 @@ __device__ __forceinline__ void grouped_tile(const TileDesc &td, int shift, int 
         const int k = q * 64 + lane;
         e[q] = q * 64 < cn && k < cn ? entry(c0 + k) : 0.0;
       }
-      T a[kLoads];
-      load_batch(a, base, ld, c0, cn, 0);
+      if (!(preloaded && c0 == 0)) load_batch(base, ld, c0, cn, 0);
 #pragma unroll
       for (int q = 0; q < kStage / 64; ++q)
         if (q * 64 < cn) lds[q * 64 + lane] = e[q];
       wave_lds_sync();
       const int nb = (cn + step - 1) / step;
       for (int sb = 0; sb < nb; ++sb) {
-        if (sb > 0) load_batch(a, base, ld, c0, cn, sb);
+        if (sb > 0) load_batch(base, ld, c0, cn, sb);
         A b[kLoads];
 #pragma unroll
         for (int u = 0; u < kLoads; ++u) {
@@ -455,7 +456,11 @@ __device__ __forceinline__ void grouped_tile(const TileDesc &td, int shift, int 
   if (np <= kPanTab) {
     int *ptab = (int *)(lds + (lds_wave_doubles<S>() - (kPanTab * 20) / 8));  // 4 ints per panel
     float *pscale = (float *)(ptab + 4 * kPanTab);
-    const PanelDesc pd0 = panels[td.panel0];
+    const PanelDesc &pd0 = td.first;
+    const S *__restrict__ base0 = vals + (pd0.val_off - shift) + (int64_t)V * c;
+    // the first matrix words are requested before the panel table is built:
+    // they need nothing but the tile descriptor
+    load_batch(base0, pd0.ld, 0, td.ncols < chunk ? td.ncols : chunk, 0);
     int ncl = 0;
     if (lane < np) {
       const PanelDesc pd = panels[td.panel0 + lane];
@@ -472,9 +477,9 @@ __device__ __forceinline__ void grouped_tile(const TileDesc &td, int shift, int 
       if (lane >= o) incl += v;
     }
     if (lane < np) ptab[4 * lane] = incl - ncl;
-    const int nc = __shfl(incl, 63, 64);
+    const int nc = td.ncols;
     wave_lds_sync();
-    run_columns(vals + (pd0.val_off - shift) + (int64_t)V * c, pd0.ld, nc, [&](int col) -> double {
+    run_columns(base0, pd0.ld, nc, [&](int col) -> double {
       int lo = 0, hi = np - 1;  // the last panel starting at or before col
       while (lo < hi) {
         const int mid = (lo + hi + 1) >> 1;
@@ -486,7 +491,7 @@ __device__ __forceinline__ void grouped_tile(const TileDesc &td, int shift, int 
       const double e = entry_of<kCoh>(ptab[4 * lo + 3], ptab[4 * lo + 1], ptab[4 * lo + 2], x, t, tp,
                                       col - ptab[4 * lo]);
       return kScaled ? e * (double)pscale[lo] : e;
-    });
+    }, true);
   } else {
     for (int p = 0; p < np; ++p) {
       const PanelDesc pd = panels[td.panel0 + p];
@@ -494,7 +499,7 @@ __device__ __forceinline__ void grouped_tile(const TileDesc &td, int shift, int 
       run_columns(vals + (pd.val_off - shift) + (int64_t)V * c, pd.ld, pd.ncols, [&](int col) -> double {
         const double e = entry_of<kCoh>(pd.vec_sel, pd.vec_off, pd.vec_chunks, x, t, tp, col);
         return kScaled ? e * sc : e;
-      });
+      }, false);
     }
   }
   // The g partial sums of every row meet in the wave's LDS region: lane
@@ -521,7 +526,7 @@ __device__ __forceinline__ void run_tile(const TileDesc &td, const PanelDesc *__
                                          const float *__restrict__ scales, const TileOut &o, int lane,
                                          double *lds) {
   constexpr int V = Wide<S>::V;
-  const int shift = (int)(panels[td.panel0].val_off % V);
+  const int shift = (int)(td.first.val_off % V);
   const int lpc = (td.nrows + shift + V - 1) / V;  // lanes per column
   if (lpc <= 32 && !(td.flags & kTileRowPerLane)) {
     grouped_tile<S, kLoads, kNT, kScaled, kCoh>(td, shift, lpc, panels, vals, x, t, tp, scales, o, lane, lds);
@@ -534,9 +539,13 @@ __device__ __forceinline__ void run_tile(const TileDesc &td, const PanelDesc *__
   }
 }
 
+// At least 4 waves per SIMD (at most 128 VGPRs): one register more drops the
+// streaming tile kernels to 3 waves per SIMD, measured 3% slower for BF16.
+#define HMVM_TILE_OCCUPANCY __attribute__((amdgpu_waves_per_eu(4)))
+
 // One wavefront per tile (see hmvm_plan.hpp for the tile model).
 template <typename S, bool kScaled, int kLoads, bool kPipe, bool kNT, bool kBV>
-__global__ __launch_bounds__(256) void tile_kernel(const TileDesc *__restrict__ tiles, int ntiles,
+__global__ __launch_bounds__(256) HMVM_TILE_OCCUPANCY void tile_kernel(const TileDesc *__restrict__ tiles, int ntiles,
                                                    const PanelDesc *__restrict__ panels,
                                                    const S *__restrict__ vals,
                                                    const double *__restrict__ x,
@@ -558,7 +567,7 @@ __global__ __launch_bounds__(256) void tile_kernel(const TileDesc *__restrict__ 
 // in-situ per-tile durations the cost model learns from.  Same results as
 // tile_kernel.
 template <typename S, bool kScaled, int kLoads, bool kPipe, bool kNT, bool kBV>
-__global__ __launch_bounds__(256) void tile_kernel_timed(const TileDesc *__restrict__ tiles, int ntiles,
+__global__ __launch_bounds__(256) HMVM_TILE_OCCUPANCY void tile_kernel_timed(const TileDesc *__restrict__ tiles, int ntiles,
                                                          const PanelDesc *__restrict__ panels,
                                                          const S *__restrict__ vals,
                                                          const double *__restrict__ x,
@@ -1011,7 +1020,7 @@ __global__ __launch_bounds__(256) void tile_multi_kernel(const TileDesc *__restr
       return;
     }
   }
-  const int shift = (int)(panels[td.panel0].val_off % V);
+  const int shift = (int)(td.first.val_off % V);
   const int lpc = (td.nrows + shift + V - 1) / V;  // lane groups (V rows each) of the tile
   for (int g0 = 0; g0 < lpc; g0 += 16 * CH) {      // pass: CH chunks of 16 lane groups
     Acc acc[CH][V][NB];

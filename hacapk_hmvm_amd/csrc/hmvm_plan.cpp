@@ -508,6 +508,15 @@ std::string build_plan(const LeafArrays &la, const PlanOptions &opt, Plan *plan)
     P.a_stage1.swap(s1);
   }
   P.tiles_b = t2;
+  // every tile carries its first panel and its column count
+  for (std::vector<TileDesc> *tl : {&P.tiles_a, &P.tiles_b})
+    for (TileDesc &t : *tl) {
+      t.first = P.panels[t.panel0];
+      int64_t nc = 0;
+      for (int q = t.panel0; q < t.panel0 + t.npanel; q++) nc += P.panels[q].ncols;
+      if (nc > INT32_MAX) return "tile columns exceed int32";
+      t.ncols = (int32_t)nc;
+    }
 
   // ---- y segment sum: strips of rows covered by the same set of slots ----------
   std::vector<int64_t> cuts;
@@ -573,6 +582,15 @@ std::string verify_plan(const Plan &P, const LeafArrays &la) {
         return "tile rows out of range";
       if (t.panel0 < 0 || t.npanel < 1 || (int64_t)t.panel0 + t.npanel > npan) return "tile panel range";
       if (t.out_off < 0) return "negative output slot";
+      {
+        const PanelDesc &f = t.first, &q = P.panels[t.panel0];
+        if (f.val_off != q.val_off || f.ld != q.ld || f.ncols != q.ncols || f.vec_off != q.vec_off ||
+            f.vec_sel != q.vec_sel || f.factor != q.factor || f.vec_chunks != q.vec_chunks)
+          return "tile's copy of its first panel differs";
+        int64_t nc = 0;
+        for (int p = t.panel0; p < t.panel0 + t.npanel; p++) nc += P.panels[p].ncols;
+        if (nc != t.ncols) return "tile column count";
+      }
       const int64_t shift0 = P.panels[t.panel0].val_off % P.lane_elems;
       for (int p = t.panel0; p < t.panel0 + t.npanel; p++) {
         const PanelDesc &d = P.panels[p];

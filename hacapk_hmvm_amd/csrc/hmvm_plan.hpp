@@ -73,15 +73,20 @@ struct PanelDesc {        // 32 bytes, read with scalar loads
   int32_t vec_chunks;     // kVecTP: (nch << 16) | stride
 };
 
-struct TileDesc {         // 32 bytes (one scalar load)
+struct TileDesc {         // 64 bytes (one scalar load)
   int32_t nrows;          // 1..Plan::max_rows
   int32_t panel0;
   int32_t npanel;
   int32_t out_off;        // first element of the tile's output slot (map_off < 0)
   int32_t map_off = -1;   // >= 0: row r of the tile goes to t[tmap[map_off + r]]
   int32_t flags = 0;      // kTileRowPerLane: lane-per-row path even when lanes per column <= 32
-  int32_t reserved[2] = {0, 0};
+  int32_t ncols = 0;      // columns of all the tile's panels together
+  int32_t pad = 0;
+  // a copy of panels[panel0]: the tile's first matrix loads need nothing but
+  // this descriptor
+  PanelDesc first{};
 };
+static_assert(sizeof(TileDesc) == 64, "tile descriptor layout");
 
 // Per-tile kernel path.  A tile needing at most 32 lanes per column runs the
 // column-split path by default (the wave's lane groups take different

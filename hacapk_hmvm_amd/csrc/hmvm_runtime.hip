@@ -395,8 +395,8 @@ struct LaunchDefaults {
 };
 const LaunchDefaults &launch_defaults(int storage) {
   static const LaunchDefaults d[3] = {
-      {32768, 16384, 6, 256, 6, 256, 256, 128},  // FP64
-      {32768, 16384, 6, 256, 6, 256, 256, 256},  // FP32
+      {32768, 32768, 6, 256, 6, 256, 256, 128},  // FP64
+      {32768, 24576, 6, 256, 6, 256, 256, 256},  // FP32
       {32768, 32768, 6, 256, 6, 256, 512, 256},  // BF16
   };
   return d[storage >= kF64 && storage <= kBF16 ? storage : 0];

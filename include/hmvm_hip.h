@@ -378,11 +378,15 @@ int hmvm_bicgstab_static(hmvm_plan *plan, const double *d_b, double *d_u, int ma
  * sum_{c<nch} slots[vec_off + c*stride + j] (2, vec_chunks = nch<<16 | stride).  Repack record: values[dst_off + r +
  * c*dst_cs] = source[src_off + r*src_rs + c*src_cs].  Segment sums:
  * out[i] (+)= sum_{e in ptr[seg[i]]..ptr[seg[i]+1]} slots[list[e] + i]. */
-typedef struct hmvm_tile { int nrows, panel0, npanel, out_off, map_off, flags, reserved[2]; } hmvm_tile;
 typedef struct hmvm_panel {
   long long val_off;
   int ld, ncols, vec_off, vec_sel, factor, vec_chunks;
 } hmvm_panel;
+/* first: a copy of panel panel0; ncols: the columns of all its panels */
+typedef struct hmvm_tile {
+  int nrows, panel0, npanel, out_off, map_off, flags, ncols, pad;
+  hmvm_panel first;
+} hmvm_tile;
 typedef struct hmvm_repack {
   long long src_off, dst_off;
   int rows, cols, src_rs, src_cs, dst_cs, factor;
