@@ -66,7 +66,7 @@ class hmvm_options(ctypes.Structure):
     _fields_ = [("storage", ctypes.c_int), ("w1", ctypes.c_int), ("w2", ctypes.c_int),
                 ("block_threads", ctypes.c_int), ("kernel", ctypes.c_int), ("order", ctypes.c_int),
                 ("mode", ctypes.c_int), ("t_mode", ctypes.c_int), ("max_rows", ctypes.c_int),
-                ("max_rows_b", ctypes.c_int)]
+                ("max_rows_b", ctypes.c_int), ("w3", ctypes.c_int)]
 
 
 class hmvm_panel(ctypes.Structure):
@@ -632,12 +632,13 @@ def bandwidth_probe(nbytes=2 << 30, iters=10):
 
 
 def options(storage="f64", w1=0, w2=0, block_threads=0, kernel=None, order=None, mode=0, t_mode=None, max_rows=0,
-            max_rows_b=0):
+            max_rows_b=0, w3=0):
     """hmvm_options; None / 0 keep the library defaults."""
     o = hmvm_options()
     lib.hmvm_options_default(ctypes.byref(o))
     o.max_rows = max_rows
     o.max_rows_b = max_rows_b
+    o.w3 = w3
     if kernel is not None:
         o.kernel = kernel
     if order is not None:

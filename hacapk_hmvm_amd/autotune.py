@@ -349,16 +349,19 @@ def learned(h, storage, model, w_choices=None, kernels=KERNELS, bts=BLOCK_THREAD
     trials = [(None, False, False)]
     for c in cands[:verify]:
         trials += [(c, False, False)] + ([(c, True, False), (c, True, True)] if reorder else [(c, False, True)])
-    best, best_ms = None, None
+    best, best_ms, trial_ms = None, None, []
     for c, ordered, paths in trials:
         p = build(c, ordered, paths)
-        ms = p.time(2, 5)["ms_stream"]
+        ms = p.time(5, 20)["ms_stream"]  # a fresh plan's first applies run slow: warm it up
         p.close()
+        trial_ms.append(round(ms, 5))
         if best_ms is None or ms < best_ms:
             best, best_ms = (c, ordered, paths), ms
     c, ordered, paths = best
     choice = {"defaults": True} if c is None else dict(c)
-    choice.update({"reordered": ordered, "tile_paths": paths, "timed_trials": len(trials)})
+    choice.update({"reordered": ordered, "tile_paths": paths, "timed_trials": len(trials), "trial_ms": trial_ms,
+                   "candidates": [(cd["w1"], cd["w2"], cd["kernel_a"], cd["bt_a"], cd["kernel_b"], cd["bt_b"],
+                                   round(cd["pred_ms"], 5)) for cd in cands[:verify]]})
     p = build(c, ordered, paths)
     if paths:
         choice["row_per_lane_tiles"] = [int(p.tile_flags(w).sum()) for w in (0, 1)]

@@ -36,6 +36,16 @@ struct AccOf { using T = float; };
 template <>
 struct AccOf<double> { using T = double; };
 
+// Element type of the slot buffers (per-tile sums): the tile's accumulation
+// type.  A lane-per-row tile's FP32 sum is stored exactly; a column-split
+// tile adds its groups' FP32 partials in FP64 and stores that sum rounded
+// once to FP32.  The segment sums that add slots into t and y accumulate in
+// FP64.  FP32 and BF16 storage thus move half the slot bytes of FP64.
+template <typename S>
+struct SlotOf { using T = float; };
+template <>
+struct SlotOf<double> { using T = double; };
+
 // BF16 values are kept as raw uint16_t bit patterns.
 __device__ __forceinline__ double load_val(const double *p) { return *p; }
 __device__ __forceinline__ float load_val(const float *p) { return *p; }
@@ -79,9 +89,9 @@ constexpr int kNumKernelVariants = 8;
 // Vector entry j of a panel.  kCoherent: the stage-1 partial slots were
 // written earlier in the same launch (persistent mode), so they are read with
 // device-coherent vector loads instead of possibly-scalar cached ones.
-template <bool kCoherent = false>
+template <bool kCoherent = false, typename P>
 __device__ __forceinline__ double vec_entry(const PanelDesc &pd, const double *__restrict__ x, const double *t,
-                                            const double *tp, int j) {
+                                            const P *tp, int j) {
   if (pd.vec_sel == kVecX) return x[pd.vec_off + j];
   if (pd.vec_sel == kVecT) {
     if constexpr (kCoherent)  // t was written earlier in the same launch
@@ -92,11 +102,11 @@ __device__ __forceinline__ double vec_entry(const PanelDesc &pd, const double *_
   const int nch = pd.vec_chunks >> 16, stride = pd.vec_chunks & 0xFFFF;
   double s = 0;
   for (int c = 0; c < nch; ++c) {
-    const double *q = tp + pd.vec_off + c * stride + j;
+    const P *q = tp + pd.vec_off + c * stride + j;
     if constexpr (kCoherent)
-      s += __hip_atomic_load(q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      s += (double)__hip_atomic_load(q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     else
-      s += *q;
+      s += (double)*q;
   }
   return s;
 }
@@ -116,17 +126,18 @@ __device__ __forceinline__ float panel_scale(const PanelDesc &pd, const float *_
 // both factor scales of that entry's leaf applied under BF16 storage
 // (tfac[e] = the leaf's a1 factor id, its a2 factor is the next id).
 struct TileOut {
-  double *out;
+  void *out;  // slots: SlotOf<S>::T of the launch's storage S
   const int32_t *tmap;
   double *tout;
   const int32_t *tfac;
 };
 
-template <bool kScaled>
+template <typename S, bool kScaled>
 __device__ __forceinline__ void store_row(const TileDesc &td, int row, double v, const TileOut &o,
                                           const float *__restrict__ scales) {
+  using Slot = typename SlotOf<S>::T;
   if (td.map_off < 0) {
-    o.out[td.out_off + row] = v;
+    static_cast<Slot *>(o.out)[td.out_off + row] = (Slot)v;
     return;
   }
   const int e = o.tmap[td.map_off + row];
@@ -180,9 +191,9 @@ __device__ __forceinline__ typename Wide<S>::T load_wide(const S *p) {
 // entry of a column is wave-uniform.
 // Lane-held vector entries: lane l holds entry jb + l of the panel (0 past
 // its end); bcast() hands entry jb + src to the whole wave.
-template <bool kCoh>
+template <bool kCoh, typename P>
 __device__ __forceinline__ double lane_entry(const PanelDesc &pd, const double *__restrict__ x,
-                                             const double *__restrict__ t, const double *tp, int j) {
+                                             const double *__restrict__ t, const P *tp, int j) {
   return j < pd.ncols ? vec_entry<kCoh>(pd, x, t, tp, j) : 0.0;
 }
 __device__ __forceinline__ double bcast(double v, int src) {
@@ -195,7 +206,8 @@ __device__ __forceinline__ double bcast(double v, int src) {
 template <typename S, int R, int kLoads, bool kPipe, bool kNT, bool kCoh, bool kBV>
 __device__ __forceinline__ void wide_panel(const S *__restrict__ vals, const PanelDesc &pd, int lane, int shift,
                                            int lpc, const double *__restrict__ x, const double *__restrict__ t,
-                                           const double *tp, typename AccOf<S>::T (&accp)[R][Wide<S>::V]) {
+                                           const typename SlotOf<S>::T *tp,
+                                           typename AccOf<S>::T (&accp)[R][Wide<S>::V]) {
   using A = typename AccOf<S>::T;
   using W = Wide<S>;
   constexpr int V = W::V;
@@ -287,8 +299,8 @@ __device__ __forceinline__ void wide_panel(const S *__restrict__ vals, const Pan
 template <typename S, int R, int kLoads, bool kPipe, bool kNT, bool kScaled, bool kCoh, bool kBV>
 __device__ __forceinline__ void wide_tile(const TileDesc &td, int shift, int lpc, const PanelDesc *__restrict__ panels,
                                           const S *__restrict__ vals, const double *__restrict__ x, const double *t,
-                                          const double *tp, const float *__restrict__ scales, const TileOut &o,
-                                          int lane) {
+                                          const typename SlotOf<S>::T *tp, const float *__restrict__ scales,
+                                          const TileOut &o, int lane) {
   using A = typename AccOf<S>::T;
   constexpr int V = Wide<S>::V;
   A acc[R][V];
@@ -318,7 +330,7 @@ __device__ __forceinline__ void wide_tile(const TileDesc &td, int shift, int lpc
 #pragma unroll
     for (int v = 0; v < V; ++v) {
       const int row = V * (lane + 64 * q) + v - shift;
-      if (row >= 0 && row < td.nrows) store_row<kScaled>(td, row, (double)acc[q][v], o, scales);
+      if (row >= 0 && row < td.nrows) store_row<S, kScaled>(td, row, (double)acc[q][v], o, scales);
     }
 }
 
@@ -344,9 +356,9 @@ __host__ __device__ constexpr int lds_wave_doubles() {
 }
 
 // Vector entry j of a panel given its fields (vec_entry's rule).
-template <bool kCoh>
+template <bool kCoh, typename P>
 __device__ __forceinline__ double entry_of(int sel, int off, int chunks, const double *__restrict__ x,
-                                           const double *t, const double *tp, int j) {
+                                           const double *t, const P *tp, int j) {
   if (sel == kVecX) return x[off + j];
   if (sel == kVecT) {
     if constexpr (kCoh)  // t was written earlier in the same launch
@@ -357,11 +369,11 @@ __device__ __forceinline__ double entry_of(int sel, int off, int chunks, const d
   const int nch = chunks >> 16, stride = chunks & 0xFFFF;
   double s = 0;
   for (int c = 0; c < nch; ++c) {
-    const double *q = tp + off + c * stride + j;
+    const P *q = tp + off + c * stride + j;
     if constexpr (kCoh)
-      s += __hip_atomic_load(q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      s += (double)__hip_atomic_load(q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     else
-      s += *q;
+      s += (double)*q;
   }
   return s;
 }
@@ -383,9 +395,9 @@ __device__ __forceinline__ double entry_of(int sel, int off, int chunks, const d
 template <typename S, int kLoads, bool kNT, bool kScaled, bool kCoh>
 __device__ __forceinline__ void grouped_tile(const TileDesc &td, int shift, int lpc,
                                              const PanelDesc *__restrict__ panels, const S *__restrict__ vals,
-                                             const double *__restrict__ x, const double *t, const double *tp,
-                                             const float *__restrict__ scales, const TileOut &o, int lane,
-                                             double *lds) {
+                                             const double *__restrict__ x, const double *t,
+                                             const typename SlotOf<S>::T *tp, const float *__restrict__ scales,
+                                             const TileOut &o, int lane, double *lds) {
   using A = typename AccOf<S>::T;
   using W = Wide<S>;
   using T = typename W::T;
@@ -515,14 +527,14 @@ __device__ __forceinline__ void grouped_tile(const TileDesc &td, int shift, int 
     double s = 0;
     for (int q = 0; q < g; ++q) s += r[q * lpc * V + ro];
     const int row = ro - shift;
-    if (row >= 0 && row < td.nrows) store_row<kScaled>(td, row, s, o, scales);
+    if (row >= 0 && row < td.nrows) store_row<S, kScaled>(td, row, s, o, scales);
   }
 }
 
 template <typename S, bool kScaled, int kLoads, bool kPipe, bool kNT, bool kCoh, bool kBV>
 __device__ __forceinline__ void run_tile(const TileDesc &td, const PanelDesc *__restrict__ panels,
                                          const S *__restrict__ vals, const double *__restrict__ x,
-                                         const double *t, const double *tp,
+                                         const double *t, const typename SlotOf<S>::T *tp,
                                          const float *__restrict__ scales, const TileOut &o, int lane,
                                          double *lds) {
   constexpr int V = Wide<S>::V;
@@ -550,7 +562,7 @@ __global__ __launch_bounds__(256) HMVM_TILE_OCCUPANCY void tile_kernel(const Til
                                                    const S *__restrict__ vals,
                                                    const double *__restrict__ x,
                                                    const double *__restrict__ t,
-                                                   const double *__restrict__ tp,
+                                                   const typename SlotOf<S>::T *__restrict__ tp,
                                                    const float *__restrict__ scales, TileOut o) {
   const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
   const int lane = (int)(threadIdx.x & 63);
@@ -572,7 +584,7 @@ __global__ __launch_bounds__(256) HMVM_TILE_OCCUPANCY void tile_kernel_timed(con
                                                          const S *__restrict__ vals,
                                                          const double *__restrict__ x,
                                                          const double *__restrict__ t,
-                                                         const double *__restrict__ tp,
+                                                         const typename SlotOf<S>::T *__restrict__ tp,
                                                          const float *__restrict__ scales, TileOut o,
                                                          long long *__restrict__ stamps) {
   const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
@@ -643,7 +655,7 @@ template <typename S, bool kScaled, int kLoads, bool kPipe, bool kNT, bool kBV>
 __global__ __launch_bounds__(256) void persistent_kernel(
     const TileDesc *__restrict__ tiles_a, int n1, int na, const TileDesc *__restrict__ tiles_b, int nt,
     int nitems, int nwaves, const PanelDesc *__restrict__ panels, const S *__restrict__ vals,
-    const double *__restrict__ x, double *t, double *slots, const float *__restrict__ scales,
+    const double *__restrict__ x, double *t, typename SlotOf<S>::T *slots, const float *__restrict__ scales,
     const int32_t *__restrict__ tmap, const int32_t *__restrict__ tfac,
     const int32_t *__restrict__ yseg, const int32_t *__restrict__ yptr, const int64_t *__restrict__ ylist,
     int nd, double *__restrict__ y, int accumulate, PersistCounters *ctr) {
@@ -678,7 +690,7 @@ __global__ __launch_bounds__(256) void persistent_kernel(
         const int sg = yseg[i];
         const int e1 = yptr[sg + 1];
         double a = 0;
-        for (int e = yptr[sg]; e < e1; ++e) a += __hip_atomic_load(slots + ylist[e] + i, __ATOMIC_RELAXED,
+        for (int e = yptr[sg]; e < e1; ++e) a += (double)__hip_atomic_load(slots + ylist[e] + i, __ATOMIC_RELAXED,
                                                                    __HIP_MEMORY_SCOPE_AGENT);
         y[i] = accumulate ? y[i] + a : a;
       }
@@ -765,11 +777,12 @@ __device__ __forceinline__ short bf16_bits(float f) {
 
 // Row `row`, right-hand side `col` of a multi-RHS tile: store_row with rows
 // of nrhs entries (slots and t in the row-major multi-RHS layout).
-template <bool kScaled>
+template <typename S, bool kScaled>
 __device__ __forceinline__ void store_row_multi(const TileDesc &td, int row, int col, int nrhs, double v,
                                                 const TileOut &o, const float *__restrict__ scales) {
+  using Slot = typename SlotOf<S>::T;
   if (td.map_off < 0) {
-    o.out[(int64_t)(td.out_off + row) * nrhs + col] = v;
+    static_cast<Slot *>(o.out)[(int64_t)(td.out_off + row) * nrhs + col] = (Slot)v;
     return;
   }
   const int e = o.tmap[td.map_off + row];
@@ -784,15 +797,15 @@ __device__ __forceinline__ void store_row_multi(const TileDesc &td, int row, int
 // MFMA (a branch per load makes the compiler wait for each one).  X and T
 // panels read `src` (the caller's choice of X or T); kTP panels sum stage-1
 // partial slots.
-template <bool kTP>
+template <bool kTP, typename P>
 __device__ __forceinline__ double vec_entry_multi(const PanelDesc &pd, const double *__restrict__ src,
-                                                  const double *__restrict__ TP, int col, int rhs, int nrhs) {
+                                                  const P *__restrict__ TP, int col, int rhs, int nrhs) {
   const int c = col < pd.ncols ? col : pd.ncols - 1;
   double v;
   if constexpr (kTP) {
     const int nch = pd.vec_chunks >> 16, stride = pd.vec_chunks & 0xFFFF;
     v = 0;
-    for (int q = 0; q < nch; ++q) v += TP[(int64_t)(pd.vec_off + q * stride + c) * nrhs + rhs];
+    for (int q = 0; q < nch; ++q) v += (double)TP[(int64_t)(pd.vec_off + q * stride + c) * nrhs + rhs];
   } else {
     v = src[(int64_t)(pd.vec_off + c) * nrhs + rhs];
   }
@@ -803,7 +816,8 @@ __device__ __forceinline__ double vec_entry_multi(const PanelDesc &pd, const dou
 // and FP32 one entry (column k0 + l/16), BF16 four (columns k0 + 4(l/16) + q).
 template <typename S, bool kTP>
 __device__ __forceinline__ typename Mfma<S>::Op b_operand(const PanelDesc &pd, const double *__restrict__ src,
-                                                          const double *__restrict__ TP, int rhs, int nrhs, int k0,
+                                                          const typename SlotOf<S>::T *__restrict__ TP, int rhs,
+                                                          int nrhs, int k0,
                                                           int lane, float scale) {
   if constexpr (sizeof(S) == 2) {
     double v[4];
@@ -878,8 +892,8 @@ struct MultiShape {
 // all its loads, then its MFMAs.
 template <typename S, int NB, bool kTP, int CH>
 __device__ __forceinline__ void multi_panel(const PanelDesc &pd, const S *const *base, int nl,
-                                            const double *__restrict__ src, const double *__restrict__ TP,
-                                            float scale, int lane,
+                                            const double *__restrict__ src,
+                                            const typename SlotOf<S>::T *__restrict__ TP, float scale, int lane,
                                             typename Mfma<S>::Acc (&acc)[CH][Wide<S>::V][NB]) {
   using M = Mfma<S>;
   constexpr int V = Wide<S>::V, K = M::K, KB = MultiShape<S, NB>::KB, nrhs = 16 * NB;
@@ -924,8 +938,9 @@ __device__ __forceinline__ S a_narrow(const S *__restrict__ vals, const PanelDes
 
 template <typename S, int NB, int NL, bool kTP, int CH>
 __device__ __forceinline__ void multi_panel_narrow(const PanelDesc &pd, const S *__restrict__ vals, int nrows,
-                                                   const double *__restrict__ src, const double *__restrict__ TP,
-                                                   float scale, int lane, typename Mfma<S>::Acc (&acc)[CH][NB]) {
+                                                   const double *__restrict__ src,
+                                                   const typename SlotOf<S>::T *__restrict__ TP, float scale, int lane,
+                                                   typename Mfma<S>::Acc (&acc)[CH][NB]) {
   using M = Mfma<S>;
   constexpr int K = M::K, KB = MultiShape<S, NB>::KB, nrhs = 16 * NB;
   for (int k0 = 0; k0 < pd.ncols; k0 += KB * K) {
@@ -956,7 +971,8 @@ __device__ __forceinline__ void multi_panel_narrow(const PanelDesc &pd, const S 
 template <typename S, bool kScaled, int NB, bool kHasTP>
 __device__ __forceinline__ void multi_tile_narrow(const TileDesc &td, const PanelDesc *__restrict__ panels,
                                                   const S *__restrict__ vals, const double *__restrict__ X,
-                                                  const double *__restrict__ T, const double *__restrict__ TP,
+                                                  const double *__restrict__ T,
+                                                  const typename SlotOf<S>::T *__restrict__ TP,
                                                   const float *__restrict__ scales, const TileOut &o,
                                                   int lane) {
   static_assert(sizeof(S) == 4, "the narrow path is FP32 only");
@@ -991,7 +1007,8 @@ __device__ __forceinline__ void multi_tile_narrow(const TileDesc &td, const Pane
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
           const int row = 16 * ch + M::row(lane, v);
-          if (row < td.nrows) store_row_multi<kScaled>(td, row, 16 * cb + (lane & 15), nrhs, (double)acc[ch][cb][v], o, scales);
+          if (row < td.nrows)
+            store_row_multi<S, kScaled>(td, row, 16 * cb + (lane & 15), nrhs, (double)acc[ch][cb][v], o, scales);
         }
 }
 
@@ -1002,7 +1019,7 @@ __global__ __launch_bounds__(256) void tile_multi_kernel(const TileDesc *__restr
                                                          const PanelDesc *__restrict__ panels,
                                                          const S *__restrict__ vals, const double *__restrict__ X,
                                                          const double *__restrict__ T,
-                                                         const double *__restrict__ TP,
+                                                         const typename SlotOf<S>::T *__restrict__ TP,
                                                          const float *__restrict__ scales, TileOut o) {
   using M = Mfma<S>;
   using Acc = typename M::Acc;
@@ -1060,7 +1077,7 @@ __global__ __launch_bounds__(256) void tile_multi_kernel(const TileDesc *__restr
             // result row i of MFMA m: row V*i+m of the chunk, minus the shift
             const int row = V * (16 * ch + g0 + M::row(lane, v)) + m - shift;
             if (row >= 0 && row < td.nrows)
-              store_row_multi<kScaled>(td, row, 16 * cb + (lane & 15), nrhs, (double)acc[ch][m][cb][v], o, scales);
+              store_row_multi<S, kScaled>(td, row, 16 * cb + (lane & 15), nrhs, (double)acc[ch][m][cb][v], o, scales);
           }
   }
 }
@@ -1069,15 +1086,17 @@ __global__ __launch_bounds__(256) void tile_multi_kernel(const TileDesc *__restr
 // e over entry i's list range (as segsum_kernel: up to 24 in flight).  A
 // thread sums W consecutive right-hand sides (W = 2: 16-byte loads and
 // stores, half the waves; buf and out 16-byte aligned, nrhs even).
-template <bool kScaled, int W>
+template <bool kScaled, int W, typename P>
 __global__ __launch_bounds__(256) void segsum_multi_kernel(int n, int nrhs, const int2 *__restrict__ rng,
                                                            const int64_t *__restrict__ list,
-                                                           const double *__restrict__ buf,
+                                                           const P *__restrict__ buf,
                                                            const int32_t *__restrict__ factor_of,
                                                            const float *__restrict__ scales,
                                                            double *__restrict__ out, int accumulate) {
   static_assert(W == 1 || W == 2, "W is 1 or 2");
-  using VT = typename std::conditional<W == 2, double2, double>::type;
+  // W slots of buf at a time (P = double or float: 16- or 8-byte loads)
+  using VT = typename std::conditional<W == 2, typename std::conditional<sizeof(P) == 8, double2, float2>::type,
+                                       P>::type;
   const int nw = nrhs / W;
   const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= (int64_t)n * nw) return;
@@ -1099,10 +1118,10 @@ __global__ __launch_bounds__(256) void segsum_multi_kernel(int n, int nrhs, cons
     for (int u = 0; u < 24; ++u)
       if (e + u < e1) {
         if constexpr (W == 2) {
-          a[0] += v[u].x;
-          a[1] += v[u].y;
+          a[0] += (double)v[u].x;
+          a[1] += (double)v[u].y;
         } else {
-          a[0] += v[u];
+          a[0] += (double)v[u];
         }
       }
   }
@@ -1145,10 +1164,10 @@ __global__ __launch_bounds__(256) void flush_kernel(double4 *__restrict__ dst, s
 // e over [rng[i].x, rng[i].y) (entry i's segment, resolved per entry on the
 // host so that the chain of dependent loads is range -> list -> slots),
 // scale(i) = product of the two factor scales of entry i's leaf (t sum only)
-template <bool kScaled>
+template <bool kScaled, typename P>
 __global__ __launch_bounds__(256) void segsum_kernel(int n, const int2 *__restrict__ rng,
                                                      const int64_t *__restrict__ list,
-                                                     const double *__restrict__ buf,
+                                                     const P *__restrict__ buf,
                                                      const int32_t *__restrict__ factor_of,
                                                      const float *__restrict__ scales,
                                                      double *__restrict__ out, int accumulate) {
@@ -1161,15 +1180,18 @@ __global__ __launch_bounds__(256) void segsum_kernel(int n, const int2 *__restri
   // adds ~18 slots per row: one round trip for the list and one for the
   // slots instead of one pair per 8), added in list order
   for (int e = r.x; e < e1; e += 24) {
+    // Entries past the range load buf[0] (the range is not empty, so
+    // neither is buf) and add nothing: the slot loads are unguarded, since a
+    // guarded FP32 load whose conversion the compiler sinks into the guard
+    // waits for each load in turn.
     int64_t l[24];
-    double v[24];
+    P v[24];
 #pragma unroll
-    for (int u = 0; u < 24; ++u) l[u] = e + u < e1 ? list[e + u] : 0;
+    for (int u = 0; u < 24; ++u) l[u] = e + u < e1 ? list[e + u] : -(int64_t)i;
 #pragma unroll
-    for (int u = 0; u < 24; ++u) v[u] = e + u < e1 ? buf[l[u] + i] : 0.0;
+    for (int u = 0; u < 24; ++u) v[u] = buf[l[u] + i];
 #pragma unroll
-    for (int u = 0; u < 24; ++u)
-      if (e + u < e1) a += v[u];
+    for (int u = 0; u < 24; ++u) a += e + u < e1 ? (double)v[u] : 0.0;
   }
   // t entry of leaf b: both factor scales of b (a1 = factor_of, a2 = the next id)
   if (kScaled) a *= (double)scales[factor_of[i]] * (double)scales[factor_of[i] + 1];

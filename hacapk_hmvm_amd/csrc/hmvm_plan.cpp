@@ -65,7 +65,9 @@ std::string build_plan(const LeafArrays &la, const PlanOptions &opt, Plan *plan)
   P.nd = la.nd;
   P.nlf = la.nlf;
   if (la.nd < 1 || la.nlf < 1) return "empty matrix";
-  if (opt.w1 < 64 || opt.w2 < 64 || opt.align < 1 || opt.t_mode < 0 || opt.t_mode > 3) return "bad plan options";
+  if (opt.w1 < 64 || opt.w2 < 64 || (opt.w3 != 0 && opt.w3 < 64) || opt.align < 1 || opt.t_mode < 0 ||
+      opt.t_mode > 3)
+    return "bad plan options";
   const int MR = opt.max_rows;
   const int MRb = opt.max_rows_b > 0 ? opt.max_rows_b : MR;  // stage-2 tiles
   const int lim = max_tile_rows_limit(lane_elems(opt.storage));
@@ -401,10 +403,11 @@ std::string build_plan(const LeafArrays &la, const PlanOptions &opt, Plan *plan)
         P.repack.push_back(r);
       }
     }
-    // dense tiles (launch A) in row chunks of <= MR rows, stage-2 tiles
-    // (launch B) of <= MRb rows
+    // dense tiles (launch A) in row chunks of <= MR rows and about w2
+    // elements, stage-2 tiles (launch B) of <= MRb rows and about w3
     for (int kind = 0; kind < 2; kind++)
       for (auto rc : row_chunks(ndl, kind == 0 ? MR : MRb)) {
+        const int64_t wtile = kind == 0 || opt.w3 <= 0 ? opt.w2 : opt.w3;
         const int r0 = rc.first, nr = rc.second;
         const std::vector<int> &lst = kind == 0 ? dense : lowrank;
         size_t q = 0;
@@ -418,7 +421,7 @@ std::string build_plan(const LeafArrays &la, const PlanOptions &opt, Plan *plan)
           while (q < lst.size()) {
             const int i = lst[q];
             const int nc = kind == 0 ? la.ndt[i] : la.kt[i];
-            if (tdsc.npanel > 0 && work + (int64_t)nr * nc > opt.w2) break;
+            if (tdsc.npanel > 0 && work + (int64_t)nr * nc > wtile) break;
             PanelDesc pd;
             pd.val_off = dbase[i] + r0;
             pd.ld = ldp;

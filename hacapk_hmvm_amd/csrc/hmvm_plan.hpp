@@ -116,7 +116,8 @@ struct SegSum {
 struct PlanOptions {
   int storage = kF64;
   int w1 = 32768;         // target elements per stage-1 tile
-  int w2 = 16384;         // target elements per dense / stage-2 tile
+  int w2 = 16384;         // target elements per dense tile
+  int w3 = 0;             // target elements per stage-2 tile (0: w2)
   int align = 16;         // factor start alignment in elements
   int order = 0;          // 0: memory order, 1: largest tiles first
   int t_mode = 0;         // stage-2 vector: 0 partial sums on the fly, 1 materialised t, 2 per leaf,

@@ -75,6 +75,12 @@ int dalloc(T **p, size_t n) {
   HIP_TRY(hipMalloc((void **)p, sizeof(T) * n));
   return HMVM_OK;
 }
+int dalloc_bytes(void **p, size_t n) {
+  *p = nullptr;
+  if (n == 0) return HMVM_OK;
+  HIP_TRY(hipMalloc(p, n));
+  return HMVM_OK;
+}
 template <class T>
 int h2d(T *d, const T *h, size_t n) {
   if (n == 0) return HMVM_OK;
@@ -96,6 +102,8 @@ std::vector<int2> seg_ranges(const SegSum &g) {
 }
 
 int elem_bytes(int storage) { return storage == kF64 ? 8 : storage == kF32 ? 4 : 2; }
+// Bytes of one slot (SlotOf<S>::T): FP64 for FP64 storage, FP32 otherwise.
+size_t slot_bytes(int storage) { return storage == kF64 ? 8 : 4; }
 
 // N events, destroyed on every return path of the timing functions.
 template <int N>
@@ -131,7 +139,8 @@ struct hmvm_plan {
   int persist_key = -1, persist_blocks = 0;  // occupancy cache (kernel variant, block size)
   // several right-hand sides: buffers sized for multi_cap columns
   int multi_cap = 0;
-  double *d_mslots = nullptr, *d_mt = nullptr, *d_mX = nullptr, *d_mY = nullptr;
+  void *d_mslots = nullptr;  // slot_bytes(storage) per slot
+  double *d_mt = nullptr, *d_mX = nullptr, *d_mY = nullptr;
   PersistCounters *d_ctr = nullptr;
   int ncu = 0;     // compute units of the device
   int device = 0;
@@ -139,7 +148,8 @@ struct hmvm_plan {
   void *d_vals = nullptr;
   TileDesc *d_tiles_a = nullptr, *d_tiles_b = nullptr;
   PanelDesc *d_panels = nullptr;
-  double *d_t = nullptr, *d_slots = nullptr;  // stage-1 partial slots, then output slots
+  double *d_t = nullptr;
+  void *d_slots = nullptr;  // stage-1 partial slots, then output slots: slot_bytes(storage) each
   int32_t *d_tfac = nullptr;
   int32_t *d_tmap = nullptr;  // t_mode 3: t index of every stage-1 tile row
   int64_t *d_tlist = nullptr;
@@ -391,13 +401,13 @@ int source_from_batch(const matrix2 *const *mats, int count, int a2_source, Sour
 // row group per lane).  Options with 0 (widths,
 // workgroup size, rows) or -1 (kernel) take these.
 struct LaunchDefaults {
-  int w1, w2, kernel_a, bt_a, kernel_b, bt_b, max_rows, max_rows_b;
+  int w1, w2, kernel_a, bt_a, kernel_b, bt_b, max_rows, max_rows_b, w3;
 };
 const LaunchDefaults &launch_defaults(int storage) {
   static const LaunchDefaults d[3] = {
-      {32768, 32768, 6, 256, 6, 256, 256, 128},  // FP64
-      {32768, 24576, 6, 256, 6, 256, 256, 256},  // FP32
-      {32768, 32768, 6, 256, 6, 256, 512, 256},  // BF16
+      {32768, 32768, 6, 256, 6, 256, 256, 128, 32768},  // FP64
+      {32768, 24576, 6, 256, 6, 256, 256, 256, 24576},  // FP32
+      {32768, 32768, 6, 256, 6, 256, 512, 256, 32768},  // BF16
   };
   return d[storage >= kF64 && storage <= kBF16 ? storage : 0];
 }
@@ -417,6 +427,7 @@ PlanOptions plan_options(const hmvm_options *o, int64_t len) {
   const int64_t wfit = std::max<int64_t>(1024, len / 4096);
   po.w1 = o->w1 > 0 ? o->w1 : (int)std::min<int64_t>(launch_defaults(po.storage).w1, wfit);
   po.w2 = o->w2 > 0 ? o->w2 : (int)std::min<int64_t>(launch_defaults(po.storage).w2, wfit);
+  po.w3 = o->w3 > 0 ? o->w3 : o->w2 > 0 ? o->w2 : (int)std::min<int64_t>(launch_defaults(po.storage).w3, wfit);
   // an explicit max_rows sets both unless max_rows_b is given; by default
   // stage-2 tiles have their own
   po.max_rows = o->max_rows > 0 ? o->max_rows : launch_defaults(po.storage).max_rows;
@@ -755,7 +766,7 @@ int create_plan(Source &src, const hmvm_options *opt, hmvm_plan **out) {
     std::vector<float> ones((size_t)P.nfactors, 1.0f);
     if ((rc = h2d(p->d_scales, ones.data(), ones.size()))) return rc;
   }
-  if ((rc = dalloc(&p->d_t, (size_t)P.n_t)) || (rc = dalloc(&p->d_slots, (size_t)(P.n_tp + P.n_z))) ||
+  if ((rc = dalloc(&p->d_t, (size_t)P.n_t)) || (rc = dalloc_bytes(&p->d_slots, slot_bytes(p->storage) * (size_t)(P.n_tp + P.n_z))) ||
       (rc = dalloc(&p->d_x, (size_t)P.nd)) ||
       (rc = dalloc(&p->d_y, (size_t)P.nd)))
     return rc;
@@ -767,7 +778,8 @@ int create_plan(Source &src, const hmvm_options *opt, hmvm_plan **out) {
   if ((rc = upload_values(p.get(), src))) return rc;
 
   p->device_bytes = eb * (size_t)P.nvals + sizeof(TileDesc) * (P.tiles_a.size() + P.tiles_b.size()) +
-                    sizeof(PanelDesc) * P.panels.size() + 8 * (size_t)(P.n_t + P.n_tp + P.n_z + 2 * P.nd) +
+                    sizeof(PanelDesc) * P.panels.size() + 8 * (size_t)(P.n_t + 2 * P.nd) +
+                    slot_bytes(p->storage) * (size_t)(P.n_tp + P.n_z) +
                     4 * (P.ysum.seg.size() + P.ysum.ptr.size()) + 4 * (size_t)P.n_t +
                     8 * (P.tsum.list.size() + P.ysum.list.size() + P.tsum.seg.size() + P.ysum.seg.size());
   if (verbose())
@@ -782,7 +794,7 @@ int create_plan(Source &src, const hmvm_options *opt, hmvm_plan **out) {
 
 // Where the tiles of one launch store their rows: the slots, and for
 // stage-1 tiles with a row map the plan's t (or the multi-RHS T).
-TileOut tile_out(const hmvm_plan *p, double *out, double *tout) {
+TileOut tile_out(const hmvm_plan *p, void *out, double *tout) {
   return TileOut{out, p->d_tmap, tout, p->d_tfac};
 }
 
@@ -790,7 +802,7 @@ TileOut tile_out(const hmvm_plan *p, double *out, double *tout) {
 // its part of the HMVM: 0 A, 1 B, 2 stage-1 part of A, 3 dense part)
 template <typename S, bool kScaled, int kVar>
 void launch_tiles_v(hmvm_plan *p, int kind, int bt, hipStream_t st, const TileDesc *d_tiles, int ntiles,
-                    const double *x, const double *t, const double *tp, double *out) {
+                    const double *x, const double *t, const void *tp, void *out) {
   (void)kind;
   constexpr int L = 8;
   constexpr bool kBV = (kVar & 1) != 0;
@@ -798,13 +810,13 @@ void launch_tiles_v(hmvm_plan *p, int kind, int bt, hipStream_t st, const TileDe
   const int wpb = bt / 64;
   const size_t lds = sizeof(double) * wpb * lds_wave_doubles<S>();
   hipLaunchKernelGGL((tile_kernel<S, kScaled, L, kPipe, kNT, kBV>), dim3((ntiles + wpb - 1) / wpb), dim3(bt), lds, st,
-                     d_tiles, ntiles, p->d_panels, (const S *)p->d_vals, x, t, tp, (const float *)p->d_scales,
-                     tile_out(p, out, p->d_t));
+                     d_tiles, ntiles, p->d_panels, (const S *)p->d_vals, x, t,
+                     (const typename SlotOf<S>::T *)tp, (const float *)p->d_scales, tile_out(p, out, p->d_t));
 }
 
 template <typename S, bool kScaled>
 void launch_tiles_s(hmvm_plan *p, int kern, int kind, int bt, hipStream_t st, const TileDesc *d_tiles, int ntiles,
-                    const double *x, const double *t, const double *tp, double *out) {
+                    const double *x, const double *t, const void *tp, void *out) {
 #define HMVM_TILE_CASE(v) \
   case v: launch_tiles_v<S, kScaled, v>(p, kind, bt, st, d_tiles, ntiles, x, t, tp, out); break;
   switch (kern) {
@@ -817,7 +829,7 @@ void launch_tiles_s(hmvm_plan *p, int kern, int kind, int bt, hipStream_t st, co
 
 template <typename S>
 void launch_tiles(hmvm_plan *p, int which, int kind, const TileDesc *d_tiles, int ntiles, const double *x,
-                  const double *t, const double *tp, double *out, bool scaled, hipStream_t st) {
+                  const double *t, const void *tp, void *out, bool scaled, hipStream_t st) {
   if (ntiles == 0) return;
   const int bt = which ? p->block_threads_b : p->block_threads;
   const int kern = which ? p->kernel_b : p->kernel;
@@ -855,20 +867,20 @@ void launch_tiles_part(hmvm_plan *p, int which, const double *x, hipStream_t st)
 // ---- per-tile timing (cost-model training data) -----------------------------
 template <typename S, bool kScaled, int kVar>
 void launch_timed_v(hmvm_plan *p, int bt, const TileDesc *d_tiles, int ntiles, const double *x, const double *t,
-                    const double *tp, double *out, long long *stamps, hipStream_t st) {
+                    const void *tp, void *out, long long *stamps, hipStream_t st) {
   constexpr int L = 8;
   constexpr bool kBV = (kVar & 1) != 0;
   constexpr bool kPipe = (kVar & 2) != 0, kNT = (kVar & 4) != 0;
   const int wpb = bt / 64;
   const size_t lds = sizeof(double) * wpb * lds_wave_doubles<S>();
   hipLaunchKernelGGL((tile_kernel_timed<S, kScaled, L, kPipe, kNT, kBV>), dim3((ntiles + wpb - 1) / wpb),
-                     dim3(bt), lds, st, d_tiles, ntiles, p->d_panels, (const S *)p->d_vals, x, t, tp,
-                     (const float *)p->d_scales, tile_out(p, out, p->d_t), stamps);
+                     dim3(bt), lds, st, d_tiles, ntiles, p->d_panels, (const S *)p->d_vals, x, t,
+                     (const typename SlotOf<S>::T *)tp, (const float *)p->d_scales, tile_out(p, out, p->d_t), stamps);
 }
 
 template <typename S, bool kScaled>
 void launch_timed_s(hmvm_plan *p, int which, const TileDesc *d_tiles, int ntiles, const double *x, const double *t,
-                    const double *tp, double *out, long long *stamps, hipStream_t st) {
+                    const void *tp, void *out, long long *stamps, hipStream_t st) {
   const int bt = which ? p->block_threads_b : p->block_threads;
   switch (which ? p->kernel_b : p->kernel) {
     case 0: launch_timed_v<S, kScaled, 0>(p, bt, d_tiles, ntiles, x, t, tp, out, stamps, st); break;
@@ -889,7 +901,7 @@ void launch_timed(hmvm_plan *p, int which, const double *x, long long *stamps, h
   const int nt = (int)(which == 0 ? P.tiles_a.size() : P.tiles_b.size());
   if (nt == 0) return;
   const double *t = which == 0 ? nullptr : p->d_t;
-  const double *tp = which == 0 ? nullptr : p->d_slots;
+  const void *tp = which == 0 ? nullptr : p->d_slots;
   if (p->storage == kF64)
     launch_timed_s<double, false>(p, which, tl, nt, x, t, tp, p->d_slots, stamps, st);
   else if (p->storage == kF32)
@@ -898,12 +910,30 @@ void launch_timed(hmvm_plan *p, int which, const double *x, long long *stamps, h
     launch_timed_s<uint16_t, true>(p, which, tl, nt, x, t, tp, p->d_slots, stamps, st);
 }
 
+// t segment sum (phase 1) or y segment sum (phase 3) over slots of type P.
+template <typename P>
+void launch_segsum(hmvm_plan *p, int phase, double *y, int accumulate, hipStream_t st) {
+  const Plan &P_ = p->plan;
+  const P *slots = (const P *)p->d_slots;
+  if (phase == 1) {
+    if (P_.t_fused || P_.tsum.n == 0) return;
+    if (p->storage == kBF16)
+      hipLaunchKernelGGL((segsum_kernel<true, P>), dim3((P_.tsum.n + 255) / 256), dim3(256), 0, st, P_.tsum.n,
+                         p->d_trng, p->d_tlist, slots, p->d_tfac, p->d_scales, p->d_t, 0);
+    else
+      hipLaunchKernelGGL((segsum_kernel<false, P>), dim3((P_.tsum.n + 255) / 256), dim3(256), 0, st, P_.tsum.n,
+                         p->d_trng, p->d_tlist, slots, (const int32_t *)nullptr, (const float *)nullptr, p->d_t, 0);
+  } else {
+    hipLaunchKernelGGL((segsum_kernel<false, P>), dim3((P_.ysum.n + 255) / 256), dim3(256), 0, st, P_.ysum.n,
+                       p->d_yrng, p->d_ylist, slots, (const int32_t *)nullptr, (const float *)nullptr, y,
+                       accumulate);
+  }
+}
+
 // Launch `phase` of one HMVM: 0 = tiles A (stage 1 + dense), 1 = t segment sum
 // (only when some leaf's rank rows straddle two row groups), 2 = tiles B
 // (stage 2), 3 = y segment sum.
 void launch_phase(hmvm_plan *p, int phase, const double *x, double *y, int accumulate, hipStream_t st) {
-  const Plan &P = p->plan;
-  const bool bf = p->storage == kBF16;
   if (phase == 0 || phase == 2) {
     const int which = phase == 0 ? 0 : 1;
     if (p->storage == kF64)
@@ -912,17 +942,10 @@ void launch_phase(hmvm_plan *p, int phase, const double *x, double *y, int accum
       launch_tiles_both<float>(p, which, x, st);
     else
       launch_tiles_both<uint16_t>(p, which, x, st);
-  } else if (phase == 1) {
-    if (P.t_fused || P.tsum.n == 0) return;
-    if (bf)
-      hipLaunchKernelGGL(segsum_kernel<true>, dim3((P.tsum.n + 255) / 256), dim3(256), 0, st, P.tsum.n, p->d_trng,
-                         p->d_tlist, p->d_slots, p->d_tfac, p->d_scales, p->d_t, 0);
-    else
-      hipLaunchKernelGGL(segsum_kernel<false>, dim3((P.tsum.n + 255) / 256), dim3(256), 0, st, P.tsum.n, p->d_trng,
-                         p->d_tlist, p->d_slots, (const int32_t *)nullptr, (const float *)nullptr, p->d_t, 0);
+  } else if (p->storage == kF64) {
+    launch_segsum<double>(p, phase, y, accumulate, st);
   } else {
-    hipLaunchKernelGGL(segsum_kernel<false>, dim3((P.ysum.n + 255) / 256), dim3(256), 0, st, P.ysum.n, p->d_yrng,
-                       p->d_ylist, p->d_slots, (const int32_t *)nullptr, (const float *)nullptr, y, accumulate);
+    launch_segsum<float>(p, phase, y, accumulate, st);
   }
 }
 
@@ -931,13 +954,14 @@ int reserve_multi(hmvm_plan *p, int nrhs) {
   if (nrhs != 16 && nrhs != 32 && nrhs != 64) return set_error(HMVM_ERR_ARG, "nrhs must be 16, 32 or 64");
   if (p->multi_cap >= nrhs) return HMVM_OK;
   const Plan &P = p->plan;
-  double *bufs[4] = {p->d_mslots, p->d_mt, p->d_mX, p->d_mY};
-  for (double *b : bufs)
+  void *bufs[4] = {p->d_mslots, p->d_mt, p->d_mX, p->d_mY};
+  for (void *b : bufs)
     if (b) (void)hipFree(b);
   p->d_mslots = p->d_mt = p->d_mX = p->d_mY = nullptr;
   p->multi_cap = 0;
   int rc;
-  if ((rc = dalloc(&p->d_mslots, (size_t)(P.n_tp + P.n_z) * nrhs)) || (rc = dalloc(&p->d_mt, (size_t)P.n_t * nrhs)) ||
+  if ((rc = dalloc_bytes(&p->d_mslots, slot_bytes(p->storage) * (size_t)(P.n_tp + P.n_z) * nrhs)) ||
+      (rc = dalloc(&p->d_mt, (size_t)P.n_t * nrhs)) ||
       (rc = dalloc(&p->d_mX, (size_t)P.nd * nrhs)) || (rc = dalloc(&p->d_mY, (size_t)P.nd * nrhs)))
     return rc;
   p->multi_cap = nrhs;
@@ -952,14 +976,15 @@ void launch_multi_tiles(hmvm_plan *p, const TileDesc *d_tiles, int ntiles, const
   if (ntiles == 0) return;
   const int wpb = p->block_threads / 64;
   const TileOut o = tile_out(p, p->d_mslots, p->d_mt);
+  const auto *TP = (const typename SlotOf<S>::T *)p->d_mslots;
   if (p->has_tp)
     hipLaunchKernelGGL((tile_multi_kernel<S, kScaled, NB, true>), dim3((ntiles + wpb - 1) / wpb),
                        dim3(p->block_threads), 0, st, d_tiles, ntiles, p->d_panels, (const S *)p->d_vals, X, T,
-                       p->d_mslots, (const float *)p->d_scales, o);
+                       TP, (const float *)p->d_scales, o);
   else
     hipLaunchKernelGGL((tile_multi_kernel<S, kScaled, NB, false>), dim3((ntiles + wpb - 1) / wpb),
                        dim3(p->block_threads), 0, st, d_tiles, ntiles, p->d_panels, (const S *)p->d_vals, X, T,
-                       p->d_mslots, (const float *)p->d_scales, o);
+                       TP, (const float *)p->d_scales, o);
 }
 
 template <typename S, bool kScaled>
@@ -973,10 +998,39 @@ void launch_multi_nb(hmvm_plan *p, const TileDesc *d_tiles, int ntiles, const do
     launch_multi_tiles<S, kScaled, 4>(p, d_tiles, ntiles, X, T, st);
 }
 
+// Multi-RHS t (phase 1) or y (phase 3) segment sum over slots of type P.
+template <typename P>
+void launch_segsum_multi(hmvm_plan *p, int phase, double *Y, int nrhs, int accumulate, hipStream_t st) {
+  const Plan &P_ = p->plan;
+  const P *slots = (const P *)p->d_mslots;
+  if (phase == 1) {
+    // two columns per thread (measured 15-20% faster than one at N=101,250);
+    // slots and t are the plan's own allocations, nrhs is even
+    if (P_.t_fused || P_.tsum.n == 0) return;
+    const int64_t tot = (int64_t)P_.tsum.n * (nrhs / 2);
+    if (p->storage == kBF16)
+      hipLaunchKernelGGL((segsum_multi_kernel<true, 2, P>), dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st,
+                         P_.tsum.n, nrhs, p->d_trng, p->d_tlist, slots, p->d_tfac, p->d_scales, p->d_mt, 0);
+    else
+      hipLaunchKernelGGL((segsum_multi_kernel<false, 2, P>), dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st,
+                         P_.tsum.n, nrhs, p->d_trng, p->d_tlist, slots, (const int32_t *)nullptr,
+                         (const float *)nullptr, p->d_mt, 0);
+  } else if (((uintptr_t)Y & 15) == 0) {  // two columns per thread where the caller's Y is 16-byte aligned
+    const int64_t tot = (int64_t)P_.ysum.n * (nrhs / 2);
+    hipLaunchKernelGGL((segsum_multi_kernel<false, 2, P>), dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st,
+                       P_.ysum.n, nrhs, p->d_yrng, p->d_ylist, slots, (const int32_t *)nullptr,
+                       (const float *)nullptr, Y, accumulate);
+  } else {
+    const int64_t tot = (int64_t)P_.ysum.n * nrhs;
+    hipLaunchKernelGGL((segsum_multi_kernel<false, 1, P>), dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st,
+                       P_.ysum.n, nrhs, p->d_yrng, p->d_ylist, slots, (const int32_t *)nullptr,
+                       (const float *)nullptr, Y, accumulate);
+  }
+}
+
 void launch_multi_phase(hmvm_plan *p, int phase, const double *X, double *Y, int nrhs, int accumulate,
                         hipStream_t st) {
   const Plan &P = p->plan;
-  const bool bf = p->storage == kBF16;
   if (phase == 0 || phase == 2) {
     const TileDesc *tl = phase == 0 ? p->d_tiles_a : p->d_tiles_b;
     const int nt = (int)(phase == 0 ? P.tiles_a.size() : P.tiles_b.size());
@@ -987,31 +1041,10 @@ void launch_multi_phase(hmvm_plan *p, int phase, const double *X, double *Y, int
       launch_multi_nb<float, false>(p, tl, nt, X, T, nrhs, st);
     else
       launch_multi_nb<uint16_t, true>(p, tl, nt, X, T, nrhs, st);
-  } else if (phase == 1) {
-    // two columns per thread (measured 15-20% faster than one at N=101,250);
-    // slots and t are the plan's own allocations, nrhs is even
-    if (P.t_fused || P.tsum.n == 0) return;
-    const int64_t tot = (int64_t)P.tsum.n * (nrhs / 2);
-    if (bf)
-      hipLaunchKernelGGL((segsum_multi_kernel<true, 2>), dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st,
-                         P.tsum.n, nrhs, p->d_trng, p->d_tlist, p->d_mslots, p->d_tfac, p->d_scales, p->d_mt, 0);
-    else
-      hipLaunchKernelGGL((segsum_multi_kernel<false, 2>), dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st,
-                         P.tsum.n, nrhs, p->d_trng, p->d_tlist, p->d_mslots, (const int32_t *)nullptr,
-                         (const float *)nullptr, p->d_mt, 0);
+  } else if (p->storage == kF64) {
+    launch_segsum_multi<double>(p, phase, Y, nrhs, accumulate, st);
   } else {
-    // two columns per thread where the caller's Y is 16-byte aligned
-    if (((uintptr_t)Y & 15) == 0) {
-      const int64_t tot = (int64_t)P.ysum.n * (nrhs / 2);
-      hipLaunchKernelGGL((segsum_multi_kernel<false, 2>), dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st,
-                         P.ysum.n, nrhs, p->d_yrng, p->d_ylist, p->d_mslots, (const int32_t *)nullptr,
-                         (const float *)nullptr, Y, accumulate);
-    } else {
-      const int64_t tot = (int64_t)P.ysum.n * nrhs;
-      hipLaunchKernelGGL((segsum_multi_kernel<false, 1>), dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st,
-                         P.ysum.n, nrhs, p->d_yrng, p->d_ylist, p->d_mslots, (const int32_t *)nullptr,
-                         (const float *)nullptr, Y, accumulate);
-    }
+    launch_segsum_multi<float>(p, phase, Y, nrhs, accumulate, st);
   }
 }
 
@@ -1046,7 +1079,7 @@ int launch_persistent_v(hmvm_plan *p, const double *x, double *y, int accumulate
   const int nt = na + (int)P.tiles_b.size();
   const int nitems = nt + (P.nd + 63) / 64;
   hipLaunchKernelGGL(kfn, dim3(blocks), dim3(p->block_threads), lds, st, p->d_tiles_a, ns1, na, p->d_tiles_b, nt,
-                     nitems, nwaves, p->d_panels, (const S *)p->d_vals, x, p->d_t, p->d_slots,
+                     nitems, nwaves, p->d_panels, (const S *)p->d_vals, x, p->d_t, (typename SlotOf<S>::T *)p->d_slots,
                      (const float *)p->d_scales, p->d_tmap, p->d_tfac, p->d_yseg, p->d_yptr, p->d_ylist, P.nd, y,
                      accumulate, p->d_ctr);
   return HMVM_OK;
@@ -1280,6 +1313,7 @@ void hmvm_options_default(hmvm_options *opt) {
   opt->t_mode = -1;  // by storage
   opt->max_rows = 0;  // by storage
   opt->max_rows_b = 0;
+  opt->w3 = 0;
 }
 
 int hmvm_plan_create(const matrix2 *mat2, int a2_source, const hmvm_options *opt, hmvm_plan **out) {
@@ -1916,6 +1950,13 @@ int hmvm_plan_profile_tiles(hmvm_plan *plan, int which, int iters, double *tile_
   long long *d_stamps = nullptr;
   int rc = dalloc(&d_stamps, 2 * nt);
   if (rc) return rc;
+  // cache eviction before every timed launch: 512 MB, twice the last-level cache
+  const size_t n4 = ((size_t)512 << 20) / sizeof(double4);
+  double4 *d_flush = nullptr;
+  if (hipMalloc(&d_flush, n4 * sizeof(double4)) != hipSuccess) {
+    (void)hipFree(d_stamps);
+    return set_error(HMVM_ERR_HIP, "hipMalloc of the 512 MB flush buffer failed");
+  }
   std::vector<long long> h(2 * nt);
   std::vector<double> acc(nt, 0.0), acc0(nt, 0.0);
   hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -1923,6 +1964,7 @@ int hmvm_plan_profile_tiles(hmvm_plan *plan, int which, int iters, double *tile_
   hipStream_t st = plan->stream;
   if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) rc = set_error(HMVM_ERR_HIP, "event");
   for (int it = 0; it <= iters && !rc; it++) {  // iteration 0 warms up
+    hipLaunchKernelGGL(flush_kernel, dim3(4096), dim3(256), 0, st, d_flush, n4, (double)it);
     if (which == 1) {
       launch_phase(plan, 0, plan->d_x, plan->d_y, 0, st);
       launch_phase(plan, 1, plan->d_x, plan->d_y, 0, st);
@@ -1952,6 +1994,7 @@ int hmvm_plan_profile_tiles(hmvm_plan *plan, int which, int iters, double *tile_
   if (e0) (void)hipEventDestroy(e0);
   if (e1) (void)hipEventDestroy(e1);
   (void)hipFree(d_stamps);
+  (void)hipFree(d_flush);
   if (rc) return rc;
   for (size_t i = 0; i < nt; i++) {
     tile_us[i] = acc[i] / iters;

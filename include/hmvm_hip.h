@@ -96,7 +96,8 @@ typedef struct hmvm_options {
   int storage;           /* HMVM_STORAGE_*: element type kept in HBM            */
   int w1;                /* target elements per stage-1 tile (0: by storage,    */
                          /* capped at max(1024, len / 4096) for small matrices) */
-  int w2;                /* target elements per dense/stage-2 tile (0: ditto)   */
+  int w2;                /* target elements per dense tile (and per stage-2     */
+                         /* tile unless w3 is set; 0: ditto)                    */
   int block_threads;     /* threads per workgroup, 64..256 step 64 (0: by       */
                          /* storage, per launch)                                */
   int kernel;            /* tile-kernel variant 0..7 (-1: by storage, per       */
@@ -123,6 +124,8 @@ typedef struct hmvm_options {
                          /* stage-2 tiles unless max_rows_b is set              */
   int max_rows_b;        /* rows per stage-2 tile of launch B (0: max_rows if   */
                          /* set, else by storage)                               */
+  int w3;                /* target elements per stage-2 tile (0: w2 if set,     */
+                         /* else by storage; capped like w2)                    */
 } hmvm_options;
 
 typedef struct hmvm_plan_info {
@@ -281,7 +284,11 @@ int hmvm_plan_time_graph(hmvm_plan *plan, int count, int warmup, int iters, hmvm
  *   column, or HMVM_TILE_ROW_PER_LANE set), 0 for the column-split path
  * hmvm_plan_profile_tiles runs the launch with a clock-stamping copy of the
  * tile kernel (same results) and returns each tile's mean duration and start
- * offset in microseconds, plus the mean launch time.
+ * offset in microseconds, plus the mean launch time.  Before each timed
+ * launch it overwrites 512 MB of scratch (twice the last-level cache), so
+ * the launch reads its matrix values from HBM even when the matrix would fit
+ * in cache: small matrices then give the durations of the streaming regime
+ * that large ones run in. 
  * hmvm_plan_set_tile_order permutes the dispatch order of a launch (results
  * are unchanged: tiles write disjoint slots).  Modes 1 and 2 need launch A as
  * "stage-1 tiles, then dense": selecting them regroups launch A so, keeping
