@@ -46,6 +46,14 @@ struct SlotOf { using T = float; };
 template <>
 struct SlotOf<double> { using T = double; };
 
+// Element type of the multi-RHS vectors the tiles read (the FP32 copy of X
+// and t): the MFMA operand type's FP32 for FP32 and BF16 storage, whose
+// operands are FP32-rounded anyway, FP64 for FP64 storage.
+template <typename S>
+struct VecOf { using T = float; };
+template <>
+struct VecOf<double> { using T = double; };
+
 // BF16 values are kept as raw uint16_t bit patterns.
 __device__ __forceinline__ double load_val(const double *p) { return *p; }
 __device__ __forceinline__ float load_val(const float *p) { return *p; }
@@ -128,7 +136,7 @@ __device__ __forceinline__ float panel_scale(const PanelDesc &pd, const float *_
 struct TileOut {
   void *out;  // slots: SlotOf<S>::T of the launch's storage S
   const int32_t *tmap;
-  double *tout;
+  void *tout;  // t: double (one right-hand side), VecOf<S>::T (several)
   const int32_t *tfac;
 };
 
@@ -142,7 +150,7 @@ __device__ __forceinline__ void store_row(const TileDesc &td, int row, double v,
   }
   const int e = o.tmap[td.map_off + row];
   if constexpr (kScaled) v *= (double)scales[o.tfac[e]] * (double)scales[o.tfac[e] + 1];
-  o.tout[e] = v;
+  static_cast<double *>(o.tout)[e] = v;
 }
 
 // ---- wide lane loads ----------------------------------------------------------
@@ -775,60 +783,123 @@ __device__ __forceinline__ short bf16_bits(float f) {
   return (short)(uint16_t)((b + 0x7FFFu + ((b >> 16) & 1u)) >> 16);
 }
 
-// Row `row`, right-hand side `col` of a multi-RHS tile: store_row with rows
-// of nrhs entries (slots and t in the row-major multi-RHS layout).
-template <typename S, bool kScaled>
-__device__ __forceinline__ void store_row_multi(const TileDesc &td, int row, int col, int nrhs, double v,
-                                                const TileOut &o, const float *__restrict__ scales) {
+// NB consecutive elements at p (p aligned to NB elements): one 4-, 8- or
+// 16-byte access per 16 bytes.
+template <int NB, typename T>
+__device__ __forceinline__ void load_n(const T *__restrict__ p, double (&v)[NB]) {
+  if constexpr (NB == 1) {
+    v[0] = (double)p[0];
+  } else if constexpr (sizeof(T) == 8) {
+#pragma unroll
+    for (int c = 0; c < NB; c += 2) {
+      const double2 w = *reinterpret_cast<const double2 *>(p + c);
+      v[c] = w.x;
+      v[c + 1] = w.y;
+    }
+  } else if constexpr (NB == 2) {
+    const float2 w = *reinterpret_cast<const float2 *>(p);
+    v[0] = (double)w.x;
+    v[1] = (double)w.y;
+  } else {
+#pragma unroll
+    for (int c = 0; c < NB; c += 4) {
+      const float4 w = *reinterpret_cast<const float4 *>(p + c);
+      v[c] = (double)w.x;
+      v[c + 1] = (double)w.y;
+      v[c + 2] = (double)w.z;
+      v[c + 3] = (double)w.w;
+    }
+  }
+}
+template <int NB, typename T>
+__device__ __forceinline__ void store_n(T *__restrict__ p, const double (&v)[NB]) {
+  if constexpr (NB == 1) {
+    p[0] = (T)v[0];
+  } else if constexpr (sizeof(T) == 8) {
+#pragma unroll
+    for (int c = 0; c < NB; c += 2) *reinterpret_cast<double2 *>(p + c) = double2{v[c], v[c + 1]};
+  } else if constexpr (NB == 2) {
+    *reinterpret_cast<float2 *>(p) = float2{(float)v[0], (float)v[1]};
+  } else {
+#pragma unroll
+    for (int c = 0; c < NB; c += 4)
+      *reinterpret_cast<float4 *>(p + c) = float4{(float)v[c], (float)v[c + 1], (float)v[c + 2], (float)v[c + 3]};
+  }
+}
+
+// Row `row`, right-hand sides rhs0 .. rhs0+NB-1 of a multi-RHS tile:
+// store_row with rows of nrhs entries (slots and t in the row-major
+// multi-RHS layout), one vector store.
+template <typename S, bool kScaled, int NB>
+__device__ __forceinline__ void store_rows_multi(const TileDesc &td, int row, int rhs0, int nrhs, double (&v)[NB],
+                                                 const TileOut &o, const float *__restrict__ scales) {
   using Slot = typename SlotOf<S>::T;
   if (td.map_off < 0) {
-    static_cast<Slot *>(o.out)[(int64_t)(td.out_off + row) * nrhs + col] = (Slot)v;
+    store_n<NB>(static_cast<Slot *>(o.out) + (int64_t)(td.out_off + row) * nrhs + rhs0, v);
     return;
   }
   const int e = o.tmap[td.map_off + row];
-  if constexpr (kScaled) v *= (double)scales[o.tfac[e]] * (double)scales[o.tfac[e] + 1];
-  o.tout[(int64_t)e * nrhs + col] = v;
+  if constexpr (kScaled) {
+    const double f = (double)scales[o.tfac[e]] * (double)scales[o.tfac[e] + 1];
+#pragma unroll
+    for (int c = 0; c < NB; ++c) v[c] *= f;
+  }
+  store_n<NB>(static_cast<typename VecOf<S>::T *>(o.tout) + (int64_t)e * nrhs + rhs0, v);
 }
 
-// Vector entry (col, rhs) of a panel in the multi-RHS layout, zero past the
-// panel's last column.  The load address is clamped into the panel so that
-// the load is issued unconditionally: a K-step's loads then sit in one basic
-// block with the rest of the batch and are all in flight before the first
-// MFMA (a branch per load makes the compiler wait for each one).  X and T
-// panels read `src` (the caller's choice of X or T); kTP panels sum stage-1
-// partial slots.
-template <bool kTP, typename P>
-__device__ __forceinline__ double vec_entry_multi(const PanelDesc &pd, const double *__restrict__ src,
-                                                  const P *__restrict__ TP, int col, int rhs, int nrhs) {
+// Vector entries (col, rhs0 .. rhs0+NB-1) of a panel in the multi-RHS
+// layout, zero past the panel's last column.  The load address is clamped
+// into the panel so that the load is issued unconditionally: a K-step's loads
+// then sit in one basic block with the rest of the batch and are all in
+// flight before the first MFMA (a branch per load makes the compiler wait for
+// each one).  X and T panels read `src` (the caller's choice of X or T); kTP
+// panels sum stage-1 partial slots.
+template <bool kTP, int NB, typename P, typename VT>
+__device__ __forceinline__ void vec_entries_multi(const PanelDesc &pd, const VT *__restrict__ src,
+                                                  const P *__restrict__ TP, int col, int rhs0, int nrhs,
+                                                  double (&v)[NB]) {
   const int c = col < pd.ncols ? col : pd.ncols - 1;
-  double v;
   if constexpr (kTP) {
     const int nch = pd.vec_chunks >> 16, stride = pd.vec_chunks & 0xFFFF;
-    v = 0;
-    for (int q = 0; q < nch; ++q) v += (double)TP[(int64_t)(pd.vec_off + q * stride + c) * nrhs + rhs];
+#pragma unroll
+    for (int r = 0; r < NB; ++r) v[r] = 0;
+    for (int q = 0; q < nch; ++q) {
+      double w[NB];
+      load_n<NB>(TP + (int64_t)(pd.vec_off + q * stride + c) * nrhs + rhs0, w);
+#pragma unroll
+      for (int r = 0; r < NB; ++r) v[r] += w[r];
+    }
   } else {
-    v = src[(int64_t)(pd.vec_off + c) * nrhs + rhs];
+    load_n<NB>(src + (int64_t)(pd.vec_off + c) * nrhs + rhs0, v);
   }
-  return col < pd.ncols ? v : 0.0;
+  if (col >= pd.ncols)
+#pragma unroll
+    for (int r = 0; r < NB; ++r) v[r] = 0.0;
 }
 
-// Vector operand of rhs block cb at K-step k0 (times the panel scale): FP64
-// and FP32 one entry (column k0 + l/16), BF16 four (columns k0 + 4(l/16) + q).
-template <typename S, bool kTP>
-__device__ __forceinline__ typename Mfma<S>::Op b_operand(const PanelDesc &pd, const double *__restrict__ src,
-                                                          const typename SlotOf<S>::T *__restrict__ TP, int rhs,
-                                                          int nrhs, int k0,
-                                                          int lane, float scale) {
+// Vector operands of the NB right-hand-side blocks at K-step k0 (times the
+// panel scale).  Block cb, MFMA column j is right-hand side NB*j + cb, so
+// lane l (j = l%16) loads its NB right-hand sides NB*j .. NB*j+NB-1 at once.
+// FP64 and FP32 one entry per block (column k0 + l/16), BF16 four (columns
+// k0 + 4(l/16) + q).
+template <typename S, bool kTP, int NB>
+__device__ __forceinline__ void b_operands(const PanelDesc &pd, const typename VecOf<S>::T *__restrict__ src,
+                                           const typename SlotOf<S>::T *__restrict__ TP, int nrhs, int k0,
+                                           int lane, float scale, typename Mfma<S>::Op (&b)[NB]) {
+  const int rhs0 = NB * (lane & 15);
   if constexpr (sizeof(S) == 2) {
-    double v[4];
+    double v[4][NB];
 #pragma unroll
-    for (int q = 0; q < 4; ++q) v[q] = vec_entry_multi<kTP>(pd, src, TP, k0 + 4 * (lane >> 4) + q, rhs, nrhs);
-    mf_s4 b;
+    for (int q = 0; q < 4; ++q) vec_entries_multi<kTP, NB>(pd, src, TP, k0 + 4 * (lane >> 4) + q, rhs0, nrhs, v[q]);
 #pragma unroll
-    for (int q = 0; q < 4; ++q) b[q] = bf16_bits((float)v[q] * scale);
-    return b;
+    for (int cb = 0; cb < NB; ++cb)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) b[cb][q] = bf16_bits((float)v[q][cb] * scale);
   } else {
-    return (S)vec_entry_multi<kTP>(pd, src, TP, k0 + (lane >> 4), rhs, nrhs);
+    double v[NB];
+    vec_entries_multi<kTP, NB>(pd, src, TP, k0 + (lane >> 4), rhs0, nrhs, v);
+#pragma unroll
+    for (int cb = 0; cb < NB; ++cb) b[cb] = (S)v[cb];
   }
 }
 
@@ -892,7 +963,7 @@ struct MultiShape {
 // all its loads, then its MFMAs.
 template <typename S, int NB, bool kTP, int CH>
 __device__ __forceinline__ void multi_panel(const PanelDesc &pd, const S *const *base, int nl,
-                                            const double *__restrict__ src,
+                                            const typename VecOf<S>::T *__restrict__ src,
                                             const typename SlotOf<S>::T *__restrict__ TP, float scale, int lane,
                                             typename Mfma<S>::Acc (&acc)[CH][Wide<S>::V][NB]) {
   using M = Mfma<S>;
@@ -905,9 +976,7 @@ __device__ __forceinline__ void multi_panel(const PanelDesc &pd, const S *const 
 #pragma unroll
       for (int ch = 0; ch < CH; ++ch)
         if (ch < nl) load_words<S>(a[kb][ch], base[ch], pd, k0 + kb * K, lane);
-#pragma unroll
-      for (int cb = 0; cb < NB; ++cb)
-        b[kb][cb] = b_operand<S, kTP>(pd, src, TP, 16 * cb + (lane & 15), nrhs, k0 + kb * K, lane, scale);
+      b_operands<S, kTP, NB>(pd, src, TP, nrhs, k0 + kb * K, lane, scale, b[kb]);
     }
 #pragma unroll
     for (int kb = 0; kb < KB; ++kb)
@@ -938,7 +1007,7 @@ __device__ __forceinline__ S a_narrow(const S *__restrict__ vals, const PanelDes
 
 template <typename S, int NB, int NL, bool kTP, int CH>
 __device__ __forceinline__ void multi_panel_narrow(const PanelDesc &pd, const S *__restrict__ vals, int nrows,
-                                                   const double *__restrict__ src,
+                                                   const typename VecOf<S>::T *__restrict__ src,
                                                    const typename SlotOf<S>::T *__restrict__ TP, float scale, int lane,
                                                    typename Mfma<S>::Acc (&acc)[CH][NB]) {
   using M = Mfma<S>;
@@ -949,9 +1018,7 @@ __device__ __forceinline__ void multi_panel_narrow(const PanelDesc &pd, const S 
     for (int kb = 0; kb < KB; ++kb) {
 #pragma unroll
       for (int ch = 0; ch < NL; ++ch) a[kb][ch] = a_narrow<S>(vals, pd, 16 * ch + (lane & 15), nrows, k0 + kb * K, lane);
-#pragma unroll
-      for (int cb = 0; cb < NB; ++cb)
-        b[kb][cb] = b_operand<S, kTP>(pd, src, TP, 16 * cb + (lane & 15), nrhs, k0 + kb * K, lane, scale);
+      b_operands<S, kTP, NB>(pd, src, TP, nrhs, k0 + kb * K, lane, scale, b[kb]);
     }
 #pragma unroll
     for (int kb = 0; kb < KB; ++kb)
@@ -970,8 +1037,9 @@ __device__ __forceinline__ void multi_panel_narrow(const PanelDesc &pd, const S 
 // the 16-byte path.
 template <typename S, bool kScaled, int NB, bool kHasTP>
 __device__ __forceinline__ void multi_tile_narrow(const TileDesc &td, const PanelDesc *__restrict__ panels,
-                                                  const S *__restrict__ vals, const double *__restrict__ X,
-                                                  const double *__restrict__ T,
+                                                  const S *__restrict__ vals,
+                                                  const typename VecOf<S>::T *__restrict__ X,
+                                                  const typename VecOf<S>::T *__restrict__ T,
                                                   const typename SlotOf<S>::T *__restrict__ TP,
                                                   const float *__restrict__ scales, const TileOut &o,
                                                   int lane) {
@@ -990,7 +1058,7 @@ __device__ __forceinline__ void multi_tile_narrow(const TileDesc &td, const Pane
     const PanelDesc pd = panels[td.panel0 + p];
     float scale = 1.0f;
     if constexpr (kScaled) scale = panel_scale(pd, scales);
-    const double *src = pd.vec_sel == kVecX ? X : T;
+    const typename VecOf<S>::T *src = pd.vec_sel == kVecX ? X : T;
     if (kHasTP && pd.vec_sel == kVecTP) {
       if (nlive == 2) multi_panel_narrow<S, NB, 2, kHasTP>(pd, vals, td.nrows, src, TP, scale, lane, acc);
       else multi_panel_narrow<S, NB, 1, kHasTP>(pd, vals, td.nrows, src, TP, scale, lane, acc);
@@ -1003,13 +1071,13 @@ __device__ __forceinline__ void multi_tile_narrow(const TileDesc &td, const Pane
   for (int ch = 0; ch < CH; ++ch)
     if (ch < nlive)
 #pragma unroll
-      for (int cb = 0; cb < NB; ++cb)
+      for (int v = 0; v < 4; ++v) {
+        const int row = 16 * ch + M::row(lane, v);
+        double r[NB];
 #pragma unroll
-        for (int v = 0; v < 4; ++v) {
-          const int row = 16 * ch + M::row(lane, v);
-          if (row < td.nrows)
-            store_row_multi<S, kScaled>(td, row, 16 * cb + (lane & 15), nrhs, (double)acc[ch][cb][v], o, scales);
-        }
+        for (int cb = 0; cb < NB; ++cb) r[cb] = (double)acc[ch][cb][v];
+        if (row < td.nrows) store_rows_multi<S, kScaled, NB>(td, row, NB * (lane & 15), nrhs, r, o, scales);
+      }
 }
 
 // nrhs = 16*NB; one wave per tile.  kHasTP: the plan has panels that sum
@@ -1017,8 +1085,9 @@ __device__ __forceinline__ void multi_tile_narrow(const TileDesc &td, const Pane
 template <typename S, bool kScaled, int NB, bool kHasTP>
 __global__ __launch_bounds__(256) void tile_multi_kernel(const TileDesc *__restrict__ tiles, int ntiles,
                                                          const PanelDesc *__restrict__ panels,
-                                                         const S *__restrict__ vals, const double *__restrict__ X,
-                                                         const double *__restrict__ T,
+                                                         const S *__restrict__ vals,
+                                                         const typename VecOf<S>::T *__restrict__ X,
+                                                         const typename VecOf<S>::T *__restrict__ T,
                                                          const typename SlotOf<S>::T *__restrict__ TP,
                                                          const float *__restrict__ scales, TileOut o) {
   using M = Mfma<S>;
@@ -1059,7 +1128,7 @@ __global__ __launch_bounds__(256) void tile_multi_kernel(const TileDesc *__restr
       const S *base[CH];
 #pragma unroll
       for (int ch = 0; ch < CH; ++ch) base[ch] = vals + (pd.val_off - shift) + (int64_t)V * grp[ch];
-      const double *src = pd.vec_sel == kVecX ? X : T;
+      const typename VecOf<S>::T *src = pd.vec_sel == kVecX ? X : T;
       if (kHasTP && pd.vec_sel == kVecTP)
         multi_panel<S, NB, kHasTP, CH>(pd, base, nlive, src, TP, scale, lane, acc);
       else
@@ -1071,14 +1140,16 @@ __global__ __launch_bounds__(256) void tile_multi_kernel(const TileDesc *__restr
 #pragma unroll
       for (int m = 0; m < V; ++m)
 #pragma unroll
-        for (int cb = 0; cb < NB; ++cb)
+        for (int v = 0; v < 4; ++v) {
+          // result row i of MFMA m: row V*i+m of the chunk, minus the shift;
+          // block cb holds right-hand side NB*(lane%16) + cb
+          const int row = V * (16 * ch + g0 + M::row(lane, v)) + m - shift;
+          double r[NB];
 #pragma unroll
-          for (int v = 0; v < 4; ++v) {
-            // result row i of MFMA m: row V*i+m of the chunk, minus the shift
-            const int row = V * (16 * ch + g0 + M::row(lane, v)) + m - shift;
-            if (row >= 0 && row < td.nrows)
-              store_row_multi<S, kScaled>(td, row, 16 * cb + (lane & 15), nrhs, (double)acc[ch][m][cb][v], o, scales);
-          }
+          for (int cb = 0; cb < NB; ++cb) r[cb] = (double)acc[ch][m][cb][v];
+          if (row >= 0 && row < td.nrows)
+            store_rows_multi<S, kScaled, NB>(td, row, NB * (lane & 15), nrhs, r, o, scales);
+        }
   }
 }
 
@@ -1086,13 +1157,13 @@ __global__ __launch_bounds__(256) void tile_multi_kernel(const TileDesc *__restr
 // e over entry i's list range (as segsum_kernel: up to 24 in flight).  A
 // thread sums W consecutive right-hand sides (W = 2: 16-byte loads and
 // stores, half the waves; buf and out 16-byte aligned, nrhs even).
-template <bool kScaled, int W, typename P>
+template <bool kScaled, int W, typename P, typename O>
 __global__ __launch_bounds__(256) void segsum_multi_kernel(int n, int nrhs, const int2 *__restrict__ rng,
                                                            const int64_t *__restrict__ list,
                                                            const P *__restrict__ buf,
                                                            const int32_t *__restrict__ factor_of,
                                                            const float *__restrict__ scales,
-                                                           double *__restrict__ out, int accumulate) {
+                                                           O *__restrict__ out, int accumulate) {
   static_assert(W == 1 || W == 2, "W is 1 or 2");
   // W slots of buf at a time (P = double or float: 16- or 8-byte loads)
   using VT = typename std::conditional<W == 2, typename std::conditional<sizeof(P) == 8, double2, float2>::type,
@@ -1127,11 +1198,11 @@ __global__ __launch_bounds__(256) void segsum_multi_kernel(int n, int nrhs, cons
   }
   double f = 1.0;
   if (kScaled) f = (double)scales[factor_of[i]] * (double)scales[factor_of[i] + 1];
-  double *o = out + (int64_t)i * nrhs + c;
+  O *o = out + (int64_t)i * nrhs + c;
 #pragma unroll
   for (int w = 0; w < W; ++w) {
     if (kScaled) a[w] *= f;
-    o[w] = accumulate ? o[w] + a[w] : a[w];
+    o[w] = (O)(accumulate ? (double)o[w] + a[w] : a[w]);
   }
 }
 
@@ -1151,6 +1222,23 @@ __global__ __launch_bounds__(256) void probe_copy_kernel(const double4 *__restri
                                                          size_t n) {
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
     dst[i] = src[i];
+}
+
+// dst[i] = (float)src[i] for n elements, W at a time (W = 2: src 16-byte and
+// dst 8-byte aligned, n even), grid-stride: the FP32 copy of a multi-RHS X.
+template <int W>
+__global__ __launch_bounds__(256) void to_float_kernel(const double *__restrict__ src, float *__restrict__ dst,
+                                                       size_t n) {
+  using DT = typename std::conditional<W == 2, double2, double>::type;
+  using FT = typename std::conditional<W == 2, float2, float>::type;
+  const size_t m = n / W;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < m; i += (size_t)gridDim.x * blockDim.x) {
+    const DT v = reinterpret_cast<const DT *>(src)[i];
+    if constexpr (W == 2)
+      reinterpret_cast<FT *>(dst)[i] = FT{(float)v.x, (float)v.y};
+    else
+      dst[i] = (float)v;
+  }
 }
 
 // Writes n double4 of dst (grid-stride): evicts L2 and the Infinity Cache
@@ -1196,6 +1284,68 @@ __global__ __launch_bounds__(256) void segsum_kernel(int n, const int2 *__restri
   // t entry of leaf b: both factor scales of b (a1 = factor_of, a2 = the next id)
   if (kScaled) a *= (double)scales[factor_of[i]] * (double)scales[factor_of[i] + 1];
   out[i] = accumulate ? out[i] + a : a;
+}
+
+// t sum with regular segments: t entry i adds nch(i) stage-1 partial slots
+// buf[first(i) + c * stride(i)], c = 0, 1, ... in order (a leaf's rank row
+// in each column chunk of its row group), desc[i] = {first, (nch << 16) |
+// stride}; the same sums in the same order as segsum_kernel over the list
+// first + c * stride, without the list loads.  Entries past nch reload the
+// last slot and add nothing (unguarded loads).
+template <bool kScaled, typename P>
+__global__ __launch_bounds__(256) void tsum_kernel(int n, const int2 *__restrict__ desc, const P *__restrict__ buf,
+                                                   const int32_t *__restrict__ factor_of,
+                                                   const float *__restrict__ scales, double *__restrict__ out) {
+  const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (i >= n) return;
+  const int2 d = desc[i];
+  const int nch = (int)((unsigned)d.y >> 16), stride = d.y & 0xFFFF;
+  double a = 0;
+  for (int c = 0; c < nch; c += 4) {
+    P v[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) v[u] = buf[d.x + (int64_t)(c + u < nch ? c + u : nch - 1) * stride];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) a += c + u < nch ? (double)v[u] : 0.0;
+  }
+  if (kScaled) a *= (double)scales[factor_of[i]] * (double)scales[factor_of[i] + 1];
+  out[i] = a;
+}
+
+// tsum_kernel for nrhs right-hand sides (rows of nrhs in buf and out): a
+// thread sums W consecutive right-hand sides with 16-byte loads (W = 2 for
+// FP64 slots, 4 for FP32).
+template <bool kScaled, typename P, typename O>
+__global__ __launch_bounds__(256) void tsum_multi_kernel(int n, int nrhs, const int2 *__restrict__ desc,
+                                                         const P *__restrict__ buf,
+                                                         const int32_t *__restrict__ factor_of,
+                                                         const float *__restrict__ scales, O *__restrict__ out) {
+  constexpr int W = 16 / sizeof(P);
+  const int nw = nrhs / W;
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= (int64_t)n * nw) return;
+  const int i = (int)(g / nw), col = (int)(g - (int64_t)i * nw) * W;
+  const int2 d = desc[i];
+  const int nch = (int)((unsigned)d.y >> 16), stride = d.y & 0xFFFF;
+  double a[W];
+#pragma unroll
+  for (int w = 0; w < W; ++w) a[w] = 0;
+  for (int c = 0; c < nch; c += 4) {
+    double v[4][W];
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+      load_n<W>(buf + (d.x + (int64_t)(c + u < nch ? c + u : nch - 1) * stride) * nrhs + col, v[u]);
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+      for (int w = 0; w < W; ++w) a[w] += c + u < nch ? v[u][w] : 0.0;
+  }
+  if (kScaled) {
+    const double f = (double)scales[factor_of[i]] * (double)scales[factor_of[i] + 1];
+#pragma unroll
+    for (int w = 0; w < W; ++w) a[w] *= f;
+  }
+  store_n<W>(out + (int64_t)i * nrhs + col, a);
 }
 
 // Per-chunk absolute maximum of the source elements (BF16 scale pass 1).

@@ -701,10 +701,16 @@ std::string verify_plan(const Plan &P, const LeafArrays &la) {
   for (int i = 0; i < P.tsum.n; i++) {
     const int s = P.tsum.seg[i];
     if (s < 0 || s >= ntseg) return "t segment id out of range";
-    for (int e = P.tsum.ptr[s]; e < P.tsum.ptr[s + 1]; e++) {
+    const int e0 = P.tsum.ptr[s], e1 = P.tsum.ptr[s + 1];
+    for (int e = e0; e < e1; e++) {
       const int64_t a = P.tsum.list[e] + i;
       if (a < 0 || a >= P.n_tp) return "t segment sum reads outside the slots";
+      // the device t sum walks a segment as first + c * stride
+      if (e > e0 + 1 && P.tsum.list[e] - P.tsum.list[e - 1] != P.tsum.list[e0 + 1] - P.tsum.list[e0])
+        return "t segment slots not at one stride";
     }
+    if (e1 - e0 > 0xFFFF || (e1 - e0 > 1 && P.tsum.list[e0 + 1] - P.tsum.list[e0] > 0xFFFF))
+      return "t segment with more than 65535 chunks or a stride above 65535";
   }
   for (int i = 0; i < P.ysum.n; i++) {
     const int s = P.ysum.seg[i];

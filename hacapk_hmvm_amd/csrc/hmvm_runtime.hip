@@ -101,9 +101,32 @@ std::vector<int2> seg_ranges(const SegSum &g) {
   return r;
 }
 
+// Per-entry {first slot, (nch << 16) | stride} of the t sum: every segment
+// lists its slots at one stride (a rank row's partial slots in the column
+// chunks of its row group are one row group apart).  False when a segment
+// does not, or its fields do not fit.
+bool t_descs(const SegSum &g, std::vector<int2> *out) {
+  out->assign(g.seg.size(), make_int2(0, 0));
+  for (size_t i = 0; i < g.seg.size(); i++) {
+    const int s = g.seg[i];
+    if (s < 0) continue;
+    const int e0 = g.ptr[s], e1 = g.ptr[s + 1], nch = e1 - e0;
+    if (nch == 0) continue;
+    const int64_t first = g.list[e0] + (int64_t)i;
+    const int64_t stride = nch > 1 ? g.list[e0 + 1] - g.list[e0] : 0;
+    if (nch > 0xFFFF || stride < 0 || stride > 0xFFFF || first < 0 || first > INT32_MAX) return false;
+    for (int e = e0 + 1; e < e1; e++)
+      if (g.list[e] != g.list[e0] + (int64_t)(e - e0) * stride) return false;
+    (*out)[i] = make_int2((int)first, (int)(((unsigned)nch << 16) | (unsigned)stride));
+  }
+  return true;
+}
+
 int elem_bytes(int storage) { return storage == kF64 ? 8 : storage == kF32 ? 4 : 2; }
 // Bytes of one slot (SlotOf<S>::T): FP64 for FP64 storage, FP32 otherwise.
 size_t slot_bytes(int storage) { return storage == kF64 ? 8 : 4; }
+// Bytes of one multi-RHS vector entry (VecOf<S>::T), likewise.
+size_t vec_bytes(int storage) { return storage == kF64 ? 8 : 4; }
 
 // N events, destroyed on every return path of the timing functions.
 template <int N>
@@ -140,7 +163,9 @@ struct hmvm_plan {
   // several right-hand sides: buffers sized for multi_cap columns
   int multi_cap = 0;
   void *d_mslots = nullptr;  // slot_bytes(storage) per slot
-  double *d_mt = nullptr, *d_mX = nullptr, *d_mY = nullptr;
+  void *d_mt = nullptr;      // t: vec_bytes(storage) per entry
+  float *d_mXf = nullptr;    // FP32 copy of X (FP32 and BF16 storage)
+  double *d_mX = nullptr, *d_mY = nullptr;
   PersistCounters *d_ctr = nullptr;
   int ncu = 0;     // compute units of the device
   int device = 0;
@@ -152,19 +177,19 @@ struct hmvm_plan {
   void *d_slots = nullptr;  // stage-1 partial slots, then output slots: slot_bytes(storage) each
   int32_t *d_tfac = nullptr;
   int32_t *d_tmap = nullptr;  // t_mode 3: t index of every stage-1 tile row
-  int64_t *d_tlist = nullptr;
   int32_t *d_yseg = nullptr, *d_yptr = nullptr;
   int64_t *d_ylist = nullptr;
-  int2 *d_trng = nullptr, *d_yrng = nullptr;  // per-entry list ranges of the segment sums
+  int2 *d_yrng = nullptr;   // per-entry list ranges of the y sum
+  int2 *d_tdesc = nullptr;  // t sum: {first slot, (nch << 16) | stride} per entry
   float *d_scales = nullptr;
   double *d_x = nullptr, *d_y = nullptr;  // staging for host-pointer calls
   size_t device_bytes = 0;
 
   ~hmvm_plan() {
     void *ptrs[] = {d_vals,  d_tiles_a, d_tiles_b, d_panels, d_t,      d_slots,
-                    d_tfac,  d_tmap,    d_tlist,   d_yseg,   d_yptr,   d_ylist,  d_scales, d_x,    d_y,
-                    d_trng,  d_yrng,
-                    d_ctr,   d_mslots,  d_mt,      d_mX,     d_mY};
+                    d_tfac,  d_tmap,    d_yseg,    d_yptr,   d_ylist,  d_scales, d_x,      d_y,
+                    d_yrng,  d_tdesc,
+                    d_ctr,   d_mslots,  d_mt,      d_mX,     d_mY,     d_mXf};
     for (void *p : ptrs)
       if (p) (void)hipFree(p);
     if (ev_fork) (void)hipEventDestroy(ev_fork);
@@ -753,11 +778,16 @@ int create_plan(Source &src, const hmvm_options *opt, hmvm_plan **out) {
   const size_t eb = (size_t)elem_bytes(p->storage);
   // schedule
   if ((rc = upload(&p->d_tiles_a, P.tiles_a)) || (rc = upload(&p->d_tiles_b, P.tiles_b)) ||
-      (rc = upload(&p->d_panels, P.panels)) || (rc = upload(&p->d_tlist, P.tsum.list)) ||
+      (rc = upload(&p->d_panels, P.panels)) ||
       (rc = upload(&p->d_yseg, P.ysum.seg)) || (rc = upload(&p->d_yptr, P.ysum.ptr)) ||
-      (rc = upload(&p->d_ylist, P.ysum.list)) || (rc = upload(&p->d_trng, seg_ranges(P.tsum))) ||
-      (rc = upload(&p->d_yrng, seg_ranges(P.ysum))))
+      (rc = upload(&p->d_ylist, P.ysum.list)) || (rc = upload(&p->d_yrng, seg_ranges(P.ysum))))
     return rc;
+  {
+    std::vector<int2> tdesc;
+    if (!t_descs(P.tsum, &tdesc))
+      return set_error(HMVM_ERR_UNSUPPORTED, "t sum segment with more than 65535 chunks or an irregular stride");
+    if ((rc = upload(&p->d_tdesc, tdesc))) return rc;
+  }
   if (p->mode != 0 && (rc = partition_a(p.get()))) return rc;
   if (P.t_direct && (rc = upload(&p->d_tmap, P.tmap))) return rc;
   if (p->storage == kBF16) {
@@ -781,7 +811,7 @@ int create_plan(Source &src, const hmvm_options *opt, hmvm_plan **out) {
                     sizeof(PanelDesc) * P.panels.size() + 8 * (size_t)(P.n_t + 2 * P.nd) +
                     slot_bytes(p->storage) * (size_t)(P.n_tp + P.n_z) +
                     4 * (P.ysum.seg.size() + P.ysum.ptr.size()) + 4 * (size_t)P.n_t +
-                    8 * (P.tsum.list.size() + P.ysum.list.size() + P.tsum.seg.size() + P.ysum.seg.size());
+                    8 * (P.ysum.list.size() + P.ysum.seg.size() + P.tsum.seg.size());
   if (verbose())
     std::fprintf(stderr,
                  "hmvm_hip: plan nd=%d nlf=%d len=%lld storage=%d tiles=%zu+%zu panels=%zu t=%lld tslots=%lld "
@@ -794,7 +824,7 @@ int create_plan(Source &src, const hmvm_options *opt, hmvm_plan **out) {
 
 // Where the tiles of one launch store their rows: the slots, and for
 // stage-1 tiles with a row map the plan's t (or the multi-RHS T).
-TileOut tile_out(const hmvm_plan *p, void *out, double *tout) {
+TileOut tile_out(const hmvm_plan *p, void *out, void *tout) {
   return TileOut{out, p->d_tmap, tout, p->d_tfac};
 }
 
@@ -917,12 +947,13 @@ void launch_segsum(hmvm_plan *p, int phase, double *y, int accumulate, hipStream
   const P *slots = (const P *)p->d_slots;
   if (phase == 1) {
     if (P_.t_fused || P_.tsum.n == 0) return;
+    const dim3 grid((P_.tsum.n + 255) / 256);
     if (p->storage == kBF16)
-      hipLaunchKernelGGL((segsum_kernel<true, P>), dim3((P_.tsum.n + 255) / 256), dim3(256), 0, st, P_.tsum.n,
-                         p->d_trng, p->d_tlist, slots, p->d_tfac, p->d_scales, p->d_t, 0);
+      hipLaunchKernelGGL((tsum_kernel<true, P>), grid, dim3(256), 0, st, P_.tsum.n, p->d_tdesc, slots, p->d_tfac,
+                         p->d_scales, p->d_t);
     else
-      hipLaunchKernelGGL((segsum_kernel<false, P>), dim3((P_.tsum.n + 255) / 256), dim3(256), 0, st, P_.tsum.n,
-                         p->d_trng, p->d_tlist, slots, (const int32_t *)nullptr, (const float *)nullptr, p->d_t, 0);
+      hipLaunchKernelGGL((tsum_kernel<false, P>), grid, dim3(256), 0, st, P_.tsum.n, p->d_tdesc, slots,
+                         (const int32_t *)nullptr, (const float *)nullptr, p->d_t);
   } else {
     hipLaunchKernelGGL((segsum_kernel<false, P>), dim3((P_.ysum.n + 255) / 256), dim3(256), 0, st, P_.ysum.n,
                        p->d_yrng, p->d_ylist, slots, (const int32_t *)nullptr, (const float *)nullptr, y,
@@ -954,16 +985,19 @@ int reserve_multi(hmvm_plan *p, int nrhs) {
   if (nrhs != 16 && nrhs != 32 && nrhs != 64) return set_error(HMVM_ERR_ARG, "nrhs must be 16, 32 or 64");
   if (p->multi_cap >= nrhs) return HMVM_OK;
   const Plan &P = p->plan;
-  void *bufs[4] = {p->d_mslots, p->d_mt, p->d_mX, p->d_mY};
+  void *bufs[5] = {p->d_mslots, p->d_mt, p->d_mX, p->d_mY, p->d_mXf};
   for (void *b : bufs)
     if (b) (void)hipFree(b);
-  p->d_mslots = p->d_mt = p->d_mX = p->d_mY = nullptr;
+  p->d_mslots = p->d_mt = nullptr;
+  p->d_mX = p->d_mY = nullptr;
+  p->d_mXf = nullptr;
   p->multi_cap = 0;
   int rc;
   if ((rc = dalloc_bytes(&p->d_mslots, slot_bytes(p->storage) * (size_t)(P.n_tp + P.n_z) * nrhs)) ||
-      (rc = dalloc(&p->d_mt, (size_t)P.n_t * nrhs)) ||
+      (rc = dalloc_bytes(&p->d_mt, vec_bytes(p->storage) * (size_t)P.n_t * nrhs)) ||
       (rc = dalloc(&p->d_mX, (size_t)P.nd * nrhs)) || (rc = dalloc(&p->d_mY, (size_t)P.nd * nrhs)))
     return rc;
+  if (p->storage != kF64 && (rc = dalloc(&p->d_mXf, (size_t)P.nd * nrhs))) return rc;
   p->multi_cap = nrhs;
   return HMVM_OK;
 }
@@ -971,12 +1005,14 @@ int reserve_multi(hmvm_plan *p, int nrhs) {
 // T: the t read by stage-2 panels (launch B; NULL in launch A, whose
 // stage-1 tiles with a row map write it instead)
 template <typename S, bool kScaled, int NB>
-void launch_multi_tiles(hmvm_plan *p, const TileDesc *d_tiles, int ntiles, const double *X, const double *T,
+void launch_multi_tiles(hmvm_plan *p, const TileDesc *d_tiles, int ntiles, const void *Xv, const void *Tv,
                         hipStream_t st) {
   if (ntiles == 0) return;
   const int wpb = p->block_threads / 64;
   const TileOut o = tile_out(p, p->d_mslots, p->d_mt);
   const auto *TP = (const typename SlotOf<S>::T *)p->d_mslots;
+  const auto *X = (const typename VecOf<S>::T *)Xv;
+  const auto *T = (const typename VecOf<S>::T *)Tv;
   if (p->has_tp)
     hipLaunchKernelGGL((tile_multi_kernel<S, kScaled, NB, true>), dim3((ntiles + wpb - 1) / wpb),
                        dim3(p->block_threads), 0, st, d_tiles, ntiles, p->d_panels, (const S *)p->d_vals, X, T,
@@ -988,7 +1024,7 @@ void launch_multi_tiles(hmvm_plan *p, const TileDesc *d_tiles, int ntiles, const
 }
 
 template <typename S, bool kScaled>
-void launch_multi_nb(hmvm_plan *p, const TileDesc *d_tiles, int ntiles, const double *X, const double *T, int nrhs,
+void launch_multi_nb(hmvm_plan *p, const TileDesc *d_tiles, int ntiles, const void *X, const void *T, int nrhs,
                      hipStream_t st) {
   if (nrhs == 16)
     launch_multi_tiles<S, kScaled, 1>(p, d_tiles, ntiles, X, T, st);
@@ -1004,26 +1040,27 @@ void launch_segsum_multi(hmvm_plan *p, int phase, double *Y, int nrhs, int accum
   const Plan &P_ = p->plan;
   const P *slots = (const P *)p->d_mslots;
   if (phase == 1) {
-    // two columns per thread (measured 15-20% faster than one at N=101,250);
-    // slots and t are the plan's own allocations, nrhs is even
+    // 16-byte loads and stores (slots and t are the plan's own allocations)
     if (P_.t_fused || P_.tsum.n == 0) return;
-    const int64_t tot = (int64_t)P_.tsum.n * (nrhs / 2);
+    // t in the multi-RHS vector type (VecOf: the slot type here)
+    P *T = (P *)p->d_mt;
+    const int64_t tw = (int64_t)P_.tsum.n * (nrhs / (16 / (int)sizeof(P)));
+    const dim3 grid((unsigned)((tw + 255) / 256));
     if (p->storage == kBF16)
-      hipLaunchKernelGGL((segsum_multi_kernel<true, 2, P>), dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st,
-                         P_.tsum.n, nrhs, p->d_trng, p->d_tlist, slots, p->d_tfac, p->d_scales, p->d_mt, 0);
+      hipLaunchKernelGGL((tsum_multi_kernel<true, P, P>), grid, dim3(256), 0, st, P_.tsum.n, nrhs, p->d_tdesc, slots,
+                         p->d_tfac, p->d_scales, T);
     else
-      hipLaunchKernelGGL((segsum_multi_kernel<false, 2, P>), dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st,
-                         P_.tsum.n, nrhs, p->d_trng, p->d_tlist, slots, (const int32_t *)nullptr,
-                         (const float *)nullptr, p->d_mt, 0);
+      hipLaunchKernelGGL((tsum_multi_kernel<false, P, P>), grid, dim3(256), 0, st, P_.tsum.n, nrhs, p->d_tdesc,
+                         slots, (const int32_t *)nullptr, (const float *)nullptr, T);
   } else if (((uintptr_t)Y & 15) == 0) {  // two columns per thread where the caller's Y is 16-byte aligned
     const int64_t tot = (int64_t)P_.ysum.n * (nrhs / 2);
-    hipLaunchKernelGGL((segsum_multi_kernel<false, 2, P>), dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st,
-                       P_.ysum.n, nrhs, p->d_yrng, p->d_ylist, slots, (const int32_t *)nullptr,
+    hipLaunchKernelGGL((segsum_multi_kernel<false, 2, P, double>), dim3((unsigned)((tot + 255) / 256)), dim3(256),
+                       0, st, P_.ysum.n, nrhs, p->d_yrng, p->d_ylist, slots, (const int32_t *)nullptr,
                        (const float *)nullptr, Y, accumulate);
   } else {
     const int64_t tot = (int64_t)P_.ysum.n * nrhs;
-    hipLaunchKernelGGL((segsum_multi_kernel<false, 1, P>), dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st,
-                       P_.ysum.n, nrhs, p->d_yrng, p->d_ylist, slots, (const int32_t *)nullptr,
+    hipLaunchKernelGGL((segsum_multi_kernel<false, 1, P, double>), dim3((unsigned)((tot + 255) / 256)), dim3(256),
+                       0, st, P_.ysum.n, nrhs, p->d_yrng, p->d_ylist, slots, (const int32_t *)nullptr,
                        (const float *)nullptr, Y, accumulate);
   }
 }
@@ -1034,13 +1071,33 @@ void launch_multi_phase(hmvm_plan *p, int phase, const double *X, double *Y, int
   if (phase == 0 || phase == 2) {
     const TileDesc *tl = phase == 0 ? p->d_tiles_a : p->d_tiles_b;
     const int nt = (int)(phase == 0 ? P.tiles_a.size() : P.tiles_b.size());
-    const double *T = phase == 0 ? nullptr : p->d_mt;
+    const void *T = phase == 0 ? nullptr : p->d_mt;
+    if (p->storage != kF64) {
+      // FP32 and BF16 operands are FP32-rounded X and t: the tiles read an
+      // FP32 copy of X (made here, before launch A) and an FP32 t
+      if (phase == 0) {
+        const size_t n = (size_t)P.nd * nrhs;
+        const unsigned blocks = (unsigned)std::min<size_t>((n / 2 + 255) / 256, 8192);
+        if (((uintptr_t)X & 15) == 0)
+          hipLaunchKernelGGL(to_float_kernel<2>, dim3(blocks), dim3(256), 0, st, X, p->d_mXf, n);
+        else
+          hipLaunchKernelGGL(to_float_kernel<1>, dim3(2 * blocks), dim3(256), 0, st, X, p->d_mXf, n);
+      }
+    }
+    if (p->storage == kF64 && nrhs > 16 && ((uintptr_t)X & 15) != 0) {
+      // the tiles load two or four right-hand sides at once (16-byte loads):
+      // an X that is not 16-byte aligned is first copied to the plan's own
+      if (phase == 0)
+        (void)hipMemcpyAsync(p->d_mX, X, sizeof(double) * (size_t)P.nd * nrhs, hipMemcpyDeviceToDevice, st);
+      X = p->d_mX;
+    }
+    const void *Xv = p->storage == kF64 ? (const void *)X : (const void *)p->d_mXf;
     if (p->storage == kF64)
-      launch_multi_nb<double, false>(p, tl, nt, X, T, nrhs, st);
+      launch_multi_nb<double, false>(p, tl, nt, Xv, T, nrhs, st);
     else if (p->storage == kF32)
-      launch_multi_nb<float, false>(p, tl, nt, X, T, nrhs, st);
+      launch_multi_nb<float, false>(p, tl, nt, Xv, T, nrhs, st);
     else
-      launch_multi_nb<uint16_t, true>(p, tl, nt, X, T, nrhs, st);
+      launch_multi_nb<uint16_t, true>(p, tl, nt, Xv, T, nrhs, st);
   } else if (p->storage == kF64) {
     launch_segsum_multi<double>(p, phase, Y, nrhs, accumulate, st);
   } else {

@@ -83,34 +83,38 @@ def test_multi_rhs_partial_slot_panels(gpu, H, hm, XY, storage, tol, t_mode):
 
 def test_multi_rhs_device_pointers_any_alignment(gpu, H, hm, XY):
     """hmvm_plan_apply_multi on caller device buffers: a Y that is only
-    8-byte aligned takes the one-column y sum, and both equal the host entry
-    point bit for bit (same sums in the same order).  Device buffers come
-    from the HIP runtime the library itself links."""
+    8-byte aligned takes the one-column y sum, an X that is only 8-byte
+    aligned is copied before the 16-byte operand loads, and every case equals
+    the host entry point bit for bit (same sums in the same order).  Device
+    buffers come from the HIP runtime the library itself links."""
     import ctypes
     with open("/proc/self/maps") as f:  # the HIP runtime the library loaded (not torch's copy)
         paths = {ln.split()[-1] for ln in f if "libamdhip64" in ln and "torch" not in ln}
     assert paths, "the HIP runtime is not loaded"
     hip = ctypes.CDLL(sorted(paths)[0])
     X, _ = XY
-    p = H.Plan(hm)
-    want = p.apply_multi_host(X)
     n = hm.nd * 32
-    dX, dbuf = ctypes.c_void_p(), ctypes.c_void_p()
-    assert hip.hipMalloc(ctypes.byref(dX), ctypes.c_size_t(8 * n)) == 0
+    dxbuf, dbuf = ctypes.c_void_p(), ctypes.c_void_p()
+    assert hip.hipMalloc(ctypes.byref(dxbuf), ctypes.c_size_t(8 * (n + 1))) == 0
     assert hip.hipMalloc(ctypes.byref(dbuf), ctypes.c_size_t(8 * (n + 1))) == 0
     try:
         Xc = np.ascontiguousarray(X)
-        assert hip.hipMemcpy(dX, Xc.ctypes.data_as(ctypes.c_void_p), ctypes.c_size_t(8 * n), 1) == 0  # H2D
-        assert H.lib.hmvm_plan_reserve_multi(p._p, 32) == 0
-        for off in (0, 1):
-            dY = ctypes.c_void_p(dbuf.value + 8 * off)
-            assert (dY.value % 16 == 0) == (off == 0)
-            assert H.lib.hmvm_plan_apply_multi(p._p, dX, dY, 32, 0, None) == 0
-            assert hip.hipDeviceSynchronize() == 0
-            got = np.empty((hm.nd, 32))
-            assert hip.hipMemcpy(got.ctypes.data_as(ctypes.c_void_p), dY, ctypes.c_size_t(8 * n), 2) == 0  # D2H
-            assert np.array_equal(got, want)
+        for storage in ("f64", "f32"):
+            p = H.Plan(hm, storage=storage)
+            want = p.apply_multi_host(X)
+            assert H.lib.hmvm_plan_reserve_multi(p._p, 32) == 0
+            for xoff in (0, 1):
+                dX = ctypes.c_void_p(dxbuf.value + 8 * xoff)
+                assert hip.hipMemcpy(dX, Xc.ctypes.data_as(ctypes.c_void_p), ctypes.c_size_t(8 * n), 1) == 0  # H2D
+                for off in (0, 1):
+                    dY = ctypes.c_void_p(dbuf.value + 8 * off)
+                    assert (dY.value % 16 == 0) == (off == 0) and (dX.value % 16 == 0) == (xoff == 0)
+                    assert H.lib.hmvm_plan_apply_multi(p._p, dX, dY, 32, 0, None) == 0
+                    assert hip.hipDeviceSynchronize() == 0
+                    got = np.empty((hm.nd, 32))
+                    assert hip.hipMemcpy(got.ctypes.data_as(ctypes.c_void_p), dY, ctypes.c_size_t(8 * n), 2) == 0
+                    assert np.array_equal(got, want), (storage, xoff, off)
+            p.close()
     finally:
-        hip.hipFree(dX)
+        hip.hipFree(dxbuf)
         hip.hipFree(dbuf)
-        p.close()
