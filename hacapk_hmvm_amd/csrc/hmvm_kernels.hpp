@@ -1153,21 +1153,20 @@ __global__ __launch_bounds__(256) void tile_multi_kernel(const TileDesc *__restr
   }
 }
 
-// out[i*nrhs+c] = (accumulate ? out : 0) + scale(i) * sum_e buf[(list[e]+i)*nrhs + c],
-// e over entry i's list range (as segsum_kernel: up to 24 in flight).  A
-// thread sums W consecutive right-hand sides (W = 2: 16-byte loads and
-// stores, half the waves; buf and out 16-byte aligned, nrhs even).
-template <bool kScaled, int W, typename P, typename O>
-__global__ __launch_bounds__(256) void segsum_multi_kernel(int n, int nrhs, const int2 *__restrict__ rng,
-                                                           const int64_t *__restrict__ list,
-                                                           const P *__restrict__ buf,
-                                                           const int32_t *__restrict__ factor_of,
-                                                           const float *__restrict__ scales,
-                                                           O *__restrict__ out, int accumulate) {
-  static_assert(W == 1 || W == 2, "W is 1 or 2");
-  // W slots of buf at a time (P = double or float: 16- or 8-byte loads)
-  using VT = typename std::conditional<W == 2, typename std::conditional<sizeof(P) == 8, double2, float2>::type,
-                                       P>::type;
+// y sum for nrhs right-hand sides: out[i*nrhs+c] = (accumulate ? out : 0) +
+// sum_e buf[(list[e]+i)*nrhs + c], e over entry i's list range in list order
+// (segsum_kernel's sums).  A thread sums W consecutive right-hand sides: W =
+// 16 / sizeof(P) (16-byte slot loads) when out is 16-byte aligned, else 1.
+// Up to U list entries and their slots are in flight at once (the y lists
+// hold 11 to 19 FP64 and 14 to 23 FP32 slots at N=101,250; measured: U = 24
+// for FP64, 16 for FP32).  The slots stay in their own type until summed, so
+// that the guarded loads are all issued before the first wait.
+template <int W, typename P>
+__global__ __launch_bounds__(256) void ysum_multi_kernel(int n, int nrhs, const int2 *__restrict__ rng,
+                                                         const int64_t *__restrict__ list,
+                                                         const P *__restrict__ buf, double *__restrict__ out,
+                                                         int accumulate) {
+  constexpr int U = sizeof(P) == 8 ? 24 : 16;
   const int nw = nrhs / W;
   const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= (int64_t)n * nw) return;
@@ -1177,33 +1176,40 @@ __global__ __launch_bounds__(256) void segsum_multi_kernel(int n, int nrhs, cons
   double a[W];
 #pragma unroll
   for (int w = 0; w < W; ++w) a[w] = 0;
-  for (int e = r.x; e < e1; e += 24) {
-    int64_t l[24];
-    VT v[24];
+  using Raw = typename std::conditional<
+      W == 1, P, typename std::conditional<sizeof(P) == 8, double2, float4>::type>::type;
+  static_assert(W == 1 || sizeof(Raw) == 16, "16-byte slot loads");
+  for (int e = r.x; e < e1; e += U) {
+    int64_t l[U];
+    Raw v[U];
 #pragma unroll
-    for (int u = 0; u < 24; ++u) l[u] = e + u < e1 ? list[e + u] : 0;
+    for (int u = 0; u < U; ++u) l[u] = e + u < e1 ? list[e + u] : 0;
 #pragma unroll
-    for (int u = 0; u < 24; ++u)
-      v[u] = e + u < e1 ? *reinterpret_cast<const VT *>(buf + (l[u] + i) * nrhs + c) : VT{};
+    for (int u = 0; u < U; ++u) v[u] = e + u < e1 ? *reinterpret_cast<const Raw *>(buf + (l[u] + i) * nrhs + c) : Raw{};
 #pragma unroll
-    for (int u = 0; u < 24; ++u)
-      if (e + u < e1) {
-        if constexpr (W == 2) {
-          a[0] += (double)v[u].x;
-          a[1] += (double)v[u].y;
-        } else {
-          a[0] += (double)v[u];
-        }
+    for (int u = 0; u < U; ++u) {
+      const bool on = e + u < e1;
+      if constexpr (W == 1) {
+        a[0] += on ? (double)v[u] : 0.0;
+      } else if constexpr (W == 2) {
+        a[0] += on ? (double)v[u].x : 0.0;
+        a[1] += on ? (double)v[u].y : 0.0;
+      } else {
+        a[0] += on ? (double)v[u].x : 0.0;
+        a[1] += on ? (double)v[u].y : 0.0;
+        a[2] += on ? (double)v[u].z : 0.0;
+        a[3] += on ? (double)v[u].w : 0.0;
       }
+    }
   }
-  double f = 1.0;
-  if (kScaled) f = (double)scales[factor_of[i]] * (double)scales[factor_of[i] + 1];
-  O *o = out + (int64_t)i * nrhs + c;
+  double *o = out + (int64_t)i * nrhs + c;
+  if (accumulate) {
+    double prev[W];
+    load_n<W>(o, prev);
 #pragma unroll
-  for (int w = 0; w < W; ++w) {
-    if (kScaled) a[w] *= f;
-    o[w] = (O)(accumulate ? (double)o[w] + a[w] : a[w]);
+    for (int w = 0; w < W; ++w) a[w] = prev[w] + a[w];
   }
+  store_n<W>(o, a);
 }
 
 // Achievable-bandwidth probes: a read-only stream (sum) and a copy, each

@@ -1052,16 +1052,15 @@ void launch_segsum_multi(hmvm_plan *p, int phase, double *Y, int nrhs, int accum
     else
       hipLaunchKernelGGL((tsum_multi_kernel<false, P, P>), grid, dim3(256), 0, st, P_.tsum.n, nrhs, p->d_tdesc,
                          slots, (const int32_t *)nullptr, (const float *)nullptr, T);
-  } else if (((uintptr_t)Y & 15) == 0) {  // two columns per thread where the caller's Y is 16-byte aligned
-    const int64_t tot = (int64_t)P_.ysum.n * (nrhs / 2);
-    hipLaunchKernelGGL((segsum_multi_kernel<false, 2, P, double>), dim3((unsigned)((tot + 255) / 256)), dim3(256),
-                       0, st, P_.ysum.n, nrhs, p->d_yrng, p->d_ylist, slots, (const int32_t *)nullptr,
-                       (const float *)nullptr, Y, accumulate);
+  } else if (((uintptr_t)Y & 15) == 0) {  // 16-byte slot loads where the caller's Y is 16-byte aligned
+    constexpr int W = 16 / sizeof(P);
+    const int64_t tot = (int64_t)P_.ysum.n * (nrhs / W);
+    hipLaunchKernelGGL((ysum_multi_kernel<W, P>), dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, P_.ysum.n,
+                       nrhs, p->d_yrng, p->d_ylist, slots, Y, accumulate);
   } else {
     const int64_t tot = (int64_t)P_.ysum.n * nrhs;
-    hipLaunchKernelGGL((segsum_multi_kernel<false, 1, P, double>), dim3((unsigned)((tot + 255) / 256)), dim3(256),
-                       0, st, P_.ysum.n, nrhs, p->d_yrng, p->d_ylist, slots, (const int32_t *)nullptr,
-                       (const float *)nullptr, Y, accumulate);
+    hipLaunchKernelGGL((ysum_multi_kernel<1, P>), dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, P_.ysum.n,
+                       nrhs, p->d_yrng, p->d_ylist, slots, Y, accumulate);
   }
 }
 
