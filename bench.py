@@ -256,6 +256,9 @@ def run_c1(args, H, np):
 def run_c3(args, H, np):
     import oracle as O
     nrhs = 32
+    # (the probe kernels also calibrate the FETCH_SIZE / WRITE_SIZE counter runs of this command)
+    probe_read, probe_copy = H.bandwidth_probe(1 << 30, 5)
+    log("bandwidth probe: read %.0f GB/s, copy %.0f GB/s" % (probe_read, probe_copy))
     h = H.generate(101250, geometry="cube", eps=1e-4)
     X = np.random.default_rng(20261204).uniform(-1.0, 1.0, (h.nd, nrhs))
     t0 = time.time()

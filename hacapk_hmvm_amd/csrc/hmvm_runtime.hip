@@ -165,6 +165,8 @@ struct hmvm_plan {
   void *d_mslots = nullptr;  // slot_bytes(storage) per slot
   void *d_mt = nullptr;      // t: vec_bytes(storage) per entry
   float *d_mXf = nullptr;    // FP32 copy of X (FP32 and BF16 storage)
+  TileDesc *d_mtiles_a = nullptr, *d_mtiles_b = nullptr;  // XCD-aware tile layouts of launches A and B
+  int mtiles_wpb = 0;                                     // waves per workgroup they were laid out for
   double *d_mX = nullptr, *d_mY = nullptr;
   PersistCounters *d_ctr = nullptr;
   int ncu = 0;     // compute units of the device
@@ -188,7 +190,7 @@ struct hmvm_plan {
   ~hmvm_plan() {
     void *ptrs[] = {d_vals,  d_tiles_a, d_tiles_b, d_panels, d_t,      d_slots,
                     d_tfac,  d_tmap,    d_yseg,    d_yptr,   d_ylist,  d_scales, d_x,      d_y,
-                    d_yrng,  d_tdesc,
+                    d_yrng,  d_tdesc,   d_mtiles_a, d_mtiles_b,
                     d_ctr,   d_mslots,  d_mt,      d_mX,     d_mY,     d_mXf};
     for (void *p : ptrs)
       if (p) (void)hipFree(p);
@@ -981,8 +983,73 @@ void launch_phase(hmvm_plan *p, int phase, const double *x, double *y, int accum
 }
 
 // ---- several right-hand sides -------------------------------------------------
+// Workgroup w of a launch runs on XCD w % kXcds: MI355X hands the workgroups
+// of a grid to its 8 XCDs round robin, and each XCD has its own L2.  A
+// multi-RHS tile reads a slice of X (launch A) or t (launch B) of nrhs
+// entries per column, which other tiles over the same columns read again.
+// Sorted by the start of that slice, the tiles are cut into kXcds runs of
+// equal work (elements), and laid out so that the workgroups of XCD k take
+// run k's tiles, largest first: each XCD then reads about one kXcds-th of the
+// vector, which its L2 holds, instead of all of it.  A workgroup whose run is
+// used up takes tiles of the run with the most left.
+constexpr int kXcds = 8;
+std::vector<TileDesc> xcd_layout(const std::vector<TileDesc> &tiles, int wpb) {
+  const size_t n = tiles.size();
+  std::vector<size_t> idx(n);
+  for (size_t i = 0; i < n; i++) idx[i] = i;
+  std::stable_sort(idx.begin(), idx.end(),
+                   [&](size_t a, size_t b) { return tiles[a].first.vec_off < tiles[b].first.vec_off; });
+  auto work = [&](size_t i) { return (double)tiles[i].nrows * (double)tiles[i].ncols; };
+  double total = 0;
+  for (size_t i = 0; i < n; i++) total += work(i);
+  std::vector<std::vector<size_t>> run(kXcds);
+  double cum = 0;
+  for (size_t i : idx) {
+    const int r = total > 0 ? std::min(kXcds - 1, (int)((cum + 0.5 * work(i)) * kXcds / total)) : 0;
+    run[r].push_back(i);
+    cum += work(i);
+  }
+  for (auto &r : run)
+    std::stable_sort(r.begin(), r.end(), [&](size_t a, size_t b) { return work(a) > work(b); });
+  std::vector<TileDesc> out;
+  out.reserve(n);
+  std::vector<size_t> pos(kXcds, 0);
+  for (int w = 0; out.size() < n; w++) {
+    for (int k = 0; k < wpb && out.size() < n; k++) {
+      int r = w % kXcds;
+      if (pos[r] == run[r].size()) {  // run used up: the run with the most tiles left
+        for (int q = 0; q < kXcds; q++)
+          if (run[q].size() - pos[q] > run[r].size() - pos[r]) r = q;
+      }
+      out.push_back(tiles[run[r][pos[r]++]]);
+    }
+  }
+  return out;
+}
+
+// Uploads the XCD-aware layouts of both launches for the plan's current
+// workgroup size (again when that changes).
+int multi_layout(hmvm_plan *p) {
+  const int wpb = p->block_threads / 64;
+  if (p->mtiles_wpb == wpb && p->d_mtiles_a) return HMVM_OK;
+  for (TileDesc **d : {&p->d_mtiles_a, &p->d_mtiles_b})
+    if (*d) {
+      (void)hipFree(*d);
+      *d = nullptr;
+    }
+  p->mtiles_wpb = 0;
+  int rc;
+  if ((rc = upload(&p->d_mtiles_a, xcd_layout(p->plan.tiles_a, wpb))) ||
+      (rc = upload(&p->d_mtiles_b, xcd_layout(p->plan.tiles_b, wpb))))
+    return rc;
+  p->mtiles_wpb = wpb;
+  return HMVM_OK;
+}
+
 int reserve_multi(hmvm_plan *p, int nrhs) {
   if (nrhs != 16 && nrhs != 32 && nrhs != 64) return set_error(HMVM_ERR_ARG, "nrhs must be 16, 32 or 64");
+  int rc = multi_layout(p);
+  if (rc) return rc;
   if (p->multi_cap >= nrhs) return HMVM_OK;
   const Plan &P = p->plan;
   void *bufs[5] = {p->d_mslots, p->d_mt, p->d_mX, p->d_mY, p->d_mXf};
@@ -992,7 +1059,6 @@ int reserve_multi(hmvm_plan *p, int nrhs) {
   p->d_mX = p->d_mY = nullptr;
   p->d_mXf = nullptr;
   p->multi_cap = 0;
-  int rc;
   if ((rc = dalloc_bytes(&p->d_mslots, slot_bytes(p->storage) * (size_t)(P.n_tp + P.n_z) * nrhs)) ||
       (rc = dalloc_bytes(&p->d_mt, vec_bytes(p->storage) * (size_t)P.n_t * nrhs)) ||
       (rc = dalloc(&p->d_mX, (size_t)P.nd * nrhs)) || (rc = dalloc(&p->d_mY, (size_t)P.nd * nrhs)))
@@ -1068,7 +1134,7 @@ void launch_multi_phase(hmvm_plan *p, int phase, const double *X, double *Y, int
                         hipStream_t st) {
   const Plan &P = p->plan;
   if (phase == 0 || phase == 2) {
-    const TileDesc *tl = phase == 0 ? p->d_tiles_a : p->d_tiles_b;
+    const TileDesc *tl = phase == 0 ? p->d_mtiles_a : p->d_mtiles_b;
     const int nt = (int)(phase == 0 ? P.tiles_a.size() : P.tiles_b.size());
     const void *T = phase == 0 ? nullptr : p->d_mt;
     if (p->storage != kF64) {
@@ -1106,6 +1172,8 @@ void launch_multi_phase(hmvm_plan *p, int phase, const double *X, double *Y, int
 
 int apply_multi(hmvm_plan *p, const double *X, double *Y, int nrhs, int accumulate, hipStream_t st) {
   if (p->multi_cap < nrhs) return set_error(HMVM_ERR_ARG, "call hmvm_plan_reserve_multi(plan, %d) first", nrhs);
+  int rc = multi_layout(p);  // no-op unless the workgroup size changed since hmvm_plan_reserve_multi
+  if (rc) return rc;
   for (int ph = 0; ph < 4; ph++) launch_multi_phase(p, ph, X, Y, nrhs, accumulate, st);
   HIP_TRY(hipGetLastError());
   return HMVM_OK;

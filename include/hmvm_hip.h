@@ -201,10 +201,16 @@ int hmvm_plan_reset_launch(hmvm_plan *plan);
  * nd x nrhs doubles, row-major (X[i*nrhs + c]); nrhs = 16, 32 or 64.  FP64
  * storage runs v_mfma_f64_16x16x4, FP32 v_mfma_f32_16x16x4 (FP32 accumulate
  * in a tile, FP64 across tiles), BF16 v_mfma_f32_16x16x16_bf16 with the
- * vector entries rounded to BF16.  hmvm_plan_reserve_multi allocates the
- * per-plan work buffers for up to nrhs columns (the apply itself allocates
- * nothing); the host form reserves by itself.  d_X and d_Y need 8-byte
- * alignment; a 16-byte-aligned d_Y gets 16-byte stores (same values). */
+ * vector entries rounded to BF16 (FP32 and BF16 storage read an FP32 copy
+ * of X that the apply makes on the device).  hmvm_plan_reserve_multi
+ * allocates the per-plan work buffers for up to nrhs columns and lays the
+ * tiles out for the 8 XCDs (workgroups over the same columns of X share an
+ * L2); the apply itself allocates nothing unless the plan's workgroup size
+ * changed since (it then lays the tiles out again, a synchronous upload:
+ * call hmvm_plan_reserve_multi again before capturing the apply in a graph).
+ * The host form reserves by itself.  d_X and d_Y need 8-byte alignment; a
+ * 16-byte-aligned d_Y gets 16-byte stores, and an FP64 plan copies a d_X
+ * that is not 16-byte aligned to its own buffer first (same values). */
 int hmvm_plan_reserve_multi(hmvm_plan *plan, int nrhs);
 int hmvm_plan_apply_multi(hmvm_plan *plan, const double *d_X, double *d_Y, int nrhs, int accumulate, void *stream);
 int hmvm_plan_apply_multi_host(hmvm_plan *plan, const double *X, double *Y, int nrhs, int accumulate);
