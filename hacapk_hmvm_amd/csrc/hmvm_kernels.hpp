@@ -46,13 +46,12 @@ struct SlotOf { using T = float; };
 template <>
 struct SlotOf<double> { using T = double; };
 
-// Element type of the multi-RHS vectors the tiles read (the FP32 copy of X
-// and t): the MFMA operand type's FP32 for FP32 and BF16 storage, whose
-// operands are FP32-rounded anyway, FP64 for FP64 storage.
+// Element type of the multi-RHS vectors the tiles read (the copy of X and
+// t): the MFMA operand type, i.e. FP64, FP32, and for BF16 storage the BF16
+// bits of the FP32-rounded value (round to nearest even twice, as the
+// operand conversion of the FP32 value would do).
 template <typename S>
-struct VecOf { using T = float; };
-template <>
-struct VecOf<double> { using T = double; };
+struct VecOf { using T = S; };
 
 // BF16 values are kept as raw uint16_t bit patterns.
 __device__ __forceinline__ double load_val(const double *p) { return *p; }
@@ -785,9 +784,29 @@ __device__ __forceinline__ short bf16_bits(float f) {
 
 // NB consecutive elements at p (p aligned to NB elements): one 4-, 8- or
 // 16-byte access per 16 bytes.
+__device__ __forceinline__ double from_bf16(uint32_t b) { return (double)__uint_as_float(b << 16); }
+__device__ __forceinline__ uint32_t to_bf16(double v) { return (uint32_t)(uint16_t)bf16_bits((float)v); }
+
 template <int NB, typename T>
 __device__ __forceinline__ void load_n(const T *__restrict__ p, double (&v)[NB]) {
-  if constexpr (NB == 1) {
+  if constexpr (std::is_same<T, uint16_t>::value) {  // BF16 bits
+    if constexpr (NB == 1) {
+      v[0] = from_bf16(p[0]);
+    } else if constexpr (NB == 2) {
+      const uint32_t w = *reinterpret_cast<const uint32_t *>(p);
+      v[0] = from_bf16(w & 0xFFFFu);
+      v[1] = from_bf16(w >> 16);
+    } else {
+#pragma unroll
+      for (int c = 0; c < NB; c += 4) {
+        const uint2 w = *reinterpret_cast<const uint2 *>(p + c);
+        v[c] = from_bf16(w.x & 0xFFFFu);
+        v[c + 1] = from_bf16(w.x >> 16);
+        v[c + 2] = from_bf16(w.y & 0xFFFFu);
+        v[c + 3] = from_bf16(w.y >> 16);
+      }
+    }
+  } else if constexpr (NB == 1) {
     v[0] = (double)p[0];
   } else if constexpr (sizeof(T) == 8) {
 #pragma unroll
@@ -813,7 +832,18 @@ __device__ __forceinline__ void load_n(const T *__restrict__ p, double (&v)[NB])
 }
 template <int NB, typename T>
 __device__ __forceinline__ void store_n(T *__restrict__ p, const double (&v)[NB]) {
-  if constexpr (NB == 1) {
+  if constexpr (std::is_same<T, uint16_t>::value) {  // BF16 bits
+    if constexpr (NB == 1) {
+      p[0] = (uint16_t)to_bf16(v[0]);
+    } else if constexpr (NB == 2) {
+      *reinterpret_cast<uint32_t *>(p) = to_bf16(v[0]) | (to_bf16(v[1]) << 16);
+    } else {
+#pragma unroll
+      for (int c = 0; c < NB; c += 4)
+        *reinterpret_cast<uint2 *>(p + c) =
+            uint2{to_bf16(v[c]) | (to_bf16(v[c + 1]) << 16), to_bf16(v[c + 2]) | (to_bf16(v[c + 3]) << 16)};
+    }
+  } else if constexpr (NB == 1) {
     p[0] = (T)v[0];
   } else if constexpr (sizeof(T) == 8) {
 #pragma unroll
@@ -887,7 +917,36 @@ __device__ __forceinline__ void b_operands(const PanelDesc &pd, const typename V
                                            const typename SlotOf<S>::T *__restrict__ TP, int nrhs, int k0,
                                            int lane, float scale, typename Mfma<S>::Op (&b)[NB]) {
   const int rhs0 = NB * (lane & 15);
-  if constexpr (sizeof(S) == 2) {
+  if constexpr (sizeof(S) == 2 && !kTP) {
+    // X and t hold BF16 bits already: one load of NB right-hand sides per
+    // column, zero past the panel's last column, and a dense panel's
+    // power-of-two scale applied exactly (no rounding: the result equals
+    // rounding the scaled FP32 value)
+    using Raw = typename std::conditional<NB == 1, uint16_t,
+                                          typename std::conditional<NB == 2, uint32_t, uint2>::type>::type;
+    Raw raw[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int col = k0 + 4 * (lane >> 4) + q;
+      const int c = col < pd.ncols ? col : pd.ncols - 1;
+      raw[q] = *reinterpret_cast<const Raw *>(src + (int64_t)(pd.vec_off + c) * nrhs + rhs0);
+      if (col >= pd.ncols) raw[q] = Raw{};
+    }
+#pragma unroll
+    for (int cb = 0; cb < NB; ++cb)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        uint32_t h;
+        if constexpr (NB == 1)
+          h = raw[q];
+        else if constexpr (NB == 2)
+          h = cb == 0 ? (raw[q] & 0xFFFFu) : (raw[q] >> 16);
+        else
+          h = ((cb < 2 ? raw[q].x : raw[q].y) >> (16 * (cb & 1))) & 0xFFFFu;
+        if (scale != 1.0f) h = __float_as_uint(__uint_as_float(h << 16) * scale) >> 16;
+        b[cb][q] = (short)(uint16_t)h;
+      }
+  } else if constexpr (sizeof(S) == 2) {
     double v[4][NB];
 #pragma unroll
     for (int q = 0; q < 4; ++q) vec_entries_multi<kTP, NB>(pd, src, TP, k0 + 4 * (lane >> 4) + q, rhs0, nrhs, v[q]);
@@ -1230,20 +1289,23 @@ __global__ __launch_bounds__(256) void probe_copy_kernel(const double4 *__restri
     dst[i] = src[i];
 }
 
-// dst[i] = (float)src[i] for n elements, W at a time (W = 2: src 16-byte and
-// dst 8-byte aligned, n even), grid-stride: the FP32 copy of a multi-RHS X.
-template <int W>
-__global__ __launch_bounds__(256) void to_float_kernel(const double *__restrict__ src, float *__restrict__ dst,
-                                                       size_t n) {
-  using DT = typename std::conditional<W == 2, double2, double>::type;
-  using FT = typename std::conditional<W == 2, float2, float>::type;
+// dst[i] = src[i] in the multi-RHS operand type O (FP32, or the BF16 bits of
+// the FP32-rounded value) for n elements, W at a time (W = 2: src 16-byte
+// aligned, n even), grid-stride: the operand copy of a multi-RHS X.
+template <int W, typename O>
+__global__ __launch_bounds__(256) void to_vec_kernel(const double *__restrict__ src, O *__restrict__ dst,
+                                                     size_t n) {
   const size_t m = n / W;
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < m; i += (size_t)gridDim.x * blockDim.x) {
-    const DT v = reinterpret_cast<const DT *>(src)[i];
-    if constexpr (W == 2)
-      reinterpret_cast<FT *>(dst)[i] = FT{(float)v.x, (float)v.y};
-    else
-      dst[i] = (float)v;
+    double v[W];
+    if constexpr (W == 2) {
+      const double2 w = reinterpret_cast<const double2 *>(src)[i];
+      v[0] = w.x;
+      v[1] = w.y;
+    } else {
+      v[0] = src[i];
+    }
+    store_n<W>(dst + i * W, v);
   }
 }
 
@@ -1379,7 +1441,10 @@ __global__ __launch_bounds__(256) void absmax_kernel(const RepackDesc *__restric
   if (threadIdx.x == 0) chunk_max[blockIdx.x] = red[0];
 }
 
-// scale[factor(d)] = max over the chunks of descriptor d (1 when all zero).
+// scale[factor(d)] = the power of two just above the factor's absolute
+// maximum (1 when all zero): the stored values lie in (-1, 1), and scaling a
+// BF16 value by the factor's scale is exact, so a BF16 vector operand rounded
+// once and then scaled equals the operand rounded after scaling.
 __global__ __launch_bounds__(256) void scale_kernel(int ndesc, const RepackDesc *__restrict__ descs,
                                                     const int32_t *__restrict__ chunk_begin,
                                                     const float *__restrict__ chunk_max,
@@ -1388,7 +1453,9 @@ __global__ __launch_bounds__(256) void scale_kernel(int ndesc, const RepackDesc 
   if (d >= ndesc) return;
   float m = 0.f;
   for (int c = chunk_begin[d]; c < chunk_begin[d + 1]; ++c) m = fmaxf(m, chunk_max[c]);
-  scales[descs[d].factor] = m > 0.f ? m : 1.f;
+  int e = 0;
+  (void)frexpf(m, &e);  // m = f * 2^e, 0.5 <= f < 1
+  scales[descs[d].factor] = m > 0.f ? ldexpf(1.0f, e) : 1.f;
 }
 
 // Source layout -> plan layout, with conversion to the storage type.

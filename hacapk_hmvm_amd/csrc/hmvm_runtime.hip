@@ -125,8 +125,8 @@ bool t_descs(const SegSum &g, std::vector<int2> *out) {
 int elem_bytes(int storage) { return storage == kF64 ? 8 : storage == kF32 ? 4 : 2; }
 // Bytes of one slot (SlotOf<S>::T): FP64 for FP64 storage, FP32 otherwise.
 size_t slot_bytes(int storage) { return storage == kF64 ? 8 : 4; }
-// Bytes of one multi-RHS vector entry (VecOf<S>::T), likewise.
-size_t vec_bytes(int storage) { return storage == kF64 ? 8 : 4; }
+// Bytes of one multi-RHS vector entry (VecOf<S>::T: the storage type).
+size_t vec_bytes(int storage) { return (size_t)elem_bytes(storage); }
 
 // N events, destroyed on every return path of the timing functions.
 template <int N>
@@ -164,7 +164,7 @@ struct hmvm_plan {
   int multi_cap = 0;
   void *d_mslots = nullptr;  // slot_bytes(storage) per slot
   void *d_mt = nullptr;      // t: vec_bytes(storage) per entry
-  float *d_mXf = nullptr;    // FP32 copy of X (FP32 and BF16 storage)
+  void *d_mXc = nullptr;     // X in the operand type (FP32 and BF16 storage)
   TileDesc *d_mtiles_a = nullptr, *d_mtiles_b = nullptr;  // XCD-aware tile layouts of launches A and B
   int mtiles_wpb = 0;                                     // waves per workgroup they were laid out for
   double *d_mX = nullptr, *d_mY = nullptr;
@@ -191,7 +191,7 @@ struct hmvm_plan {
     void *ptrs[] = {d_vals,  d_tiles_a, d_tiles_b, d_panels, d_t,      d_slots,
                     d_tfac,  d_tmap,    d_yseg,    d_yptr,   d_ylist,  d_scales, d_x,      d_y,
                     d_yrng,  d_tdesc,   d_mtiles_a, d_mtiles_b,
-                    d_ctr,   d_mslots,  d_mt,      d_mX,     d_mY,     d_mXf};
+                    d_ctr,   d_mslots,  d_mt,      d_mX,     d_mY,     d_mXc};
     for (void *p : ptrs)
       if (p) (void)hipFree(p);
     if (ev_fork) (void)hipEventDestroy(ev_fork);
@@ -1052,18 +1052,18 @@ int reserve_multi(hmvm_plan *p, int nrhs) {
   if (rc) return rc;
   if (p->multi_cap >= nrhs) return HMVM_OK;
   const Plan &P = p->plan;
-  void *bufs[5] = {p->d_mslots, p->d_mt, p->d_mX, p->d_mY, p->d_mXf};
+  void *bufs[5] = {p->d_mslots, p->d_mt, p->d_mX, p->d_mY, p->d_mXc};
   for (void *b : bufs)
     if (b) (void)hipFree(b);
   p->d_mslots = p->d_mt = nullptr;
   p->d_mX = p->d_mY = nullptr;
-  p->d_mXf = nullptr;
+  p->d_mXc = nullptr;
   p->multi_cap = 0;
   if ((rc = dalloc_bytes(&p->d_mslots, slot_bytes(p->storage) * (size_t)(P.n_tp + P.n_z) * nrhs)) ||
       (rc = dalloc_bytes(&p->d_mt, vec_bytes(p->storage) * (size_t)P.n_t * nrhs)) ||
       (rc = dalloc(&p->d_mX, (size_t)P.nd * nrhs)) || (rc = dalloc(&p->d_mY, (size_t)P.nd * nrhs)))
     return rc;
-  if (p->storage != kF64 && (rc = dalloc(&p->d_mXf, (size_t)P.nd * nrhs))) return rc;
+  if (p->storage != kF64 && (rc = dalloc_bytes(&p->d_mXc, vec_bytes(p->storage) * (size_t)P.nd * nrhs))) return rc;
   p->multi_cap = nrhs;
   return HMVM_OK;
 }
@@ -1108,13 +1108,13 @@ void launch_segsum_multi(hmvm_plan *p, int phase, double *Y, int nrhs, int accum
   if (phase == 1) {
     // 16-byte loads and stores (slots and t are the plan's own allocations)
     if (P_.t_fused || P_.tsum.n == 0) return;
-    // t in the multi-RHS vector type (VecOf: the slot type here)
-    P *T = (P *)p->d_mt;
+    // t in the multi-RHS operand type (VecOf: FP64, FP32 or BF16 bits)
     const int64_t tw = (int64_t)P_.tsum.n * (nrhs / (16 / (int)sizeof(P)));
     const dim3 grid((unsigned)((tw + 255) / 256));
+    P *T = (P *)p->d_mt;
     if (p->storage == kBF16)
-      hipLaunchKernelGGL((tsum_multi_kernel<true, P, P>), grid, dim3(256), 0, st, P_.tsum.n, nrhs, p->d_tdesc, slots,
-                         p->d_tfac, p->d_scales, T);
+      hipLaunchKernelGGL((tsum_multi_kernel<true, P, uint16_t>), grid, dim3(256), 0, st, P_.tsum.n, nrhs,
+                         p->d_tdesc, slots, p->d_tfac, p->d_scales, (uint16_t *)p->d_mt);
     else
       hipLaunchKernelGGL((tsum_multi_kernel<false, P, P>), grid, dim3(256), 0, st, P_.tsum.n, nrhs, p->d_tdesc,
                          slots, (const int32_t *)nullptr, (const float *)nullptr, T);
@@ -1138,15 +1138,22 @@ void launch_multi_phase(hmvm_plan *p, int phase, const double *X, double *Y, int
     const int nt = (int)(phase == 0 ? P.tiles_a.size() : P.tiles_b.size());
     const void *T = phase == 0 ? nullptr : p->d_mt;
     if (p->storage != kF64) {
-      // FP32 and BF16 operands are FP32-rounded X and t: the tiles read an
-      // FP32 copy of X (made here, before launch A) and an FP32 t
+      // the tiles read X and t in the operand type: a copy of X made here,
+      // before launch A, and a t the t sum writes so
       if (phase == 0) {
         const size_t n = (size_t)P.nd * nrhs;
         const unsigned blocks = (unsigned)std::min<size_t>((n / 2 + 255) / 256, 8192);
-        if (((uintptr_t)X & 15) == 0)
-          hipLaunchKernelGGL(to_float_kernel<2>, dim3(blocks), dim3(256), 0, st, X, p->d_mXf, n);
+        const bool a16 = ((uintptr_t)X & 15) == 0;
+        if (p->storage == kF32 && a16)
+          hipLaunchKernelGGL((to_vec_kernel<2, float>), dim3(blocks), dim3(256), 0, st, X, (float *)p->d_mXc, n);
+        else if (p->storage == kF32)
+          hipLaunchKernelGGL((to_vec_kernel<1, float>), dim3(2 * blocks), dim3(256), 0, st, X, (float *)p->d_mXc, n);
+        else if (a16)
+          hipLaunchKernelGGL((to_vec_kernel<2, uint16_t>), dim3(blocks), dim3(256), 0, st, X, (uint16_t *)p->d_mXc,
+                             n);
         else
-          hipLaunchKernelGGL(to_float_kernel<1>, dim3(2 * blocks), dim3(256), 0, st, X, p->d_mXf, n);
+          hipLaunchKernelGGL((to_vec_kernel<1, uint16_t>), dim3(2 * blocks), dim3(256), 0, st, X,
+                             (uint16_t *)p->d_mXc, n);
       }
     }
     if (p->storage == kF64 && nrhs > 16 && ((uintptr_t)X & 15) != 0) {
@@ -1156,7 +1163,7 @@ void launch_multi_phase(hmvm_plan *p, int phase, const double *X, double *Y, int
         (void)hipMemcpyAsync(p->d_mX, X, sizeof(double) * (size_t)P.nd * nrhs, hipMemcpyDeviceToDevice, st);
       X = p->d_mX;
     }
-    const void *Xv = p->storage == kF64 ? (const void *)X : (const void *)p->d_mXf;
+    const void *Xv = p->storage == kF64 ? (const void *)X : (const void *)p->d_mXc;
     if (p->storage == kF64)
       launch_multi_nb<double, false>(p, tl, nt, Xv, T, nrhs, st);
     else if (p->storage == kF32)
