@@ -1004,7 +1004,8 @@ __device__ __forceinline__ typename Mfma<S>::Op a_of(const MultiWords<S> &mw, in
 // re-reads of X); FP32 and BF16 two chunks when the accumulators fit 32
 // VGPRs (one for BF16 with 16 right-hand sides); FP64 batches 8 K-steps for
 // 16 right-hand sides and 4 otherwise; FP32 4 for 16 and 32 right-hand
-// sides, 2 for 64; BF16 1 (16 columns) for 16 and 32, 2 for 64.
+// sides, 2 for 64 (8 for 32: launch A 0.32 -> 0.42 ms); BF16 2 (32
+// columns; with X and t as BF16 bits, 2 beat 1 by 8% in launch A at 32).
 template <typename S, int NB>
 struct MultiShape {
   static constexpr int V = Wide<S>::V;
@@ -1012,9 +1013,7 @@ struct MultiShape {
   static constexpr int CH = sizeof(S) == 8                ? (NB <= 2 ? 4 : 2)
                             : (sizeof(S) == 2 && NB == 1) ? 1
                                                           : (V * NB * ACCW <= 32 ? 2 : 1);
-  static constexpr int KB = sizeof(S) == 8   ? (NB == 1 ? 8 : 4)
-                            : sizeof(S) == 4 ? (NB <= 2 ? 4 : 2)
-                                             : (NB <= 2 ? 1 : 2);
+  static constexpr int KB = sizeof(S) == 8 ? (NB == 1 ? 8 : 4) : sizeof(S) == 4 ? (NB <= 2 ? 4 : 2) : 2;
 };
 
 // One panel of one pass: nl (wave-uniform) live chunks; chunks past the
