@@ -123,6 +123,23 @@ def test_tile_sizes(gpu, H, O, cube3000, w):
     p.close()
 
 
+def test_stage2_width_apart_from_dense(gpu, H, O, cube3000):
+    """w3 sets the stage-2 tile width apart from w2 (dense tiles): launch A
+    keeps its tiles, launch B gets others, and the product stays exact."""
+    h = cube3000
+    b = np.random.default_rng(5).uniform(-1, 1, h.nd)
+    ref = O.seq_2(h, b)
+    base = H.Plan(h, w1=4096, w2=4096)
+    n0 = (base.info()["ntiles_a"], base.info()["ntiles_b"])
+    base.close()
+    for w3 in (256, 1 << 20):
+        p = H.Plan(h, w1=4096, w2=4096, w3=w3)
+        na, nb = p.info()["ntiles_a"], p.info()["ntiles_b"]
+        assert na == n0[0] and (nb > n0[1] if w3 == 256 else nb < n0[1])
+        assert rel(p.apply_host(b), ref) <= 1e-13
+        p.close()
+
+
 def test_plan_sources_agree(gpu, H, sphere4096):
     h = sphere4096
     b = np.random.default_rng(6).uniform(-1, 1, h.nd)

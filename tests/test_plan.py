@@ -71,6 +71,16 @@ def test_schedule_rejects_bad_descriptors(H):
     H.plan_check(h)
 
 
+def test_stage2_width_option_validated(H):
+    h = H.generate(400, geometry="sphere", leaf=16)
+    for bad in (1, 63):
+        with pytest.raises(H.HMVMError):
+            H.plan_check(h, w3=bad)
+    a = H.plan_check(h, w1=1024, w2=1024)
+    b = H.plan_check(h, w1=1024, w2=1024, w3=64)
+    assert b["ntiles_a"] == a["ntiles_a"] and b["ntiles_b"] > a["ntiles_b"]
+
+
 def test_storage_option_validated(H):
     h = H.generate(400, geometry="sphere", leaf=16)
     assert H.plan_check(h, storage="bf16")["elem_bytes"] == 2

@@ -62,7 +62,7 @@ def rel(a, b):
 
 
 @pytest.mark.parametrize("kw", [{}, {"w1": 64, "w2": 64}, {"w1": 500, "w2": 3000, "order": 1},
-                                {"w1": 1 << 20, "w2": 1 << 20}])
+                                {"w1": 1 << 20, "w2": 1 << 20}, {"w2": 3000, "w3": 64}])
 @pytest.mark.parametrize("source", ["rowmat", "rowmat_t"])
 @pytest.mark.parametrize("t_mode", [0, 1, 2, 3])
 def test_replay_matches_oracle(H, O, kw, source, t_mode):
