@@ -4,13 +4,14 @@
 // reads a materialised t, t_mode 0):
 //   tile_kernel (launch A)  stage-1 slots = stacked a1 panels x x-slices,
 //                           dense slots   = transposed dense leaves x x-slices
-//   [segsum_kernel (t)      t = sum of stage-1 slots            (fixed order)]
+//   [tsum_kernel            t = sum of stage-1 slots            (fixed order)]
 //   tile_kernel (launch B)  stage-2 slots = a2 panels x t (the materialised t,
 //                           or summed from the stage-1 slots on the fly)
-//   segsum_kernel (y)       y = y + sum of dense and stage-2 slots (fixed order)
+//   ysum_kernel             y = y + sum of dense and stage-2 slots (fixed order)
 // Separate launches are the synchronisation: a device-scope release after
 // every tile (to let the last tile of a group do the sum) costs more than a
-// launch boundary on gfx950, where it writes back the XCD's L2.
+// launch boundary on gfx950, where it writes back the XCD's L2.  The
+// persistent mode runs the same four stages in one launch with grid barriers.
 // plus the one-time repack kernels that move the caller's rowmat layout into
 // the plan's layout (and, for BF16 storage, compute per-factor scales).
 #pragma once
@@ -610,107 +611,219 @@ __global__ __launch_bounds__(256) HMVM_TILE_OCCUPANCY void tile_kernel_timed(con
   }
 }
 
+// ---- segment-sum entries (shared by the segment-sum launches and the
+// persistent launch) --------------------------------------------------------
+// t entry i: scale(i) * sum of nch(i) stage-1 partial slots buf[first(i) +
+// c * stride(i)], c = 0, 1, ... in order, desc[i] = {first, (nch << 16) |
+// stride}; scale(i) = both factor scales of entry i's leaf (a1 = factor_of,
+// a2 = the next id).  Entries past nch reload the last slot and add nothing
+// (unguarded loads).
+template <bool kScaled, typename P>
+__device__ __forceinline__ double tsum_entry(int i, const int2 *__restrict__ desc, const P *__restrict__ buf,
+                                             const int32_t *__restrict__ factor_of,
+                                             const float *__restrict__ scales) {
+  const int2 d = desc[i];
+  const int nch = (int)((unsigned)d.y >> 16), stride = d.y & 0xFFFF;
+  double a = 0;
+  for (int c = 0; c < nch; c += 4) {
+    P v[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) v[u] = buf[d.x + (int64_t)(c + u < nch ? c + u : nch - 1) * stride];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) a += c + u < nch ? (double)v[u] : 0.0;
+  }
+  if (kScaled) a *= (double)scales[factor_of[i]] * (double)scales[factor_of[i] + 1];
+  return a;
+}
+
+// y entry i: sum_e buf[list[e] + i], e over [rng[i].x, rng[i].y) in list
+// order (the range resolved per entry on the host, so that the chain of
+// dependent loads is range -> list -> slots); up to U list entries, then
+// their slots, in flight at once (the y sum adds 11 to 23 slots per row at
+// N=101,250: U = 24 in the y-sum launch; the persistent launch takes 8,
+// within the tile code's registers).  Entries past the range load buf[0]
+// (the range is not empty, so neither is buf) and add nothing: the slot
+// loads are unguarded, since a guarded FP32 load whose conversion the
+// compiler sinks into the guard waits for each load in turn.
+template <int U, typename P>
+__device__ __forceinline__ double ysum_entry(int i, const int2 *__restrict__ rng, const int64_t *__restrict__ list,
+                                             const P *__restrict__ buf) {
+  const int2 r = rng[i];
+  const int e1 = r.y;
+  double a = 0;
+  for (int e = r.x; e < e1; e += U) {
+    int64_t l[U];
+    P v[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) l[u] = e + u < e1 ? list[e + u] : -(int64_t)i;
+#pragma unroll
+    for (int u = 0; u < U; ++u) v[u] = buf[l[u] + i];
+#pragma unroll
+    for (int u = 0; u < U; ++u) a += e + u < e1 ? (double)v[u] : 0.0;
+  }
+  return a;
+}
+
 // ---- persistent single-launch mode ------------------------------------------
-// Work items, in queue order: stage-1 tiles [0, n1), dense tiles [n1, na),
-// stage-2 tiles [na, nt), y-sum items of 64 rows [nt, nitems).  A wave takes
-// the next item from a device-wide counter; a stage-2 tile first waits until
-// all stage-1 tiles have completed, a y-sum item until all tiles have.  Every
-// item a wave waits for was taken earlier by a running wave that waits for
-// nothing later in the queue, so the waits end; each wait is additionally
-// bounded (the kernel sets the fault word and carries on rather than hang).
-// The last wave to leave resets the counters for the next launch.
+// One launch of as many workgroups as the device holds at once runs the four
+// stages of an HMVM, with a grid barrier between consecutive stages: the
+// launch-A tiles, the t sum, the launch-B tiles, the y sum.  The host deals
+// each launch's tiles to the waves beforehand (longest first, each to the
+// least-loaded wave, by the tiles' durations measured in the launches: the
+// order the hardware dispatcher follows for one tile per wave), so no work
+// queue and no per-tile signal exists.  (Queues per XCD for the smallest
+// 30% of the work, taken with one atomic per tile, measured slower: FP64
+// 0.39 -> 0.53 ms at N=101,250.)
+//
+// Each XCD has its own L2, which the other XCDs do not see.  A workgroup
+// reads its XCD from the XCC_ID register and counts itself there at the
+// start (all workgroups then wait until the counts add up to the grid: they
+// are final).
+// At a barrier every workgroup waits for its own stores and counts itself on
+// its XCD's arrival counter.  The last arrival of an XCD writes that L2 back
+// (agent-scope release: buffer_wbl2), counts the XCD on the global counter,
+// waits for all XCDs, invalidates that L2 (agent-scope acquire: buffer_inv
+// sc1) and raises its XCD's flag; the other workgroups of the XCD wait for
+// the flag and invalidate only their CU's L1 (buffer_inv sc0).  One
+// writeback and one L2 invalidation per XCD and barrier, instead of one per
+// workgroup, and only eight workgroups poll the global counter.  The waits are
+// bounded: one that gives up sets the fault word and the kernel carries on
+// (the results are then invalid and the blocking entry points report it).
+// Leaving is counted per XCD as well; the last workgroup of the last XCD to
+// leave resets the counters.  Every counter has a 256-byte line of its own,
+// and no counter takes more than one XCD's workgroups (atomics on one
+// address serialise).
+constexpr int kXcds = 8;
+constexpr int kLine = 64;  // ints per counter line
+
 struct PersistCounters {
-  int head;        // next work item
-  int s1_done;     // completed stage-1 tiles
-  int tiles_done;  // completed tiles
-  int exited;      // waves that left the loop
-  int fault;       // set when a bounded wait gave up
-  int pad[3];
+  int wgs[kXcds][kLine];   // workgroups on each XCD
+  int arr[kXcds][kLine];   // arrivals per XCD, over the launch's barriers
+  int flag[kXcds][kLine];  // last barrier each XCD is through
+  int left[kXcds][kLine];  // workgroups of each XCD that left
+  int global[kLine];       // XCDs through, over the launch's barriers
+  int xleft[kLine];        // XCDs whose workgroups have all left
+  int fault;               // set when a bounded wait gave up
+  int seen;                // a barrier that gave up: 16 * barrier + XCDs it saw through
 };
 
 constexpr int kMaxSpins = 1 << 22;
 
-__device__ __forceinline__ int wave_take(int *head, int lane) {
-  int v = 0;
-  if (lane == 0) v = __hip_atomic_fetch_add(head, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  return __builtin_amdgcn_readfirstlane(v);
-}
-
-// Publishes the wave's earlier stores, then counts one completion.
-__device__ __forceinline__ void wave_signal(int *ctr, int lane) {
-  if (lane == 0) __hip_atomic_fetch_add(ctr, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-// Waits until *ctr >= target, bounded; relaxed polls, one acquire fence at
-// the end (so the caches are invalidated once, not on every poll).
-__device__ __forceinline__ void wave_wait(int *ctr, int target, int *fault, int lane) {
-  if (lane == 0) {
-    int ok = 0;
-    for (int s = 0; s < kMaxSpins; ++s) {
-      if (__hip_atomic_load(ctr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= target) {
-        ok = 1;
-        break;
-      }
-      __builtin_amdgcn_s_sleep(2);
-    }
-    if (!ok) __hip_atomic_store(fault, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+// Spins (thread 0) until *ctr >= target, bounded; false when it gave up.
+__device__ __forceinline__ bool spin_until(int *ctr, int target) {
+  for (int s = 0; s < kMaxSpins; ++s) {
+    if (__hip_atomic_load(ctr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= target) return true;
+    __builtin_amdgcn_s_sleep(2);
   }
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  return false;
 }
 
+// Barrier number `phase` (1, 2, ...) of the launch for a workgroup on XCD
+// x, which holds nwg_x workgroups of the launch; nx XCDs hold workgroups.
+__device__ __forceinline__ void grid_barrier(PersistCounters *c, int phase, int x, int nwg_x, int nx) {
+  asm volatile("s_waitcnt vmcnt(0)" ::: "memory");  // this wave's stores have reached the L2
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const int a = __hip_atomic_fetch_add(&c->arr[x][0], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) + 1;
+    bool ok;
+    if (a == phase * nwg_x) {
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");  // write this XCD's L2 back
+      __hip_atomic_fetch_add(&c->global[0], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      ok = spin_until(&c->global[0], phase * nx);
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");  // drop this XCD's stale L2 (and L1) lines
+      asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+      __hip_atomic_store(&c->flag[x][0], phase, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    } else {
+      ok = spin_until(&c->flag[x][0], phase);
+      asm volatile("buffer_inv sc0\n\ts_waitcnt vmcnt(0)" ::: "memory");  // drop this CU's stale L1 lines
+    }
+    if (!ok) {
+      __hip_atomic_fetch_or(&c->fault, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_store(&c->seen, 16 * phase + __hip_atomic_load(&c->global[0], __ATOMIC_RELAXED,
+                                                                  __HIP_MEMORY_SCOPE_AGENT) % 16,
+                         __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+  }
+  __syncthreads();
+}
+
+// Tiles of launch A (tiles_a[ptr_a[w] .. ptr_a[w+1]) for wave w) and B
+// likewise; n_t t entries (0: no t sum), nd y rows.
 template <typename S, bool kScaled, int kLoads, bool kPipe, bool kNT, bool kBV>
-__global__ __launch_bounds__(256) void persistent_kernel(
-    const TileDesc *__restrict__ tiles_a, int n1, int na, const TileDesc *__restrict__ tiles_b, int nt,
-    int nitems, int nwaves, const PanelDesc *__restrict__ panels, const S *__restrict__ vals,
-    const double *__restrict__ x, double *t, typename SlotOf<S>::T *slots, const float *__restrict__ scales,
-    const int32_t *__restrict__ tmap, const int32_t *__restrict__ tfac,
-    const int32_t *__restrict__ yseg, const int32_t *__restrict__ yptr, const int64_t *__restrict__ ylist,
-    int nd, double *__restrict__ y, int accumulate, PersistCounters *ctr) {
+__global__ __launch_bounds__(256) HMVM_TILE_OCCUPANCY void persistent_kernel(
+    const TileDesc *__restrict__ tiles_a, const int32_t *__restrict__ ptr_a, const TileDesc *__restrict__ tiles_b,
+    const int32_t *__restrict__ ptr_b, const PanelDesc *__restrict__ panels, const S *__restrict__ vals,
+    const double *__restrict__ x_vec, double *t, typename SlotOf<S>::T *slots, const float *__restrict__ scales,
+    const int32_t *__restrict__ tmap, const int32_t *__restrict__ tfac, int n_t, const int2 *__restrict__ tdesc,
+    int nd, const int2 *__restrict__ yrng, const int64_t *__restrict__ ylist, double *__restrict__ y,
+    int accumulate, PersistCounters *ctr) {
   const int lane = (int)(threadIdx.x & 63);
+  const int wave = (int)blockIdx.x * (int)(blockDim.x >> 6) + (int)(threadIdx.x >> 6);
+  const int nwg = (int)gridDim.x;
   extern __shared__ double tile_lds[];
   double *lds = tile_lds + (threadIdx.x >> 6) * lds_wave_doubles<S>();
   const TileOut o{slots, tmap, t, tfac};  // stage-1 rows with a map go to t
-  bool s1_seen = false, all_seen = false;
-  for (;;) {
-    const int item = wave_take(&ctr->head, lane);
-    if (item >= nitems) break;
-    if (item < na) {
-      const TileDesc td = tiles_a[item];
-      run_tile<S, kScaled, kLoads, kPipe, kNT, true, kBV>(td, panels, vals, x, t, slots, scales, o, lane, lds);
-      if (item < n1) wave_signal(&ctr->s1_done, lane);
-      wave_signal(&ctr->tiles_done, lane);
-    } else if (item < nt) {
-      if (!s1_seen) {
-        wave_wait(&ctr->s1_done, n1, &ctr->fault, lane);
-        s1_seen = true;
+  // join: count this workgroup on its XCD (XCC_ID register), then learn how
+  // many workgroups each XCD holds once all have joined
+  __shared__ int xcd_info[3];  // XCD, its workgroups, XCDs in use
+  if (threadIdx.x == 0) {
+    const int x = __builtin_amdgcn_s_getreg(20 | (3 << 11)) & (kXcds - 1);
+    // the join is counted only once the XCD count has been performed (its
+    // returned value gates it), so a full join count means final XCD counts
+    __hip_atomic_fetch_add(&ctr->wgs[x][0], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    int nx = 0, total = 0;
+    for (int s = 0; s < kMaxSpins; ++s) {  // until every workgroup is counted
+      nx = total = 0;
+      for (int q = 0; q < kXcds; ++q) {
+        const int n = __hip_atomic_load(&ctr->wgs[q][0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        total += n;
+        nx += n > 0;
       }
-      const TileDesc td = tiles_b[item - na];
-      run_tile<S, kScaled, kLoads, kPipe, kNT, true, kBV>(td, panels, vals, x, t, slots, scales, o, lane, lds);
-      wave_signal(&ctr->tiles_done, lane);
+      if (total >= nwg) break;
+      __builtin_amdgcn_s_sleep(2);
+    }
+    if (total < nwg) __hip_atomic_fetch_or(&ctr->fault, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    xcd_info[0] = x;
+    xcd_info[1] = __hip_atomic_load(&ctr->wgs[x][0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    xcd_info[2] = nx;
+  }
+  __syncthreads();
+  const int x = xcd_info[0], nwg_x = xcd_info[1], nx = xcd_info[2];
+  const int gt = (int)(blockIdx.x * blockDim.x + threadIdx.x), nthreads = nwg * (int)blockDim.x;
+  // the four stages in one loop body, so that the tile code is compiled
+  // once (two inlined copies needed 193 registers: 2 waves per SIMD)
+#pragma nounroll
+  for (int stage = 0; stage < 4; ++stage) {
+    if (stage == 0 || stage == 2) {
+      const TileDesc *tiles = stage == 0 ? tiles_a : tiles_b;
+      const int32_t *ptr = stage == 0 ? ptr_a : ptr_b;
+      for (int i = ptr[wave]; i < ptr[wave + 1]; ++i)
+        run_tile<S, kScaled, kLoads, kPipe, kNT, false, kBV>(tiles[i], panels, vals, x_vec, t, slots, scales, o,
+                                                             lane, lds);
+    } else if (stage == 1) {
+      for (int i = gt; i < n_t; i += nthreads) t[i] = tsum_entry<kScaled>(i, tdesc, slots, tfac, scales);
     } else {
-      if (!all_seen) {
-        wave_wait(&ctr->tiles_done, nt, &ctr->fault, lane);
-        all_seen = true;
-      }
-      const int i = (item - nt) * 64 + lane;
-      if (i < nd) {
-        const int sg = yseg[i];
-        const int e1 = yptr[sg + 1];
-        double a = 0;
-        for (int e = yptr[sg]; e < e1; ++e) a += (double)__hip_atomic_load(slots + ylist[e] + i, __ATOMIC_RELAXED,
-                                                                   __HIP_MEMORY_SCOPE_AGENT);
+      for (int i = gt; i < nd; i += nthreads) {
+        const double a = ysum_entry<8>(i, yrng, ylist, slots);
         y[i] = accumulate ? y[i] + a : a;
       }
     }
+    if (stage < 3) grid_barrier(ctr, stage + 1, x, nwg_x, nx);
   }
-  // leave; the last wave out resets the counters for the next launch
-  if (lane == 0) {
-    const int e = __hip_atomic_fetch_add(&ctr->exited, 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-    if (e == nwaves - 1) {
-      __hip_atomic_store(&ctr->head, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      __hip_atomic_store(&ctr->s1_done, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      __hip_atomic_store(&ctr->tiles_done, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      __hip_atomic_store(&ctr->exited, 0, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+  // leave; the last workgroup of the last XCD out resets the counters for
+  // the next launch
+  if (threadIdx.x == 0) {
+    const int e = __hip_atomic_fetch_add(&ctr->left[x][0], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) + 1;
+    if (e == nwg_x &&
+        __hip_atomic_fetch_add(&ctr->xleft[0], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) + 1 == nx) {
+      for (int q = 0; q < kXcds; ++q) {
+        __hip_atomic_store(&ctr->wgs[q][0], 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(&ctr->arr[q][0], 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(&ctr->flag[q][0], 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(&ctr->left[q][0], 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      }
+      __hip_atomic_store(&ctr->global[0], 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_store(&ctr->xleft[0], 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
   }
 }
@@ -1315,68 +1428,26 @@ __global__ __launch_bounds__(256) void flush_kernel(double4 *__restrict__ dst, s
     dst[i] = double4{v, v, v, v};
 }
 
-// out[i] = (accumulate ? out[i] : 0) + scale(i) * sum_e buf[list[e] + i],
-// e over [rng[i].x, rng[i].y) (entry i's segment, resolved per entry on the
-// host so that the chain of dependent loads is range -> list -> slots),
-// scale(i) = product of the two factor scales of entry i's leaf (t sum only)
-template <bool kScaled, typename P>
-__global__ __launch_bounds__(256) void segsum_kernel(int n, const int2 *__restrict__ rng,
-                                                     const int64_t *__restrict__ list,
-                                                     const P *__restrict__ buf,
-                                                     const int32_t *__restrict__ factor_of,
-                                                     const float *__restrict__ scales,
-                                                     double *__restrict__ out, int accumulate) {
+// y sum: out[i] = (accumulate ? out[i] : 0) + ysum_entry(i) for the n rows.
+template <typename P>
+__global__ __launch_bounds__(256) void ysum_kernel(int n, const int2 *__restrict__ rng,
+                                                   const int64_t *__restrict__ list, const P *__restrict__ buf,
+                                                   double *__restrict__ out, int accumulate) {
   const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
   if (i >= n) return;
-  const int2 r = rng[i];
-  const int e1 = r.y;
-  double a = 0;
-  // up to 24 list entries, then their slots, in flight at once (the y sum
-  // adds ~18 slots per row: one round trip for the list and one for the
-  // slots instead of one pair per 8), added in list order
-  for (int e = r.x; e < e1; e += 24) {
-    // Entries past the range load buf[0] (the range is not empty, so
-    // neither is buf) and add nothing: the slot loads are unguarded, since a
-    // guarded FP32 load whose conversion the compiler sinks into the guard
-    // waits for each load in turn.
-    int64_t l[24];
-    P v[24];
-#pragma unroll
-    for (int u = 0; u < 24; ++u) l[u] = e + u < e1 ? list[e + u] : -(int64_t)i;
-#pragma unroll
-    for (int u = 0; u < 24; ++u) v[u] = buf[l[u] + i];
-#pragma unroll
-    for (int u = 0; u < 24; ++u) a += e + u < e1 ? (double)v[u] : 0.0;
-  }
-  // t entry of leaf b: both factor scales of b (a1 = factor_of, a2 = the next id)
-  if (kScaled) a *= (double)scales[factor_of[i]] * (double)scales[factor_of[i] + 1];
+  const double a = ysum_entry<24>(i, rng, list, buf);
   out[i] = accumulate ? out[i] + a : a;
 }
 
-// t sum with regular segments: t entry i adds nch(i) stage-1 partial slots
-// buf[first(i) + c * stride(i)], c = 0, 1, ... in order (a leaf's rank row
-// in each column chunk of its row group), desc[i] = {first, (nch << 16) |
-// stride}; the same sums in the same order as segsum_kernel over the list
-// first + c * stride, without the list loads.  Entries past nch reload the
-// last slot and add nothing (unguarded loads).
+// t sum: t[i] = tsum_entry(i) for the n entries of t (the list of slots of
+// an entry is first + c * stride: no list loads).
 template <bool kScaled, typename P>
 __global__ __launch_bounds__(256) void tsum_kernel(int n, const int2 *__restrict__ desc, const P *__restrict__ buf,
                                                    const int32_t *__restrict__ factor_of,
                                                    const float *__restrict__ scales, double *__restrict__ out) {
   const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
   if (i >= n) return;
-  const int2 d = desc[i];
-  const int nch = (int)((unsigned)d.y >> 16), stride = d.y & 0xFFFF;
-  double a = 0;
-  for (int c = 0; c < nch; c += 4) {
-    P v[4];
-#pragma unroll
-    for (int u = 0; u < 4; ++u) v[u] = buf[d.x + (int64_t)(c + u < nch ? c + u : nch - 1) * stride];
-#pragma unroll
-    for (int u = 0; u < 4; ++u) a += c + u < nch ? (double)v[u] : 0.0;
-  }
-  if (kScaled) a *= (double)scales[factor_of[i]] * (double)scales[factor_of[i] + 1];
-  out[i] = a;
+  out[i] = tsum_entry<kScaled>(i, desc, buf, factor_of, scales);
 }
 
 // tsum_kernel for nrhs right-hand sides (rows of nrhs in buf and out): a

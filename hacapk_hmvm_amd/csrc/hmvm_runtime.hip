@@ -7,9 +7,11 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <functional>
 #include <map>
 #include <memory>
 #include <mutex>
+#include <queue>
 #include <string>
 #include <tuple>
 #include <vector>
@@ -159,7 +161,10 @@ struct hmvm_plan {
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   bool a_mixed = false;  // launch A order is not "stage-1 tiles, then dense" (modes 1 and 2 need that)
   bool has_tp = false;   // some panel sums stage-1 partial slots (kVecTP)
-  int persist_key = -1, persist_blocks = 0;  // occupancy cache (kernel variant, block size)
+  int persist_key = -1, persist_blocks = 0;  // persistent launch: its shape (kernel variant, block size) and grid
+  int tiles_version = 0, persist_version = -1;  // tile lists changed / the dealt lists are of this version
+  TileDesc *d_ptiles_a = nullptr, *d_ptiles_b = nullptr;  // persistent launch: tiles dealt to the waves
+  int32_t *d_pptr_a = nullptr, *d_pptr_b = nullptr;       // wave w's tiles: [ptr[w], ptr[w+1])
   // several right-hand sides: buffers sized for multi_cap columns
   int multi_cap = 0;
   void *d_mslots = nullptr;  // slot_bytes(storage) per slot
@@ -179,7 +184,6 @@ struct hmvm_plan {
   void *d_slots = nullptr;  // stage-1 partial slots, then output slots: slot_bytes(storage) each
   int32_t *d_tfac = nullptr;
   int32_t *d_tmap = nullptr;  // t_mode 3: t index of every stage-1 tile row
-  int32_t *d_yseg = nullptr, *d_yptr = nullptr;
   int64_t *d_ylist = nullptr;
   int2 *d_yrng = nullptr;   // per-entry list ranges of the y sum
   int2 *d_tdesc = nullptr;  // t sum: {first slot, (nch << 16) | stride} per entry
@@ -189,8 +193,8 @@ struct hmvm_plan {
 
   ~hmvm_plan() {
     void *ptrs[] = {d_vals,  d_tiles_a, d_tiles_b, d_panels, d_t,      d_slots,
-                    d_tfac,  d_tmap,    d_yseg,    d_yptr,   d_ylist,  d_scales, d_x,      d_y,
-                    d_yrng,  d_tdesc,   d_mtiles_a, d_mtiles_b,
+                    d_tfac,  d_tmap,    d_ylist,  d_scales, d_x,      d_y,
+                    d_yrng,  d_tdesc,   d_mtiles_a, d_mtiles_b, d_ptiles_a, d_ptiles_b, d_pptr_a, d_pptr_b,
                     d_ctr,   d_mslots,  d_mt,      d_mX,     d_mY,     d_mXc};
     for (void *p : ptrs)
       if (p) (void)hipFree(p);
@@ -466,10 +470,9 @@ PlanOptions plan_options(const hmvm_options *o, int64_t len) {
   // the stage-2 factors of a cluster merge into one panel, which beats summing
   // the stage-1 partials on the fly in all three storages (measured at
   // N = 101,250) despite the extra launch, and beats t_mode 3 (stage-1 bands
-  // writing t directly: thin bands over large clusters re-read x).  The
-  // persistent mode has no launch boundary for the t sum, so there the
-  // stage-2 tiles sum the partials.
-  po.t_mode = o->t_mode >= 0 ? o->t_mode : o->mode == 1 ? 0 : 1;
+  // writing t directly: thin bands over large clusters re-read x), also in
+  // the persistent mode, which runs the t sum as one of its stages.
+  po.t_mode = o->t_mode >= 0 ? o->t_mode : 1;
   return po;
 }
 
@@ -697,6 +700,8 @@ bool order_mixed(const Plan &P) {
 
 // Launch A as "stage-1 tiles, then dense", each part in its current order:
 // the layout the persistent and two-stream modes rely on.
+int persist_prepare(hmvm_plan *p);
+
 int partition_a(hmvm_plan *p) {
   Plan &P = p->plan;
   if (!order_mixed(P)) {
@@ -718,6 +723,7 @@ int partition_a(hmvm_plan *p) {
   P.tiles_a.swap(nt);
   P.a_stage1.swap(ns);
   p->a_mixed = false;
+  p->tiles_version++;
   return HMVM_OK;
 }
 
@@ -758,9 +764,6 @@ int create_plan(Source &src, const hmvm_options *opt, hmvm_plan **out) {
   err = verify_plan(p->plan, src.la);
   if (!err.empty()) return set_error(HMVM_ERR_MATRIX, "schedule check failed: %s", err.c_str());
   Plan &P = p->plan;
-  if (p->mode == 1 && !P.t_fused && !P.t_direct)
-    return set_error(HMVM_ERR_UNSUPPORTED, "mode 1 (persistent launch) needs t_mode 0 or 3: this plan sums t in a "
-                                           "launch of its own");
   for (const PanelDesc &pd : P.panels) p->has_tp = p->has_tp || pd.vec_sel == kVecTP;
   p->a_mixed = order_mixed(P);
   HIP_TRY(hipGetDevice(&p->device));
@@ -781,7 +784,6 @@ int create_plan(Source &src, const hmvm_options *opt, hmvm_plan **out) {
   // schedule
   if ((rc = upload(&p->d_tiles_a, P.tiles_a)) || (rc = upload(&p->d_tiles_b, P.tiles_b)) ||
       (rc = upload(&p->d_panels, P.panels)) ||
-      (rc = upload(&p->d_yseg, P.ysum.seg)) || (rc = upload(&p->d_yptr, P.ysum.ptr)) ||
       (rc = upload(&p->d_ylist, P.ysum.list)) || (rc = upload(&p->d_yrng, seg_ranges(P.ysum))))
     return rc;
   {
@@ -808,11 +810,12 @@ int create_plan(Source &src, const hmvm_options *opt, hmvm_plan **out) {
 
   // values: streamed to the GPU in windows and repacked there
   if ((rc = upload_values(p.get(), src))) return rc;
+  if (p->mode == 1 && (rc = persist_prepare(p.get()))) return rc;
 
   p->device_bytes = eb * (size_t)P.nvals + sizeof(TileDesc) * (P.tiles_a.size() + P.tiles_b.size()) +
                     sizeof(PanelDesc) * P.panels.size() + 8 * (size_t)(P.n_t + 2 * P.nd) +
                     slot_bytes(p->storage) * (size_t)(P.n_tp + P.n_z) +
-                    4 * (P.ysum.seg.size() + P.ysum.ptr.size()) + 4 * (size_t)P.n_t +
+                    4 * (size_t)P.n_t +
                     8 * (P.ysum.list.size() + P.ysum.seg.size() + P.tsum.seg.size());
   if (verbose())
     std::fprintf(stderr,
@@ -957,9 +960,8 @@ void launch_segsum(hmvm_plan *p, int phase, double *y, int accumulate, hipStream
       hipLaunchKernelGGL((tsum_kernel<false, P>), grid, dim3(256), 0, st, P_.tsum.n, p->d_tdesc, slots,
                          (const int32_t *)nullptr, (const float *)nullptr, p->d_t);
   } else {
-    hipLaunchKernelGGL((segsum_kernel<false, P>), dim3((P_.ysum.n + 255) / 256), dim3(256), 0, st, P_.ysum.n,
-                       p->d_yrng, p->d_ylist, slots, (const int32_t *)nullptr, (const float *)nullptr, y,
-                       accumulate);
+    hipLaunchKernelGGL((ysum_kernel<P>), dim3((P_.ysum.n + 255) / 256), dim3(256), 0, st, P_.ysum.n, p->d_yrng,
+                       p->d_ylist, slots, y, accumulate);
   }
 }
 
@@ -992,7 +994,6 @@ void launch_phase(hmvm_plan *p, int phase, const double *x, double *y, int accum
 // run k's tiles, largest first: each XCD then reads about one kXcds-th of the
 // vector, which its L2 holds, instead of all of it.  A workgroup whose run is
 // used up takes tiles of the run with the most left.
-constexpr int kXcds = 8;
 std::vector<TileDesc> xcd_layout(const std::vector<TileDesc> &tiles, int wpb) {
   const size_t n = tiles.size();
   std::vector<size_t> idx(n);
@@ -1186,38 +1187,134 @@ int apply_multi(hmvm_plan *p, const double *X, double *Y, int nrhs, int accumula
   return HMVM_OK;
 }
 
-template <typename S, bool kScaled, int kVar>
-int launch_persistent_v(hmvm_plan *p, const double *x, double *y, int accumulate, hipStream_t st) {
-  constexpr int L = 8;
-  constexpr bool kBV = (kVar & 1) != 0;
-  constexpr bool kPipe = (kVar & 2) != 0, kNT = (kVar & 4) != 0;
-  const Plan &P = p->plan;
-  auto kfn = persistent_kernel<S, kScaled, L, kPipe, kNT, kBV>;
-  const int key = kVar * 1024 + p->block_threads;
-  const size_t lds = sizeof(double) * (p->block_threads / 64) * lds_wave_doubles<S>();
-  if (p->persist_key != key) {  // resident workgroups per CU, queried once per launch shape
-    int per_cu = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kfn, p->block_threads, lds) != hipSuccess ||
-        per_cu < 1)
-      per_cu = 1;
-    p->persist_blocks = per_cu * std::max(p->ncu, 1);
-    p->persist_key = key;
+// ---- persistent launch ---------------------------------------------------------
+// A tile's fixed cost in element-equivalents (descriptor, panel table, first
+// round trip), for dealing tiles to waves when their durations could not be
+// measured.
+constexpr double kTileOverheadElems = 1024;
+
+// Per-tile cost of launch A (which = 0) or B (1): the tile's duration in the
+// launches, measured by the clock-stamping tile kernel (two timed runs);
+// elements plus a fixed overhead if that fails.  Dealt by element counts
+// alone, the waves of the persistent launch finished their launch-A tiles up
+// to 60 us apart at N=101,250 (FP64: 175 against 235 us).
+std::vector<double> tile_costs(hmvm_plan *p, int which) {
+  const std::vector<TileDesc> &tl = which == 0 ? p->plan.tiles_a : p->plan.tiles_b;
+  std::vector<double> c(tl.size()), s(tl.size());
+  double lms = 0;
+  if (!tl.empty() && hmvm_plan_profile_tiles(p, which, 2, c.data(), s.data(), &lms) == HMVM_OK) return c;
+  for (size_t i = 0; i < tl.size(); i++) c[i] = (double)tl[i].nrows * (double)tl[i].ncols + kTileOverheadElems;
+  return c;
+}
+
+// Deals a launch's tiles to nw waves, longest first, each to the wave with
+// the least work so far (what the hardware dispatcher does with one tile per
+// wave); wave w's tiles, in the order dealt, are out[ptr[w] .. ptr[w+1]).
+void deal_tiles(const std::vector<TileDesc> &tiles, const std::vector<double> &costs, int nw,
+                std::vector<TileDesc> *out, std::vector<int32_t> *ptr) {
+  const size_t n = tiles.size();
+  std::vector<size_t> idx(n);
+  for (size_t i = 0; i < n; i++) idx[i] = i;
+  std::stable_sort(idx.begin(), idx.end(), [&](size_t a, size_t b) { return costs[a] > costs[b]; });
+  std::priority_queue<std::pair<double, int>, std::vector<std::pair<double, int>>, std::greater<>> load;
+  for (int w = 0; w < nw; w++) load.push({0.0, w});
+  std::vector<std::vector<size_t>> per((size_t)nw);
+  for (size_t i : idx) {
+    const std::pair<double, int> top = load.top();
+    load.pop();
+    per[(size_t)top.second].push_back(i);
+    load.push({top.first + costs[i], top.second});
   }
-  const int blocks = p->persist_blocks;
-  const int nwaves = blocks * (p->block_threads / 64);
-  const int ns1 = P.n_stage1_tiles;
-  const int na = (int)P.tiles_a.size();
-  const int nt = na + (int)P.tiles_b.size();
-  const int nitems = nt + (P.nd + 63) / 64;
-  hipLaunchKernelGGL(kfn, dim3(blocks), dim3(p->block_threads), lds, st, p->d_tiles_a, ns1, na, p->d_tiles_b, nt,
-                     nitems, nwaves, p->d_panels, (const S *)p->d_vals, x, p->d_t, (typename SlotOf<S>::T *)p->d_slots,
-                     (const float *)p->d_scales, p->d_tmap, p->d_tfac, p->d_yseg, p->d_yptr, p->d_ylist, P.nd, y,
-                     accumulate, p->d_ctr);
-  return HMVM_OK;
+  out->clear();
+  out->reserve(n);
+  ptr->assign((size_t)nw + 1, 0);
+  for (int w = 0; w < nw; w++) {
+    (*ptr)[(size_t)w] = (int32_t)out->size();
+    for (size_t i : per[(size_t)w]) out->push_back(tiles[i]);
+  }
+  (*ptr)[(size_t)nw] = (int32_t)out->size();
+}
+
+// (four loads in flight per lane group rather than the tile launches' eight:
+// the stage loop around the tile code then fits 128 registers, 4 waves per
+// SIMD; with eight it spilled)
+template <typename S, bool kScaled, int kVar>
+auto persistent_fn() {
+  constexpr int L = 4;
+  return persistent_kernel<S, kScaled, L, (kVar & 2) != 0, (kVar & 4) != 0, (kVar & 1) != 0>;
+}
+
+// Workgroups of the persistent launch: as many as the device holds at once
+// (the grid barriers need every workgroup resident).
+template <typename S, bool kScaled, int kVar>
+int persistent_blocks_v(hmvm_plan *p) {
+  const size_t lds = sizeof(double) * (p->block_threads / 64) * lds_wave_doubles<S>();
+  int per_cu = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, persistent_fn<S, kScaled, kVar>(), p->block_threads,
+                                                   lds) != hipSuccess ||
+      per_cu < 1)
+    per_cu = 1;
+  return per_cu * std::max(p->ncu, 1);
 }
 
 template <typename S, bool kScaled>
-int launch_persistent_s(hmvm_plan *p, const double *x, double *y, int accumulate, hipStream_t st) {
+int persistent_blocks_s(hmvm_plan *p) {
+  switch (p->kernel) {
+    case 0: return persistent_blocks_v<S, kScaled, 0>(p);
+    case 1: return persistent_blocks_v<S, kScaled, 1>(p);
+    case 2: return persistent_blocks_v<S, kScaled, 2>(p);
+    case 3: return persistent_blocks_v<S, kScaled, 3>(p);
+    case 4: return persistent_blocks_v<S, kScaled, 4>(p);
+    case 5: return persistent_blocks_v<S, kScaled, 5>(p);
+    case 6: return persistent_blocks_v<S, kScaled, 6>(p);
+    default: return persistent_blocks_v<S, kScaled, 7>(p);
+  }
+}
+
+// Deals both launches' tiles to the persistent launch's waves (again when the
+// launch shape or the tile lists changed since).
+int persist_prepare(hmvm_plan *p) {
+  const int key = p->kernel * 1024 + p->block_threads;
+  if (p->persist_key == key && p->persist_version == p->tiles_version && p->d_ptiles_a) return HMVM_OK;
+  const int blocks = p->storage == kF64   ? persistent_blocks_s<double, false>(p)
+                     : p->storage == kF32 ? persistent_blocks_s<float, false>(p)
+                                          : persistent_blocks_s<uint16_t, true>(p);
+  const int nw = blocks * (p->block_threads / 64);
+  for (void *d : {(void *)p->d_ptiles_a, (void *)p->d_ptiles_b, (void *)p->d_pptr_a, (void *)p->d_pptr_b})
+    if (d) (void)hipFree(d);
+  p->d_ptiles_a = p->d_ptiles_b = nullptr;
+  p->d_pptr_a = p->d_pptr_b = nullptr;
+  p->persist_key = -1;
+  std::vector<TileDesc> ta, tb;
+  std::vector<int32_t> pa, pb;
+  deal_tiles(p->plan.tiles_a, tile_costs(p, 0), nw, &ta, &pa);
+  deal_tiles(p->plan.tiles_b, tile_costs(p, 1), nw, &tb, &pb);
+  int rc;
+  if ((rc = upload(&p->d_ptiles_a, ta)) || (rc = upload(&p->d_ptiles_b, tb)) || (rc = upload(&p->d_pptr_a, pa)) ||
+      (rc = upload(&p->d_pptr_b, pb)))
+    return rc;
+  p->persist_blocks = blocks;
+  p->persist_key = key;
+  p->persist_version = p->tiles_version;
+  if (verbose())
+    std::fprintf(stderr, "hmvm_hip: persistent launch %d workgroups of %d threads (%d CUs), %zu + %zu tiles\n",
+                 blocks, p->block_threads, p->ncu, p->plan.tiles_a.size(), p->plan.tiles_b.size());
+  return HMVM_OK;
+}
+
+template <typename S, bool kScaled, int kVar>
+void launch_persistent_v(hmvm_plan *p, const double *x, double *y, int accumulate, hipStream_t st) {
+  const Plan &P = p->plan;
+  const size_t lds = sizeof(double) * (p->block_threads / 64) * lds_wave_doubles<S>();
+  const int n_t = P.t_fused || P.t_direct ? 0 : P.tsum.n;
+  hipLaunchKernelGGL((persistent_fn<S, kScaled, kVar>()), dim3(p->persist_blocks), dim3(p->block_threads), lds, st,
+                     p->d_ptiles_a, p->d_pptr_a, p->d_ptiles_b, p->d_pptr_b, p->d_panels, (const S *)p->d_vals, x,
+                     p->d_t, (typename SlotOf<S>::T *)p->d_slots, (const float *)p->d_scales, p->d_tmap, p->d_tfac,
+                     n_t, p->d_tdesc, P.nd, p->d_yrng, p->d_ylist, y, accumulate, p->d_ctr);
+}
+
+template <typename S, bool kScaled>
+void launch_persistent_s(hmvm_plan *p, const double *x, double *y, int accumulate, hipStream_t st) {
   switch (p->kernel) {
     case 0: return launch_persistent_v<S, kScaled, 0>(p, x, y, accumulate, st);
     case 1: return launch_persistent_v<S, kScaled, 1>(p, x, y, accumulate, st);
@@ -1230,17 +1327,24 @@ int launch_persistent_s(hmvm_plan *p, const double *x, double *y, int accumulate
   }
 }
 
+// The tile lists are dealt when mode 1 is selected and whenever the launch
+// shape or the tiles change; dealing here (a synchronous upload) only
+// happens if a caller changed them behind the plan's back.
 int launch_persistent(hmvm_plan *p, const double *x, double *y, int accumulate, hipStream_t st) {
-  if (p->storage == kF64) return launch_persistent_s<double, false>(p, x, y, accumulate, st);
-  if (p->storage == kF32) return launch_persistent_s<float, false>(p, x, y, accumulate, st);
-  return launch_persistent_s<uint16_t, true>(p, x, y, accumulate, st);
+  const int rc = persist_prepare(p);
+  if (rc) return rc;
+  if (p->storage == kF64)
+    launch_persistent_s<double, false>(p, x, y, accumulate, st);
+  else if (p->storage == kF32)
+    launch_persistent_s<float, false>(p, x, y, accumulate, st);
+  else
+    launch_persistent_s<uint16_t, true>(p, x, y, accumulate, st);
+  return HMVM_OK;
 }
 
-// The persistent mode needs every stage-2 tile to read its leaf's stage-1
-// partial slots directly (no materialised t).
-bool persistent_ok(const hmvm_plan *p) {
-  return p->mode == 1 && (p->plan.t_fused || p->plan.t_direct) && !p->a_mixed;
-}
+// Mode 1 runs the persistent launch (any t_mode: it sums t itself when the
+// plan materialises t).
+bool persistent_ok(const hmvm_plan *p) { return p->mode == 1; }
 
 // Dense tiles depend only on x and write their own output slots, so in mode 2
 // they run on a second stream, concurrently with the stage-1 tiles, the t sum
@@ -1255,7 +1359,8 @@ bool overlap_ok(const hmvm_plan *p) {
 
 int apply(hmvm_plan *p, const double *d_x, double *d_y, int accumulate, hipStream_t st) {
   if (persistent_ok(p)) {
-    launch_persistent(p, d_x, d_y, accumulate, st);
+    const int rc = launch_persistent(p, d_x, d_y, accumulate, st);
+    if (rc) return rc;
   } else if (overlap_ok(p)) {
     HIP_TRY(hipEventRecord(p->ev_fork, st));
     HIP_TRY(hipStreamWaitEvent(p->stream2, p->ev_fork, 0));
@@ -1327,8 +1432,13 @@ int check_wait(hmvm_plan *p) {
   int f = 0;
   const int rc = take_fault(p, &f);
   if (rc) return rc;
-  return f ? set_error(HMVM_ERR_WAIT, "a bounded wait of the persistent launch gave up; the results are invalid")
-           : HMVM_OK;
+  if (!f) return HMVM_OK;
+  int seen = 0;  // 16 * barrier + XCDs through when a barrier gave up
+  HIP_TRY(hipMemcpy(&seen, &p->d_ctr->seen, sizeof(int), hipMemcpyDeviceToHost));
+  return set_error(HMVM_ERR_WAIT,
+                   "persistent launch of %d workgroups: a bounded wait gave up (barrier %d, %d XCDs through); the "
+                   "results are invalid",
+                   p->persist_blocks, seen / 16, seen % 16);
 }
 
 // ---------------------------------------------------------------------------
@@ -1541,7 +1651,7 @@ int hmvm_plan_set_launch(hmvm_plan *plan, int kernel, int block_threads) {
     return set_error(HMVM_ERR_ARG, "block_threads must be 64, 128, 192 or 256");
   plan->kernel = plan->kernel_b = kernel;
   plan->block_threads = plan->block_threads_b = block_threads;
-  return HMVM_OK;
+  return plan->mode == 1 ? persist_prepare(plan) : HMVM_OK;
 }
 
 int hmvm_plan_reset_launch(hmvm_plan *plan) {
@@ -1551,7 +1661,7 @@ int hmvm_plan_reset_launch(hmvm_plan *plan) {
   plan->kernel_b = ld.kernel_b;
   plan->block_threads = ld.bt_a;
   plan->block_threads_b = ld.bt_b;
-  return HMVM_OK;
+  return plan->mode == 1 ? persist_prepare(plan) : HMVM_OK;
 }
 
 int hmvm_plan_set_phase_launch(hmvm_plan *plan, int which, int kernel, int block_threads) {
@@ -1567,7 +1677,7 @@ int hmvm_plan_set_phase_launch(hmvm_plan *plan, int which, int kernel, int block
     plan->kernel_b = kernel;
     plan->block_threads_b = block_threads;
   }
-  return HMVM_OK;
+  return plan->mode == 1 ? persist_prepare(plan) : HMVM_OK;
 }
 
 int hmvm_plan_reserve_multi(hmvm_plan *plan, int nrhs) {
@@ -1651,10 +1761,12 @@ int hmvm_plan_time_multi(hmvm_plan *plan, int nrhs, int warmup, int iters, hmvm_
 int hmvm_plan_set_mode(hmvm_plan *plan, int mode) {
   if (!plan) return set_error(HMVM_ERR_ARG, "NULL plan");
   if (mode < 0 || mode > 2) return set_error(HMVM_ERR_ARG, "mode must be 0, 1 or 2");
-  if (mode == 1 && !plan->plan.t_fused && !plan->plan.t_direct)
-    return set_error(HMVM_ERR_UNSUPPORTED, "mode 1 (persistent launch) needs a plan built with t_mode 0 or 3");
   if (mode != 0) {
     const int rc = partition_a(plan);
+    if (rc) return rc;
+  }
+  if (mode == 1) {
+    const int rc = persist_prepare(plan);
     if (rc) return rc;
   }
   plan->mode = mode;
@@ -1880,7 +1992,7 @@ int hmvm_plan_time(hmvm_plan *plan, int warmup, int iters, hmvm_timing *tm) {
   for (int it = 0; it < iters && !rc; it++) {
     hipEventRecord(ev[0], st);
     if (persistent_ok(plan)) {
-      launch_persistent(plan, plan->d_x, plan->d_y, 0, st);
+      if ((rc = launch_persistent(plan, plan->d_x, plan->d_y, 0, st))) break;
       for (int phs = 0; phs < 4; phs++) hipEventRecord(ev[phs + 1], st);
     } else {
       for (int phs = 0; phs < 4; phs++) {
@@ -2259,7 +2371,8 @@ int hmvm_plan_set_tile_flags(hmvm_plan *plan, int which, const int *flags) {
   HIP_TRY(hipStreamSynchronize(plan->stream));
   HIP_TRY(hipMemcpy(d, nt.data(), sizeof(TileDesc) * nt.size(), hipMemcpyHostToDevice));
   tl.swap(nt);
-  return HMVM_OK;
+  plan->tiles_version++;
+  return plan->mode == 1 ? persist_prepare(plan) : HMVM_OK;
 }
 
 int hmvm_plan_get_tile_flags(const hmvm_plan *plan, int which, int *flags) {
@@ -2291,7 +2404,8 @@ int hmvm_plan_set_tile_order(hmvm_plan *plan, int which, const int *perm) {
     P.a_stage1.swap(ns);
     plan->a_mixed = order_mixed(P);
   }
-  return HMVM_OK;
+  plan->tiles_version++;
+  return plan->mode == 1 ? persist_prepare(plan) : HMVM_OK;
 }
 
 void hmvm_hip_1(double *v, const matrix *mat, const double *b) { run_variant(v, mat, nullptr, b, 0); }

@@ -108,17 +108,17 @@ typedef struct hmvm_options {
   int order;             /* tile order in a launch: 0 memory, 1 largest first   */
                          /* (launch A: stage-1 and dense tiles in one order)    */
   int mode;              /* 0: launches on one stream; 1: one persistent launch */
-                         /* with a device work queue (needs t_mode 0 or 3: no   */
-                         /* t sum launch; a plan whose t is summed by a launch  */
-                         /* of its own is refused with HMVM_ERR_UNSUPPORTED);   */
-                         /* 2: launches, dense tiles on a second stream         */
-                         /* concurrent with stage 1/2                           */
+                         /* of as many workgroups as the device holds, its four */
+                         /* stages separated by grid barriers, tiles dealt to   */
+                         /* the waves on the host (any t_mode); 2: launches,    */
+                         /* dense tiles on a second stream concurrent with      */
+                         /* stage 1/2                                           */
   int t_mode;            /* stage-2 vector entries: 0 sum the stage-1 partial   */
                          /* slots on the fly, 1 read a materialised t (one more */
                          /* launch), 2 choose per leaf, 3 t written directly by */
                          /* the stage-1 tiles (bands of rank rows over all of a */
                          /* column cluster's columns; no extra launch), -1      */
-                         /* default (1; 0 in mode 1)                            */
+                         /* default (1)                                         */
   int max_rows;          /* rows per tile (0: by storage); at most 192*V-V+1,  */
                          /* V = 16 bytes / element size; when set, also for the */
                          /* stage-2 tiles unless max_rows_b is set              */
@@ -216,9 +216,11 @@ int hmvm_plan_apply_multi(hmvm_plan *plan, const double *d_X, double *d_Y, int n
 int hmvm_plan_apply_multi_host(hmvm_plan *plan, const double *X, double *Y, int nrhs, int accumulate);
 int hmvm_plan_time_multi(hmvm_plan *plan, int nrhs, int warmup, int iters, hmvm_timing *timing);
 
-/* Selects the execution mode of an existing plan (see hmvm_options.mode);
- * mode 1 on a plan whose stage-2 tiles read a materialised t returns
- * HMVM_ERR_UNSUPPORTED and leaves the mode unchanged. */
+/* Selects the execution mode of an existing plan (see hmvm_options.mode).
+ * Mode 1 deals the tiles to the persistent launch's waves here, and again
+ * when hmvm_plan_set_launch / _set_phase_launch / _reset_launch /
+ * _set_tile_flags / _set_tile_order change them (a synchronous upload: make
+ * such changes before capturing applies in a graph). */
 int hmvm_plan_set_mode(hmvm_plan *plan, int mode);
 /* 1 when a bounded wait inside a persistent launch gave up since the last
  * report (the results of that launch are invalid), 0 otherwise; synchronises

@@ -255,17 +255,18 @@ def test_rank_rows_straddle_row_groups(gpu, H, O, tmp_path):
         p.close()
 
 
-@pytest.mark.parametrize("t_mode", [0, 3])
+@pytest.mark.parametrize("t_mode", [0, 1, 3])
 @pytest.mark.parametrize("storage", ["f64", "f32", "bf16"])
 def test_persistent_mode_matches_launches(gpu, H, sphere4096, storage, t_mode):
-    """The single persistent launch (device-wide work queue) computes the same
-    bits as the launches, repeatedly (its counters reset themselves), with
-    stage 2 summing the partial slots (t_mode 0) or reading the t that the
-    stage-1 tiles of the same launch wrote (t_mode 3)."""
+    """The single persistent launch (tiles dealt to the waves, grid barriers
+    between the stages) computes the same bits as the launches, repeatedly
+    (its counters reset themselves), with stage 2 summing the partial slots
+    (t_mode 0), reading the t its own t-sum stage wrote (t_mode 1), or the t
+    that the stage-1 tiles of the same launch wrote (t_mode 3)."""
     h = sphere4096
     b = np.random.default_rng(14).uniform(-1, 1, h.nd)
-    p = H.Plan(h, storage=storage, t_mode=t_mode)  # no t sum launch in either mode
-    assert p.info()["t_fused"] or p.info()["t_direct"]
+    p = H.Plan(h, storage=storage, t_mode=t_mode)
+    assert (t_mode == 0) == bool(p.info()["t_fused"]) and (t_mode == 3) == bool(p.info()["t_direct"])
     y0 = p.apply_host(b)
     p.set_mode(1)
     assert p.info()["mode"] == 1  # the persistent launch is what runs
@@ -319,25 +320,25 @@ def test_per_tile_paths(gpu, H, O, sphere4096, storage, tol):
     p.close()
 
 
-def test_persistent_mode_needs_fused_stage2(gpu, H, sphere4096):
-    """Mode 1 has no launch boundary for a t sum: a plan whose stage-2 tiles
-    read a materialised t refuses it (no silent fallback to the launches),
-    and mode 1 at creation selects t_mode 0 unless told otherwise."""
+def test_persistent_mode_default_plan(gpu, H, sphere4096):
+    """Mode 1 at creation keeps the default t_mode (a materialised t, summed
+    by a stage of the persistent launch) and gives the launches' bits; tile
+    flags set after mode 1 is selected are dealt to the waves again."""
     h = sphere4096
     b = np.random.default_rng(17).uniform(-1, 1, h.nd)
-    p = H.Plan(h)  # default t_mode 1: a materialised t
-    assert not p.info()["t_fused"] and p.info()["mode"] == 0
-    with pytest.raises(H.HMVMError, match="t_mode 0"):
-        p.set_mode(1)
-    assert p.info()["mode"] == 0
+    p = H.Plan(h)
     y0 = p.apply_host(b)
     p.close()
     p = H.Plan(h, mode=1)
-    assert p.info()["t_fused"] and p.info()["mode"] == 1
-    assert rel(p.apply_host(b), y0) <= 1e-14 and p.fault() == 0
+    assert not p.info()["t_fused"] and not p.info()["t_direct"] and p.info()["mode"] == 1
+    assert np.array_equal(p.apply_host(b), y0) and p.fault() == 0
+    f = p.tile_features(0)
+    p.set_tile_flags(0, (f[:, 6] <= 5.0).astype(np.int32))  # every eligible tile lane-per-row
+    y1 = p.apply_host(b)
+    assert rel(y1, y0) <= 1e-14 and p.fault() == 0
+    p.set_mode(0)
+    assert np.array_equal(p.apply_host(b), y1)  # the same tiles through the launches
     p.close()
-    with pytest.raises(H.HMVMError, match="t_mode 0"):
-        H.Plan(h, mode=1, t_mode=1)
 
 
 def test_modes_regroup_launch_a(gpu, H, sphere4096):
