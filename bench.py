@@ -302,7 +302,7 @@ def train_cost_model(H, np, storages, seed=0):
     Xs, ys, recs = [], [], []
     specs = [(20000, "sphere", 1e-4, 32), (36000, "cube", 1e-6, 32), (28000, "cube", 1e-4, 32),
              (44000, "sphere", 1e-6, 32), (70000, "sphere", 1e-4, 32), (130000, "cube", 1e-4, 32),
-             (40000, "sphere", 1e-6, 64), (60000, "cube", 1e-4, 64)]
+             (40000, "sphere", 1e-6, 64), (60000, "cube", 1e-4, 64), (160000, "sphere", 1e-6, 64)]
     t_start = time.time()
     for i, (n, geo, eps, leaf) in enumerate(specs):
         t0 = time.time()
