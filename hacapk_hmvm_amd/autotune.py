@@ -307,8 +307,9 @@ def learned(h, storage, model, w_choices=None, kernels=KERNELS, bts=BLOCK_THREAD
     size with the smallest predicted launch time) is predicted on the host.
     With verify > 0 the `verify` candidates predicted fastest and the library
     defaults are built and timed (a few applies each, with and without the
-    predicted-duration tile order) and the fastest is kept: the model prunes
-    the search to a handful of runs instead of timing all of them.  verify=0
+    predicted-duration tile order, and at each candidate's widths also with
+    the library's launches) and the fastest is kept: the model prunes the
+    search to a handful of runs instead of timing all of them.  verify=0
     takes the best prediction as it is."""
     t0 = time.time()
     w_choices = w_choices or list(itertools.product(W1_CHOICES, W2_CHOICES))
@@ -328,10 +329,11 @@ def learned(h, storage, model, w_choices=None, kernels=KERNELS, bts=BLOCK_THREAD
                       "bt_b": pred[1][2], "pred_ms": pred[0][0] + pred[1][0]})
     cands.sort(key=lambda c: c["pred_ms"])
 
-    def build(c, ordered, paths):
+    def build(c, ordered, paths, default_launch=False):
         p = Plan(h, storage=storage) if c is None else Plan(h, storage=storage, w1=c["w1"], w2=c["w2"])
         if c is not None:
-            _apply_choice(p, c)
+            if not default_launch:  # else the library's variants and workgroup sizes at these widths
+                _apply_choice(p, c)
             for which in (0, 1):
                 if paths:  # each eligible tile on the path predicted faster for it
                     X = p.tile_features(which)
@@ -346,23 +348,33 @@ def learned(h, storage, model, w_choices=None, kernels=KERNELS, bts=BLOCK_THREAD
     if verify <= 0:
         best = dict(cands[0], reordered=bool(reorder), tile_paths=True)
         return best, build(cands[0], reorder, True), time.time() - t0
-    trials = [(None, False, False)]
+    # per predicted width candidate: the predicted launches (in dispatch order,
+    # in predicted-duration order, and with per-tile paths), and the
+    # library's launches at those widths (the network ranks widths better
+    # than variants and workgroup sizes, whose differences show mostly in
+    # the memory system)
+    trials = [(None, False, False, False)]
     for c in cands[:verify]:
-        trials += [(c, False, False)] + ([(c, True, False), (c, True, True)] if reorder else [(c, False, True)])
+        trials += [(c, False, False, False)]
+        trials += [(c, True, False, False), (c, True, True, False)] if reorder else [(c, False, True, False)]
+        trials += [(c, False, False, True)]
     best, best_ms, trial_ms = None, None, []
-    for c, ordered, paths in trials:
-        p = build(c, ordered, paths)
-        ms = p.time(5, 20)["ms_stream"]  # a fresh plan's first applies run slow: warm it up
+    for c, ordered, paths, dl in trials:
+        p = build(c, ordered, paths, dl)
+        # a fresh plan's first applies run slow: warm it up; median of three
+        ms = float(np.median([p.time(3, 15)["ms_stream"] for _ in range(3)]))
         p.close()
         trial_ms.append(round(ms, 5))
         if best_ms is None or ms < best_ms:
-            best, best_ms = (c, ordered, paths), ms
-    c, ordered, paths = best
+            best, best_ms = (c, ordered, paths, dl), ms
+    c, ordered, paths, dl = best
     choice = {"defaults": True} if c is None else dict(c)
+    if dl:
+        choice.update({"kernel_a": None, "bt_a": None, "kernel_b": None, "bt_b": None, "default_launch": True})
     choice.update({"reordered": ordered, "tile_paths": paths, "timed_trials": len(trials), "trial_ms": trial_ms,
                    "candidates": [(cd["w1"], cd["w2"], cd["kernel_a"], cd["bt_a"], cd["kernel_b"], cd["bt_b"],
                                    round(cd["pred_ms"], 5)) for cd in cands[:verify]]})
-    p = build(c, ordered, paths)
+    p = build(c, ordered, paths, dl)
     if paths:
         choice["row_per_lane_tiles"] = [int(p.tile_flags(w).sum()) for w in (0, 1)]
     return choice, p, time.time() - t0

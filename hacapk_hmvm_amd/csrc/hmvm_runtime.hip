@@ -429,16 +429,17 @@ int source_from_batch(const matrix2 *const *mats, int count, int a2_source, Sour
 // cover, for launch A and for the stage-2 tiles of launch B.  Taller tiles
 // only pay off for BF16, whose 8-element lane loads leave short stage-2 tiles
 // with few lanes busy; FP64 stage-2 tiles run best at 128 rows (one 16-byte
-// row group per lane).  Options with 0 (widths,
-// workgroup size, rows) or -1 (kernel) take these.
+// row group per lane, 96-160 within 0.3%), FP32 at 144-160 (256: 2% slower
+// per HMVM), BF16 at 192-320 (256: 2% slower; measured with FP32 slots).
+// Options with 0 (widths, workgroup size, rows) or -1 (kernel) take these.
 struct LaunchDefaults {
   int w1, w2, kernel_a, bt_a, kernel_b, bt_b, max_rows, max_rows_b, w3;
 };
 const LaunchDefaults &launch_defaults(int storage) {
   static const LaunchDefaults d[3] = {
       {32768, 32768, 6, 256, 6, 256, 256, 128, 32768},  // FP64
-      {32768, 24576, 6, 256, 6, 256, 256, 256, 24576},  // FP32
-      {32768, 32768, 6, 256, 6, 256, 512, 256, 32768},  // BF16
+      {32768, 24576, 6, 256, 6, 256, 256, 160, 24576},  // FP32
+      {32768, 32768, 6, 256, 6, 256, 512, 192, 32768},  // BF16
   };
   return d[storage >= kF64 && storage <= kBF16 ? storage : 0];
 }
