@@ -82,6 +82,34 @@ def test_graph_replays_equal_eager(gpu, H, mats):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("storage", ["f64", "bf16"])
+def test_batch_persistent_graph_equals_launches(gpu, H, mats, storage):
+    """A batch plan in the persistent mode (one launch, grid barriers) gives
+    the bits of the four launches, eagerly and as graph replays with eager
+    work in between, and its fault word stays clear."""
+    x = np.random.default_rng(18).uniform(-1, 1, 4400)
+    p = H.Plan(mats, storage=storage)
+    y0 = p.apply_host(x)
+    p.set_mode(1)
+    assert p.info()["mode"] == 1
+    assert np.array_equal(p.apply_host(x), y0)
+    dx, dy = H.DeviceBuffer(8 * 4400), H.DeviceBuffer(8 * 4400)
+    dx.upload(x)
+    g = p.graph(dx.value, dy.value)
+    for r in range(3):
+        dy.upload(np.full(4400, np.nan))
+        g.launch()
+        H.lib.hmvm_device_synchronize()
+        assert np.array_equal(dy.download(np.empty(4400)), y0), "replay %d" % r
+        assert np.array_equal(p.apply_host(x), y0)
+    assert p.fault() == 0
+    g.close()
+    dx.close()
+    dy.close()
+    p.close()
+
+
+@pytest.mark.gpu
 def test_tile_order_and_phase_launch_keep_bits(gpu, H, mats):
     m = mats[2]
     p = H.Plan(m)
