@@ -130,14 +130,14 @@ __device__ __forceinline__ float panel_scale(const PanelDesc &pd, const float *_
 }
 
 // Where a tile's rows go: its output slot (out + out_off + row), or, for a
-// stage-1 tile with a row map (t_mode 3), the entry of t the map names, with
-// both factor scales of that entry's leaf applied under BF16 storage
-// (tfac[e] = the leaf's a1 factor id, its a2 factor is the next id).
+// stage-1 tile with a row map, the entry of t the map names, with both
+// factor scales of that entry's leaf applied under BF16 storage (mscale, one
+// product per map row, so that it loads beside the map entry).
 struct TileOut {
   void *out;  // slots: SlotOf<S>::T of the launch's storage S
   const int32_t *tmap;
   void *tout;  // t: double (one right-hand side), VecOf<S>::T (several)
-  const int32_t *tfac;
+  const double *mscale;
 };
 
 template <typename S, bool kScaled>
@@ -149,7 +149,7 @@ __device__ __forceinline__ void store_row(const TileDesc &td, int row, double v,
     return;
   }
   const int e = o.tmap[td.map_off + row];
-  if constexpr (kScaled) v *= (double)scales[o.tfac[e]] * (double)scales[o.tfac[e] + 1];
+  if constexpr (kScaled) v *= o.mscale[td.map_off + row];
   static_cast<double *>(o.tout)[e] = v;
 }
 
@@ -613,16 +613,17 @@ __global__ __launch_bounds__(256) HMVM_TILE_OCCUPANCY void tile_kernel_timed(con
 
 // ---- segment-sum entries (shared by the segment-sum launches and the
 // persistent launch) --------------------------------------------------------
-// t entry i: scale(i) * sum of nch(i) stage-1 partial slots buf[first(i) +
-// c * stride(i)], c = 0, 1, ... in order, desc[i] = {first, (nch << 16) |
-// stride}; scale(i) = both factor scales of entry i's leaf (a1 = factor_of,
-// a2 = the next id).  Entries past nch reload the last slot and add nothing
-// (unguarded loads).
+// t entry i = d.z: scale(i) * sum of nch stage-1 partial slots buf[d.x +
+// c * stride], c = 0, 1, ... in order, d.y = (nch << 16) | stride; scale(i)
+// = both factor scales of entry i's leaf (a1 = factor_of, a2 = the next id).
+// Entries past nch reload the last slot and add nothing (unguarded loads).
+// The t sum walks a list of such descriptors: the entries that stage-1 tiles
+// write themselves have none.
 template <bool kScaled, typename P>
-__device__ __forceinline__ double tsum_entry(int i, const int2 *__restrict__ desc, const P *__restrict__ buf,
+__device__ __forceinline__ double tsum_entry(const int4 d, const P *__restrict__ buf,
                                              const int32_t *__restrict__ factor_of,
                                              const float *__restrict__ scales) {
-  const int2 d = desc[i];
+  const int i = d.z;
   const int nch = (int)((unsigned)d.y >> 16), stride = d.y & 0xFFFF;
   double a = 0;
   for (int c = 0; c < nch; c += 4) {
@@ -754,7 +755,8 @@ __global__ __launch_bounds__(256) HMVM_TILE_OCCUPANCY void persistent_kernel(
     const TileDesc *__restrict__ tiles_a, const int32_t *__restrict__ ptr_a, const TileDesc *__restrict__ tiles_b,
     const int32_t *__restrict__ ptr_b, const PanelDesc *__restrict__ panels, const S *__restrict__ vals,
     const double *__restrict__ x_vec, double *t, typename SlotOf<S>::T *slots, const float *__restrict__ scales,
-    const int32_t *__restrict__ tmap, const int32_t *__restrict__ tfac, int n_t, const int2 *__restrict__ tdesc,
+    const int32_t *__restrict__ tmap, const double *__restrict__ mscale, const int32_t *__restrict__ tfac, int n_t,
+    const int4 *__restrict__ tdesc,
     int nd, const int2 *__restrict__ yrng, const int64_t *__restrict__ ylist, double *__restrict__ y,
     int accumulate, PersistCounters *ctr) {
   const int lane = (int)(threadIdx.x & 63);
@@ -762,7 +764,7 @@ __global__ __launch_bounds__(256) HMVM_TILE_OCCUPANCY void persistent_kernel(
   const int nwg = (int)gridDim.x;
   extern __shared__ double tile_lds[];
   double *lds = tile_lds + (threadIdx.x >> 6) * lds_wave_doubles<S>();
-  const TileOut o{slots, tmap, t, tfac};  // stage-1 rows with a map go to t
+  const TileOut o{slots, tmap, t, mscale};  // stage-1 rows with a map go to t
   // join: count this workgroup on its XCD (XCC_ID register), then learn how
   // many workgroups each XCD holds once all have joined
   __shared__ int xcd_info[3];  // XCD, its workgroups, XCDs in use
@@ -801,7 +803,10 @@ __global__ __launch_bounds__(256) HMVM_TILE_OCCUPANCY void persistent_kernel(
         run_tile<S, kScaled, kLoads, kPipe, kNT, false, kBV>(tiles[i], panels, vals, x_vec, t, slots, scales, o,
                                                              lane, lds);
     } else if (stage == 1) {
-      for (int i = gt; i < n_t; i += nthreads) t[i] = tsum_entry<kScaled>(i, tdesc, slots, tfac, scales);
+      for (int j = gt; j < n_t; j += nthreads) {
+        const int4 d = tdesc[j];
+        t[d.z] = tsum_entry<kScaled>(d, slots, tfac, scales);
+      }
     } else {
       for (int i = gt; i < nd; i += nthreads) {
         const double a = ysum_entry<8>(i, yrng, ylist, slots);
@@ -983,7 +988,7 @@ __device__ __forceinline__ void store_rows_multi(const TileDesc &td, int row, in
   }
   const int e = o.tmap[td.map_off + row];
   if constexpr (kScaled) {
-    const double f = (double)scales[o.tfac[e]] * (double)scales[o.tfac[e] + 1];
+    const double f = o.mscale[td.map_off + row];
 #pragma unroll
     for (int c = 0; c < NB; ++c) v[c] *= f;
   }
@@ -1439,22 +1444,23 @@ __global__ __launch_bounds__(256) void ysum_kernel(int n, const int2 *__restrict
   out[i] = accumulate ? out[i] + a : a;
 }
 
-// t sum: t[i] = tsum_entry(i) for the n entries of t (the list of slots of
-// an entry is first + c * stride: no list loads).
+// t sum: t[desc[j].z] = tsum_entry(desc[j]) for the n summed entries of t
+// (the list of slots of an entry is first + c * stride: no list loads).
 template <bool kScaled, typename P>
-__global__ __launch_bounds__(256) void tsum_kernel(int n, const int2 *__restrict__ desc, const P *__restrict__ buf,
+__global__ __launch_bounds__(256) void tsum_kernel(int n, const int4 *__restrict__ desc, const P *__restrict__ buf,
                                                    const int32_t *__restrict__ factor_of,
                                                    const float *__restrict__ scales, double *__restrict__ out) {
-  const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
-  if (i >= n) return;
-  out[i] = tsum_entry<kScaled>(i, desc, buf, factor_of, scales);
+  const int j = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (j >= n) return;
+  const int4 d = desc[j];
+  out[d.z] = tsum_entry<kScaled>(d, buf, factor_of, scales);
 }
 
 // tsum_kernel for nrhs right-hand sides (rows of nrhs in buf and out): a
 // thread sums W consecutive right-hand sides with 16-byte loads (W = 2 for
 // FP64 slots, 4 for FP32).
 template <bool kScaled, typename P, typename O>
-__global__ __launch_bounds__(256) void tsum_multi_kernel(int n, int nrhs, const int2 *__restrict__ desc,
+__global__ __launch_bounds__(256) void tsum_multi_kernel(int n, int nrhs, const int4 *__restrict__ desc,
                                                          const P *__restrict__ buf,
                                                          const int32_t *__restrict__ factor_of,
                                                          const float *__restrict__ scales, O *__restrict__ out) {
@@ -1462,8 +1468,9 @@ __global__ __launch_bounds__(256) void tsum_multi_kernel(int n, int nrhs, const 
   const int nw = nrhs / W;
   const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= (int64_t)n * nw) return;
-  const int i = (int)(g / nw), col = (int)(g - (int64_t)i * nw) * W;
-  const int2 d = desc[i];
+  const int j = (int)(g / nw), col = (int)(g - (int64_t)j * nw) * W;
+  const int4 d = desc[j];
+  const int i = d.z;  // the t entry
   const int nch = (int)((unsigned)d.y >> 16), stride = d.y & 0xFFFF;
   double a[W];
 #pragma unroll
@@ -1484,6 +1491,19 @@ __global__ __launch_bounds__(256) void tsum_multi_kernel(int n, int nrhs, const 
     for (int w = 0; w < W; ++w) a[w] *= f;
   }
   store_n<W>(out + (int64_t)i * nrhs + col, a);
+}
+
+// Both factor scales of the leaf of each row of the stage-1 row maps (BF16):
+// mscale[k] = scales[a1] * scales[a1 + 1], a1 = tfac[tmap[k]] (powers of two,
+// so the product is exact).
+__global__ __launch_bounds__(256) void map_scale_kernel(int n, const int32_t *__restrict__ tmap,
+                                                        const int32_t *__restrict__ tfac,
+                                                        const float *__restrict__ scales,
+                                                        double *__restrict__ mscale) {
+  const int k = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (k >= n) return;
+  const int f = tfac[tmap[k]];
+  mscale[k] = (double)scales[f] * (double)scales[f + 1];
 }
 
 // Per-chunk absolute maximum of the source elements (BF16 scale pass 1).

@@ -221,6 +221,7 @@ std::string build_plan(const LeafArrays &la, const PlanOptions &opt, Plan *plan)
   } else {
   // ---- stage 1 layout + tiles -----------------------------------------------------
   int64_t tpos = 0;
+  std::map<int64_t, size_t> single;  // first slot -> stage-1 tile, row groups of one column chunk
   for (auto &g : colg) {
     const int nstrtt = g.first.first, ndt = g.first.second;
     const std::vector<int> &ls = g.second;
@@ -292,6 +293,7 @@ std::string build_plan(const LeafArrays &la, const PlanOptions &opt, Plan *plan)
         P.elems_stage1 += (int64_t)nr * nc;
         P.vec_a += nc;
       }
+      if (nch == 1) single[slot0] = t1.size() - 1;
       for (const Piece &pc : groups[gi]) {
         int kk0 = -1;  // stack row of the piece's first rank, relative to the group
         for (size_t q = 0; q < ls.size(); q++)
@@ -349,9 +351,31 @@ std::string build_plan(const LeafArrays &la, const PlanOptions &opt, Plan *plan)
     });
     P.tsum.n = (int)P.n_t;
     P.tsum.seg.assign((size_t)P.n_t, -1);
+    // A row group of one column chunk whose leaves all read a materialised t
+    // has one partial slot per t entry: its tile writes t itself, through a
+    // row map (scaled like the t sum under BF16), and those t entries are
+    // left out of the t sum.
+    std::map<int64_t, bool> direct;  // first slot -> every piece of the group materialised
+    for (const PieceRec &pc : pieces)
+      if (single.count(pc.slot0)) {
+        auto it = direct.find(pc.slot0);
+        direct[pc.slot0] = (it == direct.end() || it->second) && mat[pc.leaf];
+      }
+    for (const auto &kv : direct)
+      if (kv.second) {
+        TileDesc &td1 = t1[single[kv.first]];
+        td1.map_off = (int)P.tmap.size();
+        P.tmap.resize(P.tmap.size() + (size_t)td1.nrows, -1);
+      }
     for (const PieceRec &pc : pieces) {
       if (!mat[pc.leaf]) continue;
       const int64_t t0 = t_idx[pc.leaf] + pc.k0;
+      auto dit = direct.find(pc.slot0);
+      if (dit != direct.end() && dit->second) {
+        const TileDesc &td1 = t1[single[pc.slot0]];
+        for (int r = 0; r < pc.n; r++) P.tmap[(size_t)td1.map_off + pc.kk0 + r] = (int32_t)(t0 + r);
+        continue;
+      }
       const int seg = (int)P.tsum.ptr.size();
       P.tsum.ptr.push_back((int)P.tsum.list.size());
       for (int c = 0; c < pc.nch; c++) P.tsum.list.push_back(pc.slot0 + (int64_t)c * pc.nr + pc.kk0 - t0);
@@ -577,7 +601,7 @@ std::string verify_plan(const Plan &P, const LeafArrays &la) {
   const int64_t npan = (int64_t)P.panels.size();
   std::vector<int64_t> seen2(P.nlf, 0);
   int64_t seen1 = 0;
-  std::vector<char> t_written(P.t_direct ? (size_t)P.n_t : 0, 0);
+  std::vector<char> t_written((size_t)P.n_t, 0);  // t entries a stage-1 tile writes
   for (int list = 0; list < 2; list++) {
     const std::vector<TileDesc> &tl = list == 0 ? P.tiles_a : P.tiles_b;
     for (const TileDesc &t : tl) {
@@ -632,15 +656,16 @@ std::string verify_plan(const Plan &P, const LeafArrays &la) {
         if (d.vec_sel == kVecTP && d.factor < 1) return "stage-2 panel without an a1 factor";
         if (d.vec_sel == kVecX && d.factor < 0) {  // stage-1 panel
           if (list != 0 || t.npanel != 1) return "stage-1 tile malformed";
-          if (P.t_direct) {  // rows go to t through the map, each t entry once
-            if (t.map_off < 0 || (int64_t)t.map_off + t.nrows > (int64_t)P.tmap.size()) return "t map range";
+          if (P.t_direct && t.map_off < 0) return "stage-1 tile without a t map";
+          if (t.map_off >= 0) {  // rows go to t through the map, each t entry once
+            if (P.t_fused) return "row map without t";
+            if ((int64_t)t.map_off + t.nrows > (int64_t)P.tmap.size()) return "t map range";
             for (int r = 0; r < t.nrows; r++) {
               const int32_t e = P.tmap[(size_t)t.map_off + r];
               if (e < 0 || e >= P.n_t) return "t map entry out of range";
               if (t_written[(size_t)e]++) return "t entry written twice";
             }
           } else {
-            if (t.map_off >= 0) return "row map without direct t";
             if ((int64_t)t.out_off + t.nrows > P.n_tp) return "stage-1 slot out of range";
           }
           seen1 += (int64_t)t.nrows * d.ncols;
@@ -687,8 +712,9 @@ std::string verify_plan(const Plan &P, const LeafArrays &la) {
     if (la.ltmtx[i] == 1) want1 += (int64_t)la.kt[i] * la.ndt[i];
   }
   if (seen1 != want1) return "stage-1 coverage wrong";
-  for (char w : t_written)
-    if (!w) return "t entry never written";
+  if (P.t_direct)
+    for (char w : t_written)
+      if (!w) return "t entry never written";
 
   // 3. segment sums index inside their buffers
   if ((int64_t)P.tsum.seg.size() != P.tsum.n || (int64_t)P.ysum.seg.size() != P.ysum.n || P.ysum.n != P.nd ||
@@ -700,7 +726,11 @@ std::string verify_plan(const Plan &P, const LeafArrays &la) {
   const int64_t ntseg = (int64_t)P.tsum.ptr.size() - 1, nyseg = (int64_t)P.ysum.ptr.size() - 1;
   for (int i = 0; i < P.tsum.n; i++) {
     const int s = P.tsum.seg[i];
-    if (s < 0 || s >= ntseg) return "t segment id out of range";
+    // every t entry comes from its stage-1 tile's row map or from a t sum segment
+    if (s < 0 && !t_written[(size_t)i]) return "t entry never written";
+    if (s < 0) continue;
+    if (t_written[(size_t)i]) return "t entry written by a tile and by the t sum";
+    if (s >= ntseg) return "t segment id out of range";
     const int e0 = P.tsum.ptr[s], e1 = P.tsum.ptr[s + 1];
     for (int e = e0; e < e1; e++) {
       const int64_t a = P.tsum.list[e] + i;

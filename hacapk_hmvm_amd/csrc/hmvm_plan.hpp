@@ -148,7 +148,8 @@ struct Plan {
   int nfactors = 0;
   bool t_fused = true;               // no stage-2 panel reads the materialised t (no t sum launch)
   bool t_direct = false;             // t_mode 3: stage-1 tiles write t themselves (no t sum launch)
-  std::vector<int32_t> tmap;         // t_mode 3: t index of every row of every stage-1 tile
+  std::vector<int32_t> tmap;         // t index of every row of the stage-1 tiles with a row map (all of them
+                                     // under t_mode 3; those of one-chunk row groups otherwise)
   std::vector<RepackDesc> repack;
   std::vector<int32_t> factor_of_trow;  // t index -> a1 factor id (BF16 t scaling)
   std::vector<TileDesc> tiles_a, tiles_b;  // tiles_a: stage-1 and dense tiles

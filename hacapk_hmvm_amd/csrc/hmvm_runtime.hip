@@ -103,23 +103,24 @@ std::vector<int2> seg_ranges(const SegSum &g) {
   return r;
 }
 
-// Per-entry {first slot, (nch << 16) | stride} of the t sum: every segment
-// lists its slots at one stride (a rank row's partial slots in the column
-// chunks of its row group are one row group apart).  False when a segment
-// does not, or its fields do not fit.
-bool t_descs(const SegSum &g, std::vector<int2> *out) {
-  out->assign(g.seg.size(), make_int2(0, 0));
+// {first slot, (nch << 16) | stride, t entry, 0} of every summed t entry, in
+// entry order: every segment lists its slots at one stride (a rank row's
+// partial slots in the column chunks of its row group are one row group
+// apart).  Entries without a segment are written by their stage-1 tile and
+// get no descriptor.  False when a segment is not at one stride, or its
+// fields do not fit.
+bool t_descs(const SegSum &g, std::vector<int4> *out) {
+  out->clear();
   for (size_t i = 0; i < g.seg.size(); i++) {
     const int s = g.seg[i];
     if (s < 0) continue;
     const int e0 = g.ptr[s], e1 = g.ptr[s + 1], nch = e1 - e0;
-    if (nch == 0) continue;
-    const int64_t first = g.list[e0] + (int64_t)i;
+    const int64_t first = nch > 0 ? g.list[e0] + (int64_t)i : 0;
     const int64_t stride = nch > 1 ? g.list[e0 + 1] - g.list[e0] : 0;
     if (nch > 0xFFFF || stride < 0 || stride > 0xFFFF || first < 0 || first > INT32_MAX) return false;
     for (int e = e0 + 1; e < e1; e++)
       if (g.list[e] != g.list[e0] + (int64_t)(e - e0) * stride) return false;
-    (*out)[i] = make_int2((int)first, (int)(((unsigned)nch << 16) | (unsigned)stride));
+    out->push_back(make_int4((int)first, (int)(((unsigned)nch << 16) | (unsigned)stride), (int)i, 0));
   }
   return true;
 }
@@ -183,10 +184,12 @@ struct hmvm_plan {
   double *d_t = nullptr;
   void *d_slots = nullptr;  // stage-1 partial slots, then output slots: slot_bytes(storage) each
   int32_t *d_tfac = nullptr;
-  int32_t *d_tmap = nullptr;  // t_mode 3: t index of every stage-1 tile row
+  int32_t *d_tmap = nullptr;  // t index of every row of the stage-1 tiles that write t
+  double *d_mscale = nullptr;  // BF16: both factor scales of each d_tmap row's leaf
   int64_t *d_ylist = nullptr;
   int2 *d_yrng = nullptr;   // per-entry list ranges of the y sum
-  int2 *d_tdesc = nullptr;  // t sum: {first slot, (nch << 16) | stride} per entry
+  int4 *d_tdesc = nullptr;  // t sum: {first slot, (nch << 16) | stride, entry, 0} per summed entry
+  int n_tsum = 0;           // summed t entries (descriptors in d_tdesc)
   float *d_scales = nullptr;
   double *d_x = nullptr, *d_y = nullptr;  // staging for host-pointer calls
   size_t device_bytes = 0;
@@ -195,7 +198,7 @@ struct hmvm_plan {
     void *ptrs[] = {d_vals,  d_tiles_a, d_tiles_b, d_panels, d_t,      d_slots,
                     d_tfac,  d_tmap,    d_ylist,  d_scales, d_x,      d_y,
                     d_yrng,  d_tdesc,   d_mtiles_a, d_mtiles_b, d_ptiles_a, d_ptiles_b, d_pptr_a, d_pptr_b,
-                    d_ctr,   d_mslots,  d_mt,      d_mX,     d_mY,     d_mXc};
+                    d_ctr,   d_mslots,  d_mt,      d_mX,     d_mY,     d_mXc,    d_mscale};
     for (void *p : ptrs)
       if (p) (void)hipFree(p);
     if (ev_fork) (void)hipEventDestroy(ev_fork);
@@ -788,13 +791,14 @@ int create_plan(Source &src, const hmvm_options *opt, hmvm_plan **out) {
       (rc = upload(&p->d_ylist, P.ysum.list)) || (rc = upload(&p->d_yrng, seg_ranges(P.ysum))))
     return rc;
   {
-    std::vector<int2> tdesc;
+    std::vector<int4> tdesc;
     if (!t_descs(P.tsum, &tdesc))
       return set_error(HMVM_ERR_UNSUPPORTED, "t sum segment with more than 65535 chunks or an irregular stride");
     if ((rc = upload(&p->d_tdesc, tdesc))) return rc;
+    p->n_tsum = (int)tdesc.size();
   }
   if (p->mode != 0 && (rc = partition_a(p.get()))) return rc;
-  if (P.t_direct && (rc = upload(&p->d_tmap, P.tmap))) return rc;
+  if (!P.tmap.empty() && (rc = upload(&p->d_tmap, P.tmap))) return rc;
   if (p->storage == kBF16) {
     if ((rc = upload(&p->d_tfac, P.factor_of_trow))) return rc;
     if ((rc = dalloc(&p->d_scales, (size_t)P.nfactors))) return rc;
@@ -811,13 +815,20 @@ int create_plan(Source &src, const hmvm_options *opt, hmvm_plan **out) {
 
   // values: streamed to the GPU in windows and repacked there
   if ((rc = upload_values(p.get(), src))) return rc;
+  if (p->storage == kBF16 && !P.tmap.empty()) {
+    if ((rc = dalloc(&p->d_mscale, P.tmap.size()))) return rc;
+    hipLaunchKernelGGL(map_scale_kernel, dim3((unsigned)((P.tmap.size() + 255) / 256)), dim3(256), 0, p->stream,
+                       (int)P.tmap.size(), p->d_tmap, p->d_tfac, (const float *)p->d_scales, p->d_mscale);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(p->stream));
+  }
   if (p->mode == 1 && (rc = persist_prepare(p.get()))) return rc;
 
   p->device_bytes = eb * (size_t)P.nvals + sizeof(TileDesc) * (P.tiles_a.size() + P.tiles_b.size()) +
                     sizeof(PanelDesc) * P.panels.size() + 8 * (size_t)(P.n_t + 2 * P.nd) +
                     slot_bytes(p->storage) * (size_t)(P.n_tp + P.n_z) +
-                    4 * (size_t)P.n_t +
-                    8 * (P.ysum.list.size() + P.ysum.seg.size() + P.tsum.seg.size());
+                    4 * (size_t)P.n_t + (p->d_mscale ? 12 : 4) * P.tmap.size() + sizeof(int4) * (size_t)p->n_tsum +
+                    8 * (P.ysum.list.size() + P.ysum.seg.size());
   if (verbose())
     std::fprintf(stderr,
                  "hmvm_hip: plan nd=%d nlf=%d len=%lld storage=%d tiles=%zu+%zu panels=%zu t=%lld tslots=%lld "
@@ -831,7 +842,7 @@ int create_plan(Source &src, const hmvm_options *opt, hmvm_plan **out) {
 // Where the tiles of one launch store their rows: the slots, and for
 // stage-1 tiles with a row map the plan's t (or the multi-RHS T).
 TileOut tile_out(const hmvm_plan *p, void *out, void *tout) {
-  return TileOut{out, p->d_tmap, tout, p->d_tfac};
+  return TileOut{out, p->d_tmap, tout, p->d_mscale};
 }
 
 // kind: unused by the one-wave-per-tile launch (kept so every launch names
@@ -952,13 +963,13 @@ void launch_segsum(hmvm_plan *p, int phase, double *y, int accumulate, hipStream
   const Plan &P_ = p->plan;
   const P *slots = (const P *)p->d_slots;
   if (phase == 1) {
-    if (P_.t_fused || P_.tsum.n == 0) return;
-    const dim3 grid((P_.tsum.n + 255) / 256);
+    if (P_.t_fused || p->n_tsum == 0) return;
+    const dim3 grid((p->n_tsum + 255) / 256);
     if (p->storage == kBF16)
-      hipLaunchKernelGGL((tsum_kernel<true, P>), grid, dim3(256), 0, st, P_.tsum.n, p->d_tdesc, slots, p->d_tfac,
+      hipLaunchKernelGGL((tsum_kernel<true, P>), grid, dim3(256), 0, st, p->n_tsum, p->d_tdesc, slots, p->d_tfac,
                          p->d_scales, p->d_t);
     else
-      hipLaunchKernelGGL((tsum_kernel<false, P>), grid, dim3(256), 0, st, P_.tsum.n, p->d_tdesc, slots,
+      hipLaunchKernelGGL((tsum_kernel<false, P>), grid, dim3(256), 0, st, p->n_tsum, p->d_tdesc, slots,
                          (const int32_t *)nullptr, (const float *)nullptr, p->d_t);
   } else {
     hipLaunchKernelGGL((ysum_kernel<P>), dim3((P_.ysum.n + 255) / 256), dim3(256), 0, st, P_.ysum.n, p->d_yrng,
@@ -1109,16 +1120,16 @@ void launch_segsum_multi(hmvm_plan *p, int phase, double *Y, int nrhs, int accum
   const P *slots = (const P *)p->d_mslots;
   if (phase == 1) {
     // 16-byte loads and stores (slots and t are the plan's own allocations)
-    if (P_.t_fused || P_.tsum.n == 0) return;
+    if (P_.t_fused || p->n_tsum == 0) return;
     // t in the multi-RHS operand type (VecOf: FP64, FP32 or BF16 bits)
-    const int64_t tw = (int64_t)P_.tsum.n * (nrhs / (16 / (int)sizeof(P)));
+    const int64_t tw = (int64_t)p->n_tsum * (nrhs / (16 / (int)sizeof(P)));
     const dim3 grid((unsigned)((tw + 255) / 256));
     P *T = (P *)p->d_mt;
     if (p->storage == kBF16)
-      hipLaunchKernelGGL((tsum_multi_kernel<true, P, uint16_t>), grid, dim3(256), 0, st, P_.tsum.n, nrhs,
+      hipLaunchKernelGGL((tsum_multi_kernel<true, P, uint16_t>), grid, dim3(256), 0, st, p->n_tsum, nrhs,
                          p->d_tdesc, slots, p->d_tfac, p->d_scales, (uint16_t *)p->d_mt);
     else
-      hipLaunchKernelGGL((tsum_multi_kernel<false, P, P>), grid, dim3(256), 0, st, P_.tsum.n, nrhs, p->d_tdesc,
+      hipLaunchKernelGGL((tsum_multi_kernel<false, P, P>), grid, dim3(256), 0, st, p->n_tsum, nrhs, p->d_tdesc,
                          slots, (const int32_t *)nullptr, (const float *)nullptr, T);
   } else if (((uintptr_t)Y & 15) == 0) {  // 16-byte slot loads where the caller's Y is 16-byte aligned
     constexpr int W = 16 / sizeof(P);
@@ -1307,10 +1318,10 @@ template <typename S, bool kScaled, int kVar>
 void launch_persistent_v(hmvm_plan *p, const double *x, double *y, int accumulate, hipStream_t st) {
   const Plan &P = p->plan;
   const size_t lds = sizeof(double) * (p->block_threads / 64) * lds_wave_doubles<S>();
-  const int n_t = P.t_fused || P.t_direct ? 0 : P.tsum.n;
+  const int n_t = P.t_fused || P.t_direct ? 0 : p->n_tsum;
   hipLaunchKernelGGL((persistent_fn<S, kScaled, kVar>()), dim3(p->persist_blocks), dim3(p->block_threads), lds, st,
                      p->d_ptiles_a, p->d_pptr_a, p->d_ptiles_b, p->d_pptr_b, p->d_panels, (const S *)p->d_vals, x,
-                     p->d_t, (typename SlotOf<S>::T *)p->d_slots, (const float *)p->d_scales, p->d_tmap, p->d_tfac,
+                     p->d_t, (typename SlotOf<S>::T *)p->d_slots, (const float *)p->d_scales, p->d_tmap, p->d_mscale, p->d_tfac,
                      n_t, p->d_tdesc, P.nd, p->d_yrng, p->d_ylist, y, accumulate, p->d_ctr);
 }
 

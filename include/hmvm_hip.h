@@ -115,10 +115,12 @@ typedef struct hmvm_options {
                          /* stage 1/2                                           */
   int t_mode;            /* stage-2 vector entries: 0 sum the stage-1 partial   */
                          /* slots on the fly, 1 read a materialised t (one more */
-                         /* launch), 2 choose per leaf, 3 t written directly by */
-                         /* the stage-1 tiles (bands of rank rows over all of a */
-                         /* column cluster's columns; no extra launch), -1      */
-                         /* default (1)                                         */
+                         /* launch; the stage-1 tiles of a row group of one     */
+                         /* column chunk write their t rows themselves and the  */
+                         /* t sum adds the rest), 2 choose per leaf (likewise), */
+                         /* 3 t written directly by the stage-1 tiles (bands of */
+                         /* rank rows over all of a column cluster's columns;   */
+                         /* no extra launch), -1 default (1)                    */
   int max_rows;          /* rows per tile (0: by storage); at most 192*V-V+1,  */
                          /* V = 16 bytes / element size; when set, also for the */
                          /* stage-2 tiles unless max_rows_b is set              */
@@ -388,7 +390,8 @@ int hmvm_bicgstab_static(hmvm_plan *plan, const double *d_b, double *d_u, int ma
  * The schedule a plan runs, as plain arrays: the autotuner's features, and
  * the CPU tests that replay it.  Element (r, j) of a tile's panel p is
  * values[val_off + r + j*ld]; row r of a tile goes to slot out_off + r, or to
- * t[tmap[map_off + r]] when map_off >= 0; the panel's vector entry j is
+ * t[tmap[map_off + r]] when map_off >= 0 (t entries so written have seg -1 in
+ * the t sum, which leaves them alone); the panel's vector entry j is
  * x[vec_off + j] (vec_sel 0), t[vec_off + j] (1) or
  * sum_{c<nch} slots[vec_off + c*stride + j] (2, vec_chunks = nch<<16 | stride).  Repack record: values[dst_off + r +
  * c*dst_cs] = source[src_off + r*src_rs + c*src_cs].  Segment sums:

@@ -43,18 +43,19 @@ def replay(H, h, x, source="rowmat", **kw):
             else:
                 slots[td["out_off"]:td["out_off"] + td["nrows"]] = acc
 
-    def segsum(ss):
-        out = np.zeros(ss["n"])
+    def segsum(ss, out):
         for i in range(ss["n"]):
             sg = ss["seg"][i]
-            out[i] = sum(slots[e + i] for e in ss["list"][ss["ptr"][sg]:ss["ptr"][sg + 1]])
+            if sg >= 0:  # entries without a segment were written by their stage-1 tile
+                out[i] = sum(slots[e + i] for e in ss["list"][ss["ptr"][sg]:ss["ptr"][sg + 1]])
         return out
 
     run(s["tiles_a"])
     if not s["t_fused"] and not s["t_direct"]:
-        t[:s["n_t"]] = segsum(s["tsum"])
+        segsum(s["tsum"], t)
     run(s["tiles_b"])
-    return segsum(s["ysum"]), s
+    assert all(sg >= 0 for sg in s["ysum"]["seg"])
+    return segsum(s["ysum"], np.zeros(s["ysum"]["n"])), s
 
 
 def rel(a, b):
