@@ -77,6 +77,13 @@ def test_replay_matches_oracle(H, O, kw, source, t_mode):
     if t_mode == 3:  # no partial slots, no t sum: every t entry written once by a stage-1 tile
         assert s["t_direct"] and s["n_tslots"] == 0 and s["tsum"]["n"] == 0
         assert np.array_equal(np.sort(s["tmap"]), np.arange(s["n_t"]))
+    if t_mode in (1, 2) and not s["t_fused"]:
+        # each t entry comes either from a stage-1 tile's row map or from the t sum, never both
+        seg = np.asarray(s["tsum"]["seg"])
+        summed = np.flatnonzero(seg >= 0)
+        assert np.array_equal(np.sort(np.concatenate([s["tmap"], summed])), np.arange(s["n_t"]))
+        if t_mode == 1 and not kw:  # default widths: the one-chunk row groups write t themselves
+            assert 0 < len(s["tmap"]) < s["n_t"]
     assert rel(y, O.seq_2(h, x)) <= 1e-13
 
 
