@@ -77,7 +77,7 @@ class hmvm_panel(ctypes.Structure):
 
 class hmvm_tile(ctypes.Structure):
     _fields_ = [("nrows", ctypes.c_int), ("panel0", ctypes.c_int), ("npanel", ctypes.c_int), ("out_off", ctypes.c_int),
-                ("map_off", ctypes.c_int), ("flags", ctypes.c_int), ("ncols", ctypes.c_int), ("pad", ctypes.c_int),
+                ("map_off", ctypes.c_int), ("flags", ctypes.c_int), ("ncols", ctypes.c_int), ("map_off_multi", ctypes.c_int),
                 ("first", hmvm_panel)]
 
 
@@ -676,7 +676,7 @@ def _struct_array(ptr, n, dtype):
 PANEL_DT = np.dtype([("val_off", "<i8"), ("ld", "<i4"), ("ncols", "<i4"), ("vec_off", "<i4"), ("vec_sel", "<i4"),
                      ("factor", "<i4"), ("vec_chunks", "<i4")])
 TILE_DT = np.dtype([("nrows", "<i4"), ("panel0", "<i4"), ("npanel", "<i4"), ("out_off", "<i4"), ("map_off", "<i4"),
-                    ("flags", "<i4"), ("ncols", "<i4"), ("pad", "<i4"), ("first", PANEL_DT)])
+                    ("flags", "<i4"), ("ncols", "<i4"), ("map_off_multi", "<i4"), ("first", PANEL_DT)])
 REPACK_DT = np.dtype([("src_off", "<i8"), ("dst_off", "<i8"), ("rows", "<i4"), ("cols", "<i4"), ("src_rs", "<i4"),
                       ("src_cs", "<i4"), ("dst_cs", "<i4"), ("factor", "<i4")])
 

@@ -613,17 +613,17 @@ __global__ __launch_bounds__(256) HMVM_TILE_OCCUPANCY void tile_kernel_timed(con
 
 // ---- segment-sum entries (shared by the segment-sum launches and the
 // persistent launch) --------------------------------------------------------
-// t entry i = d.z: scale(i) * sum of nch stage-1 partial slots buf[d.x +
-// c * stride], c = 0, 1, ... in order, d.y = (nch << 16) | stride; scale(i)
-// = both factor scales of entry i's leaf (a1 = factor_of, a2 = the next id).
+// t entry i: scale(i) * sum of nch stage-1 partial slots buf[d.x + c *
+// stride], c = 0, 1, ... in order, d.y = (nch << 16) | stride; scale(i) =
+// both factor scales of entry i's leaf (a1 = factor_of, a2 = the next id).
 // Entries past nch reload the last slot and add nothing (unguarded loads).
-// The t sum walks a list of such descriptors: the entries that stage-1 tiles
-// write themselves have none.
+// The one-vector t sum has a descriptor per entry (int2, entry = index); the
+// multi-RHS one lists only the entries no stage-1 row map writes (int4, entry
+// = d.z).
 template <bool kScaled, typename P>
-__device__ __forceinline__ double tsum_entry(const int4 d, const P *__restrict__ buf,
+__device__ __forceinline__ double tsum_entry(int i, const int2 d, const P *__restrict__ buf,
                                              const int32_t *__restrict__ factor_of,
                                              const float *__restrict__ scales) {
-  const int i = d.z;
   const int nch = (int)((unsigned)d.y >> 16), stride = d.y & 0xFFFF;
   double a = 0;
   for (int c = 0; c < nch; c += 4) {
@@ -756,7 +756,7 @@ __global__ __launch_bounds__(256) HMVM_TILE_OCCUPANCY void persistent_kernel(
     const int32_t *__restrict__ ptr_b, const PanelDesc *__restrict__ panels, const S *__restrict__ vals,
     const double *__restrict__ x_vec, double *t, typename SlotOf<S>::T *slots, const float *__restrict__ scales,
     const int32_t *__restrict__ tmap, const double *__restrict__ mscale, const int32_t *__restrict__ tfac, int n_t,
-    const int4 *__restrict__ tdesc,
+    const int2 *__restrict__ tdesc,
     int nd, const int2 *__restrict__ yrng, const int64_t *__restrict__ ylist, double *__restrict__ y,
     int accumulate, PersistCounters *ctr) {
   const int lane = (int)(threadIdx.x & 63);
@@ -803,10 +803,7 @@ __global__ __launch_bounds__(256) HMVM_TILE_OCCUPANCY void persistent_kernel(
         run_tile<S, kScaled, kLoads, kPipe, kNT, false, kBV>(tiles[i], panels, vals, x_vec, t, slots, scales, o,
                                                              lane, lds);
     } else if (stage == 1) {
-      for (int j = gt; j < n_t; j += nthreads) {
-        const int4 d = tdesc[j];
-        t[d.z] = tsum_entry<kScaled>(d, slots, tfac, scales);
-      }
+      for (int i = gt; i < n_t; i += nthreads) t[i] = tsum_entry<kScaled>(i, tdesc[i], slots, tfac, scales);
     } else {
       for (int i = gt; i < nd; i += nthreads) {
         const double a = ysum_entry<8>(i, yrng, ylist, slots);
@@ -1444,21 +1441,20 @@ __global__ __launch_bounds__(256) void ysum_kernel(int n, const int2 *__restrict
   out[i] = accumulate ? out[i] + a : a;
 }
 
-// t sum: t[desc[j].z] = tsum_entry(desc[j]) for the n summed entries of t
-// (the list of slots of an entry is first + c * stride: no list loads).
+// t sum: t[i] = tsum_entry(i) for the n entries of t (the list of slots of
+// an entry is first + c * stride: no list loads).
 template <bool kScaled, typename P>
-__global__ __launch_bounds__(256) void tsum_kernel(int n, const int4 *__restrict__ desc, const P *__restrict__ buf,
+__global__ __launch_bounds__(256) void tsum_kernel(int n, const int2 *__restrict__ desc, const P *__restrict__ buf,
                                                    const int32_t *__restrict__ factor_of,
                                                    const float *__restrict__ scales, double *__restrict__ out) {
-  const int j = (int)(blockIdx.x * blockDim.x + threadIdx.x);
-  if (j >= n) return;
-  const int4 d = desc[j];
-  out[d.z] = tsum_entry<kScaled>(d, buf, factor_of, scales);
+  const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (i >= n) return;
+  out[i] = tsum_entry<kScaled>(i, desc[i], buf, factor_of, scales);
 }
 
-// tsum_kernel for nrhs right-hand sides (rows of nrhs in buf and out): a
-// thread sums W consecutive right-hand sides with 16-byte loads (W = 2 for
-// FP64 slots, 4 for FP32).
+// tsum_kernel for nrhs right-hand sides (rows of nrhs in buf and out) over
+// the n entries listed in desc (entry = d.z): a thread sums W consecutive
+// right-hand sides with 16-byte loads (W = 2 for FP64 slots, 4 for FP32).
 template <bool kScaled, typename P, typename O>
 __global__ __launch_bounds__(256) void tsum_multi_kernel(int n, int nrhs, const int4 *__restrict__ desc,
                                                          const P *__restrict__ buf,

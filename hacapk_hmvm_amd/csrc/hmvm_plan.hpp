@@ -81,7 +81,10 @@ struct TileDesc {         // 64 bytes (one scalar load)
   int32_t map_off = -1;   // >= 0: row r of the tile goes to t[tmap[map_off + r]]
   int32_t flags = 0;      // kTileRowPerLane: lane-per-row path even when lanes per column <= 32
   int32_t ncols = 0;      // columns of all the tile's panels together
-  int32_t pad = 0;
+  // >= 0: in the multi-RHS launches row r goes to t[tmap[map_off_multi + r]]
+  // (a stage-1 tile of a row group of one column chunk; the one-vector
+  // launches write its slots, which the t sum adds); map_off under t_mode 3
+  int32_t map_off_multi = -1;
   // a copy of panels[panel0]: the tile's first matrix loads need nothing but
   // this descriptor
   PanelDesc first{};
@@ -149,7 +152,8 @@ struct Plan {
   bool t_fused = true;               // no stage-2 panel reads the materialised t (no t sum launch)
   bool t_direct = false;             // t_mode 3: stage-1 tiles write t themselves (no t sum launch)
   std::vector<int32_t> tmap;         // t index of every row of the stage-1 tiles with a row map (all of them
-                                     // under t_mode 3; those of one-chunk row groups otherwise)
+                                     // under t_mode 3; otherwise those of one-chunk row groups, used by the
+                                     // multi-RHS launches only)
   std::vector<RepackDesc> repack;
   std::vector<int32_t> factor_of_trow;  // t index -> a1 factor id (BF16 t scaling)
   std::vector<TileDesc> tiles_a, tiles_b;  // tiles_a: stage-1 and dense tiles

@@ -106,14 +106,14 @@ std::vector<int2> seg_ranges(const SegSum &g) {
 // {first slot, (nch << 16) | stride, t entry, 0} of every summed t entry, in
 // entry order: every segment lists its slots at one stride (a rank row's
 // partial slots in the column chunks of its row group are one row group
-// apart).  Entries without a segment are written by their stage-1 tile and
-// get no descriptor.  False when a segment is not at one stride, or its
-// fields do not fit.
-bool t_descs(const SegSum &g, std::vector<int4> *out) {
+// apart).  Entries marked in `skip` (written through a stage-1 row map) get
+// no descriptor.  False when a segment is not at one stride, or its fields do
+// not fit.
+bool t_descs(const SegSum &g, const std::vector<char> &skip, std::vector<int4> *out) {
   out->clear();
   for (size_t i = 0; i < g.seg.size(); i++) {
     const int s = g.seg[i];
-    if (s < 0) continue;
+    if (s < 0 || (!skip.empty() && skip[i])) continue;
     const int e0 = g.ptr[s], e1 = g.ptr[s + 1], nch = e1 - e0;
     const int64_t first = nch > 0 ? g.list[e0] + (int64_t)i : 0;
     const int64_t stride = nch > 1 ? g.list[e0 + 1] - g.list[e0] : 0;
@@ -188,8 +188,10 @@ struct hmvm_plan {
   double *d_mscale = nullptr;  // BF16: both factor scales of each d_tmap row's leaf
   int64_t *d_ylist = nullptr;
   int2 *d_yrng = nullptr;   // per-entry list ranges of the y sum
-  int4 *d_tdesc = nullptr;  // t sum: {first slot, (nch << 16) | stride, entry, 0} per summed entry
-  int n_tsum = 0;           // summed t entries (descriptors in d_tdesc)
+  int2 *d_tdesc = nullptr;    // t sum: {first slot, (nch << 16) | stride} per entry
+  int n_tsum = 0;             // t entries summed by the one-vector launches
+  int4 *d_tdesc_m = nullptr;  // multi-RHS t sum: {first, (nch << 16) | stride, entry, 0} of the
+  int n_tsum_m = 0;           // entries no stage-1 row map writes
   float *d_scales = nullptr;
   double *d_x = nullptr, *d_y = nullptr;  // staging for host-pointer calls
   size_t device_bytes = 0;
@@ -197,7 +199,7 @@ struct hmvm_plan {
   ~hmvm_plan() {
     void *ptrs[] = {d_vals,  d_tiles_a, d_tiles_b, d_panels, d_t,      d_slots,
                     d_tfac,  d_tmap,    d_ylist,  d_scales, d_x,      d_y,
-                    d_yrng,  d_tdesc,   d_mtiles_a, d_mtiles_b, d_ptiles_a, d_ptiles_b, d_pptr_a, d_pptr_b,
+                    d_yrng,  d_tdesc,   d_tdesc_m, d_mtiles_a, d_mtiles_b, d_ptiles_a, d_ptiles_b, d_pptr_a, d_pptr_b,
                     d_ctr,   d_mslots,  d_mt,      d_mX,     d_mY,     d_mXc,    d_mscale};
     for (void *p : ptrs)
       if (p) (void)hipFree(p);
@@ -791,11 +793,23 @@ int create_plan(Source &src, const hmvm_options *opt, hmvm_plan **out) {
       (rc = upload(&p->d_ylist, P.ysum.list)) || (rc = upload(&p->d_yrng, seg_ranges(P.ysum))))
     return rc;
   {
-    std::vector<int4> tdesc;
-    if (!t_descs(P.tsum, &tdesc))
+    std::vector<int4> all, multi;
+    if (!t_descs(P.tsum, {}, &all))
       return set_error(HMVM_ERR_UNSUPPORTED, "t sum segment with more than 65535 chunks or an irregular stride");
-    if ((rc = upload(&p->d_tdesc, tdesc))) return rc;
-    p->n_tsum = (int)tdesc.size();
+    if (all.size() != (size_t)P.tsum.n) return set_error(HMVM_ERR_UNSUPPORTED, "t entry without a t sum segment");
+    std::vector<int2> one(all.size());  // every entry, in entry order
+    for (size_t i = 0; i < all.size(); i++) one[i] = make_int2(all[i].x, all[i].y);
+    if ((rc = upload(&p->d_tdesc, one))) return rc;
+    p->n_tsum = (int)one.size();
+    if (!P.t_direct && !P.tmap.empty()) {  // multi-RHS: the entries the row maps write are not summed
+      std::vector<char> mapped((size_t)P.n_t, 0);
+      for (int32_t e : P.tmap) mapped[(size_t)e] = 1;
+      (void)t_descs(P.tsum, mapped, &multi);  // a subset of `all`, which passed
+    } else {
+      multi.swap(all);
+    }
+    if ((rc = upload(&p->d_tdesc_m, multi))) return rc;
+    p->n_tsum_m = (int)multi.size();
   }
   if (p->mode != 0 && (rc = partition_a(p.get()))) return rc;
   if (!P.tmap.empty() && (rc = upload(&p->d_tmap, P.tmap))) return rc;
@@ -827,7 +841,8 @@ int create_plan(Source &src, const hmvm_options *opt, hmvm_plan **out) {
   p->device_bytes = eb * (size_t)P.nvals + sizeof(TileDesc) * (P.tiles_a.size() + P.tiles_b.size()) +
                     sizeof(PanelDesc) * P.panels.size() + 8 * (size_t)(P.n_t + 2 * P.nd) +
                     slot_bytes(p->storage) * (size_t)(P.n_tp + P.n_z) +
-                    4 * (size_t)P.n_t + (p->d_mscale ? 12 : 4) * P.tmap.size() + sizeof(int4) * (size_t)p->n_tsum +
+                    4 * (size_t)P.n_t + (p->d_mscale ? 12 : 4) * P.tmap.size() +
+                    sizeof(int2) * (size_t)p->n_tsum + sizeof(int4) * (size_t)p->n_tsum_m +
                     8 * (P.ysum.list.size() + P.ysum.seg.size());
   if (verbose())
     std::fprintf(stderr,
@@ -1051,8 +1066,12 @@ int multi_layout(hmvm_plan *p) {
       *d = nullptr;
     }
   p->mtiles_wpb = 0;
+  // the multi-RHS launches send the rows of stage-1 tiles with a
+  // multi-RHS row map to t
+  std::vector<TileDesc> ta = p->plan.tiles_a;
+  for (TileDesc &t : ta) t.map_off = t.map_off_multi;
   int rc;
-  if ((rc = upload(&p->d_mtiles_a, xcd_layout(p->plan.tiles_a, wpb))) ||
+  if ((rc = upload(&p->d_mtiles_a, xcd_layout(ta, wpb))) ||
       (rc = upload(&p->d_mtiles_b, xcd_layout(p->plan.tiles_b, wpb))))
     return rc;
   p->mtiles_wpb = wpb;
@@ -1120,16 +1139,18 @@ void launch_segsum_multi(hmvm_plan *p, int phase, double *Y, int nrhs, int accum
   const P *slots = (const P *)p->d_mslots;
   if (phase == 1) {
     // 16-byte loads and stores (slots and t are the plan's own allocations)
-    if (P_.t_fused || p->n_tsum == 0) return;
+    const int4 *desc = p->d_tdesc_m;
+    const int n = p->n_tsum_m;
+    if (P_.t_fused || n == 0) return;
     // t in the multi-RHS operand type (VecOf: FP64, FP32 or BF16 bits)
-    const int64_t tw = (int64_t)p->n_tsum * (nrhs / (16 / (int)sizeof(P)));
+    const int64_t tw = (int64_t)n * (nrhs / (16 / (int)sizeof(P)));
     const dim3 grid((unsigned)((tw + 255) / 256));
     P *T = (P *)p->d_mt;
     if (p->storage == kBF16)
-      hipLaunchKernelGGL((tsum_multi_kernel<true, P, uint16_t>), grid, dim3(256), 0, st, p->n_tsum, nrhs,
-                         p->d_tdesc, slots, p->d_tfac, p->d_scales, (uint16_t *)p->d_mt);
+      hipLaunchKernelGGL((tsum_multi_kernel<true, P, uint16_t>), grid, dim3(256), 0, st, n, nrhs, desc, slots,
+                         p->d_tfac, p->d_scales, (uint16_t *)p->d_mt);
     else
-      hipLaunchKernelGGL((tsum_multi_kernel<false, P, P>), grid, dim3(256), 0, st, p->n_tsum, nrhs, p->d_tdesc,
+      hipLaunchKernelGGL((tsum_multi_kernel<false, P, P>), grid, dim3(256), 0, st, n, nrhs, desc,
                          slots, (const int32_t *)nullptr, (const float *)nullptr, T);
   } else if (((uintptr_t)Y & 15) == 0) {  // 16-byte slot loads where the caller's Y is 16-byte aligned
     constexpr int W = 16 / sizeof(P);

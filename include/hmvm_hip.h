@@ -115,9 +115,10 @@ typedef struct hmvm_options {
                          /* stage 1/2                                           */
   int t_mode;            /* stage-2 vector entries: 0 sum the stage-1 partial   */
                          /* slots on the fly, 1 read a materialised t (one more */
-                         /* launch; the stage-1 tiles of a row group of one     */
-                         /* column chunk write their t rows themselves and the  */
-                         /* t sum adds the rest), 2 choose per leaf (likewise), */
+                         /* launch; with several right-hand sides the stage-1   */
+                         /* tiles of a row group of one column chunk write      */
+                         /* their t rows themselves and the t sum adds the      */
+                         /* rest), 2 choose per leaf (likewise),                */
                          /* 3 t written directly by the stage-1 tiles (bands of */
                          /* rank rows over all of a column cluster's columns;   */
                          /* no extra launch), -1 default (1)                    */
@@ -390,8 +391,9 @@ int hmvm_bicgstab_static(hmvm_plan *plan, const double *d_b, double *d_u, int ma
  * The schedule a plan runs, as plain arrays: the autotuner's features, and
  * the CPU tests that replay it.  Element (r, j) of a tile's panel p is
  * values[val_off + r + j*ld]; row r of a tile goes to slot out_off + r, or to
- * t[tmap[map_off + r]] when map_off >= 0 (t entries so written have seg -1 in
- * the t sum, which leaves them alone); the panel's vector entry j is
+ * t[tmap[map_off + r]] when map_off >= 0 (in the multi-RHS launches: when
+ * map_off_multi >= 0, and their t sum leaves the entries so written alone);
+ * the panel's vector entry j is
  * x[vec_off + j] (vec_sel 0), t[vec_off + j] (1) or
  * sum_{c<nch} slots[vec_off + c*stride + j] (2, vec_chunks = nch<<16 | stride).  Repack record: values[dst_off + r +
  * c*dst_cs] = source[src_off + r*src_rs + c*src_cs].  Segment sums:
@@ -400,9 +402,10 @@ typedef struct hmvm_panel {
   long long val_off;
   int ld, ncols, vec_off, vec_sel, factor, vec_chunks;
 } hmvm_panel;
-/* first: a copy of panel panel0; ncols: the columns of all its panels */
+/* first: a copy of panel panel0; ncols: the columns of all its panels;
+ * map_off_multi: the row map the multi-RHS launches use in place of map_off */
 typedef struct hmvm_tile {
-  int nrows, panel0, npanel, out_off, map_off, flags, ncols, pad;
+  int nrows, panel0, npanel, out_off, map_off, flags, ncols, map_off_multi;
   hmvm_panel first;
 } hmvm_tile;
 typedef struct hmvm_repack {
