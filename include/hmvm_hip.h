@@ -180,8 +180,10 @@ int hmvm_plan_apply_host(hmvm_plan *plan, const double *x, double *y, int accumu
  * (kernels only) and returns the seconds per apply (the C++ hmvm_hip<T>
  * benchmark loop). */
 int hmvm_plan_bench_apply(hmvm_plan *plan, const double *x, int iters, double *seconds);
-/* Times `iters` applies on the plan's internal device vectors after `warmup`
- * untimed ones, with HIP events around every launch. */
+/* Times applies on the plan's internal device vectors after `warmup` untimed
+ * ones: ms_avg/min/median over `iters` single applies (an event pair around
+ * each), the per-launch averages over `iters` more with an event after every
+ * launch, ms_stream over `iters` applies enqueued back to back. */
 int hmvm_plan_time(hmvm_plan *plan, int warmup, int iters, hmvm_timing *timing);
 /* Times `iters` single applies (after one untimed), each preceded by a kernel
  * that writes flush_bytes of scratch memory so that no factor is still in L2
