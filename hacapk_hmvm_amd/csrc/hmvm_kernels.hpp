@@ -749,9 +749,13 @@ __device__ __forceinline__ void grid_barrier(PersistCounters *c, int phase, int 
 }
 
 // Tiles of launch A (tiles_a[ptr_a[w] .. ptr_a[w+1]) for wave w) and B
-// likewise; n_t t entries (0: no t sum), nd y rows.
+// likewise; n_t t entries (0: no t sum), nd y rows.  Waves per SIMD: 4 for
+// FP64 and FP32 (128 registers, no spills); 3 for BF16, whose scaled tile
+// code inside the stage loop spilled 84 bytes per lane at 128 registers
+// (measured at N=101,250: BF16 0.191 -> 0.168 ms at 3 waves; FP32 0.235 ->
+// 0.256 and FP64 unchanged, so they keep 4).
 template <typename S, bool kScaled, int kLoads, bool kPipe, bool kNT, bool kBV>
-__global__ __launch_bounds__(256) HMVM_TILE_OCCUPANCY void persistent_kernel(
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(S) == 2 ? 3 : 4))) void persistent_kernel(
     const TileDesc *__restrict__ tiles_a, const int32_t *__restrict__ ptr_a, const TileDesc *__restrict__ tiles_b,
     const int32_t *__restrict__ ptr_b, const PanelDesc *__restrict__ panels, const S *__restrict__ vals,
     const double *__restrict__ x_vec, double *t, typename SlotOf<S>::T *slots, const float *__restrict__ scales,
